@@ -1,0 +1,121 @@
+"""ctypes binding of libumaphip.so (the C ABI declared in include/umaphip.h).
+
+The library is built in-tree by ``cuscan_amd/build.py`` (``__graft_entry__.build``).
+Loading fails loudly when it is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libumaphip.so"
+
+OK, INVALID_INPUT, INVALID_CONFIG, MEMORY, CAPACITY, NUMERICAL, BACKEND, OVERFLOW = range(8)
+
+
+class UmapError(RuntimeError):
+    """Raised for numerical, capacity, overflow and backend failures."""
+
+
+class EpochArgs(C.Structure):
+    _fields_ = [
+        ("h", C.c_int64), ("n", C.c_int64), ("d", C.c_int64),
+        ("head", C.c_void_p), ("head_s0", C.c_int64), ("head_s1", C.c_int64),
+        ("fixed_tail", C.c_void_p), ("tail_s0", C.c_int64), ("tail_s1", C.c_int64),
+        ("offsets", C.c_void_p), ("indices", C.c_void_p),
+        ("source_order", C.c_void_p), ("negative_order", C.c_void_p),
+        ("source_ranks", C.c_void_p),
+        ("periods", C.c_void_p), ("next_positive", C.c_void_p),
+        ("negative_periods", C.c_void_p), ("next_negative", C.c_void_p),
+        ("updates", C.c_void_p), ("upd_s0", C.c_int64), ("upd_s1", C.c_int64),
+        ("m", C.c_void_p), ("m_s0", C.c_int64), ("m_s1", C.c_int64),
+        ("v", C.c_void_p), ("v_s0", C.c_int64), ("v_s1", C.c_int64),
+        ("epoch", C.c_uint32),
+        ("a", C.c_float), ("b", C.c_float), ("gamma", C.c_float), ("scale", C.c_float),
+        ("negative_range", C.c_uint32),
+        ("alpha", C.c_float), ("mu", C.c_float),
+        ("correction1", C.c_float), ("correction2", C.c_float),
+        ("momentum", C.c_int32), ("shared", C.c_int32), ("negatives", C.c_int32),
+        ("exclude_graph_neighbors", C.c_int32),
+        ("n_edges", C.c_int64),
+    ]
+
+
+class LayoutParams(C.Structure):
+    _fields_ = [
+        ("optimizer", C.c_int32), ("epochs", C.c_int32),
+        ("a", C.c_float), ("b", C.c_float), ("repulsion", C.c_float),
+        ("initial_alpha", C.c_float),
+        ("negative_sample_rate", C.c_int64), ("negative_selection_range", C.c_int64),
+        ("has_scale", C.c_int32), ("scale", C.c_float),
+        ("adaptation_samples", C.c_int64),
+        ("exclude_graph_neighbors", C.c_int32), ("good_initialization", C.c_int32),
+        ("seed", C.c_uint64),
+    ]
+
+
+class LayoutStats(C.Structure):
+    _fields_ = [
+        ("epochs_run", C.c_int64), ("adaptations", C.c_int64), ("refreshes", C.c_int64),
+        ("final_scale", C.c_float), ("device_ms", C.c_double),
+    ]
+
+
+# Every exported symbol of include/umaphip.h, with its C signature.
+_vp = C.c_void_p
+_i64 = C.c_int64
+_i32 = C.c_int32
+SIGNATURES = {
+    "umaphip_last_error": (C.c_char_p, []),
+    "umaphip_version": (C.c_int32, []),
+    "umaphip_device_count": (C.c_int32, []),
+    "umaphip_modern_epoch": (C.c_int, [C.POINTER(EpochArgs), _vp]),
+    "umaphip_debug_negatives": (C.c_int, [C.POINTER(EpochArgs), _i32, _vp, _vp, _vp, _vp]),
+    "umaphip_project": (C.c_int, [_vp, _i64, _i64, _i64, _i64, _vp, _vp, _vp]),
+    "umaphip_argsort_workspace_bytes": (C.c_int, [_i64, C.POINTER(C.c_size_t)]),
+    "umaphip_stable_argsort_f32": (C.c_int, [_vp, _i64, _vp, _vp, _vp, _vp, C.c_size_t, _vp]),
+    "umaphip_projection_ranks": (C.c_int, [_vp, _vp, _i64, _vp, _i64, _vp, _i32, _vp]),
+    "umaphip_sample_resultants": (C.c_int, [_vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i64,
+                                            C.c_uint32, C.c_float, C.c_float, C.c_float, _i32,
+                                            _vp, _vp]),
+    "umaphip_optimize_compact": (C.c_int, [_i64, _i64, _i64, _vp, _vp, _vp, _vp, _i64, _vp, _vp,
+                                           _vp, _vp, C.POINTER(LayoutParams), _i32,
+                                           C.POINTER(LayoutStats)]),
+    "umaphip_plan_create": (C.c_int, [_i64, _i64, _i64, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp,
+                                      _vp, C.POINTER(LayoutParams), _i32, C.POINTER(_vp)]),
+    "umaphip_plan_run": (C.c_int, [_vp]),
+    "umaphip_plan_download": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(LayoutStats)]),
+    "umaphip_plan_stream": (_vp, [_vp]),
+    "umaphip_plan_destroy": (None, [_vp]),
+    "umaphip_orders_at_epoch": (C.c_int, [_i64, _i64, C.POINTER(LayoutParams), _i32, _i32, _vp,
+                                          _vp]),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(
+                f"{LIB_PATH} is missing: build the HIP extension first "
+                "(python -c 'import __graft_entry__ as g; g.build()')")
+        L = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status: int) -> None:
+    if status == OK:
+        return
+    msg = lib().umaphip_last_error().decode(errors="replace")
+    if status in (INVALID_INPUT, INVALID_CONFIG):
+        raise ValueError(msg)
+    if status == MEMORY:
+        raise MemoryError(msg)
+    raise UmapError(msg)

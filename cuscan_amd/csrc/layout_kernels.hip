@@ -1,0 +1,872 @@
+// Modern-layout device kernels for gfx950 (CDNA4, wave64).
+//
+// Semantics follow crates/umap/src/kernels/standard_umap/modern_layout.rs.
+// The build disables floating-point contraction so that the f64 clocks
+// (next_negative += count * period) and every f32 force term round exactly
+// as the reference's uncontracted arithmetic does.
+#include <hip/hip_runtime.h>
+
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include <cstdint>
+
+#include "common.hpp"
+#include "layout_kernels.hpp"
+
+namespace umaphip {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+inline unsigned grid_for(int64_t n, int threads = kThreads) {
+    return static_cast<unsigned>((n + threads - 1) / threads);
+}
+
+// ------------------------------------------------------------ force terms
+// Attraction coefficient (gradient_kernel :166-175), multiplier folded as
+// factor * multiplier * (-2ab q^(b-1)) / (1 + a q^b).
+__device__ inline float attract_coeff(float q, float a, float b, float fm) {
+    float num = -2.0f * a;
+    num = num * b;
+    num = num * powf(q, b - 1.0f);
+    const float top = fm * num;
+    const float apb = a * powf(q, b);
+    return top / (1.0f + apb);
+}
+
+// Repulsion coefficient (repel :91-126); returns 0 when the reference
+// skips the contribution.
+__device__ inline float repel_coeff(float q, float a, float b, float gamma, float s,
+                                    float multiplier) {
+    if (!(q > 0.0f && gamma > 0.0f)) return 0.0f;
+    float num = s * 2.0f;
+    num = num * gamma;
+    num = num * b;
+    const float den1 = 0.001f + q;
+    const float apb = a * powf(q, b);
+    const float den = den1 * (1.0f + apb);
+    const float coeff = num / den;
+    if (!(coeff > 0.0f)) return 0.0f;
+    const float norm = sqrtf((coeff * coeff) * q);
+    if (!(norm > 0.0f)) return 0.0f;
+    const float g16 = gamma < 16.0f ? gamma : 16.0f;
+    float clip = g16 * tanhf(norm / gamma);
+    clip = clip / norm;
+    return (multiplier * coeff) * clip;
+}
+
+// Binary search of j in a sorted CSR row, or the source itself when fit
+// (excluded, :63-89).
+__device__ inline bool excluded(const int32_t *indices, int64_t begin, int64_t stop, int64_t i,
+                                int64_t j, bool shared) {
+    int64_t lo = begin, hi = stop;
+    while (lo < hi) {
+        const int64_t mid = lo + (hi - lo) / 2;
+        if (static_cast<int64_t>(static_cast<uint32_t>(indices[mid])) < j) lo = mid + 1;
+        else hi = mid;
+    }
+    bool hit = lo < stop && static_cast<int64_t>(static_cast<uint32_t>(indices[lo])) == j;
+    return hit || (shared && i == j);
+}
+
+// Negative candidate enumeration (gradient_kernel :177-250). Produces the
+// rank into negative_order of each candidate, in the reference's order.
+// e * (epoch + p + 1) mod M is advanced by adding e mod M, which equals the
+// reference's u64 product whenever that product does not wrap (E < 2^31,
+// epoch < 2^32).
+struct NegWalk {
+    uint64_t modulus;  // R (large) or N (small)
+    uint64_t step;     // e mod modulus
+    uint64_t cur;      // e*(epoch+1+p) mod modulus
+    uint64_t first;    // window start (large)
+    uint64_t left;     // samples (plain) or attempts left (retry)
+    uint64_t need;     // accepted still required (retry)
+    int mode;          // 0 retry, 1 small plain, 2 large
+    __device__ inline void init(uint64_t e, uint32_t epoch, int32_t count, bool small,
+                                bool exclude, uint64_t n, uint64_t range, int64_t rank) {
+        mode = small ? (exclude ? 0 : 1) : 2;
+        modulus = small ? n : range;
+        step = e % modulus;
+        cur = (step * ((static_cast<uint64_t>(epoch) + 1u) % modulus)) % modulus;
+        first = 0;
+        if (!small && range < n) {
+            const uint64_t rk = static_cast<uint64_t>(static_cast<uint32_t>(rank));
+            const uint64_t half = range / 2;
+            if (rk > half) first = rk - half;
+            if (first > n - range) first = n - range;
+        }
+        if (mode == 0) {
+            left = static_cast<uint64_t>(count) * n;
+            need = static_cast<uint64_t>(count);
+        } else {
+            left = static_cast<uint64_t>(count);
+            need = 0;
+        }
+    }
+    // Next candidate rank; false when done. For retry, report acceptance
+    // with accept() after checking exclusion.
+    __device__ inline bool next(uint64_t &rank) {
+        if (mode == 0) {
+            if (need == 0 || left == 0) return false;
+            rank = cur;
+            --left;
+            cur += 1;
+            if (cur >= modulus) cur -= modulus;
+            return true;
+        }
+        if (left == 0) return false;
+        rank = first + cur;
+        --left;
+        cur += step;
+        if (cur >= modulus) cur -= modulus;
+        return true;
+    }
+    __device__ inline void accept() {
+        if (mode == 0) --need;
+    }
+};
+
+// ------------------------------------------------- generic per-slot kernel
+__device__ inline float head_c(const RowwiseArgs &A, int64_t row, int64_t c) {
+    return A.head[row * A.hs0 + c * A.hs1];
+}
+__device__ inline float tgt_c(const RowwiseArgs &A, int64_t row, int64_t c) {
+    return A.shared ? A.head[row * A.hs0 + c * A.hs1] : A.tail[row * A.ts0 + c * A.ts1];
+}
+
+__device__ inline void rowwise_accumulate(const RowwiseArgs &A, int64_t i, int64_t j,
+                                          float coeff) {
+    for (int64_t c = 0; c < A.d; ++c) {
+        const float diff = head_c(A, i, c) - tgt_c(A, j, c);
+        float *u = &A.updates[i * A.us0 + c * A.us1];
+        *u = *u + coeff * diff;
+    }
+}
+
+__device__ inline float rowwise_distance(const RowwiseArgs &A, int64_t i, int64_t j) {
+    float q = 0.0f;
+    for (int64_t c = 0; c < A.d; ++c) {
+        const float z = head_c(A, i, c) - tgt_c(A, j, c);
+        q += z * z;
+    }
+    return q;
+}
+
+__global__ __launch_bounds__(64) void gradient_rowwise_kernel(RowwiseArgs A, int64_t start,
+                                                              int64_t end) {
+    const int64_t slot = start + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (slot >= end) return;
+    const int64_t i = static_cast<uint32_t>(A.source_order[slot]);
+    const bool small = A.h <= 1024;
+    const double now = static_cast<double>(A.epoch);
+    const int64_t begin = A.offsets[i], stop = A.offsets[i + 1];
+    for (int64_t e = begin; e < stop; ++e) {
+        const double npos = A.next_positive[e];
+        if (!(npos <= now)) continue;
+        const int64_t j = static_cast<uint32_t>(A.indices[e]);
+        const float q = rowwise_distance(A, i, j);
+        if (q > 0.0f) {
+            const float fm = (small ? 0.5f : 1.0f) * A.multiplier;
+            rowwise_accumulate(A, i, j, attract_coeff(q, A.a, A.b, fm));
+        }
+        A.next_positive[e] = npos + A.periods[e];
+        if (A.negatives) {
+            const double nn = A.next_negative[e];
+            const double np = A.negative_periods[e];
+            const int32_t count = static_cast<int32_t>((now - nn) / np);
+            if (count > 0) {
+                NegWalk w;
+                w.init(static_cast<uint64_t>(e), A.epoch, count, small, A.exclude != 0,
+                       static_cast<uint64_t>(A.n), A.range,
+                       (!small && A.range < static_cast<uint64_t>(A.n)) ? A.source_ranks[i] : 0);
+                uint64_t rank;
+                const float s = small ? 1.0f : (A.scale_ptr ? *A.scale_ptr : A.scale);
+                while (w.next(rank)) {
+                    const int64_t jj = static_cast<uint32_t>(A.negative_order[rank]);
+                    if (A.exclude && excluded(A.indices, begin, stop, i, jj, A.shared != 0))
+                        continue;
+                    w.accept();
+                    const float c = repel_coeff(rowwise_distance(A, i, jj), A.a, A.b, A.gamma, s,
+                                                A.multiplier);
+                    if (c != 0.0f) rowwise_accumulate(A, i, jj, c);
+                }
+            }
+            A.next_negative[e] = nn + static_cast<double>(count) * np;
+        }
+    }
+}
+
+__global__ void debug_negatives_kernel(RowwiseArgs A, int32_t cap, int32_t *count_out,
+                                       int32_t *cand_out, uint8_t *kept_out, int32_t *overflow) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= A.h) return;
+    const bool small = A.h <= 1024;
+    const double now = static_cast<double>(A.epoch);
+    const int64_t begin = A.offsets[i], stop = A.offsets[i + 1];
+    for (int64_t e = begin; e < stop; ++e) {
+        if (!(A.next_positive[e] <= now)) {
+            count_out[e] = -1;
+            continue;
+        }
+        int32_t visited = 0;
+        if (A.negatives) {
+            const int32_t count =
+                static_cast<int32_t>((now - A.next_negative[e]) / A.negative_periods[e]);
+            if (count > 0) {
+                NegWalk w;
+                w.init(static_cast<uint64_t>(e), A.epoch, count, small, A.exclude != 0,
+                       static_cast<uint64_t>(A.n), A.range,
+                       (!small && A.range < static_cast<uint64_t>(A.n)) ? A.source_ranks[i] : 0);
+                uint64_t rank;
+                while (w.next(rank)) {
+                    const int64_t jj = static_cast<uint32_t>(A.negative_order[rank]);
+                    const bool keep =
+                        !(A.exclude && excluded(A.indices, begin, stop, i, jj, A.shared != 0));
+                    if (keep) w.accept();
+                    if (visited < cap) {
+                        cand_out[e * cap + visited] = static_cast<int32_t>(jj);
+                        kept_out[e * cap + visited] = keep ? 1 : 0;
+                    } else {
+                        atomicExch(overflow, 1);
+                    }
+                    ++visited;
+                }
+            }
+        }
+        count_out[e] = visited;
+    }
+}
+
+__global__ void adam_kernel(int64_t h, int64_t d, float *head, int64_t hs0, int64_t hs1,
+                            float *upd, int64_t us0, int64_t us1, float *m, int64_t ms0,
+                            int64_t ms1, float *v, int64_t vs0, int64_t vs1, float alpha,
+                            float c1, float c2) {
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= h * d) return;
+    const int64_t row = idx / d, col = idx % d;
+    float *u = &upd[row * us0 + col * us1];
+    const float g = *u;
+    if (g != 0.0f) {
+        float *mm = &m[row * ms0 + col * ms1];
+        float *vv = &v[row * vs0 + col * vs1];
+        const float first = 0.9f * *mm + (1.0f - 0.9f) * g;
+        const float second = 0.99f * *vv + ((1.0f - 0.99f) * g) * g;
+        *mm = first;
+        *vv = second;
+        float *x = &head[row * hs0 + col * hs1];
+        *x = *x + (alpha * (first / c1)) / (sqrtf(second / c2) + 0.0001f);
+    }
+    *u = 0.0f;
+}
+
+__global__ void momentum_apply_kernel(int64_t d, float *head, int64_t hs0, int64_t hs1,
+                                      const float *vel, int64_t vs0, int64_t vs1,
+                                      const int32_t *order, int64_t start, int64_t end) {
+    const int64_t slot = start + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (slot >= end) return;
+    const int64_t row = static_cast<uint32_t>(order[slot]);
+    for (int64_t c = 0; c < d; ++c) {
+        float *x = &head[row * hs0 + c * hs1];
+        *x = *x + vel[row * vs0 + c * vs1];
+    }
+}
+
+__global__ void momentum_decay_kernel(int64_t h, int64_t d, float *vel, int64_t vs0,
+                                      int64_t vs1, float mu) {
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= h * d) return;
+    float *x = &vel[(idx / d) * vs0 + (idx % d) * vs1];
+    *x = *x * mu;
+}
+
+// ------------------------------------------------------ tiled edge sweep
+// One workgroup owns kTileRows consecutive CSR rows. Its edges stream in
+// chunks of 256 (one lane per edge, coalesced clock/index/period reads);
+// each lane computes its edge's attraction plus its negatives; per-row sums
+// are then formed in CSR order from LDS by one lane per row, so results are
+// deterministic and no global atomics are needed. The fused epilogue applies
+// the lazy Adam step into a second position buffer (Jacobi double buffer).
+constexpr int kTileRows = 64;
+
+__device__ inline float2 ld2(const float *p, int64_t row) {
+    return reinterpret_cast<const float2 *>(p)[row];
+}
+
+__global__ __launch_bounds__(kThreads) void edge_sweep2_kernel(SweepArgs A) {
+    __shared__ int32_t s_off[kTileRows + 1];
+    __shared__ float2 s_pos[kTileRows];
+    __shared__ int32_t s_rank[kTileRows];
+    __shared__ int8_t s_active[kTileRows];
+    __shared__ float2 s_contrib[kThreads];
+
+    const int tid = threadIdx.x;
+    const int64_t row0 = static_cast<int64_t>(blockIdx.x) * kTileRows;
+    const int rows = static_cast<int>(min<int64_t>(kTileRows, A.h - row0));
+    const bool small = A.h <= 1024;
+    const bool windowed = !small && static_cast<int64_t>(A.range) < A.n;
+    if (tid <= rows) s_off[tid] = A.offsets[row0 + tid];
+    if (tid < rows) {
+        s_pos[tid] = ld2(A.pos, row0 + tid);
+        s_rank[tid] = windowed ? A.ranks[row0 + tid] : 0;
+        s_active[tid] = A.block_of_row ? (A.block_of_row[row0 + tid] == A.block_id) : 1;
+    }
+    const float scale = A.scale_ptr ? *A.scale_ptr : A.scale;
+    __syncthreads();
+
+    const int32_t e_begin = s_off[0], e_end = s_off[rows];
+    const double now = static_cast<double>(A.epoch);
+    const float fm = small ? 0.5f : 1.0f;
+    const float s_neg = small ? 1.0f : scale;
+    float2 acc = make_float2(0.0f, 0.0f);
+
+    for (int32_t base = e_begin; base < e_end; base += kThreads) {
+        const int32_t e = base + tid;
+        float2 g = make_float2(0.0f, 0.0f);
+        if (e < e_end) {
+            int lo = 0, hi = rows;  // last r with s_off[r] <= e
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (s_off[mid] <= e) lo = mid;
+                else hi = mid;
+            }
+            const int r = lo;
+            const double npos = A.next_positive[e];
+            if (s_active[r] && npos <= now) {
+                const int64_t i = row0 + r;
+                const float2 xi = s_pos[r];
+                const int64_t j = static_cast<uint32_t>(A.indices[e]);
+                const float2 y = ld2(A.tgt, j);
+                const float dx = xi.x - y.x, dy = xi.y - y.y;
+                const float q = dx * dx + dy * dy;
+                if (q > 0.0f) {
+                    const float c = attract_coeff(q, A.a, A.b, fm);
+                    g.x = g.x + c * dx;
+                    g.y = g.y + c * dy;
+                }
+                A.next_positive[e] = npos + A.periods[e];
+                if (A.negatives) {
+                    const double nn = A.next_negative[e];
+                    const double np = A.negative_periods[e];
+                    const int32_t count = static_cast<int32_t>((now - nn) / np);
+                    if (count > 0) {
+                        NegWalk w;
+                        w.init(static_cast<uint64_t>(e), A.epoch, count, small, A.exclude != 0,
+                               static_cast<uint64_t>(A.n), A.range, s_rank[r]);
+                        uint64_t rank;
+                        const int64_t begin = s_off[r], stop = s_off[r + 1];
+                        while (w.next(rank)) {
+                            float2 yn;
+                            if (A.exclude) {
+                                const int64_t jj = static_cast<uint32_t>(A.order[rank]);
+                                if (excluded(A.indices, begin, stop, i, jj, A.shared != 0))
+                                    continue;
+                                w.accept();
+                                yn = A.table ? ld2(A.table, rank) : ld2(A.tgt, jj);
+                            } else {
+                                yn = A.table ? ld2(A.table, rank)
+                                             : ld2(A.tgt, static_cast<uint32_t>(A.order[rank]));
+                            }
+                            const float nx = xi.x - yn.x, ny = xi.y - yn.y;
+                            const float qn = nx * nx + ny * ny;
+                            const float c = repel_coeff(qn, A.a, A.b, A.gamma, s_neg, 1.0f);
+                            if (c != 0.0f) {
+                                g.x = g.x + c * nx;
+                                g.y = g.y + c * ny;
+                            }
+                        }
+                    }
+                    A.next_negative[e] = nn + static_cast<double>(count) * np;
+                }
+            }
+        }
+        s_contrib[tid] = g;
+        __syncthreads();
+        if (tid < rows) {
+            const int32_t lo = max(s_off[tid], base);
+            const int32_t hi = min(s_off[tid + 1], base + kThreads);
+            for (int32_t k = lo; k < hi; ++k) {
+                const float2 c = s_contrib[k - base];
+                acc.x = acc.x + c.x;
+                acc.y = acc.y + c.y;
+            }
+        }
+        __syncthreads();
+    }
+
+    if (tid >= rows || !s_active[tid]) return;
+    const int64_t i = row0 + tid;
+    if (A.fused) {
+        const float2 x = s_pos[tid];
+        float2 mm = ld2(A.m, i), vv = ld2(A.v, i);
+        float2 out = x;
+        const float gs[2] = {acc.x, acc.y};
+        float *mp = &mm.x, *vp = &vv.x, *op = &out.x;
+        const float xs[2] = {x.x, x.y};
+        for (int c = 0; c < 2; ++c) {
+            const float gc = gs[c];
+            if (gc != 0.0f) {
+                const float first = 0.9f * mp[c] + (1.0f - 0.9f) * gc;
+                const float second = 0.99f * vp[c] + ((1.0f - 0.99f) * gc) * gc;
+                mp[c] = first;
+                vp[c] = second;
+                op[c] = xs[c] + (A.alpha * (first / A.c1)) / (sqrtf(second / A.c2) + 0.0001f);
+            }
+        }
+        reinterpret_cast<float2 *>(A.m)[i] = mm;
+        reinterpret_cast<float2 *>(A.v)[i] = vv;
+        reinterpret_cast<float2 *>(A.pos_out)[i] = out;
+    } else {
+        float2 u = ld2(A.updates, i);
+        if (acc.x != 0.0f || acc.y != 0.0f) {
+            u.x = u.x + A.multiplier * acc.x;
+            u.y = u.y + A.multiplier * acc.y;
+            reinterpret_cast<float2 *>(A.updates)[i] = u;
+        }
+    }
+}
+
+// ------------------------------------------------------------ projection
+__device__ inline uint32_t radix_key(float v) {
+    uint32_t bits = __float_as_uint(v);
+    if (bits == 0x80000000u) bits = 0u;
+    return (bits & 0x80000000u) ? ~bits : (bits ^ 0x80000000u);
+}
+
+__device__ inline float key_value(uint32_t key) {
+    const uint32_t bits = (key & 0x80000000u) ? (key ^ 0x80000000u) : ~key;
+    return __uint_as_float(bits);
+}
+
+__global__ void project_kernel(const float *x, int64_t rows, int64_t d, int64_t s0, int64_t s1,
+                               const float *dir, float *out) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= rows) return;
+    float sum = 0.0f;
+    for (int64_t c = 0; c < d; ++c) sum += x[i * s0 + c * s1] * dir[c];
+    out[i] = sum;
+}
+
+__global__ void project_keys_kernel(const float *x, int64_t rows, int64_t d, const float *dir,
+                                    uint32_t *keys, float *proj, unsigned long long *status,
+                                    unsigned long long tag) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= rows) return;
+    float sum = 0.0f;
+    if (d == 2) {
+        const float2 p = reinterpret_cast<const float2 *>(x)[i];
+        sum += p.x * dir[0];
+        sum += p.y * dir[1];
+    } else {
+        for (int64_t c = 0; c < d; ++c) sum += x[i * d + c] * dir[c];
+    }
+    if (proj) proj[i] = sum;
+    if (keys) keys[i] = radix_key(sum);
+    if (!isfinite(sum)) atomicMin(status, tag | static_cast<unsigned long long>(i));
+}
+
+__global__ void keys_from_values_kernel(const float *values, int64_t n, uint32_t *keys,
+                                        unsigned long long *status) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float v = values[i];
+    keys[i] = radix_key(v);
+    if (!isfinite(v)) atomicMin(status, static_cast<unsigned long long>(i));
+}
+
+__global__ void iota_kernel(int32_t *out, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = static_cast<int32_t>(i);
+}
+
+__global__ void values_from_keys_kernel(const uint32_t *keys, int64_t n, float *values) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) values[i] = key_value(keys[i]);
+}
+
+__global__ void inverse_ranks_kernel(const int32_t *order, int64_t n, int32_t *ranks) {
+    const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j < n) ranks[static_cast<uint32_t>(order[j])] = static_cast<int32_t>(j);
+}
+
+__global__ void lower_bound_kernel(const float *sorted, int64_t n, const float *query, int64_t h,
+                                   int32_t *ranks) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= h) return;
+    const float value = query[i];
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int64_t mid = lo + (hi - lo) / 2;
+        if (sorted[mid] < value) lo = mid + 1;
+        else hi = mid;
+    }
+    ranks[i] = static_cast<int32_t>(lo < n - 1 ? lo : n - 1);
+}
+
+__global__ void rank_and_table_kernel(const int32_t *order, int64_t n, const float *src,
+                                      int32_t *ranks, float *table) {
+    const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const uint32_t id = static_cast<uint32_t>(order[r]);
+    if (ranks) ranks[id] = static_cast<int32_t>(r);
+    reinterpret_cast<float2 *>(table)[r] = reinterpret_cast<const float2 *>(src)[id];
+}
+
+__global__ void block_of_row_kernel(const int32_t *source_order, int64_t h, int64_t block,
+                                    int32_t *block_of_row) {
+    const int64_t s = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (s < h) block_of_row[static_cast<uint32_t>(source_order[s])] = static_cast<int32_t>(s / block);
+}
+
+__global__ void momentum_apply_rows_kernel(int64_t h, float *head, const float *vel,
+                                           const int32_t *block_of_row, int32_t block_id) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= h || block_of_row[i] != block_id) return;
+    float2 x = reinterpret_cast<float2 *>(head)[i];
+    const float2 v = reinterpret_cast<const float2 *>(vel)[i];
+    x.x = x.x + v.x;
+    x.y = x.y + v.y;
+    reinterpret_cast<float2 *>(head)[i] = x;
+}
+
+__global__ void status_min_kernel(unsigned long long *status, unsigned long long value) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *status = value;
+}
+
+// ------------------------------------------------------------ adaptation
+__device__ double resultant_factor(const float *emb, int64_t d, const int32_t *offsets,
+                                   const int32_t *indices, int64_t src, int64_t j, double a,
+                                   double b, double gamma, bool exclude) {
+    if (exclude) {
+        const int64_t begin = offsets[src], stop = offsets[src + 1];
+        if (excluded(indices, begin, stop, src, j, true)) return 0.0;
+    }
+    double q = 0.0;
+    for (int64_t c = 0; c < d; ++c) {
+        const double z = static_cast<double>(emb[src * d + c]) - static_cast<double>(emb[j * d + c]);
+        q += z * z;
+    }
+    const double coefficient = 2.0 * gamma * b / ((0.001 + q) * (a * pow(q, b) + 1.0));
+    const double norm = coefficient * sqrt(q);
+    if (norm > 0.0) return coefficient * gamma * tanh(norm / gamma) / norm;
+    return 0.0;
+}
+
+__global__ void resultants_kernel(const float *emb, int64_t n, int64_t d, const int32_t *order,
+                                  const int32_t *offsets, const int32_t *indices,
+                                  const int32_t *sample_ranks, const int32_t *local_offsets,
+                                  const int32_t *global_ranks, int64_t samples,
+                                  uint32_t raw_range, float a, float b, float gamma, int exclude,
+                                  double *out) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= 2 * samples) return;
+    const bool local = t >= samples;
+    const int64_t s = local ? t - samples : t;
+    const int64_t src = static_cast<uint32_t>(order[static_cast<uint32_t>(sample_ranks[s])]);
+    int64_t js[5];
+    for (int k = 0; k < 5; ++k) {
+        int64_t rank = static_cast<uint32_t>(global_ranks[s * 5 + k]);
+        if (local) {
+            const int64_t range = raw_range;
+            const int64_t source_rank = static_cast<uint32_t>(sample_ranks[s]);
+            const int64_t half = range / 2;
+            int64_t first = source_rank > half ? source_rank - half : 0;
+            if (first > n - range) first = n - range;
+            rank = first + static_cast<uint32_t>(local_offsets[s * 5 + k]);
+        }
+        js[k] = static_cast<uint32_t>(order[rank]);
+    }
+    double f[5];
+    for (int k = 0; k < 5; ++k)
+        f[k] = resultant_factor(emb, d, offsets, indices, src, js[k], static_cast<double>(a),
+                                static_cast<double>(b), static_cast<double>(gamma), exclude != 0);
+    double squared = 0.0;
+    for (int64_t c = 0; c < d; ++c) {
+        const double x = static_cast<double>(emb[src * d + c]);
+        double sum = f[0] * (x - static_cast<double>(emb[js[0] * d + c]));
+        sum = sum + f[1] * (x - static_cast<double>(emb[js[1] * d + c]));
+        sum = sum + f[2] * (x - static_cast<double>(emb[js[2] * d + c]));
+        sum = sum + f[3] * (x - static_cast<double>(emb[js[3] * d + c]));
+        sum = sum + f[4] * (x - static_cast<double>(emb[js[4] * d + c]));
+        squared += sum * sum;
+    }
+    out[t] = sqrt(squared);
+}
+
+// negative_force_ratio + NegativeScaleAdaptation::observe_ratio
+// (umap-core/src/modern_layout.rs:565-653) in one workgroup.
+__global__ __launch_bounds__(256) void ratio_update_kernel(const double *res, int64_t samples,
+                                                           double *state, float *scale_out,
+                                                           int32_t *error_flag) {
+    __shared__ double logs[kDeviceRatioMax];
+    __shared__ int count_s;
+    __shared__ int bad_s;
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        // Sequential compaction keeps the valid pairs in sample order.
+        int count = 0, bad = 0;
+        for (int64_t s = 0; s < samples; ++s) {
+            const double g = res[s], l = res[samples + s];
+            if (!isfinite(g) || !isfinite(l) || g < 0.0 || l < 0.0) bad = 1;
+            if (g > 0.0 && l > 0.0) logs[count++] = log(g) - log(l);
+        }
+        count_s = count;
+        bad_s = bad;
+    }
+    __syncthreads();
+    const int count = count_s;
+    if (bad_s) {
+        if (tid == 0) *error_flag = 1;
+        return;
+    }
+    // Bitonic sort of logs[0, pow2) padded with +inf.
+    int p2 = 1;
+    while (p2 < count) p2 <<= 1;
+    for (int k = count + tid; k < p2; k += blockDim.x) logs[k] = INFINITY;
+    __syncthreads();
+    for (int size = 2; size <= p2; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int k = tid; k < p2; k += blockDim.x) {
+                const int partner = k ^ stride;
+                if (partner > k) {
+                    const bool up = (k & size) == 0;
+                    const double x = logs[k], y = logs[partner];
+                    if ((x > y) == up) {
+                        logs[k] = y;
+                        logs[partner] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (tid != 0) return;
+    double ratio = 1.0;
+    if (count > 0) {
+        double qv[2];
+        const double qs[2] = {0.1, 0.9};
+        for (int k = 0; k < 2; ++k) {
+            const double position = static_cast<double>(count - 1) * qs[k];
+            const int lower = static_cast<int>(floor(position));
+            const int upper = static_cast<int>(ceil(position));
+            qv[k] = logs[lower] + (logs[upper] - logs[lower]) * (position - static_cast<double>(lower));
+        }
+        double sum = 0.0;
+        for (int s = 0; s < count; ++s) {
+            double v = logs[s];
+            v = v < qv[0] ? qv[0] : (v > qv[1] ? qv[1] : v);
+            sum += v;
+        }
+        ratio = exp(sum / static_cast<double>(count));
+        if (!isfinite(ratio) || ratio <= 0.0) {
+            *error_flag = 1;
+            return;
+        }
+    }
+    // state: prior, current, anchor, has_anchor
+    const double prior = state[0];
+    double current = state[1];
+    if (prior == 0.0) {
+        state[2] = 0.0;
+        state[3] = 1.0;
+        *scale_out = 0.0f;
+        return;
+    }
+    if (state[3] != 0.0) {
+        double target = state[2] * ratio;
+        const double lo = 0.5 * prior, hi = 2.0 * prior;
+        target = target < lo ? lo : (target > hi ? hi : target);
+        double change = log(target / current);
+        const double clo = log(0.9), chi = log(1.1);
+        change = change < clo ? clo : (change > chi ? chi : change);
+        current *= exp(0.1 * change);
+        state[1] = current;
+    } else {
+        const double anchor = current / ratio;
+        if (!isfinite(anchor) || anchor <= 0.0) {
+            *error_flag = 1;
+            return;
+        }
+        state[2] = anchor;
+        state[3] = 1.0;
+    }
+    *scale_out = static_cast<float>(current);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------- launchers
+void launch_gradient_rowwise(const RowwiseArgs &a, int64_t start, int64_t end, hipStream_t s) {
+    if (end <= start) return;
+    hipLaunchKernelGGL(gradient_rowwise_kernel, dim3(grid_for(end - start, 64)), dim3(64), 0, s,
+                       a, start, end);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_adam(int64_t h, int64_t d, float *head, int64_t hs0, int64_t hs1, float *upd,
+                 int64_t us0, int64_t us1, float *m, int64_t ms0, int64_t ms1, float *v,
+                 int64_t vs0, int64_t vs1, float alpha, float c1, float c2, hipStream_t s) {
+    if (h * d == 0) return;
+    hipLaunchKernelGGL(adam_kernel, dim3(grid_for(h * d)), dim3(kThreads), 0, s, h, d, head, hs0,
+                       hs1, upd, us0, us1, m, ms0, ms1, v, vs0, vs1, alpha, c1, c2);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_momentum_apply(int64_t d, float *head, int64_t hs0, int64_t hs1, const float *vel,
+                           int64_t vs0, int64_t vs1, const int32_t *order, int64_t start,
+                           int64_t end, hipStream_t s) {
+    if (end <= start) return;
+    hipLaunchKernelGGL(momentum_apply_kernel, dim3(grid_for(end - start)), dim3(kThreads), 0, s,
+                       d, head, hs0, hs1, vel, vs0, vs1, order, start, end);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_momentum_decay(int64_t h, int64_t d, float *vel, int64_t vs0, int64_t vs1, float mu,
+                           hipStream_t s) {
+    if (h * d == 0) return;
+    hipLaunchKernelGGL(momentum_decay_kernel, dim3(grid_for(h * d)), dim3(kThreads), 0, s, h, d,
+                       vel, vs0, vs1, mu);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_edge_sweep2(const SweepArgs &a, hipStream_t s) {
+    if (a.h == 0) return;
+    hipLaunchKernelGGL(edge_sweep2_kernel, dim3(grid_for(a.h, kTileRows)), dim3(kThreads), 0, s,
+                       a);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_debug_negatives(const RowwiseArgs &a, int32_t cap, int32_t *count_out,
+                            int32_t *cand_out, uint8_t *kept_out, int32_t *overflow,
+                            hipStream_t s) {
+    if (a.h == 0) return;
+    hipLaunchKernelGGL(debug_negatives_kernel, dim3(grid_for(a.h, 64)), dim3(64), 0, s, a, cap,
+                       count_out, cand_out, kept_out, overflow);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_project(const float *x, int64_t rows, int64_t d, int64_t s0, int64_t s1,
+                    const float *dir, float *out, hipStream_t s) {
+    if (rows == 0) return;
+    hipLaunchKernelGGL(project_kernel, dim3(grid_for(rows)), dim3(kThreads), 0, s, x, rows, d, s0,
+                       s1, dir, out);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_project_keys(const float *x, int64_t rows, int64_t d, const float *dir,
+                         uint32_t *keys, float *proj_out, unsigned long long *status,
+                         unsigned long long tag, hipStream_t s) {
+    if (rows == 0) return;
+    hipLaunchKernelGGL(project_keys_kernel, dim3(grid_for(rows)), dim3(kThreads), 0, s, x, rows, d,
+                       dir, keys, proj_out, status, tag);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_keys_from_values(const float *values, int64_t n, uint32_t *keys,
+                             unsigned long long *status, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(keys_from_values_kernel, dim3(grid_for(n)), dim3(kThreads), 0, s, values, n,
+                       keys, status);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_iota(int32_t *out, int64_t n, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(iota_kernel, dim3(grid_for(n)), dim3(kThreads), 0, s, out, n);
+    HIP_CHECK(hipGetLastError());
+}
+
+size_t sort_pairs_workspace(int64_t n) {
+    size_t bytes = 0;
+    HIP_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, static_cast<const uint32_t *>(nullptr),
+                                        static_cast<uint32_t *>(nullptr),
+                                        static_cast<const int32_t *>(nullptr),
+                                        static_cast<int32_t *>(nullptr),
+                                        static_cast<unsigned int>(n), 0u, 32u));
+    return bytes;
+}
+
+void launch_sort_pairs(void *ws, size_t ws_bytes, const uint32_t *keys_in, uint32_t *keys_out,
+                       const int32_t *vals_in, int32_t *vals_out, int64_t n, hipStream_t s) {
+    if (n == 0) return;
+    size_t bytes = ws_bytes;
+    HIP_CHECK(rocprim::radix_sort_pairs(ws, bytes, keys_in, keys_out, vals_in, vals_out,
+                                        static_cast<unsigned int>(n), 0u, 32u, s));
+}
+
+void launch_values_from_keys(const uint32_t *keys, int64_t n, float *values, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(values_from_keys_kernel, dim3(grid_for(n)), dim3(kThreads), 0, s, keys, n,
+                       values);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_inverse_ranks(const int32_t *order, int64_t n, int32_t *ranks, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(inverse_ranks_kernel, dim3(grid_for(n)), dim3(kThreads), 0, s, order, n,
+                       ranks);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_lower_bound(const float *sorted, int64_t n, const float *query, int64_t h,
+                        int32_t *ranks, hipStream_t s) {
+    if (h == 0) return;
+    hipLaunchKernelGGL(lower_bound_kernel, dim3(grid_for(h)), dim3(kThreads), 0, s, sorted, n,
+                       query, h, ranks);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_rank_and_table(const int32_t *order, int64_t n, const float *src, int32_t *ranks,
+                           float *table, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(rank_and_table_kernel, dim3(grid_for(n)), dim3(kThreads), 0, s, order, n,
+                       src, ranks, table);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_gather_table(const int32_t *order, int64_t n, const float *src, float *table,
+                         hipStream_t s) {
+    launch_rank_and_table(order, n, src, nullptr, table, s);
+}
+
+void launch_block_of_row(const int32_t *source_order, int64_t h, int64_t block,
+                         int32_t *block_of_row, hipStream_t s) {
+    if (h == 0) return;
+    hipLaunchKernelGGL(block_of_row_kernel, dim3(grid_for(h)), dim3(kThreads), 0, s, source_order,
+                       h, block, block_of_row);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_momentum_apply_rows(int64_t h, float *head, const float *vel,
+                                const int32_t *block_of_row, int32_t block_id, hipStream_t s) {
+    if (h == 0) return;
+    hipLaunchKernelGGL(momentum_apply_rows_kernel, dim3(grid_for(h)), dim3(kThreads), 0, s, h,
+                       head, vel, block_of_row, block_id);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_status_min_u64(unsigned long long *status, unsigned long long value, hipStream_t s) {
+    hipLaunchKernelGGL(status_min_kernel, dim3(1), dim3(64), 0, s, status, value);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_resultants(const float *emb, int64_t n, int64_t d, const int32_t *order,
+                       const int32_t *offsets, const int32_t *indices, const int32_t *sample_ranks,
+                       const int32_t *local_offsets, const int32_t *global_ranks, int64_t samples,
+                       uint32_t raw_range, float a, float b, float gamma, int exclude,
+                       double *out, hipStream_t s) {
+    if (samples == 0) return;
+    hipLaunchKernelGGL(resultants_kernel, dim3(grid_for(2 * samples, 64)), dim3(64), 0, s, emb, n,
+                       d, order, offsets, indices, sample_ranks, local_offsets, global_ranks,
+                       samples, raw_range, a, b, gamma, exclude, out);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_ratio_update(const double *resultants, int64_t samples, double *state,
+                         float *scale_out, int32_t *error_flag, hipStream_t s) {
+    hipLaunchKernelGGL(ratio_update_kernel, dim3(1), dim3(256), 0, s, resultants, samples, state,
+                       scale_out, error_flag);
+    HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace umaphip

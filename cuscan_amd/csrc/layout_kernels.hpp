@@ -1,0 +1,109 @@
+// Internal launch interface of the modern-layout kernels (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace umaphip {
+
+// Generic per-slot gradient (any d, strided coordinates, reference slot
+// order). Mirrors gradient_kernel (kernels/standard_umap/modern_layout.rs:
+// 128-257) lane-for-slot.
+struct RowwiseArgs {
+    int64_t h, n, d;
+    const float *head; int64_t hs0, hs1;
+    const float *tail; int64_t ts0, ts1;
+    const int32_t *offsets, *indices, *source_order, *negative_order, *source_ranks;
+    const double *periods; double *next_positive;
+    const double *negative_periods; double *next_negative;
+    float *updates; int64_t us0, us1;
+    uint32_t epoch; float a, b, gamma, scale; uint32_t range;
+    float multiplier;
+    int shared, negatives, exclude;
+    const float *scale_ptr;   // device scale (may be null: use `scale`)
+};
+void launch_gradient_rowwise(const RowwiseArgs &a, int64_t start, int64_t end, hipStream_t s);
+
+// Lazy Adam over [h, d] strided buffers (adam_kernel, :495-523).
+void launch_adam(int64_t h, int64_t d, float *head, int64_t hs0, int64_t hs1, float *upd,
+                 int64_t us0, int64_t us1, float *m, int64_t ms0, int64_t ms1, float *v,
+                 int64_t vs0, int64_t vs1, float alpha, float c1, float c2, hipStream_t s);
+// Momentum apply over source_order slots [start, end) (apply_kernel, :525-543).
+void launch_momentum_apply(int64_t d, float *head, int64_t hs0, int64_t hs1, const float *vel,
+                           int64_t vs0, int64_t vs1, const int32_t *order, int64_t start,
+                           int64_t end, hipStream_t s);
+// velocity *= mu (decay_kernel, :545-554).
+void launch_momentum_decay(int64_t h, int64_t d, float *vel, int64_t vs0, int64_t vs1, float mu,
+                           hipStream_t s);
+
+// Tiled edge sweep for d = 2 with contiguous float2 coordinates.
+struct SweepArgs {
+    int64_t h, n;
+    const float *pos;         // [h,2] Jacobi snapshot of the source rows
+    float *pos_out;           // fused Adam output [h,2]
+    const float *tgt;         // attraction targets: pos (fit) or fixed tail (transform)
+    const float *table;       // negatives by rank: table[r] = position of order[r]; may be null
+    const int32_t *order;     // negative order (ids by rank)
+    const int32_t *ranks;     // [h] window ranks (large windowed layouts)
+    const int32_t *offsets, *indices;
+    const double *periods; double *next_positive;
+    const double *negative_periods; double *next_negative;
+    float *updates;           // ACCUM mode: updates[i] += multiplier * g
+    float *m, *v;             // fused Adam moments
+    const int32_t *block_of_row; int32_t block_id;  // momentum block filter (may be null)
+    const float *scale_ptr;   // device scale (may be null: use `scale`)
+    float scale;
+    uint32_t epoch; float a, b, gamma; uint32_t range;
+    float multiplier, alpha, c1, c2;
+    int shared, negatives, exclude, fused;
+};
+void launch_edge_sweep2(const SweepArgs &a, hipStream_t s);
+
+// Debug: per-edge negative candidates (same selection code).
+void launch_debug_negatives(const RowwiseArgs &a, int32_t cap, int32_t *count_out,
+                            int32_t *cand_out, uint8_t *kept_out, int32_t *overflow,
+                            hipStream_t s);
+
+// Projection helpers.
+void launch_project(const float *x, int64_t rows, int64_t d, int64_t s0, int64_t s1,
+                    const float *dir, float *out, hipStream_t s);
+// keys[i] = radix key of the projection (d = 2 contiguous or generic) and
+// non-finite tracking: status = min(tag | row) over non-finite rows.
+void launch_project_keys(const float *x, int64_t rows, int64_t d, const float *dir,
+                         uint32_t *keys, float *proj_out, unsigned long long *status,
+                         unsigned long long tag, hipStream_t s);
+void launch_keys_from_values(const float *values, int64_t n, uint32_t *keys,
+                             unsigned long long *status, hipStream_t s);
+void launch_iota(int32_t *out, int64_t n, hipStream_t s);
+size_t sort_pairs_workspace(int64_t n);
+void launch_sort_pairs(void *ws, size_t ws_bytes, const uint32_t *keys_in, uint32_t *keys_out,
+                       const int32_t *vals_in, int32_t *vals_out, int64_t n, hipStream_t s);
+void launch_values_from_keys(const uint32_t *keys, int64_t n, float *values, hipStream_t s);
+void launch_inverse_ranks(const int32_t *order, int64_t n, int32_t *ranks, hipStream_t s);
+void launch_lower_bound(const float *sorted, int64_t n, const float *query, int64_t h,
+                        int32_t *ranks, hipStream_t s);
+// Fit: ranks[order[r]] = r and table[r] = src[order[r]] (d = 2) in one pass.
+void launch_rank_and_table(const int32_t *order, int64_t n, const float *src, int32_t *ranks,
+                           float *table, hipStream_t s);
+void launch_gather_table(const int32_t *order, int64_t n, const float *src, float *table,
+                         hipStream_t s);
+void launch_block_of_row(const int32_t *source_order, int64_t h, int64_t block,
+                         int32_t *block_of_row, hipStream_t s);
+void launch_momentum_apply_rows(int64_t h, float *head, const float *vel,
+                                const int32_t *block_of_row, int32_t block_id, hipStream_t s);
+void launch_status_min_u64(unsigned long long *status, unsigned long long value, hipStream_t s);
+
+// Adaptation.
+void launch_resultants(const float *emb, int64_t n, int64_t d, const int32_t *order,
+                       const int32_t *offsets, const int32_t *indices, const int32_t *sample_ranks,
+                       const int32_t *local_offsets, const int32_t *global_ranks, int64_t samples,
+                       uint32_t raw_range, float a, float b, float gamma, int exclude,
+                       double *out, hipStream_t s);
+// Device-side ratio + anchored scale update; state = {prior, current,
+// anchor, has_anchor}, writes scale_out (f32) and an error flag.
+constexpr int64_t kDeviceRatioMax = 4096;
+void launch_ratio_update(const double *resultants, int64_t samples, double *state,
+                         float *scale_out, int32_t *error_flag, hipStream_t s);
+
+}  // namespace umaphip

@@ -1,0 +1,154 @@
+"""Python front of the modern layout (host arrays in/out, HIP underneath).
+
+``optimize_compact`` mirrors cuscan's ``modern_runtime::optimize_compact``
+(crates/umap/src/standard_umap/modern_runtime.rs:1011-1035): a compact CSR
+graph with its f64 clocks, an initial head embedding and (for transform) a
+fixed tail embedding go in; the optimised embedding and the advanced clocks
+come out. ``LayoutPlan`` keeps everything resident in HBM so the same layout
+can be re-run without re-uploading (benchmarks, serving).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+
+
+@dataclass
+class LayoutParams:
+    """ModernLayoutParams (crates/umap-core/src/modern_layout.rs:9-70)."""
+
+    epochs: int
+    a: float
+    b: float
+    optimizer: str = "adam"
+    repulsion: float = 1.0
+    initial_alpha: float = 1.0
+    negative_sample_rate: int = 5
+    negative_selection_range: int = 200_000
+    negative_sample_scale: float | None = None
+    adaptation_samples: int = 128
+    exclude_graph_neighbors: bool = False
+    good_initialization: bool = False
+    seed: int = 42
+
+    def to_c(self) -> _lib.LayoutParams:
+        if self.optimizer not in ("adam", "momentum"):
+            raise ValueError("optimizer must be 'adam' or 'momentum' for the modern layout")
+        p = _lib.LayoutParams()
+        p.optimizer = 1 if self.optimizer == "momentum" else 0
+        p.epochs = int(self.epochs)
+        p.a, p.b = float(self.a), float(self.b)
+        p.repulsion = float(self.repulsion)
+        p.initial_alpha = float(self.initial_alpha)
+        p.negative_sample_rate = int(self.negative_sample_rate)
+        p.negative_selection_range = int(self.negative_selection_range)
+        p.has_scale = 0 if self.negative_sample_scale is None else 1
+        p.scale = 0.0 if self.negative_sample_scale is None else float(self.negative_sample_scale)
+        p.adaptation_samples = int(self.adaptation_samples)
+        p.exclude_graph_neighbors = int(bool(self.exclude_graph_neighbors))
+        p.good_initialization = int(bool(self.good_initialization))
+        p.seed = int(self.seed) & ((1 << 64) - 1)
+        return p
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+def _graph_arrays(graph):
+    return (np.ascontiguousarray(graph["offsets"], np.uint32),
+            np.ascontiguousarray(graph["indices"], np.uint32),
+            np.ascontiguousarray(graph["periods"], np.float64),
+            np.ascontiguousarray(graph["negative_periods"], np.float64))
+
+
+def optimize_compact(head, tail, graph, params: LayoutParams, *, shared: bool):
+    """Run every epoch on the device. Returns (embedding, next_positive,
+    next_negative, stats)."""
+    head = np.array(head, dtype=np.float32, order="C", copy=True)
+    h, d = head.shape
+    tail_arr = None
+    n = h
+    if not shared:
+        tail_arr = np.ascontiguousarray(tail, np.float32)
+        n = tail_arr.shape[0]
+        if tail_arr.shape[1] != d:
+            raise ValueError("tail embedding dimension differs from head")
+    offsets, indices, periods, negperiods = _graph_arrays(graph)
+    npos = np.array(graph["next_positive"], np.float64, copy=True)
+    nneg = np.array(graph["next_negative"], np.float64, copy=True)
+    if len(offsets) != h + 1:
+        raise ValueError("CSR offsets must have rows + 1 entries")
+    stats = _lib.LayoutStats()
+    p = params.to_c()
+    _lib.check(_lib.lib().umaphip_optimize_compact(
+        h, n, d, _ptr(head), None if tail_arr is None else _ptr(tail_arr), _ptr(offsets),
+        _ptr(indices), len(indices), _ptr(periods), _ptr(npos), _ptr(negperiods), _ptr(nneg),
+        C.byref(p), int(shared), C.byref(stats)))
+    return head, npos, nneg, {f: getattr(stats, f) for f, _ in _lib.LayoutStats._fields_}
+
+
+class LayoutPlan:
+    """A device-resident layout that can be re-run from its initial state."""
+
+    def __init__(self, head, tail, graph, params: LayoutParams, *, shared: bool):
+        self._keep = []
+        head = np.ascontiguousarray(head, np.float32)
+        self.h, self.d = head.shape
+        tail_arr = None
+        self.n = self.h
+        if not shared:
+            tail_arr = np.ascontiguousarray(tail, np.float32)
+            self.n = tail_arr.shape[0]
+        offsets, indices, periods, negperiods = _graph_arrays(graph)
+        npos = np.ascontiguousarray(graph["next_positive"], np.float64)
+        nneg = np.ascontiguousarray(graph["next_negative"], np.float64)
+        self.n_edges = len(indices)
+        p = params.to_c()
+        handle = C.c_void_p()
+        _lib.check(_lib.lib().umaphip_plan_create(
+            self.h, self.n, self.d, _ptr(head), None if tail_arr is None else _ptr(tail_arr),
+            _ptr(offsets), _ptr(indices), self.n_edges, _ptr(periods), _ptr(npos),
+            _ptr(negperiods), _ptr(nneg), C.byref(p), int(shared), C.byref(handle)))
+        self._handle = handle
+
+    @property
+    def stream(self) -> int:
+        return _lib.lib().umaphip_plan_stream(self._handle)
+
+    def run(self) -> None:
+        _lib.check(_lib.lib().umaphip_plan_run(self._handle))
+
+    def download(self, clocks: bool = False):
+        head = np.zeros((self.h, self.d), np.float32)
+        npos = np.zeros(self.n_edges, np.float64) if clocks else None
+        nneg = np.zeros(self.n_edges, np.float64) if clocks else None
+        stats = _lib.LayoutStats()
+        _lib.check(_lib.lib().umaphip_plan_download(
+            self._handle, _ptr(head), None if npos is None else _ptr(npos),
+            None if nneg is None else _ptr(nneg), C.byref(stats)))
+        out = {f: getattr(stats, f) for f, _ in _lib.LayoutStats._fields_}
+        return (head, npos, nneg, out) if clocks else (head, out)
+
+    def close(self) -> None:
+        if getattr(self, "_handle", None):
+            _lib.lib().umaphip_plan_destroy(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        self.close()
+
+
+def orders_at_epoch(h, n, params: LayoutParams, *, shared: bool, epoch: int):
+    """The (source, negative) orders in force at ``epoch`` (host RNG replay)."""
+    n = h if shared else n
+    src = np.zeros(h, np.int32)
+    neg = np.zeros(n, np.int32)
+    p = params.to_c()
+    _lib.check(_lib.lib().umaphip_orders_at_epoch(h, n, C.byref(p), int(shared), int(epoch),
+                                                  _ptr(src), _ptr(neg)))
+    return src, neg

@@ -1,0 +1,207 @@
+/*
+ * umaphip -- MI355X-native modern UMAP layout path behind a C ABI.
+ *
+ * Every entry point takes plain pointers and sizes. Device pointers are
+ * HIP device allocations; `stream` is a hipStream_t passed as void* (NULL =
+ * the null stream). Calls enqueue work in stream order and do not
+ * synchronise unless stated. No entry point allocates device memory except
+ * the plan/server constructors and the host-array drivers, which say so.
+ *
+ * Errors: every function returns a umaphip_status; umaphip_last_error()
+ * returns the calling thread's last message. The taxonomy mirrors cuscan's
+ * UmapError (crates/umap-core/src/lib.rs:26-58) and its Python mapping
+ * (crates/cuscan-python/src/lib.rs:373-384): INVALID_INPUT / INVALID_CONFIG
+ * -> ValueError, MEMORY -> MemoryError, the rest -> RuntimeError.
+ */
+#ifndef UMAPHIP_H
+#define UMAPHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    UMAPHIP_OK = 0,
+    UMAPHIP_INVALID_INPUT = 1,
+    UMAPHIP_INVALID_CONFIG = 2,
+    UMAPHIP_MEMORY = 3,
+    UMAPHIP_CAPACITY = 4,
+    UMAPHIP_NUMERICAL = 5,
+    UMAPHIP_BACKEND = 6,
+    UMAPHIP_OVERFLOW = 7
+} umaphip_status;
+
+/* Thread-local message of the last failing call on this thread. */
+const char *umaphip_last_error(void);
+/* Library ABI version (major * 10000 + minor * 100 + patch). */
+int32_t umaphip_version(void);
+/* Number of visible HIP devices (0 when none); never fails. */
+int32_t umaphip_device_count(void);
+
+/* ------------------------------------------------------------------------
+ * One modern epoch. Replaces StandardUmapOps::modern_epoch
+ * (crates/umap/src/backend.rs:144-232; CubeCL body
+ * crates/umap/src/kernels/standard_umap/modern_layout.rs:925-1026).
+ *
+ * Arguments mean what the reference's do. Device pointers; 1-D arrays are
+ * contiguous; coordinate arrays use (row, column) strides in elements.
+ * head, next_positive, next_negative, updates, m and v are overwritten in
+ * place and must not alias each other or the read-only inputs. `shared`
+ * selects fit (targets read from head); otherwise fixed_tail is read-only.
+ * Adam: the gradient is accumulated into updates (which therefore should
+ * hold zeros on entry) and one lazy Adam step consumes and clears it.
+ * Momentum: updates is the velocity; source_order is split into blocks of
+ * H (H <= 1024) or H/2 rows, each block's gradient (times alpha) is added
+ * to the velocity and applied before the next block; the velocity is then
+ * decayed by mu. Uses no scratch memory.
+ * ------------------------------------------------------------------------ */
+typedef struct {
+    int64_t h, n, d;
+    float *head;
+    int64_t head_s0, head_s1;
+    const float *fixed_tail;
+    int64_t tail_s0, tail_s1;
+    const int32_t *offsets;        /* h + 1 */
+    const int32_t *indices;        /* E, sorted within each row */
+    const int32_t *source_order;   /* h, permutation of rows */
+    const int32_t *negative_order; /* n, permutation of reference rows */
+    const int32_t *source_ranks;   /* h, read only when negative_range < n and h > 1024 */
+    const double *periods;
+    double *next_positive;
+    const double *negative_periods;
+    double *next_negative;
+    float *updates;
+    int64_t upd_s0, upd_s1;
+    float *m;
+    int64_t m_s0, m_s1;
+    float *v;
+    int64_t v_s0, v_s1;
+    uint32_t epoch;
+    float a, b, gamma, scale;
+    uint32_t negative_range;      /* effective window R in 1..=n */
+    float alpha, mu, correction1, correction2;
+    int32_t momentum, shared, negatives, exclude_graph_neighbors;
+    int64_t n_edges;              /* E = offsets[h] (host-known; checked by debug entry) */
+} umaphip_epoch_args;
+
+umaphip_status umaphip_modern_epoch(const umaphip_epoch_args *args, void *stream);
+
+/* Test hook (no reference counterpart): writes, for every edge of the
+ * epoch's graph, the negative candidates the gradient would visit at this
+ * epoch (count_out[e] = number visited, -1 when the edge does not fire;
+ * cand_out[e*cap + k] = candidate row, kept_out[e*cap + k] = 1 when not
+ * excluded). Runs the same device selection code as the gradient kernels.
+ * Reads but does not modify the clocks. Fails with CAPACITY when an edge
+ * visits more than `cap` candidates. Synchronises the stream. */
+umaphip_status umaphip_debug_negatives(const umaphip_epoch_args *args, int32_t cap,
+                                       int32_t *count_out, int32_t *cand_out,
+                                       uint8_t *kept_out, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Projection windows. Replace modern_project / modern_projection_ranks
+ * (kernels/standard_umap/modern_layout.rs:669-752) and the stable f32
+ * argsort (crates/cuscan-common/src/radix.rs:659, key :993-1003).
+ * ------------------------------------------------------------------------ */
+/* out[i] = sequential f32 sum_c x[i*s0 + c*s1] * direction[c]. */
+umaphip_status umaphip_project(const float *x, int64_t rows, int64_t d, int64_t s0, int64_t s1,
+                               const float *direction, float *out, void *stream);
+/* Workspace bytes needed by umaphip_stable_argsort_f32 for n keys. */
+umaphip_status umaphip_argsort_workspace_bytes(int64_t n, size_t *bytes);
+/* Stable ascending argsort of finite floats (-0 == +0). order[j] = row of
+ * rank j, sorted[j] = values[order[j]] (+0 for a signed zero). status_out
+ * (device, one int64) receives -1 or the lowest non-finite row. */
+umaphip_status umaphip_stable_argsort_f32(const float *values, int64_t n, int32_t *order,
+                                          float *sorted, int64_t *status_out, void *workspace,
+                                          size_t workspace_bytes, void *stream);
+/* shared: ranks[order[j]] = j (n entries). Otherwise ranks[i] =
+ * min(lower_bound(sorted, query[i]), n - 1) for h queries. */
+umaphip_status umaphip_projection_ranks(const int32_t *order, const float *sorted, int64_t n,
+                                        const float *query, int64_t h, int32_t *ranks,
+                                        int32_t shared, void *stream);
+
+/* Adaptation resultants. Replaces modern_sample_resultants
+ * (kernels/standard_umap/modern_layout.rs:754-823): 2*S float64 norms,
+ * global at [0,S), local at [S,2S). embedding is [n, d] contiguous. */
+umaphip_status umaphip_sample_resultants(const float *embedding, int64_t n, int64_t d,
+                                         const int32_t *order, const int32_t *offsets,
+                                         const int32_t *indices, const int32_t *sample_ranks,
+                                         const int32_t *local_offsets,
+                                         const int32_t *global_ranks, int64_t samples,
+                                         uint32_t raw_range, float a, float b, float gamma,
+                                         int32_t exclude, double *resultants, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Layout driver. Replaces modern_runtime::optimize_compact /
+ * optimize_compact_inner (crates/umap/src/standard_umap/modern_runtime.rs:
+ * 1011-1378): orders, projection refreshes, adaptation and the epoch loop,
+ * device resident with no per-epoch host synchronisation.
+ * ------------------------------------------------------------------------ */
+typedef struct {
+    int32_t optimizer;              /* 0 = adam, 1 = momentum */
+    int32_t epochs;
+    float a, b, repulsion, initial_alpha;
+    int64_t negative_sample_rate;
+    int64_t negative_selection_range;
+    int32_t has_scale;
+    float scale;
+    int64_t adaptation_samples;
+    int32_t exclude_graph_neighbors;
+    int32_t good_initialization;
+    uint64_t seed;
+} umaphip_layout_params;
+
+typedef struct {
+    int64_t epochs_run;
+    int64_t adaptations;
+    int64_t refreshes;
+    float final_scale;
+    double device_ms;               /* epoch loop, device events */
+} umaphip_layout_stats;
+
+/* Host arrays in, host arrays out (allocates and frees device memory).
+ * head [h,d] in/out; tail [n,d] read for transform (ignored when shared);
+ * CSR offsets [h+1], indices [E]; clocks [E] in/out. */
+umaphip_status umaphip_optimize_compact(int64_t h, int64_t n, int64_t d, float *head,
+                                        const float *tail, const uint32_t *offsets,
+                                        const uint32_t *indices, int64_t n_edges,
+                                        const double *periods, double *next_positive,
+                                        const double *negative_periods, double *next_negative,
+                                        const umaphip_layout_params *params, int32_t shared,
+                                        umaphip_layout_stats *stats);
+
+/* Resident plan: uploads once, then runs from device-resident initial
+ * state any number of times (bench / serving). */
+typedef struct umaphip_plan umaphip_plan;
+umaphip_status umaphip_plan_create(int64_t h, int64_t n, int64_t d, const float *head,
+                                   const float *tail, const uint32_t *offsets,
+                                   const uint32_t *indices, int64_t n_edges,
+                                   const double *periods, const double *next_positive,
+                                   const double *negative_periods, const double *next_negative,
+                                   const umaphip_layout_params *params, int32_t shared,
+                                   umaphip_plan **out);
+/* Resets the state to the uploaded initial state and enqueues every epoch
+ * on the plan's stream. Does not synchronise. */
+umaphip_status umaphip_plan_run(umaphip_plan *plan);
+/* Waits for the plan's stream; copies the embedding (h*d floats) and, when
+ * non-NULL, the clocks (E doubles each) to host. */
+umaphip_status umaphip_plan_download(umaphip_plan *plan, float *head, double *next_positive,
+                                     double *next_negative, umaphip_layout_stats *stats);
+/* The plan's stream (hipStream_t as void*). */
+void *umaphip_plan_stream(umaphip_plan *plan);
+void umaphip_plan_destroy(umaphip_plan *plan);
+
+/* Orders the driver would use (test hook for the host RNG): writes the
+ * source and negative orders in force at `epoch` for a layout with the
+ * given shapes/params, as the reference's ModernOrders would
+ * (modern_orders.rs:359-680). */
+umaphip_status umaphip_orders_at_epoch(int64_t h, int64_t n, const umaphip_layout_params *params,
+                                       int32_t shared, int32_t epoch, int32_t *source_out,
+                                       int32_t *negative_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,50 @@
+"""Shared builders for the GPU parity tests (graphs, tensors, metrics)."""
+import numpy as np
+
+from oracle import oracle as O
+
+
+def random_graph(n_rows, n_cols, k, seed, epochs, rate=5.0, symmetric=True):
+    """A kNN-like positive graph, compacted by the oracle (A1)."""
+    rng = np.random.default_rng(seed)
+    rows = np.repeat(np.arange(n_rows), k)
+    if symmetric:
+        assert n_rows == n_cols
+        cols = (rows + rng.integers(1, n_cols, rows.size)) % n_cols
+        w = rng.uniform(0.02, 1.0, rows.size).astype(np.float32)
+        rows, cols, w = (np.concatenate([rows, cols]), np.concatenate([cols, rows]),
+                         np.concatenate([w, w]))
+    else:
+        cols = rng.integers(0, n_cols, rows.size)
+        w = rng.uniform(0.02, 1.0, rows.size).astype(np.float32)
+    return O.compact_graph(n_rows, n_cols, rows, cols, w, epochs, rate)
+
+
+def advance_graph(g, epochs_done):
+    """Clocks as they stand after `epochs_done` epochs (data independent);
+    vectorised restatement of advance_clock (no fused multiply-adds)."""
+    npos = g["next_positive"].copy()
+    nneg = g["next_negative"].copy()
+    per, nper = g["periods"], g["negative_periods"]
+    for t in range(epochs_done):
+        fire = npos <= float(t)
+        k = np.trunc((float(t) - nneg[fire]) / nper[fire])
+        npos[fire] = npos[fire] + per[fire]
+        nneg[fire] = nneg[fire] + k * nper[fire]
+    out = dict(g)
+    out["next_positive"], out["next_negative"] = npos, nneg
+    return out
+
+
+def row_relative_error(g, g64):
+    """max over rows with ||g64|| > 1e-6 * max of ||g - g64|| / ||g64||."""
+    g = np.asarray(g, np.float64)
+    n64 = np.linalg.norm(g64, axis=1)
+    keep = n64 > 1e-6 * n64.max()
+    return float((np.linalg.norm(g - g64, axis=1)[keep] / n64[keep]).max())
+
+
+def corrections(epoch):
+    c1 = float(np.float32(1.0 - float(np.float32(0.9)) ** (epoch + 1)))
+    c2 = float(np.float32(1.0 - float(np.float32(0.99)) ** (epoch + 1)))
+    return c1, c2

@@ -1,0 +1,311 @@
+"""GPU parity of the modern layout against the oracle (CPU restatement).
+
+Bars: f64 clocks and negative candidate ids bit-exact; a single epoch's
+gradient within 1e-5 relative (per-row 2-norm, rows above 1e-6 of the
+largest) of the float64 oracle; reference-fixture coordinates within 3e-5.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from oracle import oracle as O  # noqa: E402
+from tests import fixture_cases as F  # noqa: E402
+from tests.gpu_helpers import advance_graph, corrections, random_graph, row_relative_error  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+GRAD_TOL = 1e-5
+A_CURVE, B_CURVE = 1.5769434, 0.8950609
+
+
+@pytest.fixture(scope="module", autouse=True)
+def device():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    from cuscan_amd import _lib
+
+    assert _lib.lib().umaphip_device_count() >= 1
+    return torch.device("cuda:0")
+
+
+def dev(a, dtype):
+    return torch.as_tensor(np.ascontiguousarray(a)).to(dtype=dtype, device="cuda:0")
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.detach().cpu().numpy()
+
+
+# ------------------------------------------------- reference fixture (d=16/48)
+@pytest.mark.parametrize("idx", range(32))
+def test_fixture_through_epoch_boundary(idx):
+    from cuscan_amd import device as D
+    from tests.test_oracle_fixture import kernel_epoch
+
+    case = FIX_CASES[idx]
+    x = F.inputs(case)
+    adam = case["optimizer"] == "adam"
+    fit = case["operation"] == "fit"
+    c1, c2 = corrections(case["epoch"])
+    head = dev(x["head"], torch.float32)
+    upd = dev(x["updates"], torch.float32)
+    m = torch.zeros_like(head)
+    v = torch.zeros_like(head)
+    npos = dev(case["next_positive"], torch.float64)
+    nneg = dev(case["next_negative"], torch.float64)
+    D.modern_epoch(head=head, fixed_tail=None if fit else dev(x["tail"], torch.float32),
+                   offsets=dev(case["offsets"], torch.int32), indices=dev(case["indices"], torch.int32),
+                   source_order=dev(x["from_order"], torch.int32),
+                   negative_order=dev(x["to_order"], torch.int32),
+                   source_ranks=dev(x["ranks"], torch.int32),
+                   periods=dev(case["periods"], torch.float64), next_positive=npos,
+                   negative_periods=dev(case["negative_periods"], torch.float64),
+                   next_negative=nneg, updates=upd, m=m, v=v, epoch=case["epoch"], a=1.0, b=1.0,
+                   gamma=case["gamma"], scale=case["scale"], negative_range=case["refs"],
+                   alpha=case["alpha"], mu=case["momentum"], correction1=c1, correction2=c2,
+                   momentum=not adam, shared=fit, negatives=True, exclude_graph_neighbors=False)
+    assert host(npos).tolist() == case["expected_positive"]
+    assert host(nneg).tolist() == case["expected_negative"]
+    if idx in F.UPSTREAM_BLOCKED_ONLY:
+        ep, _ = kernel_epoch(case, x)  # Jacobi reference semantics from the oracle
+        exp_head, exp_m, exp_v = ep.head, ep.m, ep.v
+    else:
+        zeros = np.zeros_like(x["head"])
+        exp_head = F.expected(case, "expected_head", x["head"])
+        exp_m = F.expected(case, "expected_first", zeros)
+        exp_v = F.expected(case, "expected_second", zeros)
+    np.testing.assert_allclose(host(head), exp_head, atol=3e-5, rtol=0)
+    if adam:
+        np.testing.assert_allclose(host(m), exp_m, atol=3e-5, rtol=0)
+        np.testing.assert_allclose(host(v), exp_v, atol=3e-5, rtol=0)
+        assert (host(upd) == 0).all()
+    else:
+        exp_u = F.expected(case, "expected_final_updates", x["updates"])
+        np.testing.assert_allclose(host(upd), exp_u, atol=3e-5, rtol=0)
+
+
+FIX_CASES = F.load()["cases"]
+
+
+# ------------------------------------------------------- one epoch, d = 2
+def epoch_inputs(h, n, d, seed, shared, epoch, *, window=None, exclude=False, scale=0.6):
+    rng = np.random.default_rng(seed)
+    g = random_graph(h, n, 6, seed, 40, symmetric=shared)
+    g = advance_graph(g, epoch)
+    head = rng.uniform(-8, 8, (h, d)).astype(np.float32)
+    tail = head if shared else rng.uniform(-8, 8, (n, d)).astype(np.float32)
+    neg_order = rng.permutation(n).astype(np.int32)
+    src_order = rng.permutation(h).astype(np.int32)
+    ranks = rng.integers(0, n, h).astype(np.int32)
+    if shared:
+        ranks = np.argsort(neg_order).astype(np.int32)  # inverse permutation
+    R = n if window is None else window
+    return dict(g=g, head=head, tail=tail, neg_order=neg_order, src_order=src_order, ranks=ranks,
+                R=R, exclude=exclude, scale=scale, epoch=epoch, shared=shared)
+
+
+def oracle_epoch(I, momentum=False, alpha=0.3, mu=0.4):
+    c1, c2 = corrections(I["epoch"])
+    g = I["g"]
+    h, d = I["head"].shape
+    return O.Epoch(head=I["head"], tail=I["tail"], offsets=g["offsets"], indices=g["indices"],
+                   source_order=I["src_order"], negative_order=I["neg_order"],
+                   source_ranks=I["ranks"], periods=g["periods"],
+                   next_positive=g["next_positive"], negative_periods=g["negative_periods"],
+                   next_negative=g["next_negative"], updates=np.zeros((h, d), np.float32),
+                   m=np.zeros((h, d)), v=np.zeros((h, d)), epoch=I["epoch"], a=A_CURVE, b=B_CURVE,
+                   gamma=1.0, scale=I["scale"], negative_range=I["R"], alpha=alpha, mu=mu,
+                   correction1=c1, correction2=c2, momentum=momentum, shared=I["shared"],
+                   negatives=True, exclude=I["exclude"])
+
+
+def device_epoch_kwargs(I, momentum=False, alpha=0.3, mu=0.4):
+    c1, c2 = corrections(I["epoch"])
+    g = I["g"]
+    head = dev(I["head"], torch.float32)
+    return dict(head=head, fixed_tail=None if I["shared"] else dev(I["tail"], torch.float32),
+                offsets=dev(g["offsets"].astype(np.int32), torch.int32),
+                indices=dev(g["indices"].astype(np.int32), torch.int32),
+                source_order=dev(I["src_order"], torch.int32),
+                negative_order=dev(I["neg_order"], torch.int32),
+                source_ranks=dev(I["ranks"], torch.int32),
+                periods=dev(g["periods"], torch.float64),
+                next_positive=dev(g["next_positive"], torch.float64),
+                negative_periods=dev(g["negative_periods"], torch.float64),
+                next_negative=dev(g["next_negative"], torch.float64),
+                updates=torch.zeros_like(head), m=torch.zeros_like(head), v=torch.zeros_like(head),
+                epoch=I["epoch"], a=A_CURVE, b=B_CURVE, gamma=1.0, scale=I["scale"],
+                negative_range=I["R"], alpha=alpha, mu=mu, correction1=c1, correction2=c2,
+                momentum=momentum, shared=I["shared"], negatives=True,
+                exclude_graph_neighbors=I["exclude"])
+
+
+CASES = [
+    # (h, n, d, shared, window, exclude, epoch)
+    (3000, 3000, 2, True, None, False, 0),      # large fit, global
+    (3000, 3000, 2, True, 500, False, 5),       # large fit, projection window
+    (3000, 3000, 2, True, 500, True, 9),        # large fit, window + exclusion
+    (800, 800, 2, True, None, True, 3),         # small fit, exclusion retries
+    (800, 800, 2, True, None, False, 17),       # small fit
+    (200, 4000, 2, False, None, False, 2),      # small transform
+    (2000, 5000, 2, False, 1000, True, 4),      # large transform, window + exclusion
+    (1500, 1500, 3, True, 300, False, 6),       # d = 3 generic kernel
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[str(c) for c in CASES])
+def test_single_epoch_adam_gradient_vs_fp64(case):
+    from cuscan_amd import device as D
+
+    h, n, d, shared, window, exclude, epoch = case
+    I = epoch_inputs(h, n, d, 100 + h, shared, epoch, window=window, exclude=exclude)
+    ref = oracle_epoch(I)
+    g64 = ref.gradient_block64(0, h, 1.0)
+    kw = device_epoch_kwargs(I)
+    print(f"[gpu] adam epoch case={case}", flush=True)
+    D.modern_epoch(**kw)
+    assert (host(kw["next_positive"]) == ref.next_positive).all()
+    assert (host(kw["next_negative"]) == ref.next_negative).all()
+    # m = (1 - 0.9f) * g from zero moments recovers the gradient.
+    g = host(kw["m"]).astype(np.float64) / float(np.float32(1.0) - np.float32(0.9))
+    assert row_relative_error(g, g64) <= GRAD_TOL
+    ref2 = oracle_epoch(I)
+    ref2.modern_epoch()
+    np.testing.assert_allclose(host(kw["head"]), ref2.head, atol=2e-4, rtol=0)
+
+
+@pytest.mark.parametrize("case", CASES[:4] + CASES[5:], ids=[str(c) for c in CASES[:4] + CASES[5:]])
+def test_single_epoch_momentum_vs_oracle(case):
+    from cuscan_amd import device as D
+
+    h, n, d, shared, window, exclude, epoch = case
+    I = epoch_inputs(h, n, d, 200 + h, shared, epoch, window=window, exclude=exclude)
+    ref = oracle_epoch(I, momentum=True)
+    ref.modern_epoch()
+    kw = device_epoch_kwargs(I, momentum=True)
+    print(f"[gpu] momentum epoch case={case}", flush=True)
+    D.modern_epoch(**kw)
+    assert (host(kw["next_positive"]) == ref.next_positive).all()
+    assert (host(kw["next_negative"]) == ref.next_negative).all()
+    np.testing.assert_allclose(host(kw["updates"]), ref.updates, atol=2e-5, rtol=1e-4)
+    np.testing.assert_allclose(host(kw["head"]), ref.head, atol=2e-5, rtol=1e-5)
+
+
+@pytest.mark.parametrize("case", CASES, ids=[str(c) for c in CASES])
+def test_negative_candidates_bit_exact(case):
+    from cuscan_amd import device as D
+
+    h, n, d, shared, window, exclude, epoch = case
+    I = epoch_inputs(h, n, d, 300 + h, shared, epoch, window=window, exclude=exclude)
+    ref = oracle_epoch(I)
+    edges, cands, kept, fires = ref.trace_negatives()
+    kw = device_epoch_kwargs(I)
+    print(f"[gpu] negatives case={case}", flush=True)
+    count, cand, kp = D.debug_negatives(cap=64, **kw)
+    count, cand, kp = host(count), host(cand), host(kp)
+    assert int((count >= 0).sum()) == fires
+    E = len(I["g"]["indices"])
+    exp_count = np.zeros(E, np.int64)
+    np.add.at(exp_count, edges, 1)
+    fired = count >= 0
+    assert (count[fired] == exp_count[fired]).all()
+    # order within an edge is the visiting order on both sides
+    pos = 0
+    starts = np.zeros(E + 1, np.int64)
+    starts[1:] = np.cumsum(exp_count)
+    order = np.argsort(edges, kind="stable")
+    se, sc, sk = edges[order], cands[order], kept[order]
+    for e in np.nonzero(exp_count)[0]:
+        k = exp_count[e]
+        assert (cand[e, :k] == sc[starts[e]:starts[e] + k]).all(), e
+        assert (kp[e, :k] == sk[starts[e]:starts[e] + k]).all(), e
+        pos += k
+    assert pos == len(edges)
+
+
+# ------------------------------------------------------------ projection
+def test_projection_sort_and_ranks():
+    from cuscan_amd import device as D
+
+    rng = np.random.default_rng(4)
+    for n in (1, 5, 1000, 300_001):
+        x = rng.normal(size=(n, 2)).astype(np.float32)
+        x[rng.integers(0, n, n // 4)] = x[0]  # ties
+        direction = O.StdRng(n).direction(2)
+        proj = D.project(dev(x, torch.float32), dev(direction, torch.float32))
+        assert (host(proj) == O.project(x, direction)).all()
+        order, sorted_vals, status = D.stable_argsort(proj)
+        eo, es = O.stable_argsort(host(proj))
+        assert (host(order) == eo).all()
+        assert (host(sorted_vals) == np.where(es == 0, np.float32(0), es)).all()
+        assert host(status)[0] == -1
+        ranks = D.projection_ranks(order, sorted_vals, shared=True)
+        assert (host(ranks) == O.inverse_ranks(eo)).all()
+        q = rng.normal(size=77).astype(np.float32)
+        lb = D.projection_ranks(order, sorted_vals, dev(q, torch.float32), shared=False)
+        assert (host(lb) == O.lower_bound_ranks(es, q)).all()
+    bad = np.array([0.0, np.inf, 1.0, np.nan], np.float32)
+    _, _, status = D.stable_argsort(dev(bad, torch.float32))
+    assert host(status)[0] == 1
+
+
+def test_resultants_vs_oracle():
+    from cuscan_amd import device as D
+
+    rng = np.random.default_rng(8)
+    n, s, q = 5000, 128, 700
+    g = random_graph(n, n, 5, 8, 50)
+    emb = rng.normal(scale=5, size=(n, 2)).astype(np.float32)
+    order = rng.permutation(n).astype(np.int32)
+    sr = rng.integers(0, n, s).astype(np.int32)
+    lo = rng.integers(0, q, 5 * s).astype(np.int32)
+    gr = rng.integers(0, n, 5 * s).astype(np.int32)
+    for exclude in (False, True):
+        exp = O.resultants(emb, order, g["offsets"], g["indices"], sr, lo, gr, q, A_CURVE,
+                           B_CURVE, 1.0, exclude)
+        got = D.sample_resultants(dev(emb, torch.float32), dev(order, torch.int32),
+                                  dev(g["offsets"].astype(np.int32), torch.int32),
+                                  dev(g["indices"].astype(np.int32), torch.int32),
+                                  dev(sr, torch.int32), dev(lo, torch.int32), dev(gr, torch.int32),
+                                  q, A_CURVE, B_CURVE, 1.0, exclude)
+        np.testing.assert_allclose(host(got), exp, rtol=1e-12, atol=0)
+
+
+# --------------------------------------------------------------- driver
+DRIVER = [
+    # (name, h, n, d, shared, optimizer, Q, exclude, epochs)
+    ("fit_adam_global_excl", 3000, 3000, 2, True, "adam", 200_000, True, 20),
+    ("fit_adam_windows_adapt", 3000, 3000, 2, True, "adam", 500, False, 24),
+    ("fit_momentum_windows", 2500, 2500, 2, True, "momentum", 600, False, 12),
+    ("fit_momentum_alias", 2000, 2000, 2, True, "momentum", 200_000, False, 12),
+    ("transform_small_proj", 200, 3000, 2, False, "adam", 500, False, 15),
+    ("transform_large_window", 1500, 4000, 2, False, "adam", 1000, True, 12),
+    ("fit_adam_d3_windows_adapt", 2200, 2200, 3, True, "adam", 400, False, 22),
+    ("fit_small_excl", 900, 900, 2, True, "adam", 200_000, True, 12),
+]
+
+
+@pytest.mark.parametrize("case", DRIVER, ids=[c[0] for c in DRIVER])
+def test_driver_matches_oracle_loop(case):
+    from cuscan_amd import layout
+
+    name, h, n, d, shared, opt, Q, exclude, epochs = case
+    rng = np.random.default_rng(hash(name) % 2**32)
+    g = random_graph(h, n, 6, 11, epochs, symmetric=shared)
+    head = rng.uniform(-10, 10, (h, d)).astype(np.float32)
+    tail = None if shared else rng.uniform(-10, 10, (n, d)).astype(np.float32)
+    kw = dict(epochs=epochs, a=A_CURVE, b=B_CURVE, optimizer=opt, negative_selection_range=Q,
+              exclude=exclude, seed=5)
+    print(f"[gpu] driver case={name}", flush=True)
+    params = layout.LayoutParams(epochs=epochs, a=A_CURVE, b=B_CURVE, optimizer=opt,
+                                 negative_selection_range=Q, exclude_graph_neighbors=exclude,
+                                 seed=5)
+    got, gp, gn, stats = layout.optimize_compact(head, tail, g, params, shared=shared)
+    exp, ep, en, est = O.optimize_compact(head, tail, g, shared=shared, threads=8, **kw)
+    assert (gp == ep).all() and (gn == en).all()
+    assert stats["adaptations"] == est["adaptations"]
+    if est["adaptations"]:
+        assert abs(stats["final_scale"] - est["final_scale"]) <= 1e-5 * est["final_scale"]
+    np.testing.assert_allclose(got, exp, atol=2e-3, rtol=0)

@@ -327,3 +327,16 @@ def test_fit_curve_default_matches_umap_learn():
     a, b = O.fit_curve(0.1, 1.0)
     # umap-learn's published defaults for min_dist=0.1, spread=1.0.
     assert abs(a - 1.5769434) < 2e-4 and abs(b - 0.8950609) < 2e-4
+
+
+def test_vectorised_clock_replay_matches_advance_clock():
+    from tests.gpu_helpers import advance_graph, random_graph
+
+    g = random_graph(50, 50, 4, 3, 30)
+    v = advance_graph(g, 13)
+    npos, nneg = g["next_positive"].copy(), g["next_negative"].copy()
+    for t in range(13):
+        for e in range(len(npos)):
+            _, npos[e], nneg[e] = O.advance_clock(npos[e], nneg[e], g["periods"][e],
+                                                  g["negative_periods"][e], t)
+    assert (npos == v["next_positive"]).all() and (nneg == v["next_negative"]).all()
