@@ -87,6 +87,9 @@ SIGNATURES = {
     "umaphip_plan_download": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(LayoutStats)]),
     "umaphip_plan_stream": (_vp, [_vp]),
     "umaphip_plan_destroy": (None, [_vp]),
+    "umaphip_knn_index_create": (C.c_int, [_vp, _i32, _i64, _i64, _i64, C.POINTER(_vp)]),
+    "umaphip_knn_query": (C.c_int, [_vp, _vp, _i64, _i32, _i32, _i64, _vp, _vp, _vp]),
+    "umaphip_knn_index_destroy": (None, [_vp]),
     "umaphip_orders_at_epoch": (C.c_int, [_i64, _i64, C.POINTER(LayoutParams), _i32, _i32, _vp,
                                           _vp]),
 }

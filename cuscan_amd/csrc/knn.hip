@@ -1,0 +1,462 @@
+// Exact k-nearest neighbours for the transform path and the fit graph:
+// a BF16 MFMA distance prefilter (FP32 accumulation) that keeps the best 64
+// candidates per query, then an exact FP32 rerank that reproduces cuscan's
+// exact_kernel arithmetic (crates/cuscan-common/src/kernels/neighbors.rs:
+// 17-76): sequential f32 sum of squared differences over the features,
+// ties broken by the lower reference index, square root at the end,
+// sentinel id n_reference and distance sqrt(f32::MAX) for unfilled slots.
+//
+// Prefilter layout (gfx950, wave64): a workgroup of 4 waves owns 128 queries
+// (32 per wave, held for the whole scan as MFMA A-fragments in VGPRs) and
+// streams one slice of the references through LDS in chunks of 32 rows.
+// Approximate distances ||r||^2 - 2 q.r are compared against each query's
+// running 64th-best threshold; survivors are appended to a per-(query,
+// slice) buffer that is compacted to its sorted best 64 whenever it could
+// overflow. Slices are merged per query, then reranked.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "common.hpp"
+#include "knn.hpp"
+
+namespace umaphip {
+
+namespace {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
+
+constexpr int kTQ = 128;      // queries per workgroup
+constexpr int kTR = 32;       // reference rows per LDS chunk
+constexpr int kCap = 128;     // candidate buffer per (query, slice)
+constexpr int kTop = 64;      // candidates kept per query
+constexpr int kPad = 8;       // LDS row padding (bf16 elements)
+
+__device__ inline uint16_t f32_to_bf16_rne(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return static_cast<uint16_t>(u >> 16);
+}
+
+__device__ inline float bf16_to_f32(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
+
+__global__ void to_bf16_kernel(const float *x, int64_t rows, int64_t d, int64_t dp, uint16_t *out,
+                               float *norms) {
+    const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.y + threadIdx.y;
+    if (r >= rows) return;
+    float acc = 0.0f;
+    for (int64_t c = threadIdx.x; c < dp; c += blockDim.x) {
+        const uint16_t h = c < d ? f32_to_bf16_rne(x[r * d + c]) : 0;
+        out[r * dp + c] = h;
+        const float v = bf16_to_f32(h);
+        acc += v * v;
+    }
+    if (norms) {
+        for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+        if (threadIdx.x == 0) norms[r] = acc;
+    }
+}
+
+// Rank-based selection of the best kTop of n <= kCap entries of one
+// (query, slice) buffer by one wave; keeps them sorted in place and returns
+// the new threshold (the kTop-th distance, or +inf when fewer).
+__device__ inline float compact_buffer(float *cd, int32_t *ci, int n, int lane) {
+    float d0 = lane < n ? cd[lane] : INFINITY;
+    int32_t i0 = lane < n ? ci[lane] : INT32_MAX;
+    float d1 = lane + 64 < n ? cd[lane + 64] : INFINITY;
+    int32_t i1 = lane + 64 < n ? ci[lane + 64] : INT32_MAX;
+    int r0 = 0, r1 = 0;
+    for (int f = 0; f < n; ++f) {
+        const int src = f & 63;
+        const float df = f < 64 ? __shfl(d0, src, 64) : __shfl(d1, src, 64);
+        const int32_t idf = f < 64 ? __shfl(i0, src, 64) : __shfl(i1, src, 64);
+        r0 += (df < d0 || (df == d0 && idf < i0)) ? 1 : 0;
+        r1 += (df < d1 || (df == d1 && idf < i1)) ? 1 : 0;
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (lane < n && r0 < kTop) {
+        cd[r0] = d0;
+        ci[r0] = i0;
+    }
+    if (lane + 64 < n && r1 < kTop) {
+        cd[r1] = d1;
+        ci[r1] = i1;
+    }
+    float tau = INFINITY;
+    if (n >= kTop) {
+        if (r0 == kTop - 1 && lane < n) tau = d0;
+        if (r1 == kTop - 1 && lane + 64 < n) tau = d1;
+        for (int off = 32; off > 0; off >>= 1) tau = fminf(tau, __shfl_xor(tau, off, 64));
+    }
+    return tau;
+}
+
+template <int KSTEPS>
+__global__ __launch_bounds__(256, 2) void knn_scan_kernel(
+    const uint16_t *__restrict__ qb, int64_t nq, const uint16_t *__restrict__ rb,
+    const float *__restrict__ rnorm, int64_t nr, int64_t per_slice, int slices,
+    float *__restrict__ cand_d, int32_t *__restrict__ cand_i, int32_t *__restrict__ cand_n,
+    int exclude_self, int64_t query_offset) {
+    constexpr int DP = KSTEPS * 32;
+    constexpr int ROW = DP + kPad;
+    __shared__ uint16_t sB[kTR * ROW];
+    __shared__ float s_tau[kTQ];
+    __shared__ int s_cnt[kTQ];
+
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int64_t q_block = static_cast<int64_t>(blockIdx.x) * kTQ;
+    const int slice = blockIdx.y;
+    const int64_t r_begin = static_cast<int64_t>(slice) * per_slice;
+    const int64_t r_end = min<int64_t>(nr, r_begin + per_slice);
+
+    // A fragments: lane holds row (lane % 16) and K elements 8*(lane/16)..+7
+    // of each 32-wide K step, for two 16-query sub-blocks.
+    bf16x8 A[2][KSTEPS];
+#pragma unroll
+    for (int qa = 0; qa < 2; ++qa) {
+        const int64_t row = q_block + wave * 32 + qa * 16 + (lane & 15);
+#pragma unroll
+        for (int ks = 0; ks < KSTEPS; ++ks) {
+            if (row < nq) {
+                const uint4 raw = *reinterpret_cast<const uint4 *>(qb + row * DP + ks * 32 + 8 * (lane >> 4));
+                A[qa][ks] = __builtin_bit_cast(bf16x8, raw);
+            } else {
+                A[qa][ks] = __builtin_bit_cast(bf16x8, make_uint4(0, 0, 0, 0));
+            }
+        }
+    }
+    if (tid < kTQ) {
+        s_tau[tid] = INFINITY;
+        s_cnt[tid] = 0;
+    }
+    const int64_t buf_stride = static_cast<int64_t>(slices) * kCap;
+
+    for (int64_t chunk = r_begin; chunk < r_end; chunk += kTR) {
+        __syncthreads();
+        // stage kTR reference rows (16-byte words)
+        constexpr int WORDS = kTR * (DP / 8);
+        for (int w = tid; w < WORDS; w += 256) {
+            const int rr = w / (DP / 8), cw = w % (DP / 8);
+            const int64_t ref = chunk + rr;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (ref < r_end) v = *reinterpret_cast<const uint4 *>(rb + ref * DP + cw * 8);
+            *reinterpret_cast<uint4 *>(&sB[rr * ROW + cw * 8]) = v;
+        }
+        __syncthreads();
+        floatx4 acc[2][2];
+#pragma unroll
+        for (int qa = 0; qa < 2; ++qa)
+#pragma unroll
+            for (int rbk = 0; rbk < 2; ++rbk) acc[qa][rbk] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int ks = 0; ks < KSTEPS; ++ks) {
+#pragma unroll
+            for (int rbk = 0; rbk < 2; ++rbk) {
+                const uint4 raw = *reinterpret_cast<const uint4 *>(
+                    &sB[(rbk * 16 + (lane & 15)) * ROW + ks * 32 + 8 * (lane >> 4)]);
+                const bf16x8 bfrag = __builtin_bit_cast(bf16x8, raw);
+#pragma unroll
+                for (int qa = 0; qa < 2; ++qa)
+                    acc[qa][rbk] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[qa][ks], bfrag,
+                                                                           acc[qa][rbk], 0, 0, 0);
+            }
+        }
+        // C layout: lane holds rows 4*(lane/16)+i (queries), column lane%16 (reference).
+#pragma unroll
+        for (int rbk = 0; rbk < 2; ++rbk) {
+            const int64_t ref = chunk + rbk * 16 + (lane & 15);
+            if (ref >= r_end) continue;
+            const float rn = rnorm[ref];
+#pragma unroll
+            for (int qa = 0; qa < 2; ++qa) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int ql = wave * 32 + qa * 16 + 4 * (lane >> 4) + i;
+                    const int64_t q = q_block + ql;
+                    if (q >= nq) continue;
+                    if (exclude_self && ref == query_offset + q) continue;
+                    const float dt = rn - 2.0f * acc[qa][rbk][i];
+                    if (dt < s_tau[ql]) {
+                        const int pos = atomicAdd(&s_cnt[ql], 1);
+                        const int64_t base = (q * slices + slice) * kCap;
+                        cand_d[base + pos] = dt;
+                        cand_i[base + pos] = static_cast<int32_t>(ref);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        // compaction of buffers that could overflow on the next chunk
+        for (int ql = wave; ql < kTQ; ql += 4) {
+            const int n = s_cnt[ql];
+            const int64_t q = q_block + ql;
+            if (q >= nq || n <= kCap - kTR) continue;
+            const int64_t base = (q * slices + slice) * kCap;
+            const float tau = compact_buffer(cand_d + base, cand_i + base, n, lane);
+            if (lane == 0) {
+                s_tau[ql] = tau;
+                s_cnt[ql] = kTop;
+            }
+        }
+    }
+    __syncthreads();
+    for (int ql = wave; ql < kTQ; ql += 4) {
+        const int64_t q = q_block + ql;
+        if (q >= nq) continue;
+        const int n = s_cnt[ql];
+        const int64_t base = (q * slices + slice) * kCap;
+        if (n > 0) compact_buffer(cand_d + base, cand_i + base, n, lane);
+        if (lane == 0) cand_n[q * slices + slice] = n < kTop ? n : kTop;
+    }
+    (void)buf_stride;
+}
+
+// Merge the sorted per-slice lists of each query into its best kTop
+// (sorted by distance, then id). One wave per query, LDS merge path.
+__global__ __launch_bounds__(64) void knn_merge_kernel(const float *cand_d, const int32_t *cand_i,
+                                                       const int32_t *cand_n, int64_t nq,
+                                                       int slices, float *top_d, int32_t *top_i,
+                                                       int32_t *top_n) {
+    __shared__ float sa_d[kTop], sb_d[kTop];
+    __shared__ int32_t sa_i[kTop], sb_i[kTop];
+    const int64_t q = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (q >= nq) return;
+    int na = cand_n[q * slices];
+    {
+        const int64_t base = (q * slices) * kCap;
+        sa_d[lane] = lane < na ? cand_d[base + lane] : INFINITY;
+        sa_i[lane] = lane < na ? cand_i[base + lane] : INT32_MAX;
+    }
+    for (int s = 1; s < slices; ++s) {
+        const int nb = cand_n[q * slices + s];
+        if (nb == 0) continue;
+        const int64_t base = (q * slices + s) * kCap;
+        sb_d[lane] = lane < nb ? cand_d[base + lane] : INFINITY;
+        sb_i[lane] = lane < nb ? cand_i[base + lane] : INT32_MAX;
+        __syncthreads();
+        const float ad = sa_d[lane], bd = sb_d[lane];
+        const int32_t ai = sa_i[lane], bi = sb_i[lane];
+        // rank of A[lane] = lane + #{B < A[lane]}; rank of B[lane] = lane + #{A < B[lane]}
+        int lo = 0, hi = kTop;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            const float md = sb_d[mid];
+            if (md < ad || (md == ad && sb_i[mid] < ai)) lo = mid + 1;
+            else hi = mid;
+        }
+        const int ra = lane + lo;
+        lo = 0;
+        hi = kTop;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            const float md = sa_d[mid];
+            if (md < bd || (md == bd && sa_i[mid] < bi)) lo = mid + 1;
+            else hi = mid;
+        }
+        const int rbk = lane + lo;
+        __syncthreads();
+        if (ra < kTop) {
+            sa_d[ra] = ad;
+            sa_i[ra] = ai;
+        }
+        if (rbk < kTop) {
+            sa_d[rbk] = bd;
+            sa_i[rbk] = bi;
+        }
+        na = min(kTop, na + nb);
+        __syncthreads();
+    }
+    top_d[q * kTop + lane] = sa_d[lane];
+    top_i[q * kTop + lane] = sa_i[lane];
+    if (lane == 0) top_n[q] = na;
+}
+
+// Exact rerank: one wave per query, one lane per candidate; the distance is
+// the reference's sequential f32 sum (no contraction in this build).
+__global__ __launch_bounds__(64) void knn_rerank_kernel(const float *query, int64_t nq,
+                                                        const float *ref, int64_t nr, int64_t d,
+                                                        const int32_t *top_i, const int32_t *top_n,
+                                                        int k, int32_t *out_i, float *out_d) {
+    const int64_t q = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (q >= nq) return;
+    const int n = top_n[q];
+    int32_t id = INT32_MAX;
+    float dist = INFINITY;
+    if (lane < n) {
+        id = top_i[q * kTop + lane];
+        const float *qv = query + q * d;
+        const float *rv = ref + static_cast<int64_t>(id) * d;
+        float acc = 0.0f;
+        for (int64_t f = 0; f < d; ++f) {
+            const float delta = qv[f] - rv[f];
+            acc += delta * delta;
+        }
+        dist = acc;
+    }
+    int rank = 0;
+    for (int f = 0; f < 64; ++f) {
+        const float df = __shfl(dist, f, 64);
+        const int32_t idf = __shfl(id, f, 64);
+        rank += (df < dist || (df == dist && idf < id)) ? 1 : 0;
+    }
+    if (lane < n && rank < k) {
+        out_i[q * k + rank] = id;
+        out_d[q * k + rank] = sqrtf(dist);
+    }
+    if (lane >= n && lane < k) {
+        // fewer candidates than k: sentinels fill the tail (rank n.. k-1)
+        out_i[q * k + lane] = static_cast<int32_t>(nr);
+        out_d[q * k + lane] = sqrtf(FLT_MAX);
+    }
+}
+
+template <int KS>
+void launch_scan(dim3 grid, hipStream_t s, const uint16_t *qb, int64_t nq, const uint16_t *rb,
+                 const float *rnorm, int64_t nr, int64_t per_slice, int slices, float *cd,
+                 int32_t *ci, int32_t *cn, int exclude_self, int64_t qoff) {
+    hipLaunchKernelGGL(knn_scan_kernel<KS>, grid, dim3(256), 0, s, qb, nq, rb, rnorm, nr, per_slice,
+                       slices, cd, ci, cn, exclude_self, qoff);
+}
+
+int ksteps_for(int64_t d) {
+    const int64_t need = (d + 31) / 32;
+    const int choices[] = {1, 2, 4, 8, 16, 25, 32};
+    for (int c : choices)
+        if (c >= need) return c;
+    return -1;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------- KnnIndex
+KnnIndex::KnnIndex(const float *ref_host, const float *ref_device, int64_t n, int64_t d, int64_t max_q,
+                   hipStream_t s)
+    : n_(n), d_(d), max_q_(max_q) {
+    ks_ = ksteps_for(d);
+    UMAPHIP_REQUIRE(ks_ > 0, UMAPHIP_INVALID_INPUT, "kNN supports at most 1024 features (got %lld)",
+                    (long long)d);
+    dp_ = static_cast<int64_t>(ks_) * 32;
+    ref_.alloc(static_cast<size_t>(n * d));
+    if (ref_device)
+        HIP_CHECK(hipMemcpyAsync(ref_.ptr, ref_device, n * d * sizeof(float), hipMemcpyDeviceToDevice, s));
+    else
+        HIP_CHECK(hipMemcpyAsync(ref_.ptr, ref_host, n * d * sizeof(float), hipMemcpyHostToDevice, s));
+    ref_bf_.alloc(static_cast<size_t>(n * dp_));
+    rnorm_.alloc(static_cast<size_t>(n));
+    hipLaunchKernelGGL(to_bf16_kernel, dim3(static_cast<unsigned>((n + 3) / 4)), dim3(64, 4), 0, s,
+                       ref_.ptr, n, d, dp_, ref_bf_.ptr, rnorm_.ptr);
+    HIP_CHECK(hipGetLastError());
+    reserve(max_q);
+}
+
+int KnnIndex::slices_for(int64_t q) const {
+    const int64_t blocks = (q + kTQ - 1) / kTQ;
+    // Enough workgroups to fill the chip (256 CUs, two resident each), but
+    // slices of at least 4096 references.
+    int64_t s = (512 + blocks - 1) / blocks;
+    s = std::min<int64_t>(s, std::max<int64_t>(1, n_ / 4096));
+    s = std::max<int64_t>(1, std::min<int64_t>(s, 256));
+    return static_cast<int>(s);
+}
+
+void KnnIndex::reserve(int64_t q) {
+    if (q <= reserved_q_) return;
+    // (query, slice) buffers are laid out per batch; size them for the
+    // largest rows x slices product of any batch up to q rows.
+    int64_t cells = 0;
+    for (int64_t qq = 1; qq <= q; ++qq) cells = std::max<int64_t>(cells, qq * slices_for(qq));
+    const size_t entries = static_cast<size_t>(cells) * kCap;
+    cand_d_.alloc(entries);
+    cand_i_.alloc(entries);
+    cand_n_.alloc(static_cast<size_t>(cells));
+    top_d_.alloc(static_cast<size_t>(q) * kTop);
+    top_i_.alloc(static_cast<size_t>(q) * kTop);
+    top_n_.alloc(static_cast<size_t>(q));
+    qbf_.alloc(static_cast<size_t>(q * dp_));
+    reserved_q_ = q;
+}
+
+void KnnIndex::query(const float *q_dev, int64_t nq, int k, bool exclude_self, int64_t query_offset,
+                     int32_t *idx_out, float *dist_out, hipStream_t s, const float *q_f32_for_rerank) {
+    UMAPHIP_REQUIRE(k >= 1 && k <= kTop, UMAPHIP_INVALID_INPUT, "k must be in 1..=%d", kTop);
+    if (nq == 0) return;
+    UMAPHIP_REQUIRE(nq <= reserved_q_, UMAPHIP_CAPACITY,
+                    "query batch %lld exceeds the reserved %lld rows", (long long)nq,
+                    (long long)reserved_q_);
+    hipLaunchKernelGGL(to_bf16_kernel, dim3(static_cast<unsigned>((nq + 3) / 4)), dim3(64, 4), 0, s,
+                       q_dev, nq, d_, dp_, qbf_.ptr, static_cast<float *>(nullptr));
+    HIP_CHECK(hipGetLastError());
+    const int slices = slices_for(nq);
+    const int64_t per_slice = ((n_ + slices - 1) / slices + kTR - 1) / kTR * kTR;
+    const dim3 grid(static_cast<unsigned>((nq + kTQ - 1) / kTQ), static_cast<unsigned>(slices));
+    const int ex = exclude_self ? 1 : 0;
+    switch (ks_) {
+        case 1: launch_scan<1>(grid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, per_slice, slices, cand_d_.ptr, cand_i_.ptr, cand_n_.ptr, ex, query_offset); break;
+        case 2: launch_scan<2>(grid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, per_slice, slices, cand_d_.ptr, cand_i_.ptr, cand_n_.ptr, ex, query_offset); break;
+        case 4: launch_scan<4>(grid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, per_slice, slices, cand_d_.ptr, cand_i_.ptr, cand_n_.ptr, ex, query_offset); break;
+        case 8: launch_scan<8>(grid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, per_slice, slices, cand_d_.ptr, cand_i_.ptr, cand_n_.ptr, ex, query_offset); break;
+        case 16: launch_scan<16>(grid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, per_slice, slices, cand_d_.ptr, cand_i_.ptr, cand_n_.ptr, ex, query_offset); break;
+        case 25: launch_scan<25>(grid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, per_slice, slices, cand_d_.ptr, cand_i_.ptr, cand_n_.ptr, ex, query_offset); break;
+        default: launch_scan<32>(grid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, per_slice, slices, cand_d_.ptr, cand_i_.ptr, cand_n_.ptr, ex, query_offset); break;
+    }
+    HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(knn_merge_kernel, dim3(static_cast<unsigned>(nq)), dim3(64), 0, s,
+                       cand_d_.ptr, cand_i_.ptr, cand_n_.ptr, nq, slices, top_d_.ptr, top_i_.ptr,
+                       top_n_.ptr);
+    HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(knn_rerank_kernel, dim3(static_cast<unsigned>(nq)), dim3(64), 0, s,
+                       q_f32_for_rerank ? q_f32_for_rerank : q_dev, nq, ref_.ptr, n_, d_,
+                       top_i_.ptr, top_n_.ptr, k, idx_out, dist_out);
+    HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace umaphip
+
+using namespace umaphip;
+
+struct umaphip_knn_index {
+    std::unique_ptr<KnnIndex> impl;
+};
+
+extern "C" {
+
+umaphip_status umaphip_knn_index_create(const float *reference, int32_t reference_on_device,
+                                        int64_t n, int64_t d, int64_t max_queries,
+                                        umaphip_knn_index **out) {
+    return guarded([&] {
+        UMAPHIP_REQUIRE(out && reference && n > 0 && d > 0 && max_queries > 0 &&
+                            n <= INT32_MAX - 1,
+                        UMAPHIP_INVALID_INPUT, "invalid kNN index arguments");
+        std::unique_ptr<umaphip_knn_index> idx(new umaphip_knn_index());
+        idx->impl.reset(new KnnIndex(reference_on_device ? nullptr : reference,
+                                     reference_on_device ? reference : nullptr, n, d, max_queries,
+                                     nullptr));
+        HIP_CHECK(hipDeviceSynchronize());
+        *out = idx.release();
+    });
+}
+
+umaphip_status umaphip_knn_query(umaphip_knn_index *index, const float *queries, int64_t q,
+                                 int32_t k, int32_t exclude_self, int64_t query_offset,
+                                 int32_t *indices, float *distances, void *stream) {
+    return guarded([&] {
+        UMAPHIP_REQUIRE(index && index->impl, UMAPHIP_INVALID_INPUT, "kNN index is null");
+        UMAPHIP_REQUIRE(q >= 0 && queries && indices && distances, UMAPHIP_INVALID_INPUT,
+                        "invalid kNN query arguments");
+        index->impl->query(queries, q, k, exclude_self != 0, query_offset, indices, distances,
+                           as_stream(stream), nullptr);
+    });
+}
+
+void umaphip_knn_index_destroy(umaphip_knn_index *index) { delete index; }
+
+}  // extern "C"

@@ -1,0 +1,37 @@
+// Resident reference set for the exact kNN (BF16 prefilter + FP32 rerank).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "common.hpp"
+
+namespace umaphip {
+
+class KnnIndex {
+public:
+    // Copies the f32 references (host or device pointer) and builds the
+    // BF16 copy (rows padded to a multiple of 32 features) and its norms.
+    KnnIndex(const float *ref_host, const float *ref_device, int64_t n, int64_t d, int64_t max_q,
+             hipStream_t s);
+    // Grows the query workspace to hold batches of up to q rows.
+    void reserve(int64_t q);
+    // k <= 64 exact neighbours of q_dev [nq, d] (device f32).
+    void query(const float *q_dev, int64_t nq, int k, bool exclude_self, int64_t query_offset,
+               int32_t *idx_out, float *dist_out, hipStream_t s,
+               const float *q_f32_for_rerank = nullptr);
+    int64_t rows() const { return n_; }
+    int64_t dims() const { return d_; }
+    const float *reference() const { return ref_.ptr; }
+    int slices_for(int64_t q) const;
+
+private:
+    int64_t n_, d_, dp_ = 0, max_q_, reserved_q_ = 0;
+    int ks_ = 0;
+    DeviceArray<float> ref_, rnorm_, cand_d_, top_d_;
+    DeviceArray<uint16_t> ref_bf_, qbf_;
+    DeviceArray<int32_t> cand_i_, cand_n_, top_i_, top_n_;
+};
+
+}  // namespace umaphip

@@ -201,6 +201,28 @@ umaphip_status umaphip_orders_at_epoch(int64_t h, int64_t n, const umaphip_layou
                                        int32_t shared, int32_t epoch, int32_t *source_out,
                                        int32_t *negative_out);
 
+/* ------------------------------------------------------------------------
+ * Exact kNN. Replaces cuscan_common::neighbors::exact_neighbors
+ * (crates/cuscan-common/src/neighbors.rs:148; kernels/neighbors.rs:270-560)
+ * as reached from crates/umap/src/standard_umap/neighbors.rs:17-77. A BF16
+ * MFMA distance prefilter with FP32 accumulation keeps 64 candidates per
+ * query; an exact FP32 rerank then reproduces the reference's sequential
+ * squared-distance sum, lower-index tie-break, square root and sentinels
+ * (id = n, distance = sqrt(f32::MAX)). k <= 64, features <= 1024.
+ * ------------------------------------------------------------------------ */
+typedef struct umaphip_knn_index umaphip_knn_index;
+/* Copies n x d f32 references (host or device pointer) into a resident
+ * index sized for batches of up to max_queries rows. Allocates. */
+umaphip_status umaphip_knn_index_create(const float *reference, int32_t reference_on_device,
+                                        int64_t n, int64_t d, int64_t max_queries,
+                                        umaphip_knn_index **out);
+/* queries [q, d] device f32; indices/distances [q, k] device outputs.
+ * exclude_self skips reference row query_offset + i for query row i. */
+umaphip_status umaphip_knn_query(umaphip_knn_index *index, const float *queries, int64_t q,
+                                 int32_t k, int32_t exclude_self, int64_t query_offset,
+                                 int32_t *indices, float *distances, void *stream);
+void umaphip_knn_index_destroy(umaphip_knn_index *index);
+
 #ifdef __cplusplus
 }
 #endif

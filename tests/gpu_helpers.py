@@ -40,6 +40,8 @@ def row_relative_error(g, g64):
     """max over rows with ||g64|| > 1e-6 * max of ||g - g64|| / ||g64||."""
     g = np.asarray(g, np.float64)
     n64 = np.linalg.norm(g64, axis=1)
+    if n64.max() == 0.0:
+        return 0.0 if not np.any(g) else float("inf")
     keep = n64 > 1e-6 * n64.max()
     return float((np.linalg.norm(g - g64, axis=1)[keep] / n64[keep]).max())
 
