@@ -60,6 +60,25 @@ public:
         return hi;
     }
 
+    // Uniform f32 in [low, high): 23 mantissa bits in [1, 2), minus one,
+    // times the span, plus low; the span shrinks by one ulp in the
+    // measure-zero case that rounding reaches high.
+    float range_f32(float low, float high) {
+        float span = high - low;
+        for (;;) {
+            const uint32_t bits = (next_u32() >> 9) | 0x3F800000u;
+            float unit;
+            std::memcpy(&unit, &bits, 4);
+            const float scaled = (unit - 1.0f) * span;
+            const float value = scaled + low;
+            if (value < high) return value;
+            uint32_t sb;
+            std::memcpy(&sb, &span, 4);
+            sb -= 1;
+            std::memcpy(&span, &sb, 4);
+        }
+    }
+
     // Box-Muller pair exactly as cuscan's gaussian_pair.
     void gaussian_pair(double &x, double &y) {
         const double r = std::sqrt(-2.0 * std::log(1.0 - next_f64()));

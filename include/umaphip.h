@@ -223,6 +223,46 @@ umaphip_status umaphip_knn_query(umaphip_knn_index *index, const float *queries,
                                  int32_t *indices, float *distances, void *stream);
 void umaphip_knn_index_destroy(umaphip_knn_index *index);
 
+/* ------------------------------------------------------------------------
+ * Host graph and initialisation steps (host arrays, multi-threaded).
+ * ------------------------------------------------------------------------ */
+/* Fuzzy simplicial set. Replaces StandardUmapOps::fuzzy_graph
+ * (crates/umap/src/backend.rs:89-111; kernels/standard_umap/graph.rs:
+ * 21-564): rho/sigma per row, directed memberships [n,k], and the
+ * symmetrised (or bipartite) COO of capacity 2*n*k grouped by head and
+ * sorted by tail; *used_out = entries written. NUMERICAL (with the
+ * -2/-2/-1 marker in slot 0) on invalid neighbour ids. */
+umaphip_status umaphip_fuzzy_graph(const int32_t *indices, const float *distances, int64_t n,
+                                   int64_t k, float local_connectivity, float bandwidth, float mix,
+                                   int32_t bipartite, int64_t n_reference, int32_t *heads,
+                                   int32_t *tails, float *weights, float *rho, float *sigma,
+                                   float *directed, int64_t *used_out);
+/* Positive-edge pruning (weights below max / prune_divisor become zero;
+ * prune_divisor <= 0 disables it; estimator.rs:2753-2798) then compact_graph
+ * (crates/umap-core/src/modern_layout.rs:303-430): CSR offsets [rows+1],
+ * column-sorted indices, merged weights, f64 periods and initial clocks
+ * (capacity n_edges each); *n_compact = E. */
+umaphip_status umaphip_compact_graph(int64_t rows, int64_t columns, int64_t n_edges,
+                                     const int32_t *heads, const int32_t *tails,
+                                     const float *weights_in, double prune_divisor, int32_t epochs,
+                                     float rate, int32_t transform, uint32_t *offsets,
+                                     uint32_t *indices, float *weights, double *periods,
+                                     double *negative_periods, double *next_positive,
+                                     double *next_negative, int64_t *n_compact);
+/* fit_curve (crates/umap-core/src/config.rs:164-330). */
+umaphip_status umaphip_fit_curve(float min_dist, float spread, float *a, float *b);
+/* Random U(-10,10) initialisation from StdRng(seed) then centring and
+ * isotropic (mode 0) or per-axis (mode 1) robust rescaling
+ * (crates/umap-core/src/init.rs:1519-1530, init_rescale.rs:77-203). */
+umaphip_status umaphip_random_init(int64_t n, int64_t d, uint64_t seed, float repulsion,
+                                   float scale_factor, int32_t mode, float *out);
+/* Transform initial placement from query neighbours and their directed
+ * memberships (crates/umap/src/standard_umap/estimator.rs:2385-2481).
+ * embedding is the fitted [n, d]; out is [q, d]. */
+umaphip_status umaphip_transform_init(const int32_t *indices, const float *directed, int64_t q,
+                                      int64_t k, const float *embedding, int64_t n, int64_t d,
+                                      float *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -165,6 +165,11 @@ def lib():
         L.orc_fit_curve.restype = C.c_int
         L.orc_fit_curve.argtypes = [C.c_float, C.c_float, C.POINTER(C.c_float),
                                     C.POINTER(C.c_float)]
+        L.orc_random_init.restype = C.c_int
+        L.orc_random_init.argtypes = [C.c_int64, C.c_int64, C.c_uint64, C.c_float, C.c_float,
+                                      C.c_int, f32p]
+        L.orc_transform_init.restype = C.c_int
+        L.orc_transform_init.argtypes = [i32p, f32p, C.c_int64, C.c_int64, f32p, C.c_int64, f32p]
         _lib = L
     return _lib
 
@@ -482,3 +487,20 @@ def fit_curve(min_dist=0.1, spread=1.0):
     if lib().orc_fit_curve(min_dist, spread, C.byref(a), C.byref(b)) != 0:
         raise ValueError("curve fitting failed")
     return a.value, b.value
+
+
+def random_init(n, d, seed, repulsion=1.0, factor=1.0, mode=0):
+    out = np.zeros((n, d), np.float32)
+    lib().orc_random_init(n, d, seed, repulsion, factor, mode, out)
+    return out
+
+
+def transform_init(indices, directed, embedding):
+    indices = np.ascontiguousarray(indices, np.int32)
+    directed = np.ascontiguousarray(directed, np.float32)
+    emb = np.ascontiguousarray(embedding, np.float32)
+    q, k = indices.shape
+    out = np.zeros((q, emb.shape[1]), np.float32)
+    if lib().orc_transform_init(indices, directed, q, k, emb, emb.shape[1], out) != 0:
+        raise FloatingPointError("transform initialization row has no positive memberships")
+    return out

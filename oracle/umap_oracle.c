@@ -1521,3 +1521,99 @@ int orc_fit_curve(float min_dist, float spread, float *a_out, float *b_out) {
     *b_out = (float)b;
     return 0;
 }
+
+/* ================================================= initialisation (fit) */
+static int cmp_dbl(const void *x, const void *y) {
+    double a = *(const double *)x, b = *(const double *)y;
+    return a < b ? -1 : (a > b ? 1 : 0);
+}
+
+static double quantile_sorted(const double *s, int64_t n, double p) {
+    double h = (double)(n - 1) * p;
+    int64_t lo = (int64_t)floor(h), hi = (int64_t)ceil(h);
+    return s[lo] + (s[hi] - s[lo]) * (h - (double)lo);
+}
+
+/* random_embedding (umap-core/src/init.rs:1519-1530) + rescale
+ * (init_rescale.rs:77-203), isotropic (mode 0) or per axis (mode 1). */
+int orc_random_init(int64_t n, int64_t d, uint64_t seed, float repulsion, float factor,
+                    int mode, float *out) {
+    if (n == 1) {
+        for (int64_t c = 0; c < d; ++c) out[c] = 0.0f;
+        return 0;
+    }
+    orc_rng rng;
+    orc_rng_seed(&rng, seed);
+    for (int64_t i = 0; i < n * d; ++i) out[i] = orc_rng_range_f32(&rng, -10.0f, 10.0f);
+    for (int64_t c = 0; c < d; ++c) {
+        float sum = 0.0f;
+        for (int64_t r = 0; r < n; ++r) sum += out[r * d + c];
+        float mean = sum / (float)n;
+        for (int64_t r = 0; r < n; ++r) out[r * d + c] -= mean;
+    }
+    double *scratch = (double *)malloc(sizeof(double) * (size_t)n);
+    double *spans = (double *)malloc(sizeof(double) * (size_t)d);
+    for (int64_t c = 0; c < d; ++c) {
+        for (int64_t r = 0; r < n; ++r) scratch[r] = out[r * d + c];
+        qsort(scratch, (size_t)n, sizeof(double), cmp_dbl);
+        double span = quantile_sorted(scratch, n, 0.95) - quantile_sorted(scratch, n, 0.05);
+        spans[c] = span == 0.0 ? 1.0 : span;
+    }
+    double target = log10((double)n) * 3.0 * log1p((double)repulsion) / 0.6931471805599453 *
+                    (double)factor;
+    if (mode == 0) {
+        double logs = 0.0;
+        for (int64_t c = 0; c < d; ++c) logs += log(spans[c]);
+        float mult = (float)(target / exp(logs / (double)d));
+        for (int64_t i = 0; i < n * d; ++i) out[i] *= mult;
+    } else {
+        for (int64_t c = 0; c < d; ++c) {
+            float mult = (float)(target / spans[c]);
+            for (int64_t r = 0; r < n; ++r) out[r * d + c] *= mult;
+        }
+    }
+    free(scratch);
+    free(spans);
+    return 0;
+}
+
+/* weighted_initialization (crates/umap/src/standard_umap/estimator.rs:
+ * 2385-2481). Returns -1 (and stops) on a row without positive membership. */
+int orc_transform_init(const int32_t *indices, const float *directed, int64_t q, int64_t k,
+                       const float *embedding, int64_t d, float *out) {
+    for (int64_t row = 0; row < q; ++row) {
+        const int32_t *ids = indices + row * k;
+        const float *w = directed + row * k;
+        int64_t selected = -1;
+        int positive = 0;
+        for (int64_t s = 0; s < k; ++s) {
+            int32_t id = ids[s];
+            if (id < 0) continue;
+            int seen = 0;
+            for (int64_t o = 0; o < s; ++o) seen |= ids[o] == id;
+            if (seen) continue;
+            float weight = 0.0f;
+            for (int64_t o = 0; o < k; ++o)
+                if (ids[o] == id) weight += w[o];
+            positive |= weight > 0.0f;
+            if (weight == 1.0f && (selected < 0 || id < selected)) selected = id;
+        }
+        if (!positive) return -1;
+        if (selected >= 0) {
+            for (int64_t c = 0; c < d; ++c) out[row * d + c] = embedding[selected * d + c];
+            continue;
+        }
+        float sum = 0.0f;
+        for (int64_t s = 0; s < k; ++s) sum += w[s];
+        float div = sum == 0.0f ? 1.0f : sum;
+        for (int64_t c = 0; c < d; ++c) out[row * d + c] = 0.0f;
+        for (int64_t s = 0; s < k; ++s) {
+            float nw = w[s] / div;
+            for (int64_t c = 0; c < d; ++c) {
+                float prod = embedding[(int64_t)ids[s] * d + c] * nw;
+                out[row * d + c] = out[row * d + c] + prod;
+            }
+        }
+    }
+    return 0;
+}

@@ -184,6 +184,10 @@ long orc_fuzzy_graph(const int32_t *indices, const float *distances, int64_t n, 
                      int64_t n_reference, int32_t *heads, int32_t *tails, float *weights,
                      float *rho, float *sigma, float *directed);
 int orc_fit_curve(float min_dist, float spread, float *a, float *b);
+int orc_random_init(int64_t n, int64_t d, uint64_t seed, float repulsion, float factor,
+                    int mode, float *out);
+int orc_transform_init(const int32_t *indices, const float *directed, int64_t q, int64_t k,
+                       const float *embedding, int64_t d, float *out);
 
 #ifdef __cplusplus
 }
