@@ -58,6 +58,8 @@ class LayoutStats(C.Structure):
     _fields_ = [
         ("epochs_run", C.c_int64), ("adaptations", C.c_int64), ("refreshes", C.c_int64),
         ("final_scale", C.c_float), ("device_ms", C.c_double),
+        ("sweep_ms", C.c_double), ("sweep_launches", C.c_int64),
+        ("refresh_ms", C.c_double), ("refresh_launches", C.c_int64),
     ]
 
 
@@ -86,6 +88,7 @@ SIGNATURES = {
     "umaphip_plan_run": (C.c_int, [_vp]),
     "umaphip_plan_download": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(LayoutStats)]),
     "umaphip_plan_stream": (_vp, [_vp]),
+    "umaphip_plan_set_profiling": (C.c_int, [_vp, _i32]),
     "umaphip_plan_destroy": (None, [_vp]),
     "umaphip_knn_index_create": (C.c_int, [_vp, _i32, _i64, _i64, _i64, C.POINTER(_vp)]),
     "umaphip_knn_query": (C.c_int, [_vp, _vp, _i64, _i32, _i32, _i64, _vp, _vp, _vp]),

@@ -209,6 +209,11 @@ struct umaphip_plan {
     std::vector<int32_t> adaptation_epochs;
     int64_t sample_count = 0;
 
+    // optional per-kernel timing (events around every edge sweep / refresh)
+    bool profiling = false;
+    std::vector<hipEvent_t> sweep_ev, refresh_ev;
+    int sweep_used = 0, refresh_used = 0;
+
     std::vector<float> alpha, mu, c1, c2;
     float scale0 = 1.0f;
     uint32_t range = 0;
@@ -217,6 +222,8 @@ struct umaphip_plan {
     bool ran = false;
 
     ~umaphip_plan() {
+        for (auto e : sweep_ev) (void)hipEventDestroy(e);
+        for (auto e : refresh_ev) (void)hipEventDestroy(e);
         if (ev_start) (void)hipEventDestroy(ev_start);
         if (ev_stop) (void)hipEventDestroy(ev_stop);
         if (stream) (void)hipStreamDestroy(stream);
@@ -224,9 +231,36 @@ struct umaphip_plan {
 
     int64_t adaptation_stride() const { return sample_count * 11; }
 
+    void mark(std::vector<hipEvent_t> &pool, int &used) {
+        if (!profiling) return;
+        if (used >= static_cast<int>(pool.size())) {
+            hipEvent_t e;
+            HIP_CHECK(hipEventCreate(&e));
+            pool.push_back(e);
+        }
+        HIP_CHECK(hipEventRecord(pool[used++], stream));
+    }
+
+    // Sum of the durations between consecutive event pairs.
+    double pairs_ms(std::vector<hipEvent_t> &pool, int used) {
+        double total = 0.0;
+        for (int i = 0; i + 1 < used; i += 2) {
+            float ms = 0.0f;
+            HIP_CHECK(hipEventElapsedTime(&ms, pool[i], pool[i + 1]));
+            total += ms;
+        }
+        return total;
+    }
+
     // One projection refresh (ProjectionOrder::refresh, modern_negatives.rs:
     // 283-351): project -> stable argsort -> ranks (+ negative table).
     void refresh(int64_t k, const float *head_now) {
+        mark(refresh_ev, refresh_used);
+        refresh_body(k, head_now);
+        mark(refresh_ev, refresh_used);
+    }
+
+    void refresh_body(int64_t k, const float *head_now) {
         const float *dir = dirs.ptr + k * d;
         const float *reference = shared ? head_now : tail.ptr;
         const unsigned long long tag = static_cast<unsigned long long>(k) << 32;
@@ -269,6 +303,8 @@ void umaphip_plan::run() {
     launch_status_min_u64(status.ptr, ~0ull, stream);
     HIP_CHECK(hipStreamSynchronize(stream));  // host staging above is pageable
 
+    sweep_used = 0;
+    refresh_used = 0;
     HIP_CHECK(hipEventRecord(ev_start, stream));
     OrderReplay orders(h, n, p.seed, alias, hard, momentum);
     const bool upload_negative = !hard;  // global mode reads host orders
@@ -366,7 +402,9 @@ void umaphip_plan::run() {
             A.multiplier = 1.0f; A.alpha = alpha[t]; A.c1 = c1[t]; A.c2 = c2[t];
             A.shared = shared; A.negatives = p.negative_sample_rate != 0;
             A.exclude = p.exclude_graph_neighbors; A.fused = 1;
+            mark(sweep_ev, sweep_used);
             launch_edge_sweep2(A, stream);
+            mark(sweep_ev, sweep_used);
             c = 1 - c;
         } else {
             const float scale_host = scale0;  // momentum never adapts the scale
@@ -618,6 +656,13 @@ umaphip_status umaphip_plan_run(umaphip_plan *plan) {
 
 void *umaphip_plan_stream(umaphip_plan *plan) { return plan ? plan->stream : nullptr; }
 
+umaphip_status umaphip_plan_set_profiling(umaphip_plan *plan, int32_t enabled) {
+    return guarded([&] {
+        UMAPHIP_REQUIRE(plan != nullptr, UMAPHIP_INVALID_INPUT, "plan is null");
+        plan->profiling = enabled != 0;
+    });
+}
+
 umaphip_status umaphip_plan_download(umaphip_plan *plan, float *head, double *next_positive,
                                      double *next_negative, umaphip_layout_stats *stats) {
     return guarded([&] {
@@ -648,6 +693,10 @@ umaphip_status umaphip_plan_download(umaphip_plan *plan, float *head, double *ne
             float ms = 0.0f;
             if (plan->ran) HIP_CHECK(hipEventElapsedTime(&ms, plan->ev_start, plan->ev_stop));
             stats->device_ms = ms;
+            stats->sweep_ms = plan->ran ? plan->pairs_ms(plan->sweep_ev, plan->sweep_used) : 0.0;
+            stats->sweep_launches = plan->sweep_used / 2;
+            stats->refresh_ms = plan->ran ? plan->pairs_ms(plan->refresh_ev, plan->refresh_used) : 0.0;
+            stats->refresh_launches = plan->refresh_used / 2;
         }
         UMAPHIP_REQUIRE(status == ~0ull, UMAPHIP_NUMERICAL,
                         "non-finite %s projection at row %llu (refresh %llu)",

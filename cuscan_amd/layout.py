@@ -116,6 +116,9 @@ class LayoutPlan:
             _ptr(negperiods), _ptr(nneg), C.byref(p), int(shared), C.byref(handle)))
         self._handle = handle
 
+    def set_profiling(self, enabled: bool = True) -> None:
+        _lib.check(_lib.lib().umaphip_plan_set_profiling(self._handle, int(bool(enabled))))
+
     @property
     def stream(self) -> int:
         return _lib.lib().umaphip_plan_stream(self._handle)

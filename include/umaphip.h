@@ -159,6 +159,10 @@ typedef struct {
     int64_t refreshes;
     float final_scale;
     double device_ms;               /* epoch loop, device events */
+    double sweep_ms;                /* sum over edge-sweep launches (profiling on) */
+    int64_t sweep_launches;
+    double refresh_ms;              /* sum over projection refreshes (profiling on) */
+    int64_t refresh_launches;
 } umaphip_layout_stats;
 
 /* Host arrays in, host arrays out (allocates and frees device memory).
@@ -189,6 +193,9 @@ umaphip_status umaphip_plan_run(umaphip_plan *plan);
  * non-NULL, the clocks (E doubles each) to host. */
 umaphip_status umaphip_plan_download(umaphip_plan *plan, float *head, double *next_positive,
                                      double *next_negative, umaphip_layout_stats *stats);
+/* Records device events around every fused edge sweep and projection
+ * refresh of later runs (per-kernel durations in the stats). */
+umaphip_status umaphip_plan_set_profiling(umaphip_plan *plan, int32_t enabled);
 /* The plan's stream (hipStream_t as void*). */
 void *umaphip_plan_stream(umaphip_plan *plan);
 void umaphip_plan_destroy(umaphip_plan *plan);

@@ -417,8 +417,10 @@ class ScaleAdaptation:
 def optimize_compact(head, tail, graph, *, epochs, a, b, repulsion=1.0, initial_alpha=1.0,
                      optimizer="adam", negative_sample_rate=5, negative_selection_range=200_000,
                      negative_sample_scale=None, adaptation_samples=128, exclude=False,
-                     good=False, seed=42, shared=True, threads=None, stop_epoch=0):
-    """Run the restated epoch loop in place on copies; returns (head, clocks, stats)."""
+                     good=False, seed=42, shared=True, threads=None, stop_epoch=0,
+                     collect_stats=True):
+    """Run the restated epoch loop in place on copies; returns (head, clocks, stats).
+    collect_stats replays each epoch's negatives once more to count them."""
     head = np.array(head, np.float32, order="C", copy=True)
     h, d = head.shape
     tail_arr = None if shared else np.ascontiguousarray(tail, np.float32)
@@ -444,7 +446,8 @@ def optimize_compact(head, tail, graph, *, epochs, a, b, repulsion=1.0, initial_
         np.ascontiguousarray(graph["indices"], np.uint32),
         np.ascontiguousarray(graph["periods"], np.float64), npos,
         np.ascontiguousarray(graph["negative_periods"], np.float64), nneg, C.byref(P),
-        int(shared), threads or threads_default(), stop_epoch, C.byref(S))
+        int(shared), threads or threads_default(), stop_epoch,
+        C.byref(S) if collect_stats else None)
     if rc != 0:
         raise RuntimeError(f"oracle layout failed ({rc})")
     stats = {f: getattr(S, f) for f, _ in LayoutStats._fields_}
@@ -504,3 +507,35 @@ def transform_init(indices, directed, embedding):
     if lib().orc_transform_init(indices, directed, q, k, emb, emb.shape[1], out) != 0:
         raise FloatingPointError("transform initialization row has no positive memberships")
     return out
+
+
+def fit(x, *, n_neighbors=15, n_components=2, min_dist=0.1, spread=1.0, n_epochs=None,
+        learning_rate=1.0, negative_sample_rate=5, repulsion=1.0, optimizer="adam",
+        negative_selection_range=200_000, exclude=False, seed=42, threads=None):
+    """CPU reference fit (init="random"): exact kNN -> fuzzy graph -> prune ->
+    compact -> random init + rescale -> modern layout. Restates the call
+    chain of crates/umap/src/standard_umap/estimator.rs:117-1005."""
+    x = np.ascontiguousarray(x, np.float32)
+    n = x.shape[0]
+    k = min(n_neighbors, n - 1)
+    idx, dist = exact_knn(x, x, k - 1, exclude_self=True, threads=threads)
+    indices = np.concatenate([np.arange(n, dtype=np.int32)[:, None], idx], 1)
+    distances = np.concatenate([np.zeros((n, 1), np.float32), dist], 1)
+    g = fuzzy_graph(indices, distances)
+    u = g["used"]
+    heads, tails, w = g["heads"][:u], g["tails"][:u], g["weights"][:u]
+    default_epochs = 500 if n <= 10_000 else 200
+    epochs = n_epochs or default_epochs
+    divisor = epochs if epochs > 10 else default_epochs
+    thr = np.float32(w.max()) / np.float32(divisor)
+    w = np.where(w < thr, np.float32(0), w)
+    graph = compact_graph(n, n, heads, tails, w, epochs, float(negative_sample_rate))
+    a, b = fit_curve(min_dist, spread)
+    init = random_init(n, n_components, seed, repulsion)
+    emb, _, _, _ = optimize_compact(init, None, graph, epochs=epochs, a=a, b=b,
+                                    repulsion=repulsion, initial_alpha=learning_rate,
+                                    optimizer=optimizer, negative_sample_rate=negative_sample_rate,
+                                    negative_selection_range=negative_selection_range,
+                                    exclude=exclude, seed=seed, threads=threads,
+                                    collect_stats=False)
+    return emb
