@@ -95,14 +95,18 @@ def build_workload(rows, dims, clusters, seed, k, epochs):
     torch.cuda.synchronize()
     log(f"data {rows}x{dims} in {time.time() - t0:.1f}s")
     t0 = time.time()
-    index = KnnIndex(x, max_queries=min(rows, 1 << 20))
-    idx_d, dist_d = index.query_device(x, k - 1, exclude_self=True)
-    torch.cuda.synchronize()
+    chunk = 131072
+    index = KnnIndex(x, max_queries=min(rows, chunk))
+    idx = np.empty((rows, k - 1), np.int32)
+    dist = np.empty((rows, k - 1), np.float32)
+    for s in range(0, rows, chunk):
+        e = min(rows, s + chunk)
+        i_d, d_d = index.query_device(x[s:e], k - 1, exclude_self=True, query_offset=s)
+        idx[s:e] = i_d.cpu().numpy()
+        dist[s:e] = d_d.cpu().numpy()
+        log(f"  kNN rows {e}/{rows} ({time.time() - t0:.1f}s)")
     knn_s = time.time() - t0
     log(f"exact kNN ({k - 1} non-self) in {knn_s:.1f}s")
-    idx = idx_d.cpu().numpy()
-    dist = dist_d.cpu().numpy()
-    del idx_d, dist_d
     index.close()
     t0 = time.time()
     indices = np.concatenate([np.arange(rows, dtype=np.int32)[:, None], idx], axis=1)
