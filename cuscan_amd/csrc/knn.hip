@@ -340,7 +340,7 @@ int ksteps_for(int64_t d) {
 // ------------------------------------------------------------- KnnIndex
 KnnIndex::KnnIndex(const float *ref_host, const float *ref_device, int64_t n, int64_t d, int64_t max_q,
                    hipStream_t s)
-    : n_(n), d_(d), max_q_(max_q) {
+    : n_(n), d_(d) {
     ks_ = ksteps_for(d);
     UMAPHIP_REQUIRE(ks_ > 0, UMAPHIP_INVALID_INPUT, "kNN supports at most 1024 features (got %lld)",
                     (long long)d);

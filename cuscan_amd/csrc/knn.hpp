@@ -27,7 +27,7 @@ public:
     int slices_for(int64_t q) const;
 
 private:
-    int64_t n_, d_, dp_ = 0, max_q_, reserved_q_ = 0;
+    int64_t n_, d_, dp_ = 0, reserved_q_ = 0;
     int ks_ = 0;
     DeviceArray<float> ref_, rnorm_, cand_d_, top_d_;
     DeviceArray<uint16_t> ref_bf_, qbf_;

@@ -403,7 +403,7 @@ void umaphip_plan::run() {
             A.shared = shared; A.negatives = p.negative_sample_rate != 0;
             A.exclude = p.exclude_graph_neighbors; A.fused = 1;
             mark(sweep_ev, sweep_used);
-            launch_edge_sweep2(A, stream);
+            launch_edge_sweep(A, stream);
             mark(sweep_ev, sweep_used);
             c = 1 - c;
         } else {
@@ -451,7 +451,7 @@ void umaphip_plan::run() {
                         A.multiplier = alpha[t];
                         A.shared = shared; A.negatives = p.negative_sample_rate != 0;
                         A.exclude = p.exclude_graph_neighbors; A.fused = 0;
-                        launch_edge_sweep2(A, stream);
+                        launch_edge_sweep(A, stream);
                         launch_momentum_apply_rows(h, pos[c].ptr, updates.ptr, block_of_row.ptr,
                                                    blk, stream);
                     } else {
@@ -822,7 +822,7 @@ umaphip_status umaphip_modern_epoch(const umaphip_epoch_args *a, void *stream) {
             A.range = a->negative_range; A.multiplier = 1.0f;
             A.shared = a->shared; A.negatives = a->negatives;
             A.exclude = a->exclude_graph_neighbors; A.fused = 0;
-            launch_edge_sweep2(A, s);
+            launch_edge_sweep(A, s);
         } else {
             R.multiplier = 1.0f;
             launch_gradient_rowwise(R, 0, h, s);

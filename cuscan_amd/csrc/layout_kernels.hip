@@ -281,34 +281,119 @@ __global__ void momentum_decay_kernel(int64_t h, int64_t d, float *vel, int64_t 
 }
 
 // ------------------------------------------------------ tiled edge sweep
-// One workgroup owns kTileRows consecutive CSR rows. Its edges stream in
-// chunks of 256 (one lane per edge, coalesced clock/index/period reads);
-// each lane computes its edge's attraction plus its negatives; per-row sums
-// are then formed in CSR order from LDS by one lane per row, so results are
-// deterministic and no global atomics are needed. The fused epilogue applies
-// the lazy Adam step into a second position buffer (Jacobi double buffer).
+// One workgroup owns kTileRows consecutive CSR rows and streams their edges
+// in chunks of 256 (one lane per edge: coalesced clock/index/period reads).
+//   A. Each lane advances its edge's f64 clocks and counts its force terms:
+//      the attraction plus one term per negative sample.
+//   B. The chunk's terms are laid out in edge order in LDS (a block-wide
+//      exclusive scan) and computed term-parallel, so every lane does the
+//      same work whether its edge fired or not; each negative term derives
+//      its candidate rank e*(epoch+p+1) mod M independently.
+//   C. One lane per row adds its row's terms in edge order, i.e. the
+//      reference's per-term sequential accumulation order.
+// The fused epilogue applies the lazy Adam step into a second position
+// buffer (Jacobi double buffer). Deterministic; no global atomics.
 constexpr int kTileRows = 64;
+constexpr int kTermCap = 2048;
 
 __device__ inline float2 ld2(const float *p, int64_t row) {
     return reinterpret_cast<const float2 *>(p)[row];
 }
 
-__global__ __launch_bounds__(kThreads) void edge_sweep2_kernel(SweepArgs A) {
+// q^b by the hardware base-2 exp/log (relative error a few ulp).
+__device__ inline float fast_pow(float q, float b) {
+    return __builtin_amdgcn_exp2f(b * __builtin_amdgcn_logf(q));
+}
+
+// tanh for x >= 0: odd Taylor polynomial below 1/8 (truncation < 2e-10
+// relative), (e^{2x} - 1) / (e^{2x} + 1) above, 1 beyond 9.
+__device__ inline float tanh_pos(float x) {
+    if (x < 0.125f) {
+        const float x2 = x * x;
+        return x * (1.0f + x2 * (-0.33333334f + x2 * (0.13333334f + x2 * -0.053968254f)));
+    }
+    if (x > 9.0f) return 1.0f;
+    const float e = __builtin_amdgcn_exp2f(x * 2.8853900817779268f);
+    return __fdividef(e - 1.0f, e + 1.0f);
+}
+
+// Attraction factor (coefficient of x_i - y): fm * -2ab q^(b-1) / (1 + a q^b).
+__device__ inline float attract_fast(float q, float a, float b, float fm) {
+    const float pb = fast_pow(q, b);
+    return __fdividef(fm * (-2.0f * a * b) * __fdividef(pb, q), 1.0f + a * pb);
+}
+
+// Repulsion factor: with c = 2*gamma*b*s / ((0.001 + q)(1 + a q^b)) and
+// norm = c*sqrt(q), c * min(gamma,16) * tanh(norm/gamma) / norm equals
+// min(gamma,16) * tanh(c*sqrt(q)/gamma) / sqrt(q).
+__device__ inline float repel_fast(float q, float a, float b, float gamma, float s) {
+    if (!(q > 0.0f && gamma > 0.0f)) return 0.0f;
+    const float c = __fdividef(s * 2.0f * gamma * b, (0.001f + q) * (1.0f + a * fast_pow(q, b)));
+    if (!(c > 0.0f)) return 0.0f;
+    const float rq = __builtin_amdgcn_rsqf(q);
+    const float g16 = gamma < 16.0f ? gamma : 16.0f;
+    return g16 * tanh_pos(__fdividef(c * q * rq, gamma)) * rq;
+}
+
+// x mod m for x < 2^64, m < 2^32, through a double-precision quotient
+// estimate corrected by at most two steps.
+__device__ inline uint64_t mod_u64(uint64_t x, uint64_t m, double inv_m) {
+    const uint64_t q = static_cast<uint64_t>(static_cast<double>(x) * inv_m);
+    int64_t r = static_cast<int64_t>(x - q * m);
+    while (r < 0) r += static_cast<int64_t>(m);
+    while (r >= static_cast<int64_t>(m)) r -= static_cast<int64_t>(m);
+    return static_cast<uint64_t>(r);
+}
+
+__device__ inline int block_exclusive_scan(int v, int tid, int *s_wave, int &total) {
+    const int lane = tid & 63, wave = tid >> 6;
+    int incl = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int up = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += up;
+    }
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    int before = 0;
+    for (int w = 0; w < wave; ++w) before += s_wave[w];
+    total = s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+    return before + incl - v;
+}
+
+__global__ __launch_bounds__(kThreads) void edge_sweep_kernel(SweepArgs A) {
     __shared__ int32_t s_off[kTileRows + 1];
     __shared__ float2 s_pos[kTileRows];
-    __shared__ int32_t s_rank[kTileRows];
+    __shared__ uint32_t s_first[kTileRows];
     __shared__ int8_t s_active[kTileRows];
-    __shared__ float2 s_contrib[kThreads];
+    __shared__ float2 s_terms[kTermCap];
+    __shared__ uint8_t s_owner[kTermCap];
+    __shared__ int32_t s_base[kThreads + 1];
+    __shared__ int32_t s_j[kThreads];
+    __shared__ uint8_t s_row[kThreads];
+    __shared__ uint32_t s_em[kThreads];
+    __shared__ int s_wave[4], s_wave2[4];
 
     const int tid = threadIdx.x;
     const int64_t row0 = static_cast<int64_t>(blockIdx.x) * kTileRows;
     const int rows = static_cast<int>(min<int64_t>(kTileRows, A.h - row0));
     const bool small = A.h <= 1024;
-    const bool windowed = !small && static_cast<int64_t>(A.range) < A.n;
+    const uint64_t n64 = static_cast<uint64_t>(A.n);
+    const uint64_t modulus = small ? n64 : A.range;
+    const bool windowed = !small && static_cast<uint64_t>(A.range) < n64;
+    const bool retry = small && A.exclude;
+    const double inv_mod = 1.0 / static_cast<double>(modulus);
+    const uint64_t t1 = (static_cast<uint64_t>(A.epoch) + 1u) % modulus;
     if (tid <= rows) s_off[tid] = A.offsets[row0 + tid];
     if (tid < rows) {
         s_pos[tid] = ld2(A.pos, row0 + tid);
-        s_rank[tid] = windowed ? A.ranks[row0 + tid] : 0;
+        uint64_t first = 0;
+        if (windowed) {
+            const uint64_t rk = static_cast<uint32_t>(A.ranks[row0 + tid]);
+            const uint64_t half = A.range / 2;
+            if (rk > half) first = rk - half;
+            if (first > n64 - A.range) first = n64 - A.range;
+        }
+        s_first[tid] = static_cast<uint32_t>(first);
         s_active[tid] = A.block_of_row ? (A.block_of_row[row0 + tid] == A.block_id) : 1;
     }
     const float scale = A.scale_ptr ? *A.scale_ptr : A.scale;
@@ -320,75 +405,203 @@ __global__ __launch_bounds__(kThreads) void edge_sweep2_kernel(SweepArgs A) {
     const float s_neg = small ? 1.0f : scale;
     float2 acc = make_float2(0.0f, 0.0f);
 
-    for (int32_t base = e_begin; base < e_end; base += kThreads) {
-        const int32_t e = base + tid;
-        float2 g = make_float2(0.0f, 0.0f);
+    // Stream registers: the next chunk's clock/index/period words are
+    // loaded while the current chunk's gathers are in flight.
+    double pf_npos = 0.0, pf_per = 0.0, pf_nneg = 0.0, pf_nper = 1.0;
+    int32_t pf_idx = 0;
+    auto prefetch = [&](int32_t chunk) {
+        const int32_t e = chunk + tid;
         if (e < e_end) {
-            int lo = 0, hi = rows;  // last r with s_off[r] <= e
+            pf_npos = A.next_positive[e];
+            pf_idx = A.indices[e];
+            pf_per = A.periods[e];
+            if (A.negatives) {
+                pf_nneg = A.next_negative[e];
+                pf_nper = A.negative_periods[e];
+            }
+        }
+    };
+    prefetch(e_begin);
+
+    for (int32_t base = e_begin; base < e_end; base += kThreads) {
+        // ---- A: clocks and term counts
+        const int32_t e = base + tid;
+        const double npos = pf_npos, per = pf_per, nn = pf_nneg, np = pf_nper;
+        const int32_t jidx = pf_idx;
+        if (base + kThreads < e_end) prefetch(base + kThreads);
+        int terms = 0, count = 0, r = 0;
+        float2 retry_sum = make_float2(0.0f, 0.0f);
+        if (e < e_end) {
+            int lo = 0, hi = rows;
             while (hi - lo > 1) {
                 const int mid = (lo + hi) >> 1;
                 if (s_off[mid] <= e) lo = mid;
                 else hi = mid;
             }
-            const int r = lo;
-            const double npos = A.next_positive[e];
+            r = lo;
             if (s_active[r] && npos <= now) {
-                const int64_t i = row0 + r;
-                const float2 xi = s_pos[r];
-                const int64_t j = static_cast<uint32_t>(A.indices[e]);
-                const float2 y = ld2(A.tgt, j);
-                const float dx = xi.x - y.x, dy = xi.y - y.y;
-                const float q = dx * dx + dy * dy;
-                if (q > 0.0f) {
-                    const float c = attract_coeff(q, A.a, A.b, fm);
-                    g.x = g.x + c * dx;
-                    g.y = g.y + c * dy;
-                }
-                A.next_positive[e] = npos + A.periods[e];
+                s_j[tid] = jidx;
+                A.next_positive[e] = npos + per;
                 if (A.negatives) {
-                    const double nn = A.next_negative[e];
-                    const double np = A.negative_periods[e];
-                    const int32_t count = static_cast<int32_t>((now - nn) / np);
-                    if (count > 0) {
-                        NegWalk w;
-                        w.init(static_cast<uint64_t>(e), A.epoch, count, small, A.exclude != 0,
-                               static_cast<uint64_t>(A.n), A.range, s_rank[r]);
-                        uint64_t rank;
-                        const int64_t begin = s_off[r], stop = s_off[r + 1];
-                        while (w.next(rank)) {
-                            float2 yn;
-                            if (A.exclude) {
-                                const int64_t jj = static_cast<uint32_t>(A.order[rank]);
-                                if (excluded(A.indices, begin, stop, i, jj, A.shared != 0))
-                                    continue;
-                                w.accept();
-                                yn = A.table ? ld2(A.table, rank) : ld2(A.tgt, jj);
-                            } else {
-                                yn = A.table ? ld2(A.table, rank)
-                                             : ld2(A.tgt, static_cast<uint32_t>(A.order[rank]));
-                            }
-                            const float nx = xi.x - yn.x, ny = xi.y - yn.y;
-                            const float qn = nx * nx + ny * ny;
-                            const float c = repel_coeff(qn, A.a, A.b, A.gamma, s_neg, 1.0f);
-                            if (c != 0.0f) {
-                                g.x = g.x + c * nx;
-                                g.y = g.y + c * ny;
-                            }
-                        }
-                    }
+                    count = static_cast<int32_t>((now - nn) / np);
                     A.next_negative[e] = nn + static_cast<double>(count) * np;
                 }
+                const int negs = count > 0 ? count : 0;
+                if (retry && negs > 0) {
+                    // Small layouts with exclusion retry rejected candidates
+                    // sequentially; the accepted forces form one term.
+                    NegWalk w;
+                    w.init(static_cast<uint64_t>(e), A.epoch, negs, true, true, n64, A.range, 0);
+                    const int64_t i = row0 + r;
+                    const float2 xi = s_pos[r];
+                    uint64_t rank;
+                    while (w.next(rank)) {
+                        const int64_t jj = static_cast<uint32_t>(A.order[rank]);
+                        if (excluded(A.indices, s_off[r], s_off[r + 1], i, jj, A.shared != 0)) continue;
+                        w.accept();
+                        const float2 y = A.table ? ld2(A.table, rank) : ld2(A.tgt, jj);
+                        const float dx = xi.x - y.x, dy = xi.y - y.y;
+                        const float c = repel_fast(dx * dx + dy * dy, A.a, A.b, A.gamma, 1.0f);
+                        retry_sum.x = retry_sum.x + c * dx;
+                        retry_sum.y = retry_sum.y + c * dy;
+                    }
+                    terms = 2;
+                } else {
+                    terms = 1 + negs;
+                }
+                s_em[tid] = static_cast<uint32_t>(static_cast<uint32_t>(e) % static_cast<uint32_t>(modulus));
             }
         }
-        s_contrib[tid] = g;
+        s_row[tid] = static_cast<uint8_t>(r);
+        int total;
+        const int my_base = block_exclusive_scan(terms, tid, s_wave, total);
+        const bool term_parallel = total <= kTermCap;
+        // Chunks with more terms than LDS holds fall back to one combined
+        // term per edge, computed by the edge's own lane.
+        int fb_total;
+        const int fb_base = block_exclusive_scan(terms > 0 ? 1 : 0, tid, s_wave2, fb_total);
+        (void)fb_total;
+        if (term_parallel) {
+            s_base[tid] = my_base;
+            for (int k = 0; k < terms; ++k) s_owner[my_base + k] = static_cast<uint8_t>(tid);
+            if (retry && terms == 2) s_terms[my_base + 1] = retry_sum;
+        } else {
+            s_base[tid] = fb_base;
+        }
+        if (tid == kThreads - 1) s_base[kThreads] = term_parallel ? total : fb_base + (terms > 0);
         __syncthreads();
+
+        // ---- B: force terms (two per thread in flight)
+        if (term_parallel) {
+            for (int item0 = tid; item0 < total; item0 += 2 * kThreads) {
+                float2 y[2], xi[2];
+                int kk[2];
+                bool live[2], keep[2];
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    const int item = item0 + u * kThreads;
+                    live[u] = item < total;
+                    keep[u] = live[u];
+                    kk[u] = 0;
+                    y[u] = make_float2(0.0f, 0.0f);
+                    xi[u] = make_float2(0.0f, 0.0f);
+                    if (!live[u]) continue;
+                    const int owner = s_owner[item];
+                    const int k = item - s_base[owner];
+                    const int rr = s_row[owner];
+                    kk[u] = k;
+                    xi[u] = s_pos[rr];
+                    if (k == 0) {
+                        y[u] = ld2(A.tgt, static_cast<uint32_t>(s_j[owner]));
+                    } else if (retry) {
+                        live[u] = false;  // combined retry term already stored
+                    } else {
+                        uint64_t f = t1 + static_cast<uint64_t>(k - 1);
+                        if (f >= modulus) f = f % modulus;
+                        const uint64_t sel =
+                            mod_u64(static_cast<uint64_t>(s_em[owner]) * f, modulus, inv_mod);
+                        const uint64_t rank = static_cast<uint64_t>(s_first[rr]) + sel;
+                        if (A.exclude) {
+                            const int64_t jj = static_cast<uint32_t>(A.order[rank]);
+                            keep[u] = !excluded(A.indices, s_off[rr], s_off[rr + 1], row0 + rr, jj,
+                                                A.shared != 0);
+                            y[u] = A.table ? ld2(A.table, rank) : ld2(A.tgt, jj);
+                        } else {
+                            y[u] = A.table ? ld2(A.table, rank)
+                                           : ld2(A.tgt, static_cast<uint32_t>(A.order[rank]));
+                        }
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    if (!live[u]) continue;
+                    float2 term = make_float2(0.0f, 0.0f);
+                    const float dx = xi[u].x - y[u].x, dy = xi[u].y - y[u].y;
+                    const float q = dx * dx + dy * dy;
+                    if (kk[u] == 0) {
+                        if (q > 0.0f) {
+                            const float c = attract_fast(q, A.a, A.b, fm);
+                            term = make_float2(c * dx, c * dy);
+                        }
+                    } else if (keep[u]) {
+                        const float c = repel_fast(q, A.a, A.b, A.gamma, s_neg);
+                        term = make_float2(c * dx, c * dy);
+                    }
+                    s_terms[item0 + u * kThreads] = term;
+                }
+            }
+        } else if (terms > 0) {
+            // fallback: the lane's whole edge as one term
+            const int64_t i = row0 + r;
+            const float2 xi = s_pos[r];
+            float2 g = make_float2(0.0f, 0.0f);
+            const float2 y = ld2(A.tgt, static_cast<uint32_t>(s_j[tid]));
+            const float dx = xi.x - y.x, dy = xi.y - y.y;
+            const float q = dx * dx + dy * dy;
+            if (q > 0.0f) {
+                const float c = attract_fast(q, A.a, A.b, fm);
+                g = make_float2(c * dx, c * dy);
+            }
+            if (retry) {
+                g.x = g.x + retry_sum.x;
+                g.y = g.y + retry_sum.y;
+            } else if (count > 0) {
+                NegWalk w;
+                w.init(static_cast<uint64_t>(e), A.epoch, count, small, false, n64, A.range,
+                       0);
+                w.first = s_first[r];
+                uint64_t rank;
+                while (w.next(rank)) {
+                    float2 yn;
+                    if (A.exclude) {
+                        const int64_t jj = static_cast<uint32_t>(A.order[rank]);
+                        if (excluded(A.indices, s_off[r], s_off[r + 1], i, jj, A.shared != 0)) continue;
+                        yn = A.table ? ld2(A.table, rank) : ld2(A.tgt, jj);
+                    } else {
+                        yn = A.table ? ld2(A.table, rank)
+                                     : ld2(A.tgt, static_cast<uint32_t>(A.order[rank]));
+                    }
+                    const float nx = xi.x - yn.x, ny = xi.y - yn.y;
+                    const float c = repel_fast(nx * nx + ny * ny, A.a, A.b, A.gamma, s_neg);
+                    g.x = g.x + c * nx;
+                    g.y = g.y + c * ny;
+                }
+            }
+            s_terms[fb_base] = g;
+        }
+        __syncthreads();
+
+        // ---- C: per-row sums in edge (and term) order
         if (tid < rows) {
-            const int32_t lo = max(s_off[tid], base);
-            const int32_t hi = min(s_off[tid + 1], base + kThreads);
-            for (int32_t k = lo; k < hi; ++k) {
-                const float2 c = s_contrib[k - base];
-                acc.x = acc.x + c.x;
-                acc.y = acc.y + c.y;
+            const int32_t lo = max(s_off[tid], base) - base;
+            const int32_t hi = min(s_off[tid + 1], base + kThreads) - base;
+            if (lo < hi) {
+                const int t_lo = s_base[lo], t_hi = s_base[hi];
+                for (int k = t_lo; k < t_hi; ++k) {
+                    const float2 c = s_terms[k];
+                    acc.x = acc.x + c.x;
+                    acc.y = acc.y + c.y;
+                }
             }
         }
         __syncthreads();
@@ -400,18 +613,15 @@ __global__ __launch_bounds__(kThreads) void edge_sweep2_kernel(SweepArgs A) {
         const float2 x = s_pos[tid];
         float2 mm = ld2(A.m, i), vv = ld2(A.v, i);
         float2 out = x;
-        const float gs[2] = {acc.x, acc.y};
-        float *mp = &mm.x, *vp = &vv.x, *op = &out.x;
-        const float xs[2] = {x.x, x.y};
-        for (int c = 0; c < 2; ++c) {
-            const float gc = gs[c];
-            if (gc != 0.0f) {
-                const float first = 0.9f * mp[c] + (1.0f - 0.9f) * gc;
-                const float second = 0.99f * vp[c] + ((1.0f - 0.99f) * gc) * gc;
-                mp[c] = first;
-                vp[c] = second;
-                op[c] = xs[c] + (A.alpha * (first / A.c1)) / (sqrtf(second / A.c2) + 0.0001f);
-            }
+        if (acc.x != 0.0f) {
+            mm.x = 0.9f * mm.x + (1.0f - 0.9f) * acc.x;
+            vv.x = 0.99f * vv.x + ((1.0f - 0.99f) * acc.x) * acc.x;
+            out.x = x.x + (A.alpha * (mm.x / A.c1)) / (sqrtf(vv.x / A.c2) + 0.0001f);
+        }
+        if (acc.y != 0.0f) {
+            mm.y = 0.9f * mm.y + (1.0f - 0.9f) * acc.y;
+            vv.y = 0.99f * vv.y + ((1.0f - 0.99f) * acc.y) * acc.y;
+            out.y = x.y + (A.alpha * (mm.y / A.c1)) / (sqrtf(vv.y / A.c2) + 0.0001f);
         }
         reinterpret_cast<float2 *>(A.m)[i] = mm;
         reinterpret_cast<float2 *>(A.v)[i] = vv;
@@ -729,9 +939,9 @@ void launch_momentum_decay(int64_t h, int64_t d, float *vel, int64_t vs0, int64_
     HIP_CHECK(hipGetLastError());
 }
 
-void launch_edge_sweep2(const SweepArgs &a, hipStream_t s) {
+void launch_edge_sweep(const SweepArgs &a, hipStream_t s) {
     if (a.h == 0) return;
-    hipLaunchKernelGGL(edge_sweep2_kernel, dim3(grid_for(a.h, kTileRows)), dim3(kThreads), 0, s,
+    hipLaunchKernelGGL(edge_sweep_kernel, dim3(grid_for(a.h, kTileRows)), dim3(kThreads), 0, s,
                        a);
     HIP_CHECK(hipGetLastError());
 }

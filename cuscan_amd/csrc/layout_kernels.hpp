@@ -58,7 +58,7 @@ struct SweepArgs {
     float multiplier, alpha, c1, c2;
     int shared, negatives, exclude, fused;
 };
-void launch_edge_sweep2(const SweepArgs &a, hipStream_t s);
+void launch_edge_sweep(const SweepArgs &a, hipStream_t s);
 
 // Debug: per-edge negative candidates (same selection code).
 void launch_debug_negatives(const RowwiseArgs &a, int32_t cap, int32_t *count_out,
