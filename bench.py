@@ -189,7 +189,7 @@ def main():
     achieved = bytes_per_launch / (sweep_avg_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
-                "kernel": "edge_sweep2_kernel (fused edge pass + lazy Adam)",
+                "kernel": "unit_sweep_kernel + unit_adam_kernel (edge pass + lazy Adam)",
                 "bytes_per_launch": int(bytes_per_launch), "avg_launch_ms": round(sweep_avg_ms, 4),
                 "firing_edges_per_epoch": F / args.epochs, "negatives_per_epoch": S / args.epochs,
                 "refresh_avg_ms": round(pstats["refresh_ms"] / max(pstats["refresh_launches"], 1), 4)}

@@ -177,6 +177,60 @@ void check_layout_params(const umaphip_layout_params *p, int64_t h, int64_t n, i
                     "negative sample rate must be nonnegative");
 }
 
+// Work partition of the unit sweep: unit u covers CSR edges
+// [u*UE, (u+1)*UE). Rows are indexed among the rows with at least one edge.
+struct UnitTables {
+    int64_t units = 0;
+    std::vector<int32_t> unit_k;     // [units][2] first/last nonzero-degree row index
+    std::vector<int32_t> unit_frag;  // [units][2] fragment slot of the first/last row, or -1
+    std::vector<int32_t> nz_rows;    // empty when every row has an edge
+    std::vector<int32_t> nz_off;     // CSR start of each nonzero-degree row, then E
+    std::vector<int32_t> row_split;  // [h] split-row index, or -1
+    std::vector<int32_t> split_off;  // [splits+1] first fragment of each split row
+};
+
+UnitTables build_unit_tables(const uint32_t *offsets, int64_t h, int64_t E, int64_t UE) {
+    UnitTables T;
+    std::vector<int32_t> rows;
+    rows.reserve(static_cast<size_t>(h));
+    for (int64_t i = 0; i < h; ++i)
+        if (offsets[i + 1] > offsets[i]) rows.push_back(static_cast<int32_t>(i));
+    const int64_t K = static_cast<int64_t>(rows.size());
+    T.nz_off.resize(static_cast<size_t>(K + 1));
+    for (int64_t k = 0; k < K; ++k) T.nz_off[k] = static_cast<int32_t>(offsets[rows[k]]);
+    T.nz_off[K] = static_cast<int32_t>(E);
+    if (K != h) T.nz_rows = rows;
+    T.units = (E + UE - 1) / UE;
+    T.unit_k.resize(static_cast<size_t>(2 * T.units));
+    T.unit_frag.assign(static_cast<size_t>(2 * T.units), -1);
+    int64_t k = 0;
+    for (int64_t u = 0; u < T.units; ++u) {
+        const int64_t e0 = u * UE, last = std::min(E, e0 + UE) - 1;
+        while (T.nz_off[k + 1] <= e0) ++k;
+        int64_t k1 = k;
+        while (T.nz_off[k1 + 1] <= last) ++k1;
+        T.unit_k[2 * u] = static_cast<int32_t>(k);
+        T.unit_k[2 * u + 1] = static_cast<int32_t>(k1);
+    }
+    T.row_split.assign(static_cast<size_t>(h), -1);
+    T.split_off.push_back(0);
+    int64_t frags = 0;
+    for (int64_t q = 0; q < K; ++q) {
+        const int64_t ua = T.nz_off[q] / UE, ub = (T.nz_off[q + 1] - 1) / UE;
+        if (ua == ub) continue;
+        T.row_split[rows[q]] = static_cast<int32_t>(T.split_off.size() - 1);
+        for (int64_t w = ua; w <= ub; ++w) {
+            const int32_t slot = static_cast<int32_t>(frags + (w - ua));
+            if (T.unit_k[2 * w] == q) T.unit_frag[2 * w] = slot;
+            else T.unit_frag[2 * w + 1] = slot;
+        }
+        frags += ub - ua + 1;
+        UMAPHIP_REQUIRE(frags <= INT32_MAX, UMAPHIP_OVERFLOW, "too many row fragments");
+        T.split_off.push_back(static_cast<int32_t>(frags));
+    }
+    return T;
+}
+
 }  // namespace
 }  // namespace umaphip
 
@@ -213,6 +267,14 @@ struct umaphip_plan {
     bool profiling = false;
     std::vector<hipEvent_t> sweep_ev, refresh_ev;
     int sweep_used = 0, refresh_used = 0;
+
+    // edge-balanced unit sweep (Adam, d = 2, H > 1024)
+    bool unit_path = false;
+    int64_t units = 0;
+    DeviceArray<double> rec;
+    DeviceArray<uint16_t> fire;
+    DeviceArray<int32_t> unit_k, unit_frag, nz_rows, nz_off, row_split, split_off;
+    DeviceArray<float> grad, frag;
 
     std::vector<float> alpha, mu, c1, c2;
     float scale0 = 1.0f;
@@ -291,6 +353,9 @@ void umaphip_plan::run() {
     HIP_CHECK(hipMemcpyAsync(pos[0].ptr, head0.ptr, head0.bytes(), hipMemcpyDeviceToDevice, stream));
     HIP_CHECK(hipMemcpyAsync(npos.ptr, npos0.ptr, npos0.bytes(), hipMemcpyDeviceToDevice, stream));
     HIP_CHECK(hipMemcpyAsync(nneg.ptr, nneg0.ptr, nneg0.bytes(), hipMemcpyDeviceToDevice, stream));
+    if (unit_path)
+        launch_unit_pack(npos0.ptr, nneg0.ptr, periods.ptr, indices.ptr, E, range, rec.ptr,
+                         fire.ptr, stream);
     HIP_CHECK(hipMemsetAsync(updates.ptr, 0, updates.bytes(), stream));
     if (adam) {
         HIP_CHECK(hipMemsetAsync(m.ptr, 0, m.bytes(), stream));
@@ -403,7 +468,22 @@ void umaphip_plan::run() {
             A.shared = shared; A.negatives = p.negative_sample_rate != 0;
             A.exclude = p.exclude_graph_neighbors; A.fused = 1;
             mark(sweep_ev, sweep_used);
-            launch_edge_sweep(A, stream);
+            if (unit_path) {
+                UnitSweepArgs U{};
+                U.s = A;
+                U.rec = rec.ptr; U.fire = fire.ptr;
+                U.unit_k = unit_k.ptr; U.unit_frag = unit_frag.ptr;
+                U.nz_rows = nz_rows.count ? nz_rows.ptr : nullptr; U.nz_off = nz_off.ptr;
+                U.grad = grad.ptr; U.frag = frag.ptr;
+                U.units = units; U.n_edges = E;
+                U.rate = static_cast<double>(p.negative_sample_rate);
+                launch_unit_sweep(U, stream);
+                launch_unit_adam(h, pos[c].ptr, pos[1 - c].ptr, grad.ptr, row_split.ptr,
+                                 split_off.ptr, frag.ptr, m.ptr, v.ptr, alpha[t], c1[t], c2[t],
+                                 stream);
+            } else {
+                launch_edge_sweep(A, stream);
+            }
             mark(sweep_ev, sweep_used);
             c = 1 - c;
         } else {
@@ -484,6 +564,7 @@ void umaphip_plan::run() {
         }
     }
     HIP_CHECK(hipEventRecord(ev_stop, stream));
+    if (unit_path) launch_unit_unpack(rec.ptr, E, npos.ptr, nneg.ptr, stream);
     cur = c;
     refresh_count = refreshes;
     ran = true;
@@ -610,6 +691,37 @@ umaphip_status umaphip_plan_create(int64_t h, int64_t n, int64_t d, const float 
             P->host_orders[0].alloc(n);
             P->host_orders[1].alloc(n);
         }
+        // Edge-balanced unit sweep: Adam, d = 2, large layouts, epochs that fit
+        // the 16-bit firing shadow, and negative periods equal to the
+        // reference formula (period / rate) * 1.5 for every edge.
+        bool unit_ok = !P->momentum && d == 2 && !P->small && E > 0 && params->epochs < 65535;
+        if (unit_ok && params->negative_sample_rate > 0) {
+            const double rate = static_cast<double>(params->negative_sample_rate);
+            for (int64_t e = 0; e < E && unit_ok; ++e)
+                unit_ok = negative_periods[e] == (periods[e] / rate) * 1.5;
+        }
+        if (unit_ok) {
+            UnitTables T = build_unit_tables(offsets, h, E, unit_edges());
+            P->unit_path = true;
+            P->units = T.units;
+            P->rec.alloc(static_cast<size_t>(4 * E));
+            P->fire.alloc(static_cast<size_t>(E));
+            auto up = [](DeviceArray<int32_t> &dst, const std::vector<int32_t> &src) {
+                if (src.empty()) return;
+                dst.alloc(src.size());
+                HIP_CHECK(hipMemcpy(dst.ptr, src.data(), src.size() * sizeof(int32_t),
+                                    hipMemcpyHostToDevice));
+            };
+            up(P->unit_k, T.unit_k);
+            up(P->unit_frag, T.unit_frag);
+            up(P->nz_rows, T.nz_rows);
+            up(P->nz_off, T.nz_off);
+            up(P->row_split, T.row_split);
+            up(P->split_off, T.split_off);
+            P->grad.alloc(static_cast<size_t>(2 * h));
+            HIP_CHECK(hipMemset(P->grad.ptr, 0, P->grad.bytes()));
+            P->frag.alloc(static_cast<size_t>(2 * std::max<int64_t>(T.split_off.back(), 1)));
+        }
         if (P->momentum) {
             P->source_dev[0].alloc(h);
             P->source_dev[1].alloc(h);
@@ -697,6 +809,7 @@ umaphip_status umaphip_plan_download(umaphip_plan *plan, float *head, double *ne
             stats->sweep_launches = plan->sweep_used / 2;
             stats->refresh_ms = plan->ran ? plan->pairs_ms(plan->refresh_ev, plan->refresh_used) : 0.0;
             stats->refresh_launches = plan->refresh_used / 2;
+            stats->edge_balanced = plan->unit_path ? 1 : 0;
         }
         UMAPHIP_REQUIRE(status == ~0ull, UMAPHIP_NUMERICAL,
                         "non-finite %s projection at row %llu (refresh %llu)",

@@ -300,6 +300,9 @@ __device__ inline float2 ld2(const float *p, int64_t row) {
     return reinterpret_cast<const float2 *>(p)[row];
 }
 
+// Hardware reciprocal (1 ulp).
+__device__ inline float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
+
 // q^b by the hardware base-2 exp/log (relative error a few ulp).
 __device__ inline float fast_pow(float q, float b) {
     return __builtin_amdgcn_exp2f(b * __builtin_amdgcn_logf(q));
@@ -314,13 +317,13 @@ __device__ inline float tanh_pos(float x) {
     }
     if (x > 9.0f) return 1.0f;
     const float e = __builtin_amdgcn_exp2f(x * 2.8853900817779268f);
-    return __fdividef(e - 1.0f, e + 1.0f);
+    return (e - 1.0f) * rcpf(e + 1.0f);
 }
 
 // Attraction factor (coefficient of x_i - y): fm * -2ab q^(b-1) / (1 + a q^b).
 __device__ inline float attract_fast(float q, float a, float b, float fm) {
     const float pb = fast_pow(q, b);
-    return __fdividef(fm * (-2.0f * a * b) * __fdividef(pb, q), 1.0f + a * pb);
+    return fm * (-2.0f * a * b) * (pb * rcpf(q)) * rcpf(1.0f + a * pb);
 }
 
 // Repulsion factor: with c = 2*gamma*b*s / ((0.001 + q)(1 + a q^b)) and
@@ -328,11 +331,11 @@ __device__ inline float attract_fast(float q, float a, float b, float fm) {
 // min(gamma,16) * tanh(c*sqrt(q)/gamma) / sqrt(q).
 __device__ inline float repel_fast(float q, float a, float b, float gamma, float s) {
     if (!(q > 0.0f && gamma > 0.0f)) return 0.0f;
-    const float c = __fdividef(s * 2.0f * gamma * b, (0.001f + q) * (1.0f + a * fast_pow(q, b)));
+    const float c = s * 2.0f * gamma * b * rcpf((0.001f + q) * (1.0f + a * fast_pow(q, b)));
     if (!(c > 0.0f)) return 0.0f;
     const float rq = __builtin_amdgcn_rsqf(q);
     const float g16 = gamma < 16.0f ? gamma : 16.0f;
-    return g16 * tanh_pos(__fdividef(c * q * rq, gamma)) * rq;
+    return g16 * tanh_pos(c * q * rq * rcpf(gamma)) * rq;
 }
 
 // x mod m for x < 2^64, m < 2^32, through a double-precision quotient
@@ -361,7 +364,9 @@ __device__ inline int block_exclusive_scan(int v, int tid, int *s_wave, int &tot
 }
 
 __global__ __launch_bounds__(kThreads) void edge_sweep_kernel(SweepArgs A) {
-    __shared__ int32_t s_off[kTileRows + 1];
+    __shared__ int32_t s_off[kTileRows + 1];   // tile-local edge prefix over the tile's rows
+    __shared__ int32_t s_seg[kTileRows];       // CSR start of each tile row
+    __shared__ int32_t s_rowid[kTileRows];
     __shared__ float2 s_pos[kTileRows];
     __shared__ uint32_t s_first[kTileRows];
     __shared__ int8_t s_active[kTileRows];
@@ -383,18 +388,33 @@ __global__ __launch_bounds__(kThreads) void edge_sweep_kernel(SweepArgs A) {
     const bool retry = small && A.exclude;
     const double inv_mod = 1.0 / static_cast<double>(modulus);
     const uint64_t t1 = (static_cast<uint64_t>(A.epoch) + 1u) % modulus;
-    if (tid <= rows) s_off[tid] = A.offsets[row0 + tid];
+    int len = 0;
     if (tid < rows) {
-        s_pos[tid] = ld2(A.pos, row0 + tid);
+        const int64_t row = row0 + tid;
+        int64_t rk = 0;
+        s_pos[tid] = ld2(A.pos, row);
+        if (windowed) rk = static_cast<uint32_t>(A.ranks[row]);
         uint64_t first = 0;
         if (windowed) {
-            const uint64_t rk = static_cast<uint32_t>(A.ranks[row0 + tid]);
             const uint64_t half = A.range / 2;
-            if (rk > half) first = rk - half;
+            if (static_cast<uint64_t>(rk) > half) first = static_cast<uint64_t>(rk) - half;
             if (first > n64 - A.range) first = n64 - A.range;
         }
+        const int32_t seg = A.offsets[row];
+        len = A.offsets[row + 1] - seg;
+        s_seg[tid] = seg;
+        s_rowid[tid] = static_cast<int32_t>(row);
         s_first[tid] = static_cast<uint32_t>(first);
-        s_active[tid] = A.block_of_row ? (A.block_of_row[row0 + tid] == A.block_id) : 1;
+        s_active[tid] = A.block_of_row ? (A.block_of_row[row] == A.block_id) : 1;
+    }
+    if (tid < 64) {
+        int incl = len;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int up = __shfl_up(incl, off, 64);
+            if (tid >= off) incl += up;
+        }
+        s_off[tid + 1] = incl;
+        if (tid == 0) s_off[0] = 0;
     }
     const float scale = A.scale_ptr ? *A.scale_ptr : A.scale;
     __syncthreads();
@@ -408,16 +428,24 @@ __global__ __launch_bounds__(kThreads) void edge_sweep_kernel(SweepArgs A) {
     // Stream registers: the next chunk's clock/index/period words are
     // loaded while the current chunk's gathers are in flight.
     double pf_npos = 0.0, pf_per = 0.0, pf_nneg = 0.0, pf_nper = 1.0;
-    int32_t pf_idx = 0;
+    int32_t pf_idx = 0, pf_e = 0, pf_r = 0;
     auto prefetch = [&](int32_t chunk) {
-        const int32_t e = chunk + tid;
-        if (e < e_end) {
-            pf_npos = A.next_positive[e];
-            pf_idx = A.indices[e];
-            pf_per = A.periods[e];
+        const int32_t v = chunk + tid;
+        if (v < e_end) {
+            int lo = 0, hi = rows;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (s_off[mid] <= v) lo = mid;
+                else hi = mid;
+            }
+            pf_r = lo;
+            pf_e = s_seg[lo] + (v - s_off[lo]);
+            pf_npos = A.next_positive[pf_e];
+            pf_idx = A.indices[pf_e];
+            pf_per = A.periods[pf_e];
             if (A.negatives) {
-                pf_nneg = A.next_negative[e];
-                pf_nper = A.negative_periods[e];
+                pf_nneg = A.next_negative[pf_e];
+                pf_nper = A.negative_periods[pf_e];
             }
         }
     };
@@ -425,20 +453,16 @@ __global__ __launch_bounds__(kThreads) void edge_sweep_kernel(SweepArgs A) {
 
     for (int32_t base = e_begin; base < e_end; base += kThreads) {
         // ---- A: clocks and term counts
-        const int32_t e = base + tid;
+        const bool has_edge = base + tid < e_end;
+        const int32_t e = pf_e;
+        const int r_cur = pf_r;
         const double npos = pf_npos, per = pf_per, nn = pf_nneg, np = pf_nper;
         const int32_t jidx = pf_idx;
         if (base + kThreads < e_end) prefetch(base + kThreads);
         int terms = 0, count = 0, r = 0;
         float2 retry_sum = make_float2(0.0f, 0.0f);
-        if (e < e_end) {
-            int lo = 0, hi = rows;
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (s_off[mid] <= e) lo = mid;
-                else hi = mid;
-            }
-            r = lo;
+        if (has_edge) {
+            r = r_cur;
             if (s_active[r] && npos <= now) {
                 s_j[tid] = jidx;
                 A.next_positive[e] = npos + per;
@@ -452,12 +476,13 @@ __global__ __launch_bounds__(kThreads) void edge_sweep_kernel(SweepArgs A) {
                     // sequentially; the accepted forces form one term.
                     NegWalk w;
                     w.init(static_cast<uint64_t>(e), A.epoch, negs, true, true, n64, A.range, 0);
-                    const int64_t i = row0 + r;
+                    const int64_t i = s_rowid[r];
                     const float2 xi = s_pos[r];
+                    const int64_t seg = s_seg[r], seg_end = seg + (s_off[r + 1] - s_off[r]);
                     uint64_t rank;
                     while (w.next(rank)) {
                         const int64_t jj = static_cast<uint32_t>(A.order[rank]);
-                        if (excluded(A.indices, s_off[r], s_off[r + 1], i, jj, A.shared != 0)) continue;
+                        if (excluded(A.indices, seg, seg_end, i, jj, A.shared != 0)) continue;
                         w.accept();
                         const float2 y = A.table ? ld2(A.table, rank) : ld2(A.tgt, jj);
                         const float dx = xi.x - y.x, dy = xi.y - y.y;
@@ -523,8 +548,9 @@ __global__ __launch_bounds__(kThreads) void edge_sweep_kernel(SweepArgs A) {
                         const uint64_t rank = static_cast<uint64_t>(s_first[rr]) + sel;
                         if (A.exclude) {
                             const int64_t jj = static_cast<uint32_t>(A.order[rank]);
-                            keep[u] = !excluded(A.indices, s_off[rr], s_off[rr + 1], row0 + rr, jj,
-                                                A.shared != 0);
+                            const int64_t seg = s_seg[rr];
+                            keep[u] = !excluded(A.indices, seg, seg + (s_off[rr + 1] - s_off[rr]),
+                                                s_rowid[rr], jj, A.shared != 0);
                             y[u] = A.table ? ld2(A.table, rank) : ld2(A.tgt, jj);
                         } else {
                             y[u] = A.table ? ld2(A.table, rank)
@@ -552,7 +578,8 @@ __global__ __launch_bounds__(kThreads) void edge_sweep_kernel(SweepArgs A) {
             }
         } else if (terms > 0) {
             // fallback: the lane's whole edge as one term
-            const int64_t i = row0 + r;
+            const int64_t i = s_rowid[r];
+            const int64_t seg = s_seg[r], seg_end = seg + (s_off[r + 1] - s_off[r]);
             const float2 xi = s_pos[r];
             float2 g = make_float2(0.0f, 0.0f);
             const float2 y = ld2(A.tgt, static_cast<uint32_t>(s_j[tid]));
@@ -575,7 +602,7 @@ __global__ __launch_bounds__(kThreads) void edge_sweep_kernel(SweepArgs A) {
                     float2 yn;
                     if (A.exclude) {
                         const int64_t jj = static_cast<uint32_t>(A.order[rank]);
-                        if (excluded(A.indices, s_off[r], s_off[r + 1], i, jj, A.shared != 0)) continue;
+                        if (excluded(A.indices, seg, seg_end, i, jj, A.shared != 0)) continue;
                         yn = A.table ? ld2(A.table, rank) : ld2(A.tgt, jj);
                     } else {
                         yn = A.table ? ld2(A.table, rank)
@@ -608,7 +635,7 @@ __global__ __launch_bounds__(kThreads) void edge_sweep_kernel(SweepArgs A) {
     }
 
     if (tid >= rows || !s_active[tid]) return;
-    const int64_t i = row0 + tid;
+    const int64_t i = s_rowid[tid];
     if (A.fused) {
         const float2 x = s_pos[tid];
         float2 mm = ld2(A.m, i), vv = ld2(A.v, i);
@@ -634,6 +661,369 @@ __global__ __launch_bounds__(kThreads) void edge_sweep_kernel(SweepArgs A) {
             reinterpret_cast<float2 *>(A.updates)[i] = u;
         }
     }
+}
+
+// ------------------------------------------------- edge-balanced unit sweep
+// Adam layouts with H > 1024 and d = 2. One wave owns kUnitEdges consecutive
+// CSR edges (a unit), so a hub row with thousands of edges is spread over
+// many waves instead of serialising one tile, and every stream is read
+// contiguously. Per unit:
+//   1. the unit's row range and the firing shadow of its edges;
+//   2. the records of the firing edges and the unit rows' positions/windows;
+//   3. force terms laid out term-parallel in edge order, all gathers of a
+//      group of 64*kUnitGroups terms in flight together, and per-row sums
+//      by a segmented wave scan into LDS accumulators.
+// A row wholly inside the unit stores its gradient; a row crossing a unit
+// boundary stores one fragment per unit, which the Adam pass adds in unit
+// order. Deterministic; no atomics.
+//
+// Edge record (32 B): next_positive, next_negative, period, and a word with
+// the column index (low 32 bits) and e mod M (high 32 bits). The negative
+// period is recomputed as (period / rate) * 1.5, the reference's formula; a
+// plan takes this path only when every stored negative period equals it.
+// The firing shadow holds min(max(ceil(next_positive), 0), 65535): for an
+// integer epoch t, next_positive <= t exactly when the shadow is <= t.
+constexpr int kUnitEPL = 2;
+constexpr int kUnitEdges = 64 * kUnitEPL;
+constexpr int kUnitTermCap = kUnitEdges * 6;
+constexpr int kUnitGroups = 4;
+
+__device__ inline uint16_t fire_shadow(double next_positive) {
+    const double c = ceil(next_positive);
+    if (!(c < 65535.0)) return 65535;   // also NaN: never fires
+    return c > 0.0 ? static_cast<uint16_t>(c) : 0;
+}
+
+// Adds the segmented (by row) inclusive sums of one 64-term group into the
+// row accumulators; `row` is -1 for lanes without a term.
+__device__ inline void segmented_accumulate(float2 term, int row, int lane, float2 *s_acc) {
+    float sx = term.x, sy = term.y;
+    for (int o = 1; o < 64; o <<= 1) {
+        const float ux = __shfl_up(sx, o, 64), uy = __shfl_up(sy, o, 64);
+        const int urow = __shfl_up(row, o, 64);
+        if (lane >= o && urow == row) {
+            sx = sx + ux;
+            sy = sy + uy;
+        }
+    }
+    const int next = __shfl_down(row, 1, 64);
+    if (row >= 0 && (lane == 63 || next != row)) {
+        float2 a = s_acc[row];
+        a.x = a.x + sx;
+        a.y = a.y + sy;
+        s_acc[row] = a;
+    }
+}
+
+__global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
+    const SweepArgs &A = U.s;
+    constexpr int EPL = kUnitEPL, UE = kUnitEdges, TCAP = kUnitTermCap, G = kUnitGroups;
+    __shared__ int32_t s_off[UE + 1];   // unit-local first edge of each unit row
+    __shared__ int32_t s_seg[UE + 1];   // CSR start of each unit row (+ end of the last)
+    __shared__ int32_t s_rowid[UE];
+    __shared__ float2 s_pos[UE];
+    __shared__ uint32_t s_first[UE];
+    __shared__ float2 s_acc[UE];
+    __shared__ uint16_t s_owner[TCAP];
+    __shared__ int32_t s_base[UE + 1];
+    __shared__ int32_t s_j[UE];
+    __shared__ uint8_t s_row[UE];
+    __shared__ uint32_t s_em[UE];
+    __shared__ uint32_t s_sel0[UE];
+    static_assert(UE <= 256, "unit rows are indexed by uint8");
+
+    const int lane = threadIdx.x;
+    const int64_t u = blockIdx.x;
+    const int64_t e0 = u * UE;
+    const int ue = static_cast<int>(min<int64_t>(UE, U.n_edges - e0));
+    const uint32_t epoch = A.epoch;
+    // 1. unit rows and firing shadow
+    const int32_t k0 = U.unit_k[2 * u], k1 = U.unit_k[2 * u + 1];
+    uint32_t fire[EPL];
+#pragma unroll
+    for (int k = 0; k < EPL; ++k) {
+        const int v = k * 64 + lane;
+        fire[k] = v < ue ? U.fire[e0 + v] : 0xffffffffu;
+    }
+    // 2. firing records, then the row data
+    double r_npos[EPL], r_nn[EPL], r_per[EPL];
+    uint64_t r_w[EPL];
+#pragma unroll
+    for (int k = 0; k < EPL; ++k) {
+        r_npos[k] = 0.0; r_nn[k] = 0.0; r_per[k] = 0.0; r_w[k] = 0;
+        if (fire[k] <= epoch) {
+            const double4 rec = reinterpret_cast<const double4 *>(U.rec)[e0 + k * 64 + lane];
+            r_npos[k] = rec.x;
+            r_nn[k] = rec.y;
+            r_per[k] = rec.z;
+            r_w[k] = static_cast<uint64_t>(__double_as_longlong(rec.w));
+        }
+    }
+    const int nr = k1 - k0 + 1;
+    const uint64_t n64 = static_cast<uint64_t>(A.n);
+    const uint64_t modulus = A.range;
+    const bool windowed = modulus < n64;
+    for (int lr = lane; lr < nr; lr += 64) {
+        const int32_t k = k0 + lr;
+        const int32_t row = U.nz_rows ? U.nz_rows[k] : k;
+        const int32_t seg = U.nz_off[k];
+        s_seg[lr] = seg;
+        s_off[lr] = static_cast<int32_t>(max<int64_t>(seg, e0) - e0);
+        s_rowid[lr] = row;
+        s_pos[lr] = ld2(A.pos, row);
+        uint64_t first = 0;
+        if (windowed) {
+            const uint64_t rk = static_cast<uint32_t>(A.ranks[row]);
+            const uint64_t half = modulus / 2;
+            if (rk > half) first = rk - half;
+            if (first > n64 - modulus) first = n64 - modulus;
+        }
+        s_first[lr] = static_cast<uint32_t>(first);
+        s_acc[lr] = make_float2(0.0f, 0.0f);
+    }
+    if (lane == 0) {
+        s_off[nr] = ue;
+        s_seg[nr] = U.nz_off[k1 + 1];
+    }
+    const float scale = A.scale_ptr ? *A.scale_ptr : A.scale;
+    const double now = static_cast<double>(epoch);
+    __syncthreads();
+
+    // clocks (bit-exact f64, reference order of operations) and term counts
+    int running = 0;
+    int t_cnt[EPL], t_base[EPL];
+#pragma unroll
+    for (int k = 0; k < EPL; ++k) {
+        const int slot = k * 64 + lane;
+        int t = 0, r = 0;
+        if (slot < ue) {
+            int lo = 0, hi = nr;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (s_off[mid] <= slot) lo = mid;
+                else hi = mid;
+            }
+            r = lo;
+            if (fire[k] <= epoch) {
+                const int64_t e = e0 + slot;
+                const double per = r_per[k], nn = r_nn[k];
+                const double npos2 = r_npos[k] + per;
+                int32_t count = 0;
+                double nn2 = nn;
+                if (A.negatives) {
+                    const double np = (per / U.rate) * 1.5;
+                    count = static_cast<int32_t>((now - nn) / np);
+                    nn2 = nn + static_cast<double>(count) * np;
+                }
+                reinterpret_cast<double2 *>(U.rec)[2 * e] = make_double2(npos2, nn2);
+                U.fire[e] = fire_shadow(npos2);
+                const int negs = count > 0 ? count : 0;
+                t = 1 + negs;
+                const uint32_t em = static_cast<uint32_t>(r_w[k] >> 32);
+                s_j[slot] = static_cast<int32_t>(static_cast<uint32_t>(r_w[k]));
+                s_em[slot] = em;
+                s_sel0[slot] = negs ? static_cast<uint32_t>(mod_u64(
+                                          static_cast<uint64_t>(em) * U.t1, modulus, U.inv_mod))
+                                    : 0u;
+            }
+        }
+        s_row[slot] = static_cast<uint8_t>(r);
+        int incl = t;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int up = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += up;
+        }
+        t_cnt[k] = t;
+        t_base[k] = running + incl - t;
+        s_base[slot] = t_base[k];
+        running += __shfl(incl, 63, 64);
+    }
+    const int total = running;
+    if (lane == 0) s_base[UE] = total;
+    const bool term_parallel = total <= TCAP;
+    if (term_parallel) {
+#pragma unroll
+        for (int k = 0; k < EPL; ++k)
+            for (int q = 0; q < t_cnt[k]; ++q)
+                s_owner[t_base[k] + q] = static_cast<uint16_t>(k * 64 + lane);
+    }
+    __syncthreads();
+
+    if (term_parallel) {
+        // 3. all terms of a pass gathered together, then computed and summed
+        for (int p0 = 0; p0 < total; p0 += 64 * G) {
+            float2 y[G];
+            bool keep[G];
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                const int item = p0 + g * 64 + lane;
+                y[g] = make_float2(0.0f, 0.0f);
+                keep[g] = true;
+                if (item >= total) continue;
+                const int owner = s_owner[item];
+                const int kq = item - s_base[owner];
+                if (kq == 0) {
+                    y[g] = ld2(A.tgt, static_cast<uint32_t>(s_j[owner]));
+                    continue;
+                }
+                // e*(epoch+kq) mod M from e*(epoch+1) mod M by adding e mod M
+                uint64_t sel = static_cast<uint64_t>(s_sel0[owner]) +
+                               static_cast<uint64_t>(kq - 1) * s_em[owner];
+                if (kq <= 4) {
+                    while (sel >= modulus) sel -= modulus;
+                } else {
+                    sel = mod_u64(sel, modulus, U.inv_mod);
+                }
+                const int rr = s_row[owner];
+                const uint64_t rank = static_cast<uint64_t>(s_first[rr]) + sel;
+                if (A.exclude) {
+                    const int64_t jj = static_cast<uint32_t>(A.order[rank]);
+                    keep[g] = !excluded(A.indices, s_seg[rr], s_seg[rr + 1], s_rowid[rr], jj,
+                                        A.shared != 0);
+                    y[g] = A.table ? ld2(A.table, rank) : ld2(A.tgt, jj);
+                } else {
+                    y[g] = A.table ? ld2(A.table, rank)
+                                   : ld2(A.tgt, static_cast<uint32_t>(A.order[rank]));
+                }
+            }
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                if (p0 + g * 64 >= total) break;   // wave-uniform
+                const int item = p0 + g * 64 + lane;
+                float2 term = make_float2(0.0f, 0.0f);
+                int row = -1;
+                if (item < total) {
+                    const int owner = s_owner[item];
+                    const int kq = item - s_base[owner];
+                    row = s_row[owner];
+                    const float2 xi = s_pos[row];
+                    const float dx = xi.x - y[g].x, dy = xi.y - y[g].y;
+                    const float q = dx * dx + dy * dy;
+                    if (kq == 0) {
+                        if (q > 0.0f) {
+                            const float c = attract_fast(q, A.a, A.b, 1.0f);
+                            term = make_float2(c * dx, c * dy);
+                        }
+                    } else if (keep[g]) {
+                        const float c = repel_fast(q, A.a, A.b, A.gamma, scale);
+                        term = make_float2(c * dx, c * dy);
+                    }
+                }
+                segmented_accumulate(term, row, lane, s_acc);
+            }
+        }
+    } else {
+        // More terms than LDS holds: each lane sums its edges' terms in
+        // order, then the per-edge sums are reduced per row.
+#pragma unroll
+        for (int k = 0; k < EPL; ++k) {
+            const int slot = k * 64 + lane;
+            float2 g = make_float2(0.0f, 0.0f);
+            const int rr = slot < ue ? s_row[slot] : 0;
+            if (t_cnt[k] > 0) {
+                const float2 xi = s_pos[rr];
+                const float2 y = ld2(A.tgt, static_cast<uint32_t>(s_j[slot]));
+                const float dx = xi.x - y.x, dy = xi.y - y.y;
+                const float q = dx * dx + dy * dy;
+                if (q > 0.0f) {
+                    const float c = attract_fast(q, A.a, A.b, 1.0f);
+                    g = make_float2(c * dx, c * dy);
+                }
+                uint64_t sel = s_sel0[slot];
+                const uint64_t em = s_em[slot];
+                for (int p = 0; p + 1 < t_cnt[k]; ++p) {
+                    if (p > 0) {
+                        sel += em;
+                        if (sel >= modulus) sel -= modulus;
+                    }
+                    const uint64_t rank = static_cast<uint64_t>(s_first[rr]) + sel;
+                    float2 yn;
+                    if (A.exclude) {
+                        const int64_t jj = static_cast<uint32_t>(A.order[rank]);
+                        if (excluded(A.indices, s_seg[rr], s_seg[rr + 1], s_rowid[rr], jj,
+                                     A.shared != 0))
+                            continue;
+                        yn = A.table ? ld2(A.table, rank) : ld2(A.tgt, jj);
+                    } else {
+                        yn = A.table ? ld2(A.table, rank)
+                                     : ld2(A.tgt, static_cast<uint32_t>(A.order[rank]));
+                    }
+                    const float nx = xi.x - yn.x, ny = xi.y - yn.y;
+                    const float c = repel_fast(nx * nx + ny * ny, A.a, A.b, A.gamma, scale);
+                    g.x = g.x + c * nx;
+                    g.y = g.y + c * ny;
+                }
+            }
+            segmented_accumulate(g, slot < ue ? rr : -1, lane, s_acc);
+        }
+    }
+    __syncthreads();
+    for (int lr = lane; lr < nr; lr += 64) {
+        const float2 g = s_acc[lr];
+        int slot = -1;
+        if (lr == 0) slot = U.unit_frag[2 * u];
+        if (lr == nr - 1 && slot < 0) slot = U.unit_frag[2 * u + 1];
+        if (slot >= 0) reinterpret_cast<float2 *>(U.frag)[slot] = g;
+        else reinterpret_cast<float2 *>(U.grad)[s_rowid[lr]] = g;
+    }
+}
+
+__global__ void unit_pack_kernel(const double *npos, const double *nneg, const double *periods,
+                                 const int32_t *indices, int64_t E, uint64_t modulus,
+                                 double *rec, uint16_t *fire) {
+    const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    const uint64_t em = static_cast<uint64_t>(e) % modulus;
+    const uint64_t w = (em << 32) | static_cast<uint32_t>(indices[e]);
+    reinterpret_cast<double4 *>(rec)[e] =
+        make_double4(npos[e], nneg[e], periods[e], __longlong_as_double(static_cast<long long>(w)));
+    fire[e] = fire_shadow(npos[e]);
+}
+
+__global__ void unit_unpack_kernel(const double *rec, int64_t E, double *npos, double *nneg) {
+    const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    const double2 c = reinterpret_cast<const double2 *>(rec)[2 * e];
+    npos[e] = c.x;
+    nneg[e] = c.y;
+}
+
+// Lazy Adam over rows; rows split across units add their fragments in unit
+// order first.
+__global__ void unit_adam_kernel(int64_t h, const float *pos, float *pos_out, const float *grad,
+                                 const int32_t *row_split, const int32_t *split_off,
+                                 const float *frag, float *m, float *v, float alpha, float c1,
+                                 float c2) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= h) return;
+    const int32_t s = row_split[i];
+    float2 g;
+    if (s < 0) {
+        g = ld2(grad, i);
+    } else {
+        g = make_float2(0.0f, 0.0f);
+        for (int32_t q = split_off[s]; q < split_off[s + 1]; ++q) {
+            const float2 f = ld2(frag, q);
+            g.x = g.x + f.x;
+            g.y = g.y + f.y;
+        }
+    }
+    const float2 x = ld2(pos, i);
+    float2 mm = ld2(m, i), vv = ld2(v, i);
+    float2 out = x;
+    if (g.x != 0.0f) {
+        mm.x = 0.9f * mm.x + (1.0f - 0.9f) * g.x;
+        vv.x = 0.99f * vv.x + ((1.0f - 0.99f) * g.x) * g.x;
+        out.x = x.x + (alpha * (mm.x / c1)) / (sqrtf(vv.x / c2) + 0.0001f);
+    }
+    if (g.y != 0.0f) {
+        mm.y = 0.9f * mm.y + (1.0f - 0.9f) * g.y;
+        vv.y = 0.99f * vv.y + ((1.0f - 0.99f) * g.y) * g.y;
+        out.y = x.y + (alpha * (mm.y / c1)) / (sqrtf(vv.y / c2) + 0.0001f);
+    }
+    reinterpret_cast<float2 *>(m)[i] = mm;
+    reinterpret_cast<float2 *>(v)[i] = vv;
+    reinterpret_cast<float2 *>(pos_out)[i] = out;
 }
 
 // ------------------------------------------------------------ projection
@@ -943,6 +1333,42 @@ void launch_edge_sweep(const SweepArgs &a, hipStream_t s) {
     if (a.h == 0) return;
     hipLaunchKernelGGL(edge_sweep_kernel, dim3(grid_for(a.h, kTileRows)), dim3(kThreads), 0, s,
                        a);
+    HIP_CHECK(hipGetLastError());
+}
+
+int64_t unit_edges() { return kUnitEdges; }
+
+void launch_unit_sweep(UnitSweepArgs a, hipStream_t s) {
+    if (a.units == 0) return;
+    const uint64_t modulus = a.s.range;
+    a.t1 = (static_cast<uint64_t>(a.s.epoch) + 1u) % modulus;
+    a.inv_mod = 1.0 / static_cast<double>(modulus);
+    hipLaunchKernelGGL(unit_sweep_kernel, dim3(static_cast<unsigned>(a.units)), dim3(64), 0, s, a);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_unit_pack(const double *npos, const double *nneg, const double *periods,
+                      const int32_t *indices, int64_t E, uint64_t modulus, double *rec,
+                      uint16_t *fire, hipStream_t s) {
+    if (E == 0) return;
+    hipLaunchKernelGGL(unit_pack_kernel, dim3(grid_for(E)), dim3(kThreads), 0, s, npos, nneg,
+                       periods, indices, E, modulus, rec, fire);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_unit_unpack(const double *rec, int64_t E, double *npos, double *nneg, hipStream_t s) {
+    if (E == 0) return;
+    hipLaunchKernelGGL(unit_unpack_kernel, dim3(grid_for(E)), dim3(kThreads), 0, s, rec, E, npos,
+                       nneg);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_unit_adam(int64_t h, const float *pos, float *pos_out, const float *grad,
+                      const int32_t *row_split, const int32_t *split_off, const float *frag,
+                      float *m, float *v, float alpha, float c1, float c2, hipStream_t s) {
+    if (h == 0) return;
+    hipLaunchKernelGGL(unit_adam_kernel, dim3(grid_for(h)), dim3(kThreads), 0, s, h, pos, pos_out,
+                       grad, row_split, split_off, frag, m, v, alpha, c1, c2);
     HIP_CHECK(hipGetLastError());
 }
 

@@ -60,6 +60,33 @@ struct SweepArgs {
 };
 void launch_edge_sweep(const SweepArgs &a, hipStream_t s);
 
+// Edge-balanced unit sweep for Adam layouts with H > 1024 and d = 2: one
+// wave per unit_edges() consecutive CSR edges; writes per-row gradients
+// (grad) and, for rows crossing a unit boundary, per-unit fragments.
+struct UnitSweepArgs {
+    SweepArgs s;               // positions, negatives and parameters (pos_out, m, v unused)
+    double *rec;               // [E] records {next_positive, next_negative, period, j | e mod M << 32}
+    uint16_t *fire;            // [E] firing shadow
+    const int32_t *unit_k;     // [units][2] first/last nonzero-degree row index of each unit
+    const int32_t *unit_frag;  // [units][2] fragment slot of the unit's first/last row, or -1
+    const int32_t *nz_rows;    // nonzero-degree row ids (null when every row has an edge)
+    const int32_t *nz_off;     // CSR start of each nonzero-degree row, then E
+    float *grad, *frag;
+    int64_t units, n_edges;
+    double rate;               // negative_sample_rate
+    uint64_t t1;               // (epoch + 1) mod M (set by the launcher)
+    double inv_mod;            // 1 / M (set by the launcher)
+};
+int64_t unit_edges();
+void launch_unit_sweep(UnitSweepArgs a, hipStream_t s);
+void launch_unit_pack(const double *npos, const double *nneg, const double *periods,
+                      const int32_t *indices, int64_t E, uint64_t modulus, double *rec,
+                      uint16_t *fire, hipStream_t s);
+void launch_unit_unpack(const double *rec, int64_t E, double *npos, double *nneg, hipStream_t s);
+void launch_unit_adam(int64_t h, const float *pos, float *pos_out, const float *grad,
+                      const int32_t *row_split, const int32_t *split_off, const float *frag,
+                      float *m, float *v, float alpha, float c1, float c2, hipStream_t s);
+
 // Debug: per-edge negative candidates (same selection code).
 void launch_debug_negatives(const RowwiseArgs &a, int32_t cap, int32_t *count_out,
                             int32_t *cand_out, uint8_t *kept_out, int32_t *overflow,

@@ -163,6 +163,7 @@ typedef struct {
     int64_t sweep_launches;
     double refresh_ms;              /* sum over projection refreshes (profiling on) */
     int64_t refresh_launches;
+    int64_t edge_balanced;          /* 1 when epochs ran the edge-balanced unit sweep */
 } umaphip_layout_stats;
 
 /* Host arrays in, host arrays out (allocates and frees device memory).

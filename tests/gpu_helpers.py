@@ -20,6 +20,23 @@ def random_graph(n_rows, n_cols, k, seed, epochs, rate=5.0, symmetric=True):
     return O.compact_graph(n_rows, n_cols, rows, cols, w, epochs, rate)
 
 
+def hub_graph(n, k, seed, epochs, rate=5.0, hubs=(7, 1234), hub_degree=1500, isolated=(3, 4, 2999)):
+    """random_graph plus hub rows joined to `hub_degree` others (so a hub's
+    CSR row spans many edge units) and rows with no edge at all."""
+    rng = np.random.default_rng(seed)
+    rows = np.repeat(np.arange(n), k)
+    cols = (rows + rng.integers(1, n, rows.size)) % n
+    extra_r = np.concatenate([np.full(hub_degree, h) for h in hubs])
+    extra_c = np.concatenate([rng.choice(np.setdiff1d(np.arange(n), [h]), hub_degree, replace=False)
+                              for h in hubs])
+    rows, cols = np.concatenate([rows, extra_r]), np.concatenate([cols, extra_c])
+    keep = ~np.isin(rows, isolated) & ~np.isin(cols, isolated)
+    rows, cols = rows[keep], cols[keep]
+    w = rng.uniform(0.02, 1.0, rows.size).astype(np.float32)
+    rows, cols, w = np.concatenate([rows, cols]), np.concatenate([cols, rows]), np.concatenate([w, w])
+    return O.compact_graph(n, n, rows, cols, w, epochs, rate)
+
+
 def advance_graph(g, epochs_done):
     """Clocks as they stand after `epochs_done` epochs (data independent);
     vectorised restatement of advance_clock (no fused multiply-adds)."""

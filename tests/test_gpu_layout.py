@@ -11,7 +11,8 @@ torch = pytest.importorskip("torch")
 
 from oracle import oracle as O  # noqa: E402
 from tests import fixture_cases as F  # noqa: E402
-from tests.gpu_helpers import advance_graph, corrections, random_graph, row_relative_error  # noqa: E402
+from tests.gpu_helpers import (advance_graph, corrections, hub_graph, random_graph,  # noqa: E402
+                               row_relative_error)
 
 pytestmark = pytest.mark.gpu
 
@@ -284,7 +285,15 @@ DRIVER = [
     ("transform_large_window", 1500, 4000, 2, False, "adam", 1000, True, 12),
     ("fit_adam_d3_windows_adapt", 2200, 2200, 3, True, "adam", 400, False, 22),
     ("fit_small_excl", 900, 900, 2, True, "adam", 200_000, True, 12),
+    # edge-balanced unit sweep: hub rows over many units, rows without edges,
+    # and a negative rate whose terms overflow a unit's term capacity
+    ("fit_adam_global_hubs", 3000, 3000, 2, True, "adam", 200_000, False, 16),
+    ("fit_adam_windows_hubs_excl", 3000, 3000, 2, True, "adam", 700, True, 16),
+    ("fit_adam_global_rate24_hubs", 3000, 3000, 2, True, "adam", 200_000, True, 10),
 ]
+# cases that must run the edge-balanced unit sweep
+UNIT_PATH = {"fit_adam_global_excl", "fit_adam_windows_adapt", "transform_large_window",
+             "fit_adam_global_hubs", "fit_adam_windows_hubs_excl", "fit_adam_global_rate24_hubs"}
 
 
 @pytest.mark.parametrize("case", DRIVER, ids=[c[0] for c in DRIVER])
@@ -304,22 +313,27 @@ def test_driver_matches_oracle_loop(case):
     name, h, n, d, shared, opt, Q, exclude, epochs = case
     projection = Q < n
     rng = np.random.default_rng(abs(hash(name)) % 2**32)
-    g = random_graph(h, n, 6, 11, epochs, symmetric=shared)
+    rate = 24 if "rate24" in name else 5
+    if "hubs" in name:
+        g = hub_graph(h, 6, 11, epochs, rate=float(rate))
+    else:
+        g = random_graph(h, n, 6, 11, epochs, symmetric=shared)
     head = rng.uniform(-10, 10, (h, d)).astype(np.float32)
     tail = None if shared else rng.uniform(-10, 10, (n, d)).astype(np.float32)
     print(f"[gpu] driver case={name}", flush=True)
 
     def both(total):
         params = layout.LayoutParams(epochs=total, a=A_CURVE, b=B_CURVE, optimizer=opt,
-                                     negative_selection_range=Q,
+                                     negative_selection_range=Q, negative_sample_rate=rate,
                                      exclude_graph_neighbors=exclude, seed=5)
         gr = layout.optimize_compact(head, tail, g, params, shared=shared)
         orc = O.optimize_compact(head, tail, g, shared=shared, threads=8, epochs=total,
-                                 a=A_CURVE, b=B_CURVE, optimizer=opt,
+                                 a=A_CURVE, b=B_CURVE, optimizer=opt, negative_sample_rate=rate,
                                  negative_selection_range=Q, exclude=exclude, seed=5)
         return gr, orc
 
     (got, gp, gn, stats), (exp, ep, en, est) = both(epochs)
+    assert stats["edge_balanced"] == (name in UNIT_PATH)
     assert (gp == ep).all() and (gn == en).all()
     assert stats["adaptations"] == est["adaptations"]
     assert stats["refreshes"] == est["refreshes"]
