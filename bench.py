@@ -9,12 +9,14 @@ come from per-epoch random-projection windows) and scale adaptation.
 
 A step is one complete 200-epoch layout from the same resident initial
 state (graph, clocks, init and directions already in HBM). value =
-E * 200 * steps / seconds. The roofline object prices the dominant kernel
-(the fused edge sweep + Adam step) with the algorithmic stream bytes of
-each epoch, divided by that kernel's average launch duration measured
-with device events on the plan's stream. The CPU baseline times the oracle
-(C restatement, OpenMP over rows) on the same graph and init for the first
-epochs of the same layout.
+E * 200 * steps / seconds. The roofline object prices the dominant kernels
+(the edge-balanced unit sweep + its Adam step) with the algorithmic stream
+bytes of each epoch, divided by their average duration per epoch measured
+with device events on the plan's stream; `traffic` is the measured HBM
+bytes per epoch of the same kernels from the committed rocprofv3 counter
+summary. The CPU baseline times the oracle (C restatement, OpenMP over
+rows) on the same graph and init for the first epochs of the same layout
+and extrapolates by force terms per epoch.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--rows N]
 """
@@ -58,7 +60,8 @@ def make_data(rows, dims, clusters, seed, device):
 def stream_bytes(graph, epochs, h, d, windowed):
     """Sum over epochs of the algorithmic stream bytes B_stream(t):
     8E + F_t*44 + S_t*4 + H*(4 + 4*windowed) + 4H + H*d*4*6, with F_t (firing
-    edges) and S_t (negatives drawn) replayed exactly from the clocks."""
+    edges) and S_t (negatives drawn) replayed exactly from the clocks.
+    Returns (total bytes, sum F_t, sum S_t, per-epoch F_t + S_t)."""
     import torch
 
     dev = torch.device("cuda:0")
@@ -69,6 +72,7 @@ def stream_bytes(graph, epochs, h, d, windowed):
     E = npos.numel()
     total = 0
     F = S = 0
+    terms = []
     for t in range(epochs):
         fire = npos <= float(t)
         k = torch.trunc((float(t) - nneg) / nper)
@@ -79,8 +83,9 @@ def stream_bytes(graph, epochs, h, d, windowed):
         nneg = torch.where(fire, nneg + cnt * nper, nneg)
         F += f_t
         S += s_t
+        terms.append(f_t + s_t)
         total += 8 * E + f_t * 44 + s_t * 4 + h * (4 + 4 * int(windowed)) + 4 * h + h * d * 24
-    return total, F, S
+    return total, F, S, terms
 
 
 def build_workload(rows, dims, clusters, seed, k, epochs):
@@ -121,7 +126,21 @@ def build_workload(rows, dims, clusters, seed, k, epochs):
     return x, compact, init, a, b, knn_s, graph_s
 
 
-def cpu_baseline(compact, init, a, b, epochs, sample_epochs):
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(compact, init, a, b, epochs, sample_epochs, terms):
+    """Times the oracle (C restatement, OpenMP over rows) on epochs
+    0..sample_epochs-1 of the same layout and extrapolates to the whole
+    layout by the exact per-epoch force-term counts (its cost scales with
+    the firing edges and negatives drawn, which are data independent)."""
     from oracle import oracle as O
 
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -130,11 +149,41 @@ def cpu_baseline(compact, init, a, b, epochs, sample_epochs):
                        seed=42, threads=threads, stop_epoch=sample_epochs, collect_stats=False)
     secs = time.time() - t0
     E = len(compact["indices"])
-    return {"value": E * sample_epochs / secs, "unit": "edge-epochs/s", "cores": threads,
-            "kind": "port",
-            "sample": f"oracle C restatement (OpenMP, {threads} threads) running epochs "
-                      f"0..{sample_epochs - 1} of the same 200-epoch layout on the same graph "
-                      f"and init ({secs:.1f}s)"}
+    sample_terms = sum(terms[:sample_epochs]) / sample_epochs
+    mean_terms = sum(terms) / len(terms)
+    ratio = mean_terms / sample_terms
+    raw = E * sample_epochs / secs
+    return {"value": raw / ratio, "unit": "edge-epochs/s", "cores": threads, "kind": "port",
+            "sample": f"oracle C restatement (OpenMP, {threads} threads, {cpu_model()}, "
+                      f"nproc {os.cpu_count()}) timed on epochs 0..{sample_epochs - 1} of the same "
+                      f"200-epoch layout, graph and init ({secs:.1f}s, raw {raw:.3e} edge-epochs/s); "
+                      f"extrapolated to the layout average by force terms per epoch "
+                      f"({mean_terms:.4g} average vs {sample_terms:.4g} in the sample)"}
+
+
+def copy_calibration(gib):
+    """A plain device copy of a known size, for scaling the FETCH_SIZE and
+    WRITE_SIZE counters (run under rocprofv3 --pmc)."""
+    import torch
+
+    n = int(gib * (1 << 30)) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n * 4}), flush=True)
+
+
+def measured_traffic():
+    """HBM bytes per epoch of the edge pass + Adam step from the committed
+    rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE per dispatch,
+    scaled by the copy calibration), or None."""
+    path = ROOT / "profiles" / "unit_sweep_1m_traffic.json"
+    if not path.exists():
+        return None
+    t = json.loads(path.read_text())
+    return int(t["bytes_per_epoch"])
 
 
 def main():
@@ -145,10 +194,15 @@ def main():
     ap.add_argument("--rows", type=int, default=1_000_000)
     ap.add_argument("--dims", type=int, default=784)
     ap.add_argument("--epochs", type=int, default=200)
-    ap.add_argument("--cpu-epochs", type=int, default=10)
+    ap.add_argument("--cpu-epochs", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quality", action="store_true", help="report trustworthiness@15 (10k sample)")
+    ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
+                    help="only run a plain device copy of GIB GiB (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
@@ -184,11 +238,12 @@ def main():
     _, pstats = plan.download()
     plan.set_profiling(False)
     sweep_avg_ms = pstats["sweep_ms"] / max(pstats["sweep_launches"], 1)
-    total_bytes, F, S = stream_bytes(compact, args.epochs, H, 2, windowed=True)
+    total_bytes, F, S, terms = stream_bytes(compact, args.epochs, H, 2, windowed=True)
     bytes_per_launch = total_bytes / args.epochs
     achieved = bytes_per_launch / (sweep_avg_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                "traffic": measured_traffic() if args.rows == 1_000_000 else None,
                 "kernel": "unit_sweep_kernel + unit_adam_kernel (edge pass + lazy Adam)",
                 "bytes_per_launch": int(bytes_per_launch), "avg_launch_ms": round(sweep_avg_ms, 4),
                 "firing_edges_per_epoch": F / args.epochs, "negatives_per_epoch": S / args.epochs,
@@ -215,7 +270,7 @@ def main():
         out["config"]["knn_preservation@15"] = round(pres, 4)
     if not args.no_cpu:
         log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(compact, init, a, b, args.epochs, args.cpu_epochs)
+        out["cpu_baseline"] = cpu_baseline(compact, init, a, b, args.epochs, args.cpu_epochs, terms)
     print(json.dumps(out), flush=True)
 
 
