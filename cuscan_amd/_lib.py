@@ -41,6 +41,19 @@ class EpochArgs(C.Structure):
     ]
 
 
+class ClassicArgs(C.Structure):
+    _fields_ = [
+        ("h", C.c_int64), ("n", C.c_int64), ("d", C.c_int64), ("n_edges", C.c_int64),
+        ("head", C.c_void_p), ("tail", C.c_void_p), ("tails", C.c_void_p), ("weights", C.c_void_p),
+        ("epochs_per_sample", C.c_void_p), ("epoch_of_next_sample", C.c_void_p),
+        ("epochs_per_negative_sample", C.c_void_p), ("epoch_of_next_negative_sample", C.c_void_p),
+        ("rng_state", C.c_void_p), ("head_offsets", C.c_void_p), ("head_edges", C.c_void_p),
+        ("mask", C.c_void_p), ("epoch", C.c_uint32),
+        ("a", C.c_float), ("b", C.c_float), ("gamma", C.c_float), ("alpha", C.c_float),
+        ("negative_sample_rate", C.c_float), ("move_other", C.c_int32), ("shared", C.c_int32),
+    ]
+
+
 class LayoutParams(C.Structure):
     _fields_ = [
         ("optimizer", C.c_int32), ("epochs", C.c_int32),
@@ -103,6 +116,10 @@ SIGNATURES = {
                                     C.POINTER(C.c_float)]),
     "umaphip_random_init": (C.c_int, [_i64, _i64, C.c_uint64, C.c_float, C.c_float, _i32, _vp]),
     "umaphip_transform_init": (C.c_int, [_vp, _vp, _i64, _i64, _vp, _i64, _i64, _vp]),
+    "umaphip_classic_epoch": (C.c_int, [_vp, _vp]),
+    "umaphip_classic_optimize": (C.c_int, [_i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _i64, _i32,
+                                           _i32, C.c_float, C.c_float, C.c_float, C.c_float, _i32,
+                                           C.c_uint64, _i32, _vp, _vp, _vp]),
     "umaphip_orders_at_epoch": (C.c_int, [_i64, _i64, C.POINTER(LayoutParams), _i32, _i32, _vp,
                                           _vp]),
 }

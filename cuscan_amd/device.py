@@ -133,3 +133,40 @@ def sample_resultants(embedding, order, offsets, indices, sample_ranks, local_of
         int(raw_range), float(a), float(b), float(gamma), int(bool(exclude)), _p(out),
         _stream()))
     return out
+
+
+def classic_epoch(*, head, tail, tails, weights, epochs_per_sample, epoch_of_next_sample,
+                  epochs_per_negative_sample, epoch_of_next_negative_sample, rng_state,
+                  head_offsets, head_edges, mask, epoch, a, b, gamma, alpha,
+                  negative_sample_rate, move_other, shared):
+    """StandardUmapOps::optimize_epoch (crates/umap/src/backend.rs:234-262) on
+    device tensors; head, clocks and rng_state are updated in place."""
+    _need(head, torch.float32, "head")
+    for name, t in (("tails", tails), ("head_offsets", head_offsets),
+                    ("head_edges", head_edges), ("mask", mask)):
+        _need(t, torch.int32, name)
+    for name, t in (("epochs_per_sample", epochs_per_sample),
+                    ("epoch_of_next_sample", epoch_of_next_sample),
+                    ("epochs_per_negative_sample", epochs_per_negative_sample),
+                    ("epoch_of_next_negative_sample", epoch_of_next_negative_sample)):
+        _need(t, torch.float64, name)
+    _need(weights, torch.float32, "weights")
+    _need(rng_state, torch.int64, "rng_state")
+    A = _lib.ClassicArgs()
+    A.h, A.d = head.shape
+    A.n = A.h if shared else tail.shape[0]
+    A.n_edges = tails.numel()
+    A.head = _p(head)
+    A.tail = None if shared else _p(tail)
+    A.tails, A.weights = _p(tails), _p(weights)
+    A.epochs_per_sample = _p(epochs_per_sample)
+    A.epoch_of_next_sample = _p(epoch_of_next_sample)
+    A.epochs_per_negative_sample = _p(epochs_per_negative_sample)
+    A.epoch_of_next_negative_sample = _p(epoch_of_next_negative_sample)
+    A.rng_state = _p(rng_state)
+    A.head_offsets, A.head_edges, A.mask = _p(head_offsets), _p(head_edges), _p(mask)
+    A.epoch = int(epoch)
+    A.a, A.b, A.gamma, A.alpha = float(a), float(b), float(gamma), float(alpha)
+    A.negative_sample_rate = float(negative_sample_rate)
+    A.move_other, A.shared = int(bool(move_other)), int(bool(shared))
+    _lib.check(_lib.lib().umaphip_classic_epoch(C.byref(A), _stream()))

@@ -7,8 +7,8 @@ device (BF16 MFMA prefilter + FP32 rerank), the fuzzy graph and compaction
 on host threads, and the modern layout (Adam or momentum) on the device.
 
 Scope of this build: ``init="random"``; ``optimizer`` in {"adam",
-"momentum"} (``"compatibility"`` is accepted and runs the classic SGD epoch
-on the device); neighbour search is always exact, which equals the
+"momentum", "compatibility"} (the last runs the classic SGD epoch on the
+device); neighbour search is always exact, which equals the
 reference with ``neighbor_search="exact"`` (its "approximate" NN-descent is
 not part of this path); model serialisation is not provided.
 """
@@ -234,11 +234,12 @@ class UMAP:
         if self.optimizer == "compatibility":
             from .classic import optimize_classic
 
-            emb = optimize_classic(init, None, g["heads"], g["tails"], g["weights"], n_rows=n,
-                                   n_cols=n, epochs=n_epochs, prune_divisor=divisor, a=a, b=b,
-                                   alpha=self.learning_rate, gamma=self.repulsion_strength,
-                                   negative_sample_rate=self.negative_sample_rate, seed=seed,
-                                   fit=True)
+            emb = optimize_classic(init, None, g["heads"], g["tails"], g["weights"],
+                                   n_epochs=n_epochs, default_epochs=default_epochs, a=a, b=b,
+                                   learning_rate=self.learning_rate,
+                                   gamma=self.repulsion_strength,
+                                   negative_sample_rate=int(self.negative_sample_rate), seed=seed,
+                                   shared=True)
         else:
             compact = G.compact_graph(n, n, g["heads"], g["tails"], g["weights"],
                                       epochs=n_epochs, rate=self.negative_sample_rate,
@@ -303,12 +304,14 @@ class UMAP:
         if self.optimizer == "compatibility":
             from .classic import optimize_classic
 
+            # transform schedule: prepare_schedule(.., n_epochs, n_epochs, ..)
+            # (estimator.rs:2158-2166), learning rate / 4, transform_seed
             return optimize_classic(init, self.embedding_, g["heads"], g["tails"], g["weights"],
-                                    n_rows=q, n_cols=n, epochs=epochs, prune_divisor=float(epochs),
-                                    a=self._a, b=self._b, alpha=self.learning_rate / 4.0,
+                                    n_epochs=epochs, default_epochs=epochs, a=self._a, b=self._b,
+                                    learning_rate=self.learning_rate / 4.0,
                                     gamma=self.repulsion_strength,
-                                    negative_sample_rate=self.negative_sample_rate, seed=tseed,
-                                    fit=False)
+                                    negative_sample_rate=int(self.negative_sample_rate),
+                                    seed=tseed, shared=False)
         compact = G.compact_graph(q, n, g["heads"], g["tails"], g["weights"], epochs=epochs,
                                   rate=self.negative_sample_rate, prune_divisor=float(epochs),
                                   transform=True)

@@ -271,6 +271,57 @@ umaphip_status umaphip_transform_init(const int32_t *indices, const float *direc
                                       int64_t k, const float *embedding, int64_t n, int64_t d,
                                       float *out);
 
+/* ------------------------------------------------------------------------
+ * Compatibility (classic) SGD layout, optimizer="compatibility".
+ * ------------------------------------------------------------------------ */
+/* One epoch. Replaces StandardUmapOps::optimize_epoch
+ * (crates/umap/src/backend.rs:234-262; CubeCL body
+ * kernels/standard_umap/layout.rs:795-1060). Device pointers, contiguous
+ * [rows, d] coordinates. Clocks and rng_state [h,3] are advanced in place;
+ * head is updated in place with atomic adds (fit also moves attraction
+ * tails, which are the same buffer). head_offsets/head_edges list each head
+ * row's positive-weight edge ids in ascending order. mask is an
+ * n_edges-entry i32 workspace. move_other must equal shared, as in the
+ * reference. The coordinates are not unique (concurrent updates); the clocks
+ * and the generator states are. */
+typedef struct {
+    int64_t h, n, d, n_edges;
+    float *head;                       /* [h,d] in/out */
+    const float *tail;                 /* [n,d] fixed tail (transform); ignored when shared */
+    const int32_t *tails;              /* [n_edges] */
+    const float *weights;              /* [n_edges] (zero: never fires) */
+    const double *epochs_per_sample;
+    double *epoch_of_next_sample;
+    const double *epochs_per_negative_sample;
+    double *epoch_of_next_negative_sample;
+    int64_t *rng_state;                /* [h,3] */
+    const int32_t *head_offsets;       /* h + 1 */
+    const int32_t *head_edges;
+    int32_t *mask;                     /* workspace [n_edges] */
+    uint32_t epoch;
+    float a, b, gamma, alpha, negative_sample_rate;
+    int32_t move_other, shared;
+} umaphip_classic_args;
+umaphip_status umaphip_classic_epoch(const umaphip_classic_args *args, void *stream);
+
+/* Whole compatibility layout from host arrays (allocates device memory).
+ * Replaces estimator::optimize (crates/umap/src/standard_umap/estimator.rs:
+ * 2868-2982) with its schedule (prepare_schedule :2753-2798: prune below
+ * max / (n_epochs > 10 ? n_epochs : default_epochs), f64 intervals
+ * max / w, negative intervals / max(rate, 1)), head CSR (:3080-3131),
+ * generator seeding from StdRng(seed) and each row's first coordinate
+ * (:2800-2829) and the linear learning-rate decay. heads/tails/weights are
+ * the fuzzy graph's COO. The optional outputs receive the final clocks (E)
+ * and generator states (3h). */
+umaphip_status umaphip_classic_optimize(int64_t h, int64_t n, int64_t d, float *head,
+                                        const float *tail, const int32_t *heads,
+                                        const int32_t *tails, const float *weights,
+                                        int64_t n_edges, int32_t n_epochs, int32_t default_epochs,
+                                        float a, float b, float learning_rate, float gamma,
+                                        int32_t negative_sample_rate, uint64_t seed,
+                                        int32_t shared, double *next_positive_out,
+                                        double *next_negative_out, int64_t *rng_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -170,6 +170,17 @@ def lib():
                                       C.c_int, f32p]
         L.orc_transform_init.restype = C.c_int
         L.orc_transform_init.argtypes = [i32p, f32p, C.c_int64, C.c_int64, f32p, C.c_int64, f32p]
+        L.orc_taus_next.restype = C.c_int64
+        L.orc_taus_next.argtypes = [i64p]
+        L.orc_classic_epoch.argtypes = [C.c_int64, C.c_int64, C.c_int64, f32p, v, i32p, f32p,
+                                        C.c_int64, f64p, f64p, f64p, f64p, i64p, i32p, i32p,
+                                        C.c_uint32, C.c_float, C.c_float, C.c_float, C.c_float,
+                                        C.c_float, C.c_int]
+        L.orc_classic_optimize.restype = C.c_int
+        L.orc_classic_optimize.argtypes = [C.c_int64, C.c_int64, C.c_int64, f32p, v, i32p, i32p,
+                                           f32p, C.c_int64, C.c_int32, C.c_int32, C.c_float,
+                                           C.c_float, C.c_float, C.c_float, C.c_int32, C.c_uint64,
+                                           C.c_int, f64p, f64p, i64p]
         _lib = L
     return _lib
 
@@ -509,6 +520,43 @@ def transform_init(indices, directed, embedding):
     return out
 
 
+def taus_next(state):
+    """Advance a 3-word Tausworthe state in place; returns the signed draw."""
+    assert state.dtype == np.int64 and state.flags.c_contiguous and state.size == 3
+    return int(lib().orc_taus_next(state))
+
+
+def classic_epoch(head, tail, tails, weights, per, npos, nper, nneg, rng, head_offsets,
+                  head_edges, *, epoch, a, b, gamma, alpha, rate, shared):
+    """One serial compatibility epoch in place on the given arrays."""
+    h, d = head.shape
+    n = h if shared else tail.shape[0]
+    tail_arr = None if shared else np.ascontiguousarray(tail, np.float32)
+    lib().orc_classic_epoch(h, n, d, head, None if tail_arr is None else tail_arr.ctypes.data,
+                            tails, weights, len(tails), per, npos, nper, nneg, rng, head_offsets,
+                            head_edges, epoch, a, b, gamma, alpha, rate, int(shared))
+
+
+def classic_optimize(head, tail, heads, tails, weights, *, n_epochs, default_epochs, a, b,
+                     learning_rate=1.0, gamma=1.0, rate=5, seed=42, shared=True):
+    """Whole serial compatibility layout. Returns (head, npos, nneg, rng)."""
+    head = np.array(head, np.float32, order="C", copy=True)
+    h, d = head.shape
+    tail_arr = None if shared else np.ascontiguousarray(tail, np.float32)
+    E = len(heads)
+    npos = np.zeros(max(E, 1), np.float64)
+    nneg = np.zeros(max(E, 1), np.float64)
+    rng = np.zeros(3 * h, np.int64)
+    lib().orc_classic_optimize(h, h if shared else tail_arr.shape[0], d, head,
+                               None if tail_arr is None else tail_arr.ctypes.data,
+                               np.ascontiguousarray(heads, np.int32),
+                               np.ascontiguousarray(tails, np.int32),
+                               np.ascontiguousarray(weights, np.float32), E, n_epochs,
+                               default_epochs, a, b, learning_rate, gamma, rate, seed,
+                               int(shared), npos, nneg, rng)
+    return head, npos[:E], nneg[:E], rng.reshape(h, 3)
+
+
 def fit(x, *, n_neighbors=15, n_components=2, min_dist=0.1, spread=1.0, n_epochs=None,
         learning_rate=1.0, negative_sample_rate=5, repulsion=1.0, optimizer="adam",
         negative_selection_range=200_000, exclude=False, seed=42, threads=None):
@@ -529,9 +577,15 @@ def fit(x, *, n_neighbors=15, n_components=2, min_dist=0.1, spread=1.0, n_epochs
     divisor = epochs if epochs > 10 else default_epochs
     thr = np.float32(w.max()) / np.float32(divisor)
     w = np.where(w < thr, np.float32(0), w)
-    graph = compact_graph(n, n, heads, tails, w, epochs, float(negative_sample_rate))
     a, b = fit_curve(min_dist, spread)
     init = random_init(n, n_components, seed, repulsion)
+    if optimizer == "compatibility":
+        emb, _, _, _ = classic_optimize(init, None, heads, tails, g["weights"][:u], n_epochs=epochs,
+                                        default_epochs=default_epochs, a=a, b=b,
+                                        learning_rate=learning_rate, gamma=repulsion,
+                                        rate=negative_sample_rate, seed=seed, shared=True)
+        return emb
+    graph = compact_graph(n, n, heads, tails, w, epochs, float(negative_sample_rate))
     emb, _, _, _ = optimize_compact(init, None, graph, epochs=epochs, a=a, b=b,
                                     repulsion=repulsion, initial_alpha=learning_rate,
                                     optimizer=optimizer, negative_sample_rate=negative_sample_rate,

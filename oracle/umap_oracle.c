@@ -1617,3 +1617,167 @@ int orc_transform_init(const int32_t *indices, const float *directed, int64_t q,
     }
     return 0;
 }
+
+
+/* ------------------------------------------------------------------------
+ * Compatibility (classic) SGD.
+ * Follows crates/umap/src/kernels/standard_umap/layout.rs:68-336 (clock
+ * pass :68-100, vertex pass :230-336) and the driver
+ * crates/umap/src/standard_umap/estimator.rs:2753-2982.
+ * ------------------------------------------------------------------------ */
+
+/* The three Tausworthe components of layout.rs:291-303 on signed 64-bit
+ * state: ((s & keep) << l) & 0xffffffff, xor ((((s << m) & 0xffffffff) ^ s)
+ * >> r) with an arithmetic right shift. */
+static int64_t orc_taus_component(int64_t s, uint64_t keep, int l, int m, int r) {
+    uint64_t u = (uint64_t)s;
+    uint64_t left = ((u & keep) << l) & 0xffffffffull;
+    uint64_t mixed = ((u << m) & 0xffffffffull) ^ u;
+    int64_t right = (int64_t)mixed;
+    /* arithmetic shift written without relying on implementation-defined >> */
+    right = right < 0 ? ~(~right >> r) : right >> r;
+    return (int64_t)left ^ right;
+}
+
+int64_t orc_taus_next(int64_t st[3]) {
+    st[0] = orc_taus_component(st[0], 4294967294ull, 12, 13, 19);
+    st[1] = orc_taus_component(st[1], 4294967288ull, 4, 2, 25);
+    st[2] = orc_taus_component(st[2], 4294967280ull, 17, 3, 11);
+    return (int64_t)(int32_t)(uint32_t)(uint64_t)(st[0] ^ st[1] ^ st[2]);
+}
+
+static float orc_classic_dist(const float *head, const float *other, int64_t d) {
+    float q = 0.0f;
+    for (int64_t c = 0; c < d; ++c) {
+        float diff = head[c] - other[c];
+        q = q + diff * diff;
+    }
+    return q;
+}
+
+static void orc_classic_move(float *head, float *other, int64_t d, float coef, float alpha,
+                             int move_other) {
+    for (int64_t c = 0; c < d; ++c) {
+        float g = coef * (head[c] - other[c]);
+        if (g < -4.0f) g = -4.0f;
+        if (g > 4.0f) g = 4.0f;
+        float upd = g * alpha;
+        head[c] = head[c] + upd;
+        if (move_other) other[c] = other[c] - upd;
+    }
+}
+
+void orc_classic_epoch(int64_t h, int64_t n, int64_t d, float *head, const float *tail,
+                       const int32_t *tails, const float *weights, int64_t E,
+                       const double *per, double *npos, const double *nper, double *nneg,
+                       int64_t *rng, const int32_t *head_offsets, const int32_t *head_edges,
+                       uint32_t epoch, float a, float b, float gamma, float alpha, float rate,
+                       int shared) {
+    int32_t *mask = (int32_t *)malloc(sizeof(int32_t) * (size_t)(E > 0 ? E : 1));
+    double now = (double)epoch;
+    for (int64_t e = 0; e < E; ++e) {
+        int32_t fired = -1;
+        if (weights[e] > 0.0f && npos[e] <= now) {
+            fired = 0;
+            npos[e] = npos[e] + per[e];
+            if (rate > 0.0f) {
+                int32_t count = (int32_t)((now - nneg[e]) / nper[e]);
+                if (count > 0) fired = count;
+                nneg[e] = nneg[e] + (double)count * nper[e];
+            }
+        }
+        mask[e] = fired;
+    }
+    float *tail_rows = shared ? head : (float *)tail;
+    for (int64_t v = 0; v < h; ++v) {
+        int64_t *st = rng + 3 * v;
+        for (int32_t slot = head_offsets[v]; slot < head_offsets[v + 1]; ++slot) {
+            int32_t e = head_edges[slot];
+            if (mask[e] < 0) continue;
+            int64_t t = tails[e];
+            if (t < n && (!shared || t != v)) {
+                float q = orc_classic_dist(head + v * d, tail_rows + t * d, d);
+                if (q > 0.0f) {
+                    float coef = (-2.0f * a * b) * powf(q, b - 1.0f) / (a * powf(q, b) + 1.0f);
+                    orc_classic_move(head + v * d, tail_rows + t * d, d, coef, alpha, shared);
+                }
+            }
+            for (int32_t k = 0; k < mask[e]; ++k) {
+                int32_t draw = (int32_t)orc_taus_next(st);
+                int32_t rem = draw % (int32_t)n;
+                if (rem < 0) rem += (int32_t)n;
+                if (shared && rem == v) continue;
+                float q = orc_classic_dist(head + v * d, tail_rows + (int64_t)rem * d, d);
+                if (q > 0.0f) {
+                    float coef = (2.0f * gamma * b) / ((0.001f + q) * (a * powf(q, b) + 1.0f));
+                    orc_classic_move(head + v * d, tail_rows + (int64_t)rem * d, d, coef, alpha, 0);
+                }
+            }
+        }
+    }
+    free(mask);
+}
+
+int orc_classic_optimize(int64_t h, int64_t n, int64_t d, float *head, const float *tail,
+                         const int32_t *heads, const int32_t *tails, const float *weights,
+                         int64_t E, int32_t n_epochs, int32_t default_epochs, float a, float b,
+                         float learning_rate, float gamma, int32_t rate, uint64_t seed,
+                         int shared, double *npos_out, double *nneg_out, int64_t *rng_out) {
+    if (shared) n = h;
+    size_t Ec = (size_t)(E > 0 ? E : 1);
+    float *pruned = (float *)malloc(sizeof(float) * Ec);
+    double *per = (double *)malloc(sizeof(double) * Ec), *npos = (double *)malloc(sizeof(double) * Ec);
+    double *nper = (double *)malloc(sizeof(double) * Ec), *nneg = (double *)malloc(sizeof(double) * Ec);
+    int32_t *offsets = (int32_t *)calloc((size_t)h + 1, sizeof(int32_t));
+    int32_t *ids = (int32_t *)malloc(sizeof(int32_t) * Ec);
+    int64_t *rng = (int64_t *)malloc(sizeof(int64_t) * (size_t)(3 * h));
+    /* prepare_schedule (estimator.rs:2753-2798) */
+    float wmax = 0.0f;
+    for (int64_t e = 0; e < E; ++e) wmax = weights[e] > wmax ? weights[e] : wmax;
+    float threshold = wmax / (float)(n_epochs > 10 ? n_epochs : default_epochs);
+    double r = (double)(rate > 1 ? rate : 1);
+    for (int64_t e = 0; e < E; ++e) {
+        pruned[e] = weights[e] < threshold ? 0.0f : weights[e];
+        double safe = pruned[e] == 0.0f ? 1.0 : (double)pruned[e];
+        per[e] = (double)wmax / safe;
+        npos[e] = per[e];
+        nper[e] = per[e] / r;
+        nneg[e] = nper[e];
+        if (pruned[e] > 0.0f) offsets[heads[e] + 1] += 1;
+    }
+    /* build_head_csr (estimator.rs:3080-3131): edge ids grouped by head */
+    for (int64_t i = 0; i < h; ++i) offsets[i + 1] += offsets[i];
+    {
+        int32_t *cursor = (int32_t *)malloc(sizeof(int32_t) * (size_t)(h > 0 ? h : 1));
+        memcpy(cursor, offsets, sizeof(int32_t) * (size_t)h);
+        for (int64_t e = 0; e < E; ++e)
+            if (pruned[e] > 0.0f) ids[cursor[heads[e]]++] = (int32_t)e;
+        free(cursor);
+    }
+    /* optimizer_rng_from_first_coordinates (estimator.rs:2800-2829):
+     * StdRng(seed) random_range(i32::MIN..i32::MAX) three times, plus the
+     * bits of each row's first coordinate widened to f64 */
+    orc_rng g;
+    orc_rng_seed(&g, seed);
+    int64_t base[3];
+    for (int k = 0; k < 3; ++k)
+        base[k] = (int64_t)(int32_t)(0x80000000u + orc_rng_range_u32(&g, 0xFFFFFFFFu));
+    for (int64_t i = 0; i < h; ++i) {
+        double first = (double)head[i * d];
+        uint64_t bits;
+        memcpy(&bits, &first, sizeof(bits));
+        for (int k = 0; k < 3; ++k) rng[3 * i + k] = (int64_t)((uint64_t)base[k] + bits);
+    }
+    /* optimize (estimator.rs:2868-2982): linear learning-rate decay */
+    float alpha = learning_rate;
+    for (int32_t t = 0; t < n_epochs; ++t) {
+        orc_classic_epoch(h, n, d, head, tail, tails, pruned, E, per, npos, nper, nneg, rng,
+                          offsets, ids, (uint32_t)t, a, b, gamma, alpha, (float)rate, shared);
+        alpha = learning_rate * (1.0f - (float)t / (float)(n_epochs > 1 ? n_epochs : 1));
+    }
+    if (npos_out) memcpy(npos_out, npos, sizeof(double) * (size_t)E);
+    if (nneg_out) memcpy(nneg_out, nneg, sizeof(double) * (size_t)E);
+    if (rng_out) memcpy(rng_out, rng, sizeof(int64_t) * (size_t)(3 * h));
+    free(pruned); free(per); free(npos); free(nper); free(nneg); free(offsets); free(ids); free(rng);
+    return 0;
+}

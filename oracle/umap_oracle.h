@@ -189,6 +189,22 @@ int orc_random_init(int64_t n, int64_t d, uint64_t seed, float repulsion, float 
 int orc_transform_init(const int32_t *indices, const float *directed, int64_t q, int64_t k,
                        const float *embedding, int64_t d, float *out);
 
+/* ---- compatibility (classic) SGD: kernels/standard_umap/layout.rs:68-336,
+ * estimator.rs:2753-2982. Serial over head vertices in ascending order, each
+ * vertex's positive edges in edge-id order. */
+int64_t orc_taus_next(int64_t state[3]);  /* advances; returns the signed 32-bit draw */
+void orc_classic_epoch(int64_t h, int64_t n, int64_t d, float *head, const float *tail,
+                       const int32_t *tails, const float *weights, int64_t E,
+                       const double *per, double *npos, const double *nper, double *nneg,
+                       int64_t *rng, const int32_t *head_offsets, const int32_t *head_edges,
+                       uint32_t epoch, float a, float b, float gamma, float alpha, float rate,
+                       int shared);
+int orc_classic_optimize(int64_t h, int64_t n, int64_t d, float *head, const float *tail,
+                         const int32_t *heads, const int32_t *tails, const float *weights,
+                         int64_t E, int32_t n_epochs, int32_t default_epochs, float a, float b,
+                         float learning_rate, float gamma, int32_t rate, uint64_t seed,
+                         int shared, double *npos_out, double *nneg_out, int64_t *rng_out);
+
 #ifdef __cplusplus
 }
 #endif
