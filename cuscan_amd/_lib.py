@@ -54,6 +54,28 @@ class ClassicArgs(C.Structure):
     ]
 
 
+class ServerParams(C.Structure):
+    _fields_ = [
+        ("n_neighbors", C.c_int32), ("epochs", C.c_int32),
+        ("a", C.c_float), ("b", C.c_float), ("learning_rate", C.c_float),
+        ("repulsion", C.c_float), ("local_connectivity", C.c_float),
+        ("set_op_mix_ratio", C.c_float),
+        ("negative_sample_rate", C.c_int64), ("negative_selection_range", C.c_int64),
+        ("has_scale", C.c_int32), ("scale", C.c_float), ("adaptation_samples", C.c_int64),
+        ("exclude_graph_neighbors", C.c_int32), ("transform_seed", C.c_uint64),
+        ("max_batch_rows", C.c_int64), ("memory_bytes", C.c_int64),
+    ]
+
+
+class ServerCounters(C.Structure):
+    _fields_ = [
+        ("requests", C.c_int64), ("rows", C.c_int64), ("steps", C.c_int64),
+        ("knn_batches", C.c_int64), ("mean_active", C.c_double), ("peak_active", C.c_int64),
+        ("admission_waits", C.c_int64), ("memory_budget", C.c_int64),
+        ("memory_in_use", C.c_int64),
+    ]
+
+
 class LayoutParams(C.Structure):
     _fields_ = [
         ("optimizer", C.c_int32), ("epochs", C.c_int32),
@@ -117,6 +139,11 @@ SIGNATURES = {
     "umaphip_random_init": (C.c_int, [_i64, _i64, C.c_uint64, C.c_float, C.c_float, _i32, _vp]),
     "umaphip_transform_init": (C.c_int, [_vp, _vp, _i64, _i64, _vp, _i64, _i64, _vp]),
     "umaphip_classic_epoch": (C.c_int, [_vp, _vp]),
+    "umaphip_server_create": (C.c_int, [_vp, _i64, _i64, _vp, _i64, _vp, _vp]),
+    "umaphip_server_submit": (C.c_int, [_vp, _vp, _i64, _vp]),
+    "umaphip_server_wait": (C.c_int, [_vp, _i64, C.c_double, _vp, _vp, _vp]),
+    "umaphip_server_stats": (C.c_int, [_vp, _vp]),
+    "umaphip_server_destroy": (None, [_vp]),
     "umaphip_classic_optimize": (C.c_int, [_i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _i64, _i32,
                                            _i32, C.c_float, C.c_float, C.c_float, C.c_float, _i32,
                                            C.c_uint64, _i32, _vp, _vp, _vp]),

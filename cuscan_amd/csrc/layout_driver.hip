@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "layout_host.hpp"
 #include "layout_kernels.hpp"
 #include "stdrng.hpp"
 
@@ -37,8 +38,6 @@ void set_error(const char *fmt, ...) {
     va_end(ap);
     g_last_error = buf;
 }
-
-namespace {
 
 // ---------------------------------------------------------- host helpers
 // adam_schedule / momentum_schedule (umap-core/src/modern_layout.rs:110-165),
@@ -114,48 +113,16 @@ void projection_direction(StdRng &rng, int64_t d, float *dir) {
     }
 }
 
-// Host replay of ModernOrders (modern_orders.rs:359-680) for a layout.
-struct OrderReplay {
-    StdRng rng;
-    std::vector<int32_t> source, negative;
-    bool alias, negative_live, source_needed;
-    OrderReplay(int64_t h, int64_t n, uint64_t seed, bool alias_, bool hard, bool momentum)
-        : rng(seed), alias(alias_) {
-        // The negative order is shuffled only while neither aliased nor retired
-        // (retirement happens before the first shuffle in projection mode).
-        negative_live = !alias && !hard;
-        // Adam never reads the source order (whole-epoch Jacobi), so its
-        // shuffles only matter for the words they consume before a live
-        // negative shuffle.
-        source_needed = momentum;
-        source.resize(static_cast<size_t>(h));
-        for (int64_t i = 0; i < h; ++i) source[i] = static_cast<int32_t>(i);
-        if (!alias) {
-            negative.resize(static_cast<size_t>(n));
-            for (int64_t i = 0; i < n; ++i) negative[i] = static_cast<int32_t>(i);
-        }
+void bias_corrections(int epochs, std::vector<float> &c1, std::vector<float> &c2) {
+    c1.resize(epochs);
+    c2.resize(epochs);
+    for (int t = 0; t < epochs; ++t) {
+        c1[t] = static_cast<float>(1.0 - std::pow(static_cast<double>(0.9f), t + 1.0));
+        c2[t] = static_cast<float>(1.0 - std::pow(static_cast<double>(0.99f), t + 1.0));
     }
-    bool active() const { return source_needed || negative_live; }
-    // prepare_next: source first, then the live negative order.
-    void advance() {
-        if (!active()) return;
-        rng.shuffle(source.data(), source.size(), source_needed);
-        if (negative_live) rng.shuffle(negative.data(), negative.size(), true);
-    }
-    const std::vector<int32_t> &neg() const { return alias ? source : negative; }
-};
+}
 
-struct Pinned {
-    void *ptr = nullptr;
-    explicit Pinned(size_t bytes) {
-        if (bytes) HIP_CHECK(hipHostMalloc(&ptr, bytes, hipHostMallocDefault));
-    }
-    ~Pinned() {
-        if (ptr) (void)hipHostFree(ptr);
-    }
-    Pinned(const Pinned &) = delete;
-    Pinned &operator=(const Pinned &) = delete;
-};
+namespace {
 
 void check_layout_params(const umaphip_layout_params *p, int64_t h, int64_t n, int64_t d,
                          int64_t E) {
@@ -621,12 +588,7 @@ umaphip_status umaphip_plan_create(int64_t h, int64_t n, int64_t d, const float 
                                    params->scale, params->adaptation_samples > 0);
         schedules(params->epochs, params->initial_alpha, params->good_initialization != 0,
                   P->momentum, P->alpha, P->mu);
-        P->c1.resize(params->epochs);
-        P->c2.resize(params->epochs);
-        for (int t = 0; t < params->epochs; ++t) {
-            P->c1[t] = static_cast<float>(1.0 - std::pow(static_cast<double>(0.9f), t + 1.0));
-            P->c2[t] = static_cast<float>(1.0 - std::pow(static_cast<double>(0.99f), t + 1.0));
-        }
+        bias_corrections(params->epochs, P->c1, P->c2);
         HIP_CHECK(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
         HIP_CHECK(hipEventCreate(&P->ev_start));
         HIP_CHECK(hipEventCreate(&P->ev_stop));

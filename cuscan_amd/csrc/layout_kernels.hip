@@ -363,7 +363,7 @@ __device__ inline int block_exclusive_scan(int v, int tid, int *s_wave, int &tot
     return before + incl - v;
 }
 
-__global__ __launch_bounds__(kThreads) void edge_sweep_kernel(SweepArgs A) {
+__device__ inline void sweep_tile(const SweepArgs &A, int64_t tile) {
     __shared__ int32_t s_off[kTileRows + 1];   // tile-local edge prefix over the tile's rows
     __shared__ int32_t s_seg[kTileRows];       // CSR start of each tile row
     __shared__ int32_t s_rowid[kTileRows];
@@ -379,7 +379,7 @@ __global__ __launch_bounds__(kThreads) void edge_sweep_kernel(SweepArgs A) {
     __shared__ int s_wave[4], s_wave2[4];
 
     const int tid = threadIdx.x;
-    const int64_t row0 = static_cast<int64_t>(blockIdx.x) * kTileRows;
+    const int64_t row0 = tile * kTileRows;
     const int rows = static_cast<int>(min<int64_t>(kTileRows, A.h - row0));
     const bool small = A.h <= 1024;
     const uint64_t n64 = static_cast<uint64_t>(A.n);
@@ -661,6 +661,29 @@ __global__ __launch_bounds__(kThreads) void edge_sweep_kernel(SweepArgs A) {
             reinterpret_cast<float2 *>(A.updates)[i] = u;
         }
     }
+}
+
+__global__ __launch_bounds__(kThreads) void edge_sweep_kernel(SweepArgs A) {
+    sweep_tile(A, blockIdx.x);
+}
+
+// Many independent layouts in one launch: workgroup b runs tile
+// b - tile0[s] of the segment s whose tile range holds b (segments are in
+// ascending tile0 order). Each segment carries its own arguments, so its
+// rows are computed exactly as a launch of that segment alone would.
+__global__ __launch_bounds__(kThreads) void segment_sweep_kernel(const SweepSegment *segs,
+                                                                 int32_t n_segs) {
+    const int64_t b = blockIdx.x;
+    int lo = 0, hi = n_segs;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (segs[mid].tile0 <= b) lo = mid;
+        else hi = mid;
+    }
+    const SweepArgs A = segs[lo].args;
+    const int64_t tile = b - segs[lo].tile0;
+    if (tile * kTileRows >= A.h) return;
+    sweep_tile(A, tile);
 }
 
 // ------------------------------------------------- edge-balanced unit sweep
@@ -1344,6 +1367,16 @@ void launch_unit_sweep(UnitSweepArgs a, hipStream_t s) {
     a.t1 = (static_cast<uint64_t>(a.s.epoch) + 1u) % modulus;
     a.inv_mod = 1.0 / static_cast<double>(modulus);
     hipLaunchKernelGGL(unit_sweep_kernel, dim3(static_cast<unsigned>(a.units)), dim3(64), 0, s, a);
+    HIP_CHECK(hipGetLastError());
+}
+
+int64_t tile_rows() { return kTileRows; }
+
+void launch_segment_sweep(const SweepSegment *segs, int32_t n_segs, int64_t total_tiles,
+                          hipStream_t s) {
+    if (n_segs == 0 || total_tiles == 0) return;
+    hipLaunchKernelGGL(segment_sweep_kernel, dim3(static_cast<unsigned>(total_tiles)),
+                       dim3(kThreads), 0, s, segs, n_segs);
     HIP_CHECK(hipGetLastError());
 }
 

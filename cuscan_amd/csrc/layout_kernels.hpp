@@ -60,6 +60,16 @@ struct SweepArgs {
 };
 void launch_edge_sweep(const SweepArgs &a, hipStream_t s);
 
+// Batched tile sweep: segment s owns global tiles [tile0, tile0 + ceil(h /
+// tile_rows())); segments are in ascending tile0 order.
+struct SweepSegment {
+    SweepArgs args;
+    int64_t tile0;
+};
+int64_t tile_rows();
+void launch_segment_sweep(const SweepSegment *segs, int32_t n_segs, int64_t total_tiles,
+                          hipStream_t s);
+
 // Edge-balanced unit sweep for Adam layouts with H > 1024 and d = 2: one
 // wave per unit_edges() consecutive CSR edges; writes per-row gradients
 // (grad) and, for rows crossing a unit boundary, per-unit fragments.

@@ -1,0 +1,714 @@
+// Continuous-batching transform server over a resident fitted model.
+//
+// A transform request (cuscan FittedUMAP::transform, crates/umap/src/
+// standard_umap/estimator.rs:1367-2264) is: exact kNN of its rows against
+// the fitted references, a bipartite fuzzy graph, a weighted initial
+// placement, then E_tr Adam epochs against the fixed fitted embedding. Two
+// facts make requests batchable without changing any result:
+//   * every query row's kNN depends only on that row, and
+//   * in a transform layout every row moves alone against the fixed tail,
+//     and the tail's projection order for each epoch depends only on the
+//     tail and on StdRng(transform_seed) (modern_negatives.rs:283-351), not
+//     on the request.
+// The server therefore keeps the references (BF16 + f32), the embedding and
+// the per-direction projection tables (order, sorted keys, tail positions by
+// rank) resident, and runs a step loop: each step advances every active
+// request by one epoch of its own (its own epoch index, schedule and
+// negative tables) in one batched tile-sweep launch. New requests are
+// admitted whenever HBM in a fixed arena and a kNN batch slot are free;
+// their kNN runs on a second stream while the step loop continues, and they
+// join at the next step. A request's rows are computed exactly as the
+// single-request transform computes them (same kernels, same per-row
+// summation order), so results do not depend on what else is in flight.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "common.hpp"
+#include "knn.hpp"
+#include "layout_host.hpp"
+#include "layout_kernels.hpp"
+#include "stdrng.hpp"
+
+namespace umaphip {
+namespace {
+
+double now_seconds() {
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+size_t align256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
+
+// First-in first-out allocator over one device buffer. Transform requests
+// all run the same number of epochs and are admitted in order, so they
+// finish nearly in order; a block is released when it and every older block
+// have been freed (a request that fails before its layout frees early).
+struct FifoArena {
+    struct Block {
+        size_t off, bytes;
+        bool freed;
+    };
+    uint8_t *base = nullptr;
+    size_t cap = 0, head = 0, start = 0;
+    std::deque<Block> live;
+    bool alloc(size_t bytes, size_t &off) {
+        bytes = align256(bytes);
+        if (bytes > cap) return false;
+        if (live.empty()) head = start = 0;
+        if (live.empty() || head > start) {
+            if (cap - head >= bytes) {
+                off = head;
+            } else if (start >= bytes) {
+                off = 0;   // wrap; [head, cap) stays unused until start passes it
+            } else {
+                return false;
+            }
+        } else {
+            if (start - head < bytes) return false;
+            off = head;
+        }
+        head = off + bytes;
+        live.push_back({off, bytes, false});
+        return true;
+    }
+    void release(size_t off) {
+        for (auto &b : live)
+            if (b.off == off && !b.freed) {
+                b.freed = true;
+                break;
+            }
+        while (!live.empty() && live.front().freed) live.pop_front();
+        if (live.empty()) head = start = 0;
+        else start = live.front().off;
+    }
+    size_t in_use() const {
+        size_t s = 0;
+        for (auto &l : live) s += l.bytes;
+        return s;
+    }
+};
+
+struct Request {
+    int64_t ticket = 0, q = 0;
+    std::vector<float> x;
+    double t_submit = 0, t_join = 0, t_done = 0;
+    // layout state (device, in the arena)
+    size_t arena_off = 0;
+    int64_t E = 0;
+    bool small = true;
+    int32_t *offsets = nullptr, *indices = nullptr, *ranks = nullptr;
+    double *per = nullptr, *npos = nullptr, *nper = nullptr, *nneg = nullptr;
+    float *pos[2] = {nullptr, nullptr}, *m = nullptr, *v = nullptr, *proj = nullptr;
+    unsigned long long *status = nullptr;
+    int epoch = 0, cur = 0;
+    hipEvent_t ev_final = nullptr;
+    // result
+    std::vector<float> out;
+    bool host_ready = false;   // out already holds the result (no layout ran)
+    umaphip_status code = UMAPHIP_OK;
+    std::string error;
+    bool done = false;         // guarded by the server mutex
+};
+
+// Device bytes of one request's layout state (graph of at most q*k edges).
+size_t request_bytes(int64_t q, int64_t k) {
+    const size_t E = static_cast<size_t>(q * k);
+    return align256((q + 1) * 4) + align256(E * 4) + 4 * align256(E * 8) +
+           4 * align256(q * 2 * 4) + align256(q * 4) + align256(q * 4) + 256;
+}
+
+}  // namespace
+}  // namespace umaphip
+
+using namespace umaphip;
+
+struct umaphip_server {
+    // model
+    int64_t n = 0, D = 0, d = 2, k = 15, epochs = 0;
+    float a = 0, b = 0, gamma = 1, alpha0 = 0.25f, local_connectivity = 0, mix = 1;
+    int32_t rate = 5, exclude = 0;
+    uint32_t range = 0;
+    float scale = 1;
+    std::unique_ptr<KnnIndex> index;
+    DeviceArray<float> tail, dirs;
+    std::vector<float> tail_host, dirs_host;
+    DeviceArray<int32_t> iota, cache_order;
+    DeviceArray<float> cache_sorted, cache_table;
+    std::vector<float> alpha, c1, c2;
+    // streams and staging
+    hipStream_t sl = nullptr, sk = nullptr;
+    int64_t max_batch = 0;
+    DeviceArray<float> q_dev, d_dist;
+    DeviceArray<int32_t> d_idx;
+    std::unique_ptr<Pinned> q_pin, idx_pin, dist_pin, up_pin;
+    size_t up_cap = 0;
+    hipEvent_t ev_knn = nullptr, ev_up = nullptr;
+    static constexpr int kWindow = 8;
+    std::unique_ptr<Pinned> seg_pin[kWindow];
+    DeviceArray<SweepSegment> seg_dev[kWindow];
+    hipEvent_t ev_step[kWindow] = {};
+    int32_t seg_cap = 0;
+    int64_t step_index = 0;
+    DeviceArray<uint8_t> arena_mem;
+    FifoArena arena;
+    // queues
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<std::shared_ptr<Request>> pending;
+    std::vector<std::shared_ptr<Request>> knn_batch, active;
+    std::deque<std::shared_ptr<Request>> finishing;
+    std::unordered_map<int64_t, std::shared_ptr<Request>> tickets;
+    int64_t next_ticket = 1;
+    bool stop = false;
+    std::string fatal;
+    std::thread worker;
+    // statistics
+    int64_t st_requests = 0, st_rows = 0, st_steps = 0, st_knn_batches = 0, st_seg_sum = 0,
+            st_peak_active = 0, st_rejected_wait = 0;
+
+    ~umaphip_server() {
+        {
+            std::lock_guard<std::mutex> g(mu);
+            stop = true;
+        }
+        cv_work.notify_all();
+        if (worker.joinable()) worker.join();
+        if (sl) (void)hipStreamSynchronize(sl);
+        if (sk) (void)hipStreamSynchronize(sk);
+        for (auto &e : ev_step)
+            if (e) (void)hipEventDestroy(e);
+        if (ev_knn) (void)hipEventDestroy(ev_knn);
+        if (ev_up) (void)hipEventDestroy(ev_up);
+        for (auto &r : active)
+            if (r->ev_final) (void)hipEventDestroy(r->ev_final);
+        for (auto &r : finishing)
+            if (r->ev_final) (void)hipEventDestroy(r->ev_final);
+        if (sl) (void)hipStreamDestroy(sl);
+        if (sk) (void)hipStreamDestroy(sk);
+    }
+
+    int device = 0;
+
+    // Marks a request as failed; it retires through the finishing queue.
+    void fail_request(const std::shared_ptr<Request> &r, umaphip_status code, const std::string &m) {
+        r->code = code;
+        r->error = m;
+    }
+
+    // ---- admission: a kNN batch of pending requests that fit the arena
+    void start_knn_batch() {
+        std::vector<std::shared_ptr<Request>> batch;
+        int64_t rows = 0;
+        {
+            std::lock_guard<std::mutex> g(mu);
+            while (!pending.empty()) {
+                auto &r = pending.front();
+                if (rows + r->q > max_batch) break;
+                size_t off;
+                if (!arena.alloc(request_bytes(r->q, k), off)) {
+                    ++st_rejected_wait;
+                    break;
+                }
+                r->arena_off = off;
+                rows += r->q;
+                batch.push_back(r);
+                pending.pop_front();
+            }
+        }
+        if (batch.empty()) return;
+        float *qp = static_cast<float *>(q_pin->ptr);
+        int64_t row = 0;
+        for (auto &r : batch) {
+            std::memcpy(qp + row * D, r->x.data(), r->q * D * sizeof(float));
+            row += r->q;
+        }
+        HIP_CHECK(hipMemcpyAsync(q_dev.ptr, qp, rows * D * sizeof(float), hipMemcpyHostToDevice, sk));
+        index->query(q_dev.ptr, rows, static_cast<int>(k), false, 0, d_idx.ptr, d_dist.ptr, sk);
+        HIP_CHECK(hipMemcpyAsync(idx_pin->ptr, d_idx.ptr, rows * k * sizeof(int32_t),
+                                 hipMemcpyDeviceToHost, sk));
+        HIP_CHECK(hipMemcpyAsync(dist_pin->ptr, d_dist.ptr, rows * k * sizeof(float),
+                                 hipMemcpyDeviceToHost, sk));
+        HIP_CHECK(hipEventRecord(ev_knn, sk));
+        knn_batch = std::move(batch);
+        ++st_knn_batches;
+    }
+
+    // ---- host graph for a finished kNN batch, upload, join the step loop
+    void join_knn_batch() {
+        HIP_CHECK(hipEventSynchronize(ev_up));  // previous uploads left the staging buffer
+        const int32_t *idx_all = static_cast<const int32_t *>(idx_pin->ptr);
+        const float *dist_all = static_cast<const float *>(dist_pin->ptr);
+        uint8_t *up = static_cast<uint8_t *>(up_pin->ptr);
+        size_t up_used = 0;
+        int64_t row = 0;
+        std::vector<std::shared_ptr<Request>> joined, failed;
+        for (auto &r : knn_batch) {
+            const int32_t *idx = idx_all + row * k;
+            const float *dist = dist_all + row * k;
+            row += r->q;
+            const int64_t cap = 2 * r->q * k;
+            std::vector<int32_t> heads(cap), tails(cap);
+            std::vector<float> w(cap), rho(r->q), sigma(r->q), directed(r->q * k);
+            int64_t used = 0;
+            umaphip_status st = umaphip_fuzzy_graph(idx, dist, r->q, k, local_connectivity, 1.0f,
+                                                    mix, 1, n, heads.data(), tails.data(), w.data(),
+                                                    rho.data(), sigma.data(), directed.data(), &used);
+            bool any = false;
+            for (int64_t e = 0; e < used && !any; ++e) any = w[e] > 0.0f;
+            if (st == UMAPHIP_OK && !any) {
+                st = UMAPHIP_INVALID_INPUT;
+                set_error("transform fuzzy graph is empty after positive-edge pruning");
+            }
+            std::vector<float> init(r->q * d);
+            if (st == UMAPHIP_OK)
+                st = umaphip_transform_init(idx, directed.data(), r->q, k, tail_host.data(), n, d,
+                                            init.data());
+            if (st != UMAPHIP_OK) {
+                fail_request(r, st, umaphip_last_error());
+                failed.push_back(r);
+                continue;
+            }
+            if (epochs == 0) {
+                r->out = init;
+                r->host_ready = true;
+                failed.push_back(r);  // finished without a layout
+                continue;
+            }
+            std::vector<uint32_t> off(r->q + 1), ind(std::max<int64_t>(used, 1));
+            std::vector<float> cw(std::max<int64_t>(used, 1));
+            std::vector<double> per(std::max<int64_t>(used, 1)), nper(per.size()), npos(per.size()),
+                nneg(per.size());
+            int64_t E = 0;
+            st = umaphip_compact_graph(r->q, n, used, heads.data(), tails.data(), w.data(),
+                                       static_cast<double>(epochs), static_cast<int32_t>(epochs),
+                                       static_cast<float>(rate), 1, off.data(), ind.data(),
+                                       cw.data(), per.data(), nper.data(), npos.data(), nneg.data(),
+                                       &E);
+            if (st != UMAPHIP_OK) {
+                fail_request(r, st, umaphip_last_error());
+                failed.push_back(r);
+                continue;
+            }
+            // carve the request's arena block
+            uint8_t *p = arena.base + r->arena_off;
+            auto take = [&](size_t bytes) {
+                uint8_t *q = p;
+                p += align256(bytes);
+                return q;
+            };
+            r->E = E;
+            r->small = r->q <= 1024;
+            r->offsets = reinterpret_cast<int32_t *>(take((r->q + 1) * 4));
+            r->indices = reinterpret_cast<int32_t *>(take(r->q * k * 4));
+            r->per = reinterpret_cast<double *>(take(r->q * k * 8));
+            r->npos = reinterpret_cast<double *>(take(r->q * k * 8));
+            r->nper = reinterpret_cast<double *>(take(r->q * k * 8));
+            r->nneg = reinterpret_cast<double *>(take(r->q * k * 8));
+            r->pos[0] = reinterpret_cast<float *>(take(r->q * d * 4));
+            r->pos[1] = reinterpret_cast<float *>(take(r->q * d * 4));
+            r->m = reinterpret_cast<float *>(take(r->q * d * 4));
+            r->v = reinterpret_cast<float *>(take(r->q * d * 4));
+            r->ranks = reinterpret_cast<int32_t *>(take(r->q * 4));
+            r->proj = reinterpret_cast<float *>(take(r->q * 4));
+            r->status = reinterpret_cast<unsigned long long *>(take(8));
+            auto stage = [&](void *dst, const void *src, size_t bytes) {
+                std::memcpy(up + up_used, src, bytes);
+                HIP_CHECK(hipMemcpyAsync(dst, up + up_used, bytes, hipMemcpyHostToDevice, sl));
+                up_used += align256(bytes);
+            };
+            stage(r->offsets, off.data(), (r->q + 1) * 4);
+            stage(r->indices, ind.data(), E * 4);
+            stage(r->per, per.data(), E * 8);
+            stage(r->npos, npos.data(), E * 8);
+            stage(r->nper, nper.data(), E * 8);
+            stage(r->nneg, nneg.data(), E * 8);
+            stage(r->pos[0], init.data(), r->q * d * 4);
+            HIP_CHECK(hipMemsetAsync(r->m, 0, r->q * d * 4, sl));
+            HIP_CHECK(hipMemsetAsync(r->v, 0, r->q * d * 4, sl));
+            launch_status_min_u64(r->status, ~0ull, sl);
+            r->epoch = 0;
+            r->cur = 0;
+            r->t_join = now_seconds();
+            joined.push_back(r);
+        }
+        HIP_CHECK(hipEventRecord(ev_up, sl));
+        std::lock_guard<std::mutex> g(mu);
+        // failed requests still hold their arena block in FIFO order: they
+        // retire through the finishing queue behind earlier requests
+        for (auto &r : knn_batch) {
+            if (r->code != UMAPHIP_OK || r->host_ready) {
+                HIP_CHECK(hipEventCreateWithFlags(&r->ev_final, hipEventDisableTiming));
+                HIP_CHECK(hipEventRecord(r->ev_final, sl));
+                finishing.push_back(r);
+            } else {
+                active.push_back(r);
+            }
+        }
+        st_peak_active = std::max<int64_t>(st_peak_active, static_cast<int64_t>(active.size()));
+        knn_batch.clear();
+    }
+
+    // ---- one step: every active request advances one epoch
+    void step() {
+        const int slot = static_cast<int>(step_index % kWindow);
+        HIP_CHECK(hipEventSynchronize(ev_step[slot]));
+        const int32_t ns = static_cast<int32_t>(active.size());
+        if (ns > seg_cap) {
+            seg_cap = std::max<int32_t>(ns, 2 * seg_cap);
+            for (int w = 0; w < kWindow; ++w) {
+                HIP_CHECK(hipEventSynchronize(ev_step[w]));
+                seg_pin[w].reset(new Pinned(sizeof(SweepSegment) * seg_cap));
+                seg_dev[w].alloc(seg_cap);
+            }
+        }
+        SweepSegment *segs = static_cast<SweepSegment *>(seg_pin[slot]->ptr);
+        int64_t tiles = 0;
+        for (int32_t s = 0; s < ns; ++s) {
+            Request &r = *active[s];
+            const int t = r.epoch;
+            SweepArgs A{};
+            A.h = r.q; A.n = n;
+            A.pos = r.pos[r.cur]; A.pos_out = r.pos[1 - r.cur];
+            A.tgt = tail.ptr;
+            if (r.small) {
+                // small layouts: epoch 0 identity order, epoch t >= 1 the
+                // order of direction t - 1 (modern_runtime.rs:1128-1135)
+                A.table = t == 0 ? tail.ptr : cache_table.ptr + (t - 1) * n * 2;
+                A.order = t == 0 ? iota.ptr : cache_order.ptr + (t - 1) * n;
+                A.ranks = nullptr;
+            } else {
+                const float *dir = dirs.ptr + t * d;
+                launch_project_keys(r.pos[r.cur], r.q, d, dir, nullptr, r.proj, r.status,
+                                    static_cast<unsigned long long>(t) << 32, sl);
+                launch_lower_bound(cache_sorted.ptr + t * n, n, r.proj, r.q, r.ranks, sl);
+                A.table = cache_table.ptr + t * n * 2;
+                A.order = cache_order.ptr + t * n;
+                A.ranks = r.ranks;
+            }
+            A.offsets = r.offsets; A.indices = r.indices;
+            A.periods = r.per; A.next_positive = r.npos;
+            A.negative_periods = r.nper; A.next_negative = r.nneg;
+            A.m = r.m; A.v = r.v;
+            A.scale = scale;
+            A.epoch = static_cast<uint32_t>(t);
+            A.a = a; A.b = b; A.gamma = gamma; A.range = range;
+            A.multiplier = 1.0f; A.alpha = alpha[t]; A.c1 = c1[t]; A.c2 = c2[t];
+            A.shared = 0; A.negatives = rate != 0; A.exclude = exclude; A.fused = 1;
+            segs[s].args = A;
+            segs[s].tile0 = tiles;
+            tiles += (r.q + tile_rows() - 1) / tile_rows();
+        }
+        HIP_CHECK(hipMemcpyAsync(seg_dev[slot].ptr, segs, sizeof(SweepSegment) * ns,
+                                 hipMemcpyHostToDevice, sl));
+        launch_segment_sweep(seg_dev[slot].ptr, ns, tiles, sl);
+        HIP_CHECK(hipEventRecord(ev_step[slot], sl));
+        ++step_index;
+        ++st_steps;
+        st_seg_sum += ns;
+        std::vector<std::shared_ptr<Request>> still;
+        still.reserve(active.size());
+        for (auto &rp : active) {
+            rp->cur = 1 - rp->cur;
+            rp->epoch += 1;
+            if (rp->epoch == epochs) {
+                HIP_CHECK(hipEventCreateWithFlags(&rp->ev_final, hipEventDisableTiming));
+                HIP_CHECK(hipEventRecord(rp->ev_final, sl));
+                std::lock_guard<std::mutex> g(mu);
+                finishing.push_back(rp);
+            } else {
+                still.push_back(rp);
+            }
+        }
+        active.swap(still);
+    }
+
+    // ---- retire finished requests in FIFO order
+    bool retire() {
+        bool any = false;
+        for (;;) {
+            std::shared_ptr<Request> r;
+            {
+                std::lock_guard<std::mutex> g(mu);
+                if (finishing.empty()) break;
+                r = finishing.front();
+            }
+            if (hipEventQuery(r->ev_final) != hipSuccess) break;
+            (void)hipEventDestroy(r->ev_final);
+            r->ev_final = nullptr;
+            if (r->code == UMAPHIP_OK && !r->host_ready) {
+                r->out.resize(r->q * d);
+                unsigned long long status = ~0ull;
+                HIP_CHECK(hipMemcpy(r->out.data(), r->pos[r->cur], r->q * d * sizeof(float),
+                                    hipMemcpyDeviceToHost));
+                HIP_CHECK(hipMemcpy(&status, r->status, sizeof(status), hipMemcpyDeviceToHost));
+                bool finite = status == ~0ull;
+                for (float x : r->out) finite = finite && std::isfinite(x);
+                if (!finite) fail_request(r, UMAPHIP_NUMERICAL, "non-finite transform coordinates");
+            }
+            std::lock_guard<std::mutex> g(mu);
+            r->t_done = now_seconds();
+            r->done = true;
+            finishing.pop_front();
+            arena.release(r->arena_off);
+            ++st_requests;
+            st_rows += r->q;
+            any = true;
+        }
+        if (any) cv_done.notify_all();
+        return any;
+    }
+
+    void loop() {
+        (void)hipSetDevice(device);
+        try {
+            for (;;) {
+                bool busy = false;
+                {
+                    std::unique_lock<std::mutex> g(mu);
+                    if (stop) break;
+                    if (pending.empty() && active.empty() && knn_batch.empty() && finishing.empty())
+                        cv_work.wait_for(g, std::chrono::milliseconds(5));
+                    if (stop) break;
+                }
+                if (knn_batch.empty()) start_knn_batch();
+                if (!knn_batch.empty() && hipEventQuery(ev_knn) == hipSuccess) {
+                    join_knn_batch();
+                    busy = true;
+                }
+                if (!active.empty()) {
+                    step();
+                    busy = true;
+                }
+                busy = retire() || busy;
+                if (!busy && !finishing.empty())
+                    std::this_thread::sleep_for(std::chrono::microseconds(50));
+                else if (!busy && !knn_batch.empty())
+                    std::this_thread::sleep_for(std::chrono::microseconds(20));
+            }
+        } catch (const Fail &) {
+            std::lock_guard<std::mutex> g(mu);
+            fatal = umaphip_last_error();
+            auto fail_all = [&](auto &list) {
+                for (auto &r : list) {
+                    r->code = UMAPHIP_BACKEND;
+                    r->error = fatal;
+                    r->done = true;
+                }
+            };
+            fail_all(active);
+            fail_all(knn_batch);
+            fail_all(pending);
+            fail_all(finishing);
+            stop = true;
+            cv_done.notify_all();
+        }
+    }
+};
+
+extern "C" {
+
+umaphip_status umaphip_server_create(const float *reference, int64_t n, int64_t features,
+                                     const float *embedding, int64_t d,
+                                     const umaphip_server_params *params, umaphip_server **out) {
+    return guarded([&] {
+        UMAPHIP_REQUIRE(out && reference && embedding && params, UMAPHIP_INVALID_INPUT,
+                        "server arguments are null");
+        *out = nullptr;
+        UMAPHIP_REQUIRE(n > 0 && features > 0 && n <= INT32_MAX - 1, UMAPHIP_INVALID_INPUT,
+                        "server needs a non-empty reference set");
+        UMAPHIP_REQUIRE(d == 2, UMAPHIP_INVALID_CONFIG, "the transform server supports 2-D embeddings");
+        const umaphip_server_params &p = *params;
+        UMAPHIP_REQUIRE(p.n_neighbors >= 1 && p.epochs >= 0 && p.epochs < 65535 &&
+                            p.negative_sample_rate >= 0 && p.negative_selection_range > 0 &&
+                            p.max_batch_rows >= 1,
+                        UMAPHIP_INVALID_CONFIG, "invalid server parameters");
+        std::unique_ptr<umaphip_server> S(new umaphip_server());
+        S->n = n; S->D = features; S->d = d;
+        S->k = std::min<int64_t>(p.n_neighbors, n);
+        UMAPHIP_REQUIRE(S->k <= 64, UMAPHIP_INVALID_CONFIG, "n_neighbors above 64 is not supported");
+        S->epochs = p.epochs;
+        S->a = p.a; S->b = p.b; S->gamma = p.repulsion;
+        S->alpha0 = p.learning_rate / 4.0f;
+        S->local_connectivity = std::max(p.local_connectivity - 1.0f, 0.0f);
+        S->mix = p.set_op_mix_ratio;
+        S->rate = p.negative_sample_rate;
+        S->exclude = p.exclude_graph_neighbors;
+        const uint64_t Q = static_cast<uint64_t>(p.negative_selection_range);
+        UMAPHIP_REQUIRE(Q < static_cast<uint64_t>(n), UMAPHIP_INVALID_CONFIG,
+                        "the server caches projection windows: negative_selection_range must be "
+                        "below the reference rows");
+        S->range = static_cast<uint32_t>(effective_range(Q, static_cast<uint64_t>(n)));
+        S->scale = resolved_scale(Q, static_cast<uint64_t>(n), p.has_scale != 0, p.scale,
+                                  p.adaptation_samples > 0);
+        std::vector<float> mu;
+        schedules(static_cast<int>(S->epochs), S->alpha0, false, false, S->alpha, mu);
+        bias_corrections(static_cast<int>(S->epochs), S->c1, S->c2);
+        S->max_batch = p.max_batch_rows;
+        HIP_CHECK(hipStreamCreateWithFlags(&S->sl, hipStreamNonBlocking));
+        HIP_CHECK(hipStreamCreateWithFlags(&S->sk, hipStreamNonBlocking));
+        HIP_CHECK(hipEventCreateWithFlags(&S->ev_knn, hipEventDisableTiming));
+        HIP_CHECK(hipEventCreateWithFlags(&S->ev_up, hipEventDisableTiming));
+        HIP_CHECK(hipEventRecord(S->ev_up, S->sl));
+        for (auto &e : S->ev_step) {
+            HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            HIP_CHECK(hipEventRecord(e, S->sl));
+        }
+        // resident model
+        S->index.reset(new KnnIndex(reference, nullptr, n, features, S->max_batch, S->sk));
+        S->tail_host.assign(embedding, embedding + n * d);
+        S->tail.alloc(n * d);
+        HIP_CHECK(hipMemcpy(S->tail.ptr, embedding, n * d * sizeof(float), hipMemcpyHostToDevice));
+        S->iota.alloc(n);
+        launch_iota(S->iota.ptr, n, S->sk);
+        // per-direction projection tables of the fixed tail (directions of
+        // StdRng(transform_seed), one per epoch)
+        const int64_t nd = std::max<int64_t>(S->epochs, 1);
+        S->dirs_host.resize(nd * d);
+        StdRng rng(p.transform_seed);
+        for (int64_t t = 0; t < nd; ++t) projection_direction(rng, d, S->dirs_host.data() + t * d);
+        S->dirs.alloc(nd * d);
+        HIP_CHECK(hipMemcpy(S->dirs.ptr, S->dirs_host.data(), nd * d * sizeof(float),
+                            hipMemcpyHostToDevice));
+        S->cache_order.alloc(nd * n);
+        S->cache_sorted.alloc(nd * n);
+        S->cache_table.alloc(nd * n * 2);
+        {
+            DeviceArray<uint32_t> keys(n), keys_sorted(n);
+            DeviceArray<uint8_t> ws(sort_pairs_workspace(n));
+            DeviceArray<unsigned long long> status(1);
+            launch_status_min_u64(status.ptr, ~0ull, S->sk);
+            for (int64_t t = 0; t < nd; ++t) {
+                launch_project_keys(S->tail.ptr, n, d, S->dirs.ptr + t * d, keys.ptr, nullptr,
+                                    status.ptr, static_cast<unsigned long long>(t) << 32, S->sk);
+                launch_sort_pairs(ws.ptr, ws.count, keys.ptr, keys_sorted.ptr, S->iota.ptr,
+                                  S->cache_order.ptr + t * n, n, S->sk);
+                launch_values_from_keys(keys_sorted.ptr, n, S->cache_sorted.ptr + t * n, S->sk);
+                launch_gather_table(S->cache_order.ptr + t * n, n, S->tail.ptr,
+                                    S->cache_table.ptr + t * n * 2, S->sk);
+            }
+            unsigned long long st = 0;
+            HIP_CHECK(hipMemcpyAsync(&st, status.ptr, sizeof(st), hipMemcpyDeviceToHost, S->sk));
+            HIP_CHECK(hipStreamSynchronize(S->sk));
+            UMAPHIP_REQUIRE(st == ~0ull, UMAPHIP_NUMERICAL, "non-finite fitted embedding at row %llu",
+                            st & 0xFFFFFFFFull);
+        }
+        // kNN staging
+        S->q_dev.alloc(S->max_batch * features);
+        S->d_idx.alloc(S->max_batch * S->k);
+        S->d_dist.alloc(S->max_batch * S->k);
+        S->q_pin.reset(new Pinned(S->max_batch * features * sizeof(float)));
+        S->idx_pin.reset(new Pinned(S->max_batch * S->k * sizeof(int32_t)));
+        S->dist_pin.reset(new Pinned(S->max_batch * S->k * sizeof(float)));
+        // per row: offsets, k indices, 4k clocks/periods, init; per request:
+        // seven 256-byte alignments
+        S->up_cap = static_cast<size_t>(S->max_batch) * (S->k * 36 + 16 + 7 * 256);
+        S->up_pin.reset(new Pinned(S->up_cap));
+        // admission arena: explicit size, or a quarter of the free HBM up to 1 GiB
+        size_t free_b = 0, total_b = 0;
+        HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+        size_t cap = p.memory_bytes > 0 ? static_cast<size_t>(p.memory_bytes)
+                                        : std::min<size_t>(free_b / 4, size_t(1) << 30);
+        UMAPHIP_REQUIRE(cap >= request_bytes(1, S->k) && cap <= free_b, UMAPHIP_MEMORY,
+                        "server memory budget %zu exceeds free device memory %zu", cap, free_b);
+        S->arena_mem.alloc(cap);
+        S->arena.base = S->arena_mem.ptr;
+        S->arena.cap = cap;
+        HIP_CHECK(hipDeviceSynchronize());
+        HIP_CHECK(hipGetDevice(&S->device));
+        umaphip_server *raw = S.get();
+        S->worker = std::thread([raw] { raw->loop(); });
+        *out = S.release();
+    });
+}
+
+umaphip_status umaphip_server_submit(umaphip_server *server, const float *queries, int64_t q,
+                                     int64_t *ticket) {
+    return guarded([&] {
+        UMAPHIP_REQUIRE(server && ticket && (q == 0 || queries), UMAPHIP_INVALID_INPUT,
+                        "server submit arguments are null");
+        UMAPHIP_REQUIRE(q >= 1 && q <= server->max_batch, UMAPHIP_CAPACITY,
+                        "request rows must be in 1..=%lld", (long long)server->max_batch);
+        UMAPHIP_REQUIRE(request_bytes(q, server->k) <= server->arena.cap, UMAPHIP_MEMORY,
+                        "request exceeds the server memory budget");
+        auto r = std::make_shared<Request>();
+        r->q = q;
+        r->x.assign(queries, queries + q * server->D);
+        for (float v : r->x)
+            UMAPHIP_REQUIRE(std::isfinite(v), UMAPHIP_INVALID_INPUT, "query rows must be finite");
+        r->t_submit = now_seconds();
+        {
+            std::lock_guard<std::mutex> g(server->mu);
+            UMAPHIP_REQUIRE(server->fatal.empty(), UMAPHIP_BACKEND, "server stopped: %s",
+                            server->fatal.c_str());
+            r->ticket = server->next_ticket++;
+            server->tickets[r->ticket] = r;
+            server->pending.push_back(r);
+        }
+        server->cv_work.notify_all();
+        *ticket = r->ticket;
+    });
+}
+
+umaphip_status umaphip_server_wait(umaphip_server *server, int64_t ticket, double timeout_s,
+                                   float *out, double *latency_ms, int32_t *done) {
+    return guarded([&] {
+        UMAPHIP_REQUIRE(server && done, UMAPHIP_INVALID_INPUT, "server wait arguments are null");
+        *done = 0;
+        std::shared_ptr<Request> r;
+        std::unique_lock<std::mutex> g(server->mu);
+        auto it = server->tickets.find(ticket);
+        UMAPHIP_REQUIRE(it != server->tickets.end(), UMAPHIP_INVALID_INPUT, "unknown ticket %lld",
+                        (long long)ticket);
+        r = it->second;
+        const auto deadline = std::chrono::steady_clock::now() +
+                              std::chrono::duration<double>(std::max(timeout_s, 0.0));
+        while (!r->done) {
+            if (timeout_s <= 0.0 ||
+                server->cv_done.wait_until(g, deadline) == std::cv_status::timeout) {
+                if (!r->done) break;
+            }
+        }
+        *done = r->done ? 1 : 0;
+        if (!r->done) return;
+        server->tickets.erase(ticket);
+        g.unlock();
+        if (latency_ms) *latency_ms = (r->t_done - r->t_submit) * 1e3;
+        UMAPHIP_REQUIRE(r->code == UMAPHIP_OK, r->code, "%s", r->error.c_str());
+        if (out) std::memcpy(out, r->out.data(), r->out.size() * sizeof(float));
+    });
+}
+
+umaphip_status umaphip_server_stats(umaphip_server *server, umaphip_server_counters *stats) {
+    return guarded([&] {
+        UMAPHIP_REQUIRE(server && stats, UMAPHIP_INVALID_INPUT, "server stats arguments are null");
+        std::lock_guard<std::mutex> g(server->mu);
+        stats->requests = server->st_requests;
+        stats->rows = server->st_rows;
+        stats->steps = server->st_steps;
+        stats->knn_batches = server->st_knn_batches;
+        stats->mean_active = server->st_steps ? static_cast<double>(server->st_seg_sum) /
+                                                    static_cast<double>(server->st_steps)
+                                              : 0.0;
+        stats->peak_active = server->st_peak_active;
+        stats->admission_waits = server->st_rejected_wait;
+        stats->memory_budget = static_cast<int64_t>(server->arena.cap);
+        stats->memory_in_use = static_cast<int64_t>(server->arena.in_use());
+    });
+}
+
+void umaphip_server_destroy(umaphip_server *server) { delete server; }
+
+}  // extern "C"

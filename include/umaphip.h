@@ -322,6 +322,56 @@ umaphip_status umaphip_classic_optimize(int64_t h, int64_t n, int64_t d, float *
                                         int32_t shared, double *next_positive_out,
                                         double *next_negative_out, int64_t *rng_out);
 
+/* ------------------------------------------------------------------------
+ * Continuous-batching transform server (no reference counterpart: it serves
+ * FittedUMAP::transform, crates/umap/src/standard_umap/estimator.rs:
+ * 1367-2264, for many concurrent requests). The fitted references, the
+ * embedding and the per-epoch projection tables of the embedding stay
+ * resident; requests are admitted while a fixed device arena has room and
+ * advance one epoch per batched step. A request's result equals what
+ * UMAP.transform computes for it alone (same kernels and per-row order for
+ * requests of up to 1024 rows).
+ * ------------------------------------------------------------------------ */
+typedef struct umaphip_server umaphip_server;
+typedef struct {
+    int32_t n_neighbors;            /* k = min(n_neighbors, n) */
+    int32_t epochs;                 /* transform epochs */
+    float a, b;                     /* fitted curve */
+    float learning_rate;            /* fit learning rate; the transform uses / 4 */
+    float repulsion;                /* gamma */
+    float local_connectivity;       /* fit value; the transform uses max(lc - 1, 0) */
+    float set_op_mix_ratio;
+    int64_t negative_sample_rate;
+    int64_t negative_selection_range;   /* must be below n (projection windows) */
+    int32_t has_scale;
+    float scale;
+    int64_t adaptation_samples;     /* only selects the default scale formula */
+    int32_t exclude_graph_neighbors;
+    uint64_t transform_seed;
+    int64_t max_batch_rows;         /* rows per kNN batch and per request */
+    int64_t memory_bytes;           /* admission arena; 0 = min(free/4, 1 GiB) */
+} umaphip_server_params;
+typedef struct {
+    int64_t requests, rows, steps, knn_batches;
+    double mean_active;             /* requests per step */
+    int64_t peak_active, admission_waits, memory_budget, memory_in_use;
+} umaphip_server_counters;
+/* Builds the resident model (copies reference [n, features] and embedding
+ * [n, d] from host memory) and starts the server thread. */
+umaphip_status umaphip_server_create(const float *reference, int64_t n, int64_t features,
+                                     const float *embedding, int64_t d,
+                                     const umaphip_server_params *params, umaphip_server **out);
+/* Queues q rows (host [q, features], copied); returns a ticket. */
+umaphip_status umaphip_server_submit(umaphip_server *server, const float *queries, int64_t q,
+                                     int64_t *ticket);
+/* Waits up to timeout_s (0: poll) for a ticket. When *done = 1 the result
+ * [q, d] is in out, latency_ms is submit-to-completion, and the ticket is
+ * released; a failed request returns its status. */
+umaphip_status umaphip_server_wait(umaphip_server *server, int64_t ticket, double timeout_s,
+                                   float *out, double *latency_ms, int32_t *done);
+umaphip_status umaphip_server_stats(umaphip_server *server, umaphip_server_counters *stats);
+void umaphip_server_destroy(umaphip_server *server);
+
 #ifdef __cplusplus
 }
 #endif
