@@ -41,12 +41,17 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def make_data(rows, dims, clusters, seed, device):
+def make_data(rows, dims, clusters, seed, device, row_seed=None):
+    """Rows of the 10-cluster 0..255 mixture; centres come from `seed`, the
+    rows from `row_seed` (default: the same generator as the centres)."""
     import torch
 
     g = torch.Generator(device=device)
     g.manual_seed(seed)
     centers = 32.0 + 192.0 * torch.rand((clusters, dims), generator=g, device=device)
+    if row_seed is not None:
+        g = torch.Generator(device=device)
+        g.manual_seed(row_seed)
     labels = torch.randint(0, clusters, (rows,), generator=g, device=device)
     x = torch.empty((rows, dims), dtype=torch.float32, device=device)
     step = 1 << 17
@@ -112,7 +117,6 @@ def build_workload(rows, dims, clusters, seed, k, epochs):
         log(f"  kNN rows {e}/{rows} ({time.time() - t0:.1f}s)")
     knn_s = time.time() - t0
     log(f"exact kNN ({k - 1} non-self) in {knn_s:.1f}s")
-    index.close()
     t0 = time.time()
     indices = np.concatenate([np.arange(rows, dtype=np.int32)[:, None], idx], axis=1)
     distances = np.concatenate([np.zeros((rows, 1), np.float32), dist], axis=1)
@@ -123,7 +127,131 @@ def build_workload(rows, dims, clusters, seed, k, epochs):
     init = G.random_init(rows, 2, seed)
     graph_s = time.time() - t0
     log(f"fuzzy graph + compaction: E={len(compact['indices'])} in {graph_s:.1f}s")
-    return x, compact, init, a, b, knn_s, graph_s
+    return x, compact, init, a, b, knn_s, graph_s, index
+
+
+def serve_benchmark(x, emb, compact_index, a, b, args):
+    """Transform serving on the resident 1M-row model: requests of
+    round(lognormal(ln 64, 1)) rows clipped to 1..4096 from the same mixture
+    (rows seed 43), 100-epoch Adam transform (lr/4, transform_seed 42)."""
+    import threading
+
+    import torch
+
+    from cuscan_amd import graph as G
+    from cuscan_amd import layout as L
+    from cuscan_amd.serve import TransformServer
+
+    rng = np.random.default_rng(43)
+    n_plain, n_served = args.serve_plain, args.serve_requests
+    sizes = np.clip(np.round(rng.lognormal(np.log(64), 1.0, n_served + n_plain)), 1, 4096).astype(int)
+    pool = make_data(int(sizes.sum()), x.shape[1], 10, 42, x.device, row_seed=43).cpu().numpy()
+    starts = np.concatenate([[0], np.cumsum(sizes)])
+    req = [pool[starts[i]:starts[i + 1]] for i in range(len(sizes))]
+    epochs, k = 100, 15
+    emb = np.ascontiguousarray(emb, np.float32)
+
+    # ---- plain per-request transform (the UMAP.transform call chain)
+    params = L.LayoutParams(epochs=epochs, a=a, b=b, initial_alpha=0.25, seed=42)
+
+    def plain(xq):
+        idx, dist = compact_index.query(xq, k)
+        g = G.fuzzy_graph(idx, dist, local_connectivity=0.0, bipartite=True, n_reference=len(emb))
+        init = G.transform_init(idx, g["directed"], emb)
+        cg = G.compact_graph(len(xq), len(emb), g["heads"], g["tails"], g["weights"], epochs=epochs,
+                             rate=5.0, prune_divisor=float(epochs), transform=True)
+        out, _, _, _ = L.optimize_compact(init, emb, cg, params, shared=False)
+        return out
+
+    plain(req[-1])  # warm
+    t_plain, plain_out = [], []
+    for i in range(n_plain):
+        t0 = time.perf_counter()
+        plain_out.append(plain(req[n_served + i]))
+        t_plain.append((time.perf_counter() - t0) * 1e3)
+    plain_rows = int(sizes[n_served:].sum())
+    plain_rps = plain_rows / (sum(t_plain) / 1e3)
+    log(f"plain transform: {n_plain} requests, {plain_rows} rows, {plain_rps:.0f} rows/s, "
+        f"p50 {np.percentile(t_plain, 50):.2f} ms")
+
+    t0 = time.time()
+    srv = TransformServer(x.cpu().numpy(), emb, a=a, b=b, n_neighbors=k, epochs=epochs,
+                          learning_rate=1.0, negative_selection_range=200_000, transform_seed=42,
+                          max_batch_rows=8192)
+    setup_s = time.time() - t0
+    log(f"server setup {setup_s:.1f}s")
+    # equal accuracy: the served coordinates of the plain requests
+    same = True
+    for i in range(n_plain):
+        out, _ = srv.wait(srv.submit(req[n_served + i]))
+        same = same and bool((out == plain_out[i]).all()) if sizes[n_served + i] <= 1024 else same
+        if sizes[n_served + i] > 1024:
+            dev = np.linalg.norm(out - plain_out[i], axis=1)
+            same = same and bool(np.median(dev) < 1e-2)
+
+    def run(rate):
+        """Poisson arrivals at `rate` requests/s (inf: all at once)."""
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, n_served)) if np.isfinite(rate) else \
+            np.zeros(n_served)
+        tickets = [None] * n_served
+        t_start = time.perf_counter()
+
+        def submitter():
+            for i in range(n_served):
+                delay = t_start + arrivals[i] - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                tickets[i] = srv.submit(req[i])
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        lat = []
+        for i in range(n_served):
+            while tickets[i] is None:
+                time.sleep(0.0002)
+            res = srv.wait(tickets[i], 120.0)
+            lat.append(res[1])
+        t_end = time.perf_counter()
+        th.join()
+        rows = int(sizes[:n_served].sum())
+        return {"rate_req_s": None if not np.isfinite(rate) else rate,
+                "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                "rows_per_s": round(rows / (t_end - t_start), 1),
+                "requests": n_served}
+
+    sweep = []
+    for rate in args.serve_rates:
+        sweep.append(run(rate))
+        log(f"served rate {rate}: {sweep[-1]}")
+    sat = run(float("inf"))
+    log(f"served saturation: {sat}")
+    stats = srv.stats()
+    srv.close()
+
+    # BF16 prefilter recall@15 against fp32 brute force on 256 query rows
+    qs = torch.from_numpy(pool[:256]).to(x.device)
+    cand, counts = compact_index.candidates_device(qs)
+    d2 = torch.cdist(qs, x) ** 2
+    exact = torch.topk(d2, k, largest=False).indices.to(torch.int32)
+    top15 = cand[:, :k]
+    recall = (top15.unsqueeze(2) == exact.unsqueeze(1)).any(2).float().mean().item()
+    coverage = (cand.unsqueeze(2) == exact.unsqueeze(1)).any(1).float().mean().item()
+    return {
+        "metric": "transform p50/p99 ms and rows/s, served vs plain",
+        "workload": "resident 1M x 784 model; requests of round(lognormal(ln 64, 1)) rows in "
+                    "1..4096 from the same mixture (rows seed 43), Poisson arrivals; exact k=15 "
+                    "kNN (BF16 MFMA prefilter + FP32 rerank), 100-epoch Adam transform",
+        "plain": {"requests": n_plain, "rows_per_s": round(plain_rps, 1),
+                  "p50_ms": round(float(np.percentile(t_plain, 50)), 3),
+                  "p99_ms": round(float(np.percentile(t_plain, 99)), 3)},
+        "served_sweep": sweep, "served_saturation": sat,
+        "speedup_rows_per_s": round(sat["rows_per_s"] / plain_rps, 2),
+        "served_equals_plain": same, "server_setup_s": round(setup_s, 2),
+        "mean_active_requests_per_step": round(stats["mean_active"], 2),
+        "peak_active_requests": stats["peak_active"],
+        "prefilter_recall_at_15": round(recall, 5), "prefilter_candidate_coverage": round(coverage, 5),
+    }
 
 
 def cpu_model():
@@ -197,6 +325,10 @@ def main():
     ap.add_argument("--cpu-epochs", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quality", action="store_true", help="report trustworthiness@15 (10k sample)")
+    ap.add_argument("--no-serve", action="store_true", help="skip the transform serving leg")
+    ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--serve-plain", type=int, default=40)
+    ap.add_argument("--serve-rates", type=float, nargs="*", default=[50.0, 200.0, 800.0])
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only run a plain device copy of GIB GiB (counter calibration)")
     args = ap.parse_args()
@@ -213,8 +345,8 @@ def main():
     if not torch.cuda.is_available():
         print(json.dumps({"error": "no HIP device"}))
         sys.exit(1)
-    x, compact, init, a, b, knn_s, graph_s = build_workload(args.rows, args.dims, 10, 42, 15,
-                                                            args.epochs)
+    x, compact, init, a, b, knn_s, graph_s, index = build_workload(args.rows, args.dims, 10, 42,
+                                                                   15, args.epochs)
     E = len(compact["indices"])
     H = args.rows
     params = L.LayoutParams(epochs=args.epochs, a=a, b=b, seed=42)
@@ -268,6 +400,9 @@ def main():
         tw, pres = trustworthiness_device(x.cpu().numpy(), emb, k=15, sample=10_000)
         out["config"]["trustworthiness@15"] = round(tw, 4)
         out["config"]["knn_preservation@15"] = round(pres, 4)
+    if not args.no_serve:
+        log("transform serving ...")
+        out["serving"] = serve_benchmark(x, emb, index, a, b, args)
     if not args.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(compact, init, a, b, args.epochs, args.cpu_epochs, terms)

@@ -386,7 +386,8 @@ void KnnIndex::reserve(int64_t q) {
 }
 
 void KnnIndex::query(const float *q_dev, int64_t nq, int k, bool exclude_self, int64_t query_offset,
-                     int32_t *idx_out, float *dist_out, hipStream_t s, const float *q_f32_for_rerank) {
+                     int32_t *idx_out, float *dist_out, hipStream_t s, const float *q_f32_for_rerank,
+                     int32_t *cand_ids, int32_t *cand_counts) {
     UMAPHIP_REQUIRE(k >= 1 && k <= kTop, UMAPHIP_INVALID_INPUT, "k must be in 1..=%d", kTop);
     if (nq == 0) return;
     UMAPHIP_REQUIRE(nq <= reserved_q_, UMAPHIP_CAPACITY,
@@ -413,6 +414,14 @@ void KnnIndex::query(const float *q_dev, int64_t nq, int k, bool exclude_self, i
                        cand_d_.ptr, cand_i_.ptr, cand_n_.ptr, nq, slices, top_d_.ptr, top_i_.ptr,
                        top_n_.ptr);
     HIP_CHECK(hipGetLastError());
+    if (cand_ids) {
+        // prefilter lists only (best first), no rerank
+        HIP_CHECK(hipMemcpyAsync(cand_ids, top_i_.ptr, nq * kTop * sizeof(int32_t),
+                                 hipMemcpyDeviceToDevice, s));
+        HIP_CHECK(hipMemcpyAsync(cand_counts, top_n_.ptr, nq * sizeof(int32_t),
+                                 hipMemcpyDeviceToDevice, s));
+        return;
+    }
     hipLaunchKernelGGL(knn_rerank_kernel, dim3(static_cast<unsigned>(nq)), dim3(64), 0, s,
                        q_f32_for_rerank ? q_f32_for_rerank : q_dev, nq, ref_.ptr, n_, d_,
                        top_i_.ptr, top_n_.ptr, k, idx_out, dist_out);
@@ -454,6 +463,17 @@ umaphip_status umaphip_knn_query(umaphip_knn_index *index, const float *queries,
                         "invalid kNN query arguments");
         index->impl->query(queries, q, k, exclude_self != 0, query_offset, indices, distances,
                            as_stream(stream), nullptr);
+    });
+}
+
+umaphip_status umaphip_knn_candidates(umaphip_knn_index *index, const float *queries, int64_t q,
+                                      int32_t *candidates, int32_t *counts, void *stream) {
+    return guarded([&] {
+        UMAPHIP_REQUIRE(index && index->impl, UMAPHIP_INVALID_INPUT, "kNN index is null");
+        UMAPHIP_REQUIRE(q >= 0 && queries && candidates && counts, UMAPHIP_INVALID_INPUT,
+                        "invalid kNN candidate arguments");
+        index->impl->query(queries, q, 1, false, 0, nullptr, nullptr, as_stream(stream), nullptr,
+                           candidates, counts);
     });
 }
 

@@ -18,9 +18,12 @@ public:
     // Grows the query workspace to hold batches of up to q rows.
     void reserve(int64_t q);
     // k <= 64 exact neighbours of q_dev [nq, d] (device f32).
+    // With cand_ids set, writes the merged prefilter lists instead ([nq, 64]
+    // ids best first, counts [nq]) and skips the rerank.
     void query(const float *q_dev, int64_t nq, int k, bool exclude_self, int64_t query_offset,
                int32_t *idx_out, float *dist_out, hipStream_t s,
-               const float *q_f32_for_rerank = nullptr);
+               const float *q_f32_for_rerank = nullptr, int32_t *cand_ids = nullptr,
+               int32_t *cand_counts = nullptr);
     int64_t rows() const { return n_; }
     int64_t dims() const { return d_; }
     const float *reference() const { return ref_.ptr; }

@@ -49,6 +49,21 @@ class KnnIndex:
             torch.cuda.current_stream().cuda_stream))
         return idx, dist
 
+    def candidates_device(self, queries):
+        """The BF16 prefilter's merged candidate lists before the exact rerank:
+        (ids [q, 64] best first, counts [q]) as device tensors."""
+        import torch
+
+        q = queries.contiguous()
+        if q.dtype != torch.float32 or not q.is_cuda or q.shape[1] != self.d:
+            raise ValueError("queries must be a float32 [q, d] tensor on the HIP device")
+        ids = torch.full((q.shape[0], 64), -1, dtype=torch.int32, device=q.device)
+        counts = torch.empty(q.shape[0], dtype=torch.int32, device=q.device)
+        _lib.check(_lib.lib().umaphip_knn_candidates(
+            self._handle, q.data_ptr(), q.shape[0], ids.data_ptr(), counts.data_ptr(),
+            torch.cuda.current_stream().cuda_stream))
+        return ids, counts
+
     def query(self, queries, k: int, *, exclude_self=False, query_offset=0, batch=None):
         """Host arrays in and out; batches of at most max_queries rows."""
         import torch

@@ -229,6 +229,12 @@ umaphip_status umaphip_knn_index_create(const float *reference, int32_t referenc
 umaphip_status umaphip_knn_query(umaphip_knn_index *index, const float *queries, int64_t q,
                                  int32_t k, int32_t exclude_self, int64_t query_offset,
                                  int32_t *indices, float *distances, void *stream);
+/* Diagnostics (no reference counterpart): the BF16 prefilter's merged
+ * candidate lists before the exact rerank, candidates [q, 64] best first by
+ * approximate distance and counts [q] (device pointers); used to report the
+ * prefilter's recall. */
+umaphip_status umaphip_knn_candidates(umaphip_knn_index *index, const float *queries, int64_t q,
+                                      int32_t *candidates, int32_t *counts, void *stream);
 void umaphip_knn_index_destroy(umaphip_knn_index *index);
 
 /* ------------------------------------------------------------------------
