@@ -145,6 +145,13 @@ SIGNATURES = {
     "umaphip_server_wait": (C.c_int, [_vp, _i64, C.c_double, _vp, _vp, _vp]),
     "umaphip_server_stats": (C.c_int, [_vp, _vp]),
     "umaphip_server_destroy": (None, [_vp]),
+    "umaphip_batch_create": (C.c_int, [_i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64,
+                                       _vp, _vp, _vp]),
+    "umaphip_batch_tune": (C.c_int, [_vp, _i32, _vp, _vp]),
+    "umaphip_batch_set_shape": (C.c_int, [_vp, _i32]),
+    "umaphip_batch_run": (C.c_int, [_vp, _i32]),
+    "umaphip_batch_download": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "umaphip_batch_destroy": (None, [_vp]),
     "umaphip_classic_optimize": (C.c_int, [_i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _i64, _i32,
                                            _i32, C.c_float, C.c_float, C.c_float, C.c_float, _i32,
                                            C.c_uint64, _i32, _vp, _vp, _vp]),

@@ -1,0 +1,343 @@
+// Batched layouts: many independent fit layouts (graphs of uneven size) run
+// as one launch per epoch, and the whole epoch loop is captured once as a
+// hipGraph and replayed.
+//
+// Each graph is a modern Adam fit in global negative mode (its rows do not
+// exceed negative_selection_range), so everything that is not a position is
+// known before the first epoch: the per-epoch negative orders come from the
+// graph's own StdRng(seed) shuffles (modern_orders.rs:359-680) and are
+// generated on host threads up front, and each epoch's arguments (schedule
+// value, bias corrections, order, Jacobi buffers) are fixed. Every epoch is
+// one segment-sweep launch in which each graph is a segment of tiles; a
+// graph's rows are computed exactly as a launch of that graph alone would
+// compute them. The tile shape (rows per tile x lanes) is chosen by timing
+// the candidate shapes on the first epochs (umaphip_batch_tune).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <thread>
+#include <vector>
+
+#include "common.hpp"
+#include "layout_host.hpp"
+#include "layout_kernels.hpp"
+#include "stdrng.hpp"
+
+using namespace umaphip;
+
+struct umaphip_batch {
+    int32_t G = 0;
+    int64_t d = 2, total_rows = 0, total_edges = 0, total_order = 0;
+    int epochs = 0;
+    umaphip_layout_params p{};
+    std::vector<int64_t> rows, edges, row_start, edge_start, off_start, order_start;
+    std::vector<float> scale;
+    std::vector<uint32_t> range;
+    std::vector<float> alpha, mu, c1, c2;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    DeviceArray<float> pos[2], m, v, head0;
+    DeviceArray<int32_t> offsets, indices, orders;
+    DeviceArray<double> periods, negperiods, npos, nneg, npos0, nneg0;
+    DeviceArray<SweepSegment> segs;
+    int shape = 0;
+    int64_t tiles = 0;
+    bool built = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    float last_ms = 0.0f;
+    bool ran = false;
+
+    ~umaphip_batch() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    // Segment arguments of every epoch for a tile shape.
+    void build(int s) {
+        if (built && shape == s) return;
+        if (exec) {
+            HIP_CHECK(hipGraphExecDestroy(exec));
+            exec = nullptr;
+        }
+        if (graph) {
+            HIP_CHECK(hipGraphDestroy(graph));
+            graph = nullptr;
+        }
+        shape = s;
+        const int64_t tr = tile_rows(s);
+        std::vector<SweepSegment> host(static_cast<size_t>(epochs) * G);
+        tiles = 0;
+        std::vector<int64_t> tile0(G);
+        for (int32_t g = 0; g < G; ++g) {
+            tile0[g] = tiles;
+            tiles += (rows[g] + tr - 1) / tr;
+        }
+        for (int t = 0; t < epochs; ++t) {
+            for (int32_t g = 0; g < G; ++g) {
+                SweepArgs A{};
+                const int64_t r0 = row_start[g], e0 = edge_start[g];
+                A.h = rows[g];
+                A.n = rows[g];
+                A.pos = pos[t & 1].ptr + r0 * d;
+                A.pos_out = pos[(t + 1) & 1].ptr + r0 * d;
+                A.tgt = A.pos;
+                A.table = nullptr;
+                A.order = orders.ptr + order_start[g] + static_cast<int64_t>(t) * rows[g];
+                A.ranks = nullptr;
+                A.offsets = offsets.ptr + off_start[g];
+                A.indices = indices.ptr + e0;
+                A.periods = periods.ptr + e0;
+                A.next_positive = npos.ptr + e0;
+                A.negative_periods = negperiods.ptr + e0;
+                A.next_negative = nneg.ptr + e0;
+                A.m = m.ptr + r0 * d;
+                A.v = v.ptr + r0 * d;
+                A.scale = scale[g];
+                A.epoch = static_cast<uint32_t>(t);
+                A.a = p.a; A.b = p.b; A.gamma = p.repulsion; A.range = range[g];
+                A.multiplier = 1.0f; A.alpha = alpha[t]; A.c1 = c1[t]; A.c2 = c2[t];
+                A.shared = 1; A.negatives = p.negative_sample_rate != 0;
+                A.exclude = p.exclude_graph_neighbors; A.fused = 1;
+                host[static_cast<size_t>(t) * G + g].args = A;
+                host[static_cast<size_t>(t) * G + g].tile0 = tile0[g];
+            }
+        }
+        segs.alloc(std::max<size_t>(host.size(), 1));
+        if (!host.empty())
+            HIP_CHECK(hipMemcpy(segs.ptr, host.data(), host.size() * sizeof(SweepSegment),
+                                hipMemcpyHostToDevice));
+        built = true;
+    }
+
+    void reset() {
+        HIP_CHECK(hipMemcpyAsync(pos[0].ptr, head0.ptr, head0.bytes(), hipMemcpyDeviceToDevice, stream));
+        HIP_CHECK(hipMemsetAsync(m.ptr, 0, m.bytes(), stream));
+        HIP_CHECK(hipMemsetAsync(v.ptr, 0, v.bytes(), stream));
+        if (total_edges) {
+            HIP_CHECK(hipMemcpyAsync(npos.ptr, npos0.ptr, npos0.bytes(), hipMemcpyDeviceToDevice, stream));
+            HIP_CHECK(hipMemcpyAsync(nneg.ptr, nneg0.ptr, nneg0.bytes(), hipMemcpyDeviceToDevice, stream));
+        }
+    }
+
+    void enqueue_epochs(int count) {
+        for (int t = 0; t < count; ++t)
+            launch_segment_sweep(segs.ptr + static_cast<size_t>(t) * G, G, tiles, stream, shape);
+    }
+
+    void capture() {
+        if (exec) return;
+        HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        enqueue_epochs(epochs);
+        HIP_CHECK(hipStreamEndCapture(stream, &graph));
+        HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    }
+
+    void run(bool use_graph) {
+        if (!built) build(shape);
+        if (use_graph) capture();
+        reset();
+        HIP_CHECK(hipEventRecord(ev0, stream));
+        if (use_graph) HIP_CHECK(hipGraphLaunch(exec, stream));
+        else enqueue_epochs(epochs);
+        HIP_CHECK(hipEventRecord(ev1, stream));
+        ran = true;
+    }
+};
+
+extern "C" {
+
+umaphip_status umaphip_batch_create(int32_t G, const int64_t *rows, const int64_t *n_edges,
+                                    const uint32_t *offsets, const uint32_t *indices,
+                                    const double *periods, const double *next_positive,
+                                    const double *negative_periods, const double *next_negative,
+                                    const float *head, int64_t d,
+                                    const umaphip_layout_params *params, const uint64_t *seeds,
+                                    umaphip_batch **out) {
+    return guarded([&] {
+        UMAPHIP_REQUIRE(out && rows && n_edges && offsets && head && params && seeds,
+                        UMAPHIP_INVALID_INPUT, "batch arguments are null");
+        *out = nullptr;
+        UMAPHIP_REQUIRE(G >= 1, UMAPHIP_INVALID_INPUT, "a batch needs at least one graph");
+        UMAPHIP_REQUIRE(d == 2, UMAPHIP_INVALID_CONFIG, "batched layouts support 2-D embeddings");
+        const umaphip_layout_params &p = *params;
+        UMAPHIP_REQUIRE(p.optimizer == 0, UMAPHIP_INVALID_CONFIG, "batched layouts run Adam");
+        UMAPHIP_REQUIRE(p.epochs >= 0 && p.negative_selection_range > 0 &&
+                            p.negative_sample_rate >= 0,
+                        UMAPHIP_INVALID_CONFIG, "invalid layout parameters");
+        std::unique_ptr<umaphip_batch> B(new umaphip_batch());
+        B->G = G; B->d = d; B->p = p; B->epochs = p.epochs;
+        B->rows.assign(rows, rows + G);
+        B->edges.assign(n_edges, n_edges + G);
+        B->row_start.resize(G + 1);
+        B->edge_start.resize(G + 1);
+        B->off_start.resize(G + 1);
+        B->order_start.resize(G + 1);
+        B->row_start[0] = B->edge_start[0] = B->off_start[0] = B->order_start[0] = 0;
+        const uint64_t Q = static_cast<uint64_t>(p.negative_selection_range);
+        for (int32_t g = 0; g < G; ++g) {
+            const int64_t h = rows[g], E = n_edges[g];
+            UMAPHIP_REQUIRE(h >= 1 && h <= INT32_MAX && E >= 0 && E <= INT32_MAX,
+                            UMAPHIP_INVALID_INPUT, "graph %d: invalid sizes", g);
+            UMAPHIP_REQUIRE(static_cast<uint64_t>(h) <= Q, UMAPHIP_INVALID_CONFIG,
+                            "graph %d: %lld rows exceed negative_selection_range; batched "
+                            "layouts run global negative sampling", g, (long long)h);
+            const uint32_t *off = offsets + B->off_start[g];
+            UMAPHIP_REQUIRE(off[0] == 0 && static_cast<int64_t>(off[h]) == E, UMAPHIP_INVALID_INPUT,
+                            "graph %d: CSR offsets must run from 0 to E", g);
+            for (int64_t i = 0; i < h; ++i)
+                UMAPHIP_REQUIRE(off[i] <= off[i + 1], UMAPHIP_INVALID_INPUT,
+                                "graph %d: CSR offsets must be nondecreasing", g);
+            for (int64_t e = 0; e < E; ++e)
+                UMAPHIP_REQUIRE(static_cast<int64_t>(indices[B->edge_start[g] + e]) < h,
+                                UMAPHIP_INVALID_INPUT, "graph %d: CSR index out of range", g);
+            B->row_start[g + 1] = B->row_start[g] + h;
+            B->edge_start[g + 1] = B->edge_start[g] + E;
+            B->off_start[g + 1] = B->off_start[g] + h + 1;
+            B->order_start[g + 1] = B->order_start[g] + h * std::max(p.epochs, 1);
+            B->range.push_back(static_cast<uint32_t>(effective_range(Q, h)));
+            B->scale.push_back(resolved_scale(Q, h, p.has_scale != 0, p.scale,
+                                              p.adaptation_samples > 0));
+        }
+        B->total_rows = B->row_start[G];
+        B->total_edges = B->edge_start[G];
+        B->total_order = B->order_start[G];
+        schedules(p.epochs, p.initial_alpha, p.good_initialization != 0, false, B->alpha, B->mu);
+        bias_corrections(p.epochs, B->c1, B->c2);
+
+        // negative orders of every epoch, one host thread per group of graphs
+        std::vector<int32_t> orders(static_cast<size_t>(std::max<int64_t>(B->total_order, 1)));
+        {
+            const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+            std::vector<std::thread> pool;
+            for (unsigned w = 0; w < hw; ++w)
+                pool.emplace_back([&, w] {
+                    for (int32_t g = static_cast<int32_t>(w); g < G; g += static_cast<int32_t>(hw)) {
+                        const int64_t h = B->rows[g];
+                        OrderReplay rep(h, h, seeds[g], false, false, false);
+                        for (int t = 0; t < p.epochs; ++t) {
+                            std::memcpy(orders.data() + B->order_start[g] + t * h, rep.neg().data(),
+                                        h * sizeof(int32_t));
+                            rep.advance();
+                        }
+                    }
+                });
+            for (auto &t : pool) t.join();
+        }
+        HIP_CHECK(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
+        HIP_CHECK(hipEventCreate(&B->ev0));
+        HIP_CHECK(hipEventCreate(&B->ev1));
+        const size_t hd = static_cast<size_t>(B->total_rows * d);
+        B->head0.alloc(hd);
+        B->pos[0].alloc(hd);
+        B->pos[1].alloc(hd);
+        B->m.alloc(hd);
+        B->v.alloc(hd);
+        HIP_CHECK(hipMemcpy(B->head0.ptr, head, hd * sizeof(float), hipMemcpyHostToDevice));
+        B->offsets.alloc(B->off_start[G]);
+        HIP_CHECK(hipMemcpy(B->offsets.ptr, offsets, B->off_start[G] * sizeof(int32_t),
+                            hipMemcpyHostToDevice));
+        const size_t Ec = static_cast<size_t>(std::max<int64_t>(B->total_edges, 1));
+        B->indices.alloc(Ec);
+        B->periods.alloc(Ec);
+        B->negperiods.alloc(Ec);
+        B->npos.alloc(Ec);
+        B->nneg.alloc(Ec);
+        B->npos0.alloc(Ec);
+        B->nneg0.alloc(Ec);
+        if (B->total_edges) {
+            const size_t E = static_cast<size_t>(B->total_edges);
+            HIP_CHECK(hipMemcpy(B->indices.ptr, indices, E * 4, hipMemcpyHostToDevice));
+            HIP_CHECK(hipMemcpy(B->periods.ptr, periods, E * 8, hipMemcpyHostToDevice));
+            HIP_CHECK(hipMemcpy(B->negperiods.ptr, negative_periods, E * 8, hipMemcpyHostToDevice));
+            HIP_CHECK(hipMemcpy(B->npos0.ptr, next_positive, E * 8, hipMemcpyHostToDevice));
+            HIP_CHECK(hipMemcpy(B->nneg0.ptr, next_negative, E * 8, hipMemcpyHostToDevice));
+        }
+        B->orders.alloc(orders.size());
+        HIP_CHECK(hipMemcpy(B->orders.ptr, orders.data(), orders.size() * sizeof(int32_t),
+                            hipMemcpyHostToDevice));
+        *out = B.release();
+    });
+}
+
+umaphip_status umaphip_batch_tune(umaphip_batch *batch, int32_t probe_epochs, int32_t *shape_out,
+                                  double *ms_out) {
+    return guarded([&] {
+        UMAPHIP_REQUIRE(batch && shape_out, UMAPHIP_INVALID_INPUT, "batch tune arguments are null");
+        const int probe = std::max(1, std::min(probe_epochs, batch->epochs));
+        double best = 1e300;
+        int best_shape = 0;
+        for (int s = 0; s < kTileShapes; ++s) {
+            batch->build(s);
+            batch->reset();
+            batch->enqueue_epochs(probe);   // warm
+            batch->reset();
+            HIP_CHECK(hipEventRecord(batch->ev0, batch->stream));
+            batch->enqueue_epochs(probe);
+            HIP_CHECK(hipEventRecord(batch->ev1, batch->stream));
+            HIP_CHECK(hipEventSynchronize(batch->ev1));
+            float ms = 0.0f;
+            HIP_CHECK(hipEventElapsedTime(&ms, batch->ev0, batch->ev1));
+            if (ms_out) ms_out[s] = ms;
+            if (ms < best) {
+                best = ms;
+                best_shape = s;
+            }
+        }
+        batch->build(best_shape);
+        batch->ran = false;
+        *shape_out = best_shape;
+    });
+}
+
+umaphip_status umaphip_batch_set_shape(umaphip_batch *batch, int32_t shape) {
+    return guarded([&] {
+        UMAPHIP_REQUIRE(batch, UMAPHIP_INVALID_INPUT, "batch is null");
+        UMAPHIP_REQUIRE(shape >= 0 && shape < kTileShapes, UMAPHIP_INVALID_CONFIG,
+                        "tile shape must be in 0..%d", kTileShapes - 1);
+        batch->build(shape);
+    });
+}
+
+umaphip_status umaphip_batch_run(umaphip_batch *batch, int32_t use_graph) {
+    return guarded([&] {
+        UMAPHIP_REQUIRE(batch, UMAPHIP_INVALID_INPUT, "batch is null");
+        batch->run(use_graph != 0);
+    });
+}
+
+umaphip_status umaphip_batch_download(umaphip_batch *batch, float *head, double *next_positive,
+                                      double *next_negative, double *device_ms) {
+    return guarded([&] {
+        UMAPHIP_REQUIRE(batch, UMAPHIP_INVALID_INPUT, "batch is null");
+        HIP_CHECK(hipStreamSynchronize(batch->stream));
+        if (device_ms) {
+            float ms = 0.0f;
+            if (batch->ran) HIP_CHECK(hipEventElapsedTime(&ms, batch->ev0, batch->ev1));
+            *device_ms = ms;
+        }
+        const float *src = batch->ran ? batch->pos[batch->epochs & 1].ptr : batch->head0.ptr;
+        if (head)
+            HIP_CHECK(hipMemcpy(head, src, batch->total_rows * batch->d * sizeof(float),
+                                hipMemcpyDeviceToHost));
+        if (batch->total_edges) {
+            if (next_positive)
+                HIP_CHECK(hipMemcpy(next_positive, batch->ran ? batch->npos.ptr : batch->npos0.ptr,
+                                    batch->total_edges * sizeof(double), hipMemcpyDeviceToHost));
+            if (next_negative)
+                HIP_CHECK(hipMemcpy(next_negative, batch->ran ? batch->nneg.ptr : batch->nneg0.ptr,
+                                    batch->total_edges * sizeof(double), hipMemcpyDeviceToHost));
+        }
+    });
+}
+
+void umaphip_batch_destroy(umaphip_batch *batch) { delete batch; }
+
+}  // extern "C"

@@ -348,6 +348,7 @@ __device__ inline uint64_t mod_u64(uint64_t x, uint64_t m, double inv_m) {
     return static_cast<uint64_t>(r);
 }
 
+template <int WAVES>
 __device__ inline int block_exclusive_scan(int v, int tid, int *s_wave, int &total) {
     const int lane = tid & 63, wave = tid >> 6;
     int incl = v;
@@ -359,28 +360,36 @@ __device__ inline int block_exclusive_scan(int v, int tid, int *s_wave, int &tot
     __syncthreads();
     int before = 0;
     for (int w = 0; w < wave; ++w) before += s_wave[w];
-    total = s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+    total = 0;
+#pragma unroll
+    for (int w = 0; w < WAVES; ++w) total += s_wave[w];
     return before + incl - v;
 }
 
+// ROWS rows per tile, THREADS lanes per workgroup (chunks of THREADS edges).
+template <int ROWS, int THREADS>
 __device__ inline void sweep_tile(const SweepArgs &A, int64_t tile) {
-    __shared__ int32_t s_off[kTileRows + 1];   // tile-local edge prefix over the tile's rows
-    __shared__ int32_t s_seg[kTileRows];       // CSR start of each tile row
-    __shared__ int32_t s_rowid[kTileRows];
-    __shared__ float2 s_pos[kTileRows];
-    __shared__ uint32_t s_first[kTileRows];
-    __shared__ int8_t s_active[kTileRows];
-    __shared__ float2 s_terms[kTermCap];
-    __shared__ uint8_t s_owner[kTermCap];
-    __shared__ int32_t s_base[kThreads + 1];
-    __shared__ int32_t s_j[kThreads];
-    __shared__ uint8_t s_row[kThreads];
-    __shared__ uint32_t s_em[kThreads];
-    __shared__ int s_wave[4], s_wave2[4];
+    constexpr int TCAP = 8 * THREADS;
+    constexpr int WAVES = THREADS / 64;
+    static_assert(ROWS <= 64 && ROWS <= THREADS && THREADS % 64 == 0 && THREADS <= 256,
+                  "tile shape");
+    __shared__ int32_t s_off[ROWS + 1];   // tile-local edge prefix over the tile's rows
+    __shared__ int32_t s_seg[ROWS];       // CSR start of each tile row
+    __shared__ int32_t s_rowid[ROWS];
+    __shared__ float2 s_pos[ROWS];
+    __shared__ uint32_t s_first[ROWS];
+    __shared__ int8_t s_active[ROWS];
+    __shared__ float2 s_terms[TCAP];
+    __shared__ uint8_t s_owner[TCAP];
+    __shared__ int32_t s_base[THREADS + 1];
+    __shared__ int32_t s_j[THREADS];
+    __shared__ uint8_t s_row[THREADS];
+    __shared__ uint32_t s_em[THREADS];
+    __shared__ int s_wave[WAVES], s_wave2[WAVES];
 
     const int tid = threadIdx.x;
-    const int64_t row0 = tile * kTileRows;
-    const int rows = static_cast<int>(min<int64_t>(kTileRows, A.h - row0));
+    const int64_t row0 = tile * ROWS;
+    const int rows = static_cast<int>(min<int64_t>(ROWS, A.h - row0));
     const bool small = A.h <= 1024;
     const uint64_t n64 = static_cast<uint64_t>(A.n);
     const uint64_t modulus = small ? n64 : A.range;
@@ -413,7 +422,7 @@ __device__ inline void sweep_tile(const SweepArgs &A, int64_t tile) {
             const int up = __shfl_up(incl, off, 64);
             if (tid >= off) incl += up;
         }
-        s_off[tid + 1] = incl;
+        if (tid < ROWS) s_off[tid + 1] = incl;
         if (tid == 0) s_off[0] = 0;
     }
     const float scale = A.scale_ptr ? *A.scale_ptr : A.scale;
@@ -451,14 +460,14 @@ __device__ inline void sweep_tile(const SweepArgs &A, int64_t tile) {
     };
     prefetch(e_begin);
 
-    for (int32_t base = e_begin; base < e_end; base += kThreads) {
+    for (int32_t base = e_begin; base < e_end; base += THREADS) {
         // ---- A: clocks and term counts
         const bool has_edge = base + tid < e_end;
         const int32_t e = pf_e;
         const int r_cur = pf_r;
         const double npos = pf_npos, per = pf_per, nn = pf_nneg, np = pf_nper;
         const int32_t jidx = pf_idx;
-        if (base + kThreads < e_end) prefetch(base + kThreads);
+        if (base + THREADS < e_end) prefetch(base + THREADS);
         int terms = 0, count = 0, r = 0;
         float2 retry_sum = make_float2(0.0f, 0.0f);
         if (has_edge) {
@@ -499,12 +508,12 @@ __device__ inline void sweep_tile(const SweepArgs &A, int64_t tile) {
         }
         s_row[tid] = static_cast<uint8_t>(r);
         int total;
-        const int my_base = block_exclusive_scan(terms, tid, s_wave, total);
-        const bool term_parallel = total <= kTermCap;
+        const int my_base = block_exclusive_scan<WAVES>(terms, tid, s_wave, total);
+        const bool term_parallel = total <= TCAP;
         // Chunks with more terms than LDS holds fall back to one combined
         // term per edge, computed by the edge's own lane.
         int fb_total;
-        const int fb_base = block_exclusive_scan(terms > 0 ? 1 : 0, tid, s_wave2, fb_total);
+        const int fb_base = block_exclusive_scan<WAVES>(terms > 0 ? 1 : 0, tid, s_wave2, fb_total);
         (void)fb_total;
         if (term_parallel) {
             s_base[tid] = my_base;
@@ -513,18 +522,18 @@ __device__ inline void sweep_tile(const SweepArgs &A, int64_t tile) {
         } else {
             s_base[tid] = fb_base;
         }
-        if (tid == kThreads - 1) s_base[kThreads] = term_parallel ? total : fb_base + (terms > 0);
+        if (tid == THREADS - 1) s_base[THREADS] = term_parallel ? total : fb_base + (terms > 0);
         __syncthreads();
 
         // ---- B: force terms (two per thread in flight)
         if (term_parallel) {
-            for (int item0 = tid; item0 < total; item0 += 2 * kThreads) {
+            for (int item0 = tid; item0 < total; item0 += 2 * THREADS) {
                 float2 y[2], xi[2];
                 int kk[2];
                 bool live[2], keep[2];
 #pragma unroll
                 for (int u = 0; u < 2; ++u) {
-                    const int item = item0 + u * kThreads;
+                    const int item = item0 + u * THREADS;
                     live[u] = item < total;
                     keep[u] = live[u];
                     kk[u] = 0;
@@ -573,7 +582,7 @@ __device__ inline void sweep_tile(const SweepArgs &A, int64_t tile) {
                         const float c = repel_fast(q, A.a, A.b, A.gamma, s_neg);
                         term = make_float2(c * dx, c * dy);
                     }
-                    s_terms[item0 + u * kThreads] = term;
+                    s_terms[item0 + u * THREADS] = term;
                 }
             }
         } else if (terms > 0) {
@@ -621,7 +630,7 @@ __device__ inline void sweep_tile(const SweepArgs &A, int64_t tile) {
         // ---- C: per-row sums in edge (and term) order
         if (tid < rows) {
             const int32_t lo = max(s_off[tid], base) - base;
-            const int32_t hi = min(s_off[tid + 1], base + kThreads) - base;
+            const int32_t hi = min(s_off[tid + 1], base + THREADS) - base;
             if (lo < hi) {
                 const int t_lo = s_base[lo], t_hi = s_base[hi];
                 for (int k = t_lo; k < t_hi; ++k) {
@@ -664,15 +673,16 @@ __device__ inline void sweep_tile(const SweepArgs &A, int64_t tile) {
 }
 
 __global__ __launch_bounds__(kThreads) void edge_sweep_kernel(SweepArgs A) {
-    sweep_tile(A, blockIdx.x);
+    sweep_tile<kTileRows, kThreads>(A, blockIdx.x);
 }
 
 // Many independent layouts in one launch: workgroup b runs tile
 // b - tile0[s] of the segment s whose tile range holds b (segments are in
 // ascending tile0 order). Each segment carries its own arguments, so its
 // rows are computed exactly as a launch of that segment alone would.
-__global__ __launch_bounds__(kThreads) void segment_sweep_kernel(const SweepSegment *segs,
-                                                                 int32_t n_segs) {
+template <int ROWS, int THREADS>
+__global__ __launch_bounds__(THREADS) void segment_sweep_kernel(const SweepSegment *segs,
+                                                                int32_t n_segs) {
     const int64_t b = blockIdx.x;
     int lo = 0, hi = n_segs;
     while (hi - lo > 1) {
@@ -682,8 +692,8 @@ __global__ __launch_bounds__(kThreads) void segment_sweep_kernel(const SweepSegm
     }
     const SweepArgs A = segs[lo].args;
     const int64_t tile = b - segs[lo].tile0;
-    if (tile * kTileRows >= A.h) return;
-    sweep_tile(A, tile);
+    if (tile * ROWS >= A.h) return;
+    sweep_tile<ROWS, THREADS>(A, tile);
 }
 
 // ------------------------------------------------- edge-balanced unit sweep
@@ -1370,13 +1380,30 @@ void launch_unit_sweep(UnitSweepArgs a, hipStream_t s) {
     HIP_CHECK(hipGetLastError());
 }
 
-int64_t tile_rows() { return kTileRows; }
+int64_t tile_rows(int shape) {
+    switch (shape) {
+        case 1: return 32;
+        case 2: return 16;
+        default: return kTileRows;
+    }
+}
 
 void launch_segment_sweep(const SweepSegment *segs, int32_t n_segs, int64_t total_tiles,
-                          hipStream_t s) {
+                          hipStream_t s, int shape) {
     if (n_segs == 0 || total_tiles == 0) return;
-    hipLaunchKernelGGL(segment_sweep_kernel, dim3(static_cast<unsigned>(total_tiles)),
-                       dim3(kThreads), 0, s, segs, n_segs);
+    const dim3 grid(static_cast<unsigned>(total_tiles));
+    switch (shape) {
+        case 1:
+            hipLaunchKernelGGL((segment_sweep_kernel<32, 128>), grid, dim3(128), 0, s, segs, n_segs);
+            break;
+        case 2:
+            hipLaunchKernelGGL((segment_sweep_kernel<16, 64>), grid, dim3(64), 0, s, segs, n_segs);
+            break;
+        default:
+            hipLaunchKernelGGL((segment_sweep_kernel<kTileRows, kThreads>), grid, dim3(kThreads), 0,
+                               s, segs, n_segs);
+            break;
+    }
     HIP_CHECK(hipGetLastError());
 }
 

@@ -66,9 +66,11 @@ struct SweepSegment {
     SweepArgs args;
     int64_t tile0;
 };
-int64_t tile_rows();
+// Tile shapes: 0 = 64 rows x 256 lanes, 1 = 32 x 128, 2 = 16 x 64.
+constexpr int kTileShapes = 3;
+int64_t tile_rows(int shape = 0);
 void launch_segment_sweep(const SweepSegment *segs, int32_t n_segs, int64_t total_tiles,
-                          hipStream_t s);
+                          hipStream_t s, int shape = 0);
 
 // Edge-balanced unit sweep for Adam layouts with H > 1024 and d = 2: one
 // wave per unit_edges() consecutive CSR edges; writes per-row gradients

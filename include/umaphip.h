@@ -378,6 +378,40 @@ umaphip_status umaphip_server_wait(umaphip_server *server, int64_t ticket, doubl
 umaphip_status umaphip_server_stats(umaphip_server *server, umaphip_server_counters *stats);
 void umaphip_server_destroy(umaphip_server *server);
 
+/* ------------------------------------------------------------------------
+ * Batched layouts (no reference counterpart: many independent
+ * optimize_compact fits, crates/umap/src/standard_umap/modern_runtime.rs:
+ * 1011-1378, in one launch per epoch). Every graph is an Adam fit in global
+ * negative mode (rows <= negative_selection_range) with its own seed; the
+ * per-epoch orders are generated up front and the whole epoch loop is
+ * captured once as a hipGraph. Graph g's arrays are concatenated: offsets
+ * holds rows[g] + 1 local entries per graph, indices/clocks n_edges[g]
+ * local entries, head rows[g] * d floats.
+ * ------------------------------------------------------------------------ */
+typedef struct umaphip_batch umaphip_batch;
+umaphip_status umaphip_batch_create(int32_t graphs, const int64_t *rows, const int64_t *n_edges,
+                                    const uint32_t *offsets, const uint32_t *indices,
+                                    const double *periods, const double *next_positive,
+                                    const double *negative_periods, const double *next_negative,
+                                    const float *head, int64_t d,
+                                    const umaphip_layout_params *params, const uint64_t *seeds,
+                                    umaphip_batch **out);
+/* Autotuner: times every tile shape (0: 64 rows x 256 lanes, 1: 32 x 128,
+ * 2: 16 x 64 -- one wave per tile) over the first probe_epochs epochs and
+ * keeps the fastest; ms_out (optional) receives the three timings. */
+umaphip_status umaphip_batch_tune(umaphip_batch *batch, int32_t probe_epochs, int32_t *shape_out,
+                                  double *ms_out);
+umaphip_status umaphip_batch_set_shape(umaphip_batch *batch, int32_t shape);
+/* Resets to the uploaded initial state and enqueues every epoch: as one
+ * hipGraph launch (use_graph = 1, captured on first use) or as plain
+ * launches. Does not synchronise. */
+umaphip_status umaphip_batch_run(umaphip_batch *batch, int32_t use_graph);
+/* Synchronises; copies the concatenated head (and optionally clocks) to
+ * host; device_ms = duration of the last run's epochs. */
+umaphip_status umaphip_batch_download(umaphip_batch *batch, float *head, double *next_positive,
+                                      double *next_negative, double *device_ms);
+void umaphip_batch_destroy(umaphip_batch *batch);
+
 #ifdef __cplusplus
 }
 #endif
