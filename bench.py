@@ -254,6 +254,85 @@ def serve_benchmark(x, emb, compact_index, a, b, args):
     }
 
 
+def batch_benchmark(args):
+    """256 independent fits: rows round(lognormal(ln 5000 - 0.5, 1)) clipped to
+    500..50000, 32-D mixtures of U{5..20} unit-variance clusters (centres in
+    [-10, 10]), exact k=15 graphs, seeds 42+g, 200 Adam epochs (global
+    negatives). Batched (autotuned tile shape, one hipGraph launch) vs the
+    same layouts as sequential single-graph plans."""
+    import torch
+
+    from cuscan_amd import graph as G
+    from cuscan_amd import layout as L
+    from cuscan_amd.batch import SHAPES, LayoutBatch
+    from cuscan_amd.knn import KnnIndex
+
+    n_graphs, epochs = args.batch_graphs, 200
+    t0 = time.time()
+    graphs, heads, seeds = [], [], []
+    size_rng = np.random.default_rng(42)
+    sizes = np.clip(np.round(size_rng.lognormal(np.log(5000) - 0.5, 1.0, n_graphs)), 500,
+                    50_000).astype(int)
+    for g in range(n_graphs):
+        rng = np.random.default_rng(42 + g)
+        c = int(rng.integers(5, 21))
+        centers = rng.uniform(-10, 10, (c, 32))
+        x = (centers[rng.integers(0, c, sizes[g])] + rng.normal(0, 1, (sizes[g], 32))).astype(np.float32)
+        index = KnnIndex(torch.from_numpy(x).cuda(), max_queries=int(sizes[g]))
+        idx, dist = index.query(x, 14, exclude_self=True)
+        index.close()
+        n = sizes[g]
+        indices = np.concatenate([np.arange(n, dtype=np.int32)[:, None], idx], 1)
+        distances = np.concatenate([np.zeros((n, 1), np.float32), dist], 1)
+        fg = G.fuzzy_graph(indices, distances)
+        graphs.append(G.compact_graph(n, n, fg["heads"], fg["tails"], fg["weights"], epochs=epochs,
+                                      rate=5.0, prune_divisor=float(epochs)))
+        heads.append(G.random_init(n, 2, 42 + g))
+        seeds.append(42 + g)
+    a, b = G.fit_curve(0.1, 1.0)
+    build_s = time.time() - t0
+    total_e = sum(len(gr["indices"]) for gr in graphs)
+    log(f"batched workload: {n_graphs} graphs, {int(sizes.sum())} rows, {total_e} edges "
+        f"({build_s:.1f}s)")
+    params = L.LayoutParams(epochs=epochs, a=a, b=b)
+    batch = LayoutBatch(graphs, heads, params, seeds)
+    shape, shape_ms = batch.tune(probe_epochs=20)
+    batch.run(use_graph=True)   # capture + warm
+    batch.download()
+    times = []
+    for _ in range(args.batch_steps):
+        t1 = time.perf_counter()
+        batch.run(use_graph=True)
+        _, dev_ms = batch.download()
+        times.append(time.perf_counter() - t1)
+    batched_s = min(times)
+    batch.close()
+    # the same layouts one graph at a time (resident plans; runs timed)
+    plans = [L.LayoutPlan(h, None, gr, L.LayoutParams(epochs=epochs, a=a, b=b, seed=sd), shared=True)
+             for gr, h, sd in zip(graphs, heads, seeds)]
+    plans[0].run()
+    plans[0].download()
+    t1 = time.perf_counter()
+    for pl in plans:
+        pl.run()
+        pl.download()
+    seq_s = time.perf_counter() - t1
+    for pl in plans:
+        pl.close()
+    value = total_e * epochs / batched_s
+    seq = total_e * epochs / seq_s
+    log(f"batched {batched_s * 1e3:.1f} ms ({value:.3e} edge-epochs/s, shape {SHAPES[shape]}) vs "
+        f"sequential {seq_s * 1e3:.1f} ms ({seq:.3e})")
+    return {"metric": "batched layouts edge-epochs/s (sum E_g * 200 / t)",
+            "workload": f"{n_graphs} graphs, rows lognormal(ln 5000 - 0.5, 1) in 500..50000 "
+                        "(32-D, U{5..20} clusters, k=15, seeds 42+g), 200 Adam epochs",
+            "graphs": n_graphs, "rows": int(sizes.sum()), "edges": int(total_e),
+            "value": value, "ms": round(batched_s * 1e3, 2), "device_ms": round(dev_ms, 2),
+            "sequential_value": seq, "sequential_ms": round(seq_s * 1e3, 2),
+            "speedup": round(seq_s / batched_s, 2), "tile_shape": SHAPES[shape],
+            "tune_ms": [round(t, 3) for t in shape_ms], "one_hipgraph_launch": True}
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -326,6 +405,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quality", action="store_true", help="report trustworthiness@15 (10k sample)")
     ap.add_argument("--no-serve", action="store_true", help="skip the transform serving leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-layouts leg")
+    ap.add_argument("--batch-graphs", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--serve-plain", type=int, default=40)
     ap.add_argument("--serve-rates", type=float, nargs="*", default=[50.0, 200.0, 800.0])
@@ -400,6 +482,9 @@ def main():
         tw, pres = trustworthiness_device(x.cpu().numpy(), emb, k=15, sample=10_000)
         out["config"]["trustworthiness@15"] = round(tw, 4)
         out["config"]["knn_preservation@15"] = round(pres, 4)
+    if not args.no_batch:
+        log("batched layouts ...")
+        out["batched"] = batch_benchmark(args)
     if not args.no_serve:
         log("transform serving ...")
         out["serving"] = serve_benchmark(x, emb, index, a, b, args)
