@@ -140,6 +140,11 @@ SIGNATURES = {
     "umaphip_random_init": (C.c_int, [_i64, _i64, C.c_uint64, C.c_float, C.c_float, _i32, _vp]),
     "umaphip_transform_init": (C.c_int, [_vp, _vp, _i64, _i64, _vp, _i64, _i64, _vp]),
     "umaphip_classic_epoch": (C.c_int, [_vp, _vp]),
+    "umaphip_plan_set_objective": (C.c_int, [_vp, _vp, _i32]),
+    "umaphip_plan_objective": (C.c_int, [_vp, _vp, _i64, _vp]),
+    "umaphip_cross_entropy_workspace_bytes": (C.c_int, [_i64, _vp]),
+    "umaphip_cross_entropy": (C.c_int, [_vp, _i64, _vp, _i64, _i64, _vp, _vp, _vp, C.c_float,
+                                        C.c_float, _vp, C.c_size_t, _vp, _vp]),
     "umaphip_server_create": (C.c_int, [_vp, _i64, _i64, _vp, _i64, _vp, _vp]),
     "umaphip_server_submit": (C.c_int, [_vp, _vp, _i64, _vp]),
     "umaphip_server_wait": (C.c_int, [_vp, _i64, C.c_double, _vp, _vp, _vp]),

@@ -235,6 +235,13 @@ struct umaphip_plan {
     std::vector<hipEvent_t> sweep_ev, refresh_ev;
     int sweep_used = 0, refresh_used = 0;
 
+    // optional FP64 objective trace (every obj_every epochs and at the end)
+    int32_t obj_every = 0;
+    int64_t obj_count = 0;
+    DeviceArray<float> obj_weights;
+    DeviceArray<double> obj_ws, obj_vals;
+    std::vector<int32_t> obj_epochs;
+
     // edge-balanced unit sweep (Adam, d = 2, H > 1024)
     bool unit_path = false;
     int64_t units = 0;
@@ -337,6 +344,7 @@ void umaphip_plan::run() {
 
     sweep_used = 0;
     refresh_used = 0;
+    obj_count = 0;
     HIP_CHECK(hipEventRecord(ev_start, stream));
     OrderReplay orders(h, n, p.seed, alias, hard, momentum);
     const bool upload_negative = !hard;  // global mode reads host orders
@@ -509,6 +517,12 @@ void umaphip_plan::run() {
                 }
                 launch_momentum_decay(h, d, updates.ptr, d, 1, mu[t], stream);
             }
+        }
+        if (obj_every > 0 && ((t + 1) % obj_every == 0 || t + 1 == epochs)) {
+            launch_cross_entropy(pos[c].ptr, h, shared ? pos[c].ptr : tail.ptr, d, offsets.ptr,
+                                 indices.ptr, obj_weights.ptr, p.a, p.b, obj_ws.ptr,
+                                 obj_vals.ptr + obj_count, stream);
+            ++obj_count;
         }
         // ---- next orders, then the projection refresh
         orders.advance();
@@ -729,6 +743,62 @@ umaphip_status umaphip_plan_run(umaphip_plan *plan) {
 }
 
 void *umaphip_plan_stream(umaphip_plan *plan) { return plan ? plan->stream : nullptr; }
+
+umaphip_status umaphip_plan_set_objective(umaphip_plan *plan, const float *weights,
+                                          int32_t every) {
+    return guarded([&] {
+        UMAPHIP_REQUIRE(plan != nullptr, UMAPHIP_INVALID_INPUT, "plan is null");
+        UMAPHIP_REQUIRE(every >= 0, UMAPHIP_INVALID_CONFIG, "objective interval must be >= 0");
+        plan->obj_every = every;
+        if (every == 0) return;
+        UMAPHIP_REQUIRE(weights != nullptr || plan->E == 0, UMAPHIP_INVALID_INPUT,
+                        "objective weights are null");
+        plan->obj_weights.alloc(std::max<int64_t>(plan->E, 1));
+        if (plan->E)
+            HIP_CHECK(hipMemcpy(plan->obj_weights.ptr, weights, plan->E * sizeof(float),
+                                hipMemcpyHostToDevice));
+        plan->obj_ws.alloc(cross_entropy_workspace(plan->h));
+        const int64_t slots = plan->p.epochs / every + 1;
+        plan->obj_vals.alloc(std::max<int64_t>(slots, 1));
+    });
+}
+
+umaphip_status umaphip_plan_objective(umaphip_plan *plan, double *values, int64_t capacity,
+                                      int64_t *count) {
+    return guarded([&] {
+        UMAPHIP_REQUIRE(plan != nullptr && count != nullptr, UMAPHIP_INVALID_INPUT,
+                        "plan objective arguments are null");
+        HIP_CHECK(hipStreamSynchronize(plan->stream));
+        *count = plan->obj_count;
+        if (values && plan->obj_count)
+            HIP_CHECK(hipMemcpy(values, plan->obj_vals.ptr,
+                                std::min(capacity, plan->obj_count) * sizeof(double),
+                                hipMemcpyDeviceToHost));
+    });
+}
+
+umaphip_status umaphip_cross_entropy_workspace_bytes(int64_t h, size_t *bytes) {
+    return guarded([&] {
+        UMAPHIP_REQUIRE(bytes && h >= 0, UMAPHIP_INVALID_INPUT, "invalid workspace query");
+        *bytes = static_cast<size_t>(cross_entropy_workspace(h)) * sizeof(double);
+    });
+}
+
+umaphip_status umaphip_cross_entropy(const float *head, int64_t h, const float *tail, int64_t n,
+                                     int64_t d, const int32_t *offsets, const int32_t *indices,
+                                     const float *weights, float a, float b, void *workspace,
+                                     size_t workspace_bytes, double *out, void *stream) {
+    return guarded([&] {
+        UMAPHIP_REQUIRE(head && offsets && out && workspace && h >= 0 && d > 0,
+                        UMAPHIP_INVALID_INPUT, "invalid objective arguments");
+        UMAPHIP_REQUIRE(workspace_bytes >= static_cast<size_t>(cross_entropy_workspace(h)) *
+                                               sizeof(double),
+                        UMAPHIP_CAPACITY, "objective workspace too small");
+        (void)n;
+        launch_cross_entropy(head, h, tail ? tail : head, d, offsets, indices, weights, a, b,
+                             static_cast<double *>(workspace), out, as_stream(stream));
+    });
+}
 
 umaphip_status umaphip_plan_set_profiling(umaphip_plan *plan, int32_t enabled) {
     return guarded([&] {

@@ -8,6 +8,7 @@
 
 #include <rocprim/device/device_radix_sort.hpp>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "common.hpp"
@@ -1059,6 +1060,66 @@ __global__ void unit_adam_kernel(int64_t h, const float *pos, float *pos_out, co
     reinterpret_cast<float2 *>(pos_out)[i] = out;
 }
 
+// ------------------------------------------------ FP64 cross-entropy objective
+// Fuzzy-set cross entropy of the embedding over the graph's positive edges:
+//   sum_e -w_e ln(max(phi_e, eps)) - (1 - w_e) ln(max(1 - phi_e, eps)),
+//   phi_e = 1 / (1 + a q_e^b), q_e = |x_i - y_j|^2, eps = 1e-4,
+// every term and sum in FP64 (coordinates widened from f32). One lane per
+// row sums its edges in order; workgroups reduce by a fixed tree and a
+// second pass adds the workgroup partials in order, so the value is
+// deterministic.
+constexpr int kObjThreads = 256;
+
+__device__ inline double cross_entropy_term(double q, double w, double a, double b) {
+    const double p = q > 0.0 ? a * pow(q, b) : 0.0;
+    const double phi = 1.0 / (1.0 + p);
+    const double rest = p / (1.0 + p);
+    const double eps = 1e-4;
+    return -w * log(phi > eps ? phi : eps) - (1.0 - w) * log(rest > eps ? rest : eps);
+}
+
+__global__ __launch_bounds__(kObjThreads) void cross_entropy_kernel(
+    const float *head, int64_t h, const float *tail, int64_t d, const int32_t *offsets,
+    const int32_t *indices, const float *weights, double a, double b, double *partial) {
+    __shared__ double s_sum[kObjThreads];
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kObjThreads + threadIdx.x;
+    double acc = 0.0;
+    if (i < h) {
+        for (int32_t e = offsets[i]; e < offsets[i + 1]; ++e) {
+            const int64_t j = static_cast<uint32_t>(indices[e]);
+            double q = 0.0;
+            for (int64_t c = 0; c < d; ++c) {
+                const double diff = static_cast<double>(head[i * d + c]) -
+                                    static_cast<double>(tail[j * d + c]);
+                q += diff * diff;
+            }
+            acc += cross_entropy_term(q, static_cast<double>(weights[e]), a, b);
+        }
+    }
+    s_sum[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = kObjThreads / 2; w > 0; w >>= 1) {
+        if (static_cast<int>(threadIdx.x) < w) s_sum[threadIdx.x] += s_sum[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = s_sum[0];
+}
+
+__global__ __launch_bounds__(kObjThreads) void sum_partials_kernel(const double *partial,
+                                                                   int64_t count, double *out) {
+    __shared__ double s_sum[kObjThreads];
+    double acc = 0.0;
+    // lane l adds partials l, l + 256, ... in order
+    for (int64_t k = threadIdx.x; k < count; k += kObjThreads) acc += partial[k];
+    s_sum[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = kObjThreads / 2; w > 0; w >>= 1) {
+        if (static_cast<int>(threadIdx.x) < w) s_sum[threadIdx.x] += s_sum[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = s_sum[0];
+}
+
 // ------------------------------------------------------------ projection
 __device__ inline uint32_t radix_key(float v) {
     uint32_t bits = __float_as_uint(v);
@@ -1404,6 +1465,23 @@ void launch_segment_sweep(const SweepSegment *segs, int32_t n_segs, int64_t tota
                                s, segs, n_segs);
             break;
     }
+    HIP_CHECK(hipGetLastError());
+}
+
+int64_t cross_entropy_workspace(int64_t h) { return std::max<int64_t>((h + kObjThreads - 1) / kObjThreads, 1); }
+
+void launch_cross_entropy(const float *head, int64_t h, const float *tail, int64_t d,
+                          const int32_t *offsets, const int32_t *indices, const float *weights,
+                          float a, float b, double *workspace, double *out, hipStream_t s) {
+    const int64_t blocks = cross_entropy_workspace(h);
+    if (h > 0) {
+        hipLaunchKernelGGL(cross_entropy_kernel, dim3(static_cast<unsigned>(blocks)),
+                           dim3(kObjThreads), 0, s, head, h, tail, d, offsets, indices, weights,
+                           static_cast<double>(a), static_cast<double>(b), workspace);
+        HIP_CHECK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(kObjThreads), 0, s, workspace,
+                       h > 0 ? blocks : 0, out);
     HIP_CHECK(hipGetLastError());
 }
 

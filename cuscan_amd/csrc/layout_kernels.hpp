@@ -99,6 +99,13 @@ void launch_unit_adam(int64_t h, const float *pos, float *pos_out, const float *
                       const int32_t *row_split, const int32_t *split_off, const float *frag,
                       float *m, float *v, float alpha, float c1, float c2, hipStream_t s);
 
+// FP64 fuzzy-set cross entropy over the positive edges (doubles of workspace:
+// cross_entropy_workspace(h)); *out is a device double.
+int64_t cross_entropy_workspace(int64_t h);
+void launch_cross_entropy(const float *head, int64_t h, const float *tail, int64_t d,
+                          const int32_t *offsets, const int32_t *indices, const float *weights,
+                          float a, float b, double *workspace, double *out, hipStream_t s);
+
 // Debug: per-edge negative candidates (same selection code).
 void launch_debug_negatives(const RowwiseArgs &a, int32_t cap, int32_t *count_out,
                             int32_t *cand_out, uint8_t *kept_out, int32_t *overflow,

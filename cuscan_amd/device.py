@@ -170,3 +170,22 @@ def classic_epoch(*, head, tail, tails, weights, epochs_per_sample, epoch_of_nex
     A.negative_sample_rate = float(negative_sample_rate)
     A.move_other, A.shared = int(bool(move_other)), int(bool(shared))
     _lib.check(_lib.lib().umaphip_classic_epoch(C.byref(A), _stream()))
+
+
+def cross_entropy(*, head, tail, offsets, indices, weights, a, b) -> float:
+    """FP64 fuzzy-set cross entropy of the embedding over the positive edges
+    (tail None for a fit); returns a Python float."""
+    _need(head, torch.float32, "head")
+    _need(offsets, torch.int32, "offsets")
+    _need(indices, torch.int32, "indices")
+    _need(weights, torch.float32, "weights")
+    h, d = head.shape
+    nbytes = C.c_size_t()
+    _lib.check(_lib.lib().umaphip_cross_entropy_workspace_bytes(h, C.byref(nbytes)))
+    ws = torch.empty(max(nbytes.value // 8, 1), dtype=torch.float64, device=head.device)
+    out = torch.zeros(1, dtype=torch.float64, device=head.device)
+    n = h if tail is None else tail.shape[0]
+    _lib.check(_lib.lib().umaphip_cross_entropy(
+        _p(head), h, _p(tail), n, d, _p(offsets), _p(indices), _p(weights), float(a), float(b),
+        _p(ws), nbytes.value, _p(out), _stream()))
+    return float(out.item())

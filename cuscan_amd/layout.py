@@ -116,6 +116,26 @@ class LayoutPlan:
             _ptr(negperiods), _ptr(nneg), C.byref(p), int(shared), C.byref(handle)))
         self._handle = handle
 
+    def set_objective(self, weights, every: int) -> None:
+        """Evaluate the FP64 cross entropy every `every` epochs (0 disables)."""
+        w = None if weights is None else np.ascontiguousarray(weights, np.float32)
+        if w is not None and len(w) != self.n_edges:
+            raise ValueError("objective weights need one value per edge")
+        self._keep.append(w)
+        _lib.check(_lib.lib().umaphip_plan_set_objective(self._handle,
+                                                         None if w is None else _ptr(w),
+                                                         int(every)))
+
+    def objective(self):
+        """The last run's objective trace (FP64)."""
+        count = C.c_int64()
+        _lib.check(_lib.lib().umaphip_plan_objective(self._handle, None, 0, C.byref(count)))
+        out = np.zeros(count.value, np.float64)
+        if count.value:
+            _lib.check(_lib.lib().umaphip_plan_objective(self._handle, _ptr(out), count.value,
+                                                         C.byref(count)))
+        return out
+
     def set_profiling(self, enabled: bool = True) -> None:
         _lib.check(_lib.lib().umaphip_plan_set_profiling(self._handle, int(bool(enabled))))
 

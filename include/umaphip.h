@@ -101,6 +101,22 @@ umaphip_status umaphip_debug_negatives(const umaphip_epoch_args *args, int32_t c
                                        uint8_t *kept_out, void *stream);
 
 /* ------------------------------------------------------------------------
+ * FP64 objective (no reference counterpart; the reference computes no
+ * objective). Fuzzy-set cross entropy of the embedding over the graph's
+ * positive edges:
+ *   sum_e -w_e ln(max(phi_e, 1e-4)) - (1 - w_e) ln(max(1 - phi_e, 1e-4)),
+ *   phi_e = 1 / (1 + a q_e^b), q_e = |head_i - tail_j|^2,
+ * with every term and sum in FP64 and a deterministic reduction order.
+ * tail = NULL for a fit (targets are head rows). Device pointers; *out is
+ * a device double.
+ * ------------------------------------------------------------------------ */
+umaphip_status umaphip_cross_entropy_workspace_bytes(int64_t h, size_t *bytes);
+umaphip_status umaphip_cross_entropy(const float *head, int64_t h, const float *tail, int64_t n,
+                                     int64_t d, const int32_t *offsets, const int32_t *indices,
+                                     const float *weights, float a, float b, void *workspace,
+                                     size_t workspace_bytes, double *out, void *stream);
+
+/* ------------------------------------------------------------------------
  * Projection windows. Replace modern_project / modern_projection_ranks
  * (kernels/standard_umap/modern_layout.rs:669-752) and the stable f32
  * argsort (crates/cuscan-common/src/radix.rs:659, key :993-1003).
@@ -187,6 +203,15 @@ umaphip_status umaphip_plan_create(int64_t h, int64_t n, int64_t d, const float 
                                    const double *negative_periods, const double *next_negative,
                                    const umaphip_layout_params *params, int32_t shared,
                                    umaphip_plan **out);
+/* Objective trace (no reference counterpart): with every > 0, each run
+ * evaluates the FP64 cross entropy (umaphip_cross_entropy) of the embedding
+ * after every `every`-th epoch and after the last one, on the plan's
+ * stream. weights [n_edges] are the CSR edge weights; every = 0 disables. */
+umaphip_status umaphip_plan_set_objective(umaphip_plan *plan, const float *weights,
+                                          int32_t every);
+/* Synchronises; copies up to `capacity` values of the last run's trace. */
+umaphip_status umaphip_plan_objective(umaphip_plan *plan, double *values, int64_t capacity,
+                                      int64_t *count);
 /* Resets the state to the uploaded initial state and enqueues every epoch
  * on the plan's stream. Does not synchronise. */
 umaphip_status umaphip_plan_run(umaphip_plan *plan);

@@ -170,6 +170,9 @@ def lib():
                                       C.c_int, f32p]
         L.orc_transform_init.restype = C.c_int
         L.orc_transform_init.argtypes = [i32p, f32p, C.c_int64, C.c_int64, f32p, C.c_int64, f32p]
+        L.orc_cross_entropy.restype = C.c_double
+        L.orc_cross_entropy.argtypes = [f32p, C.c_int64, v, C.c_int64, u32p, u32p, f32p,
+                                        C.c_double, C.c_double]
         L.orc_taus_next.restype = C.c_int64
         L.orc_taus_next.argtypes = [i64p]
         L.orc_classic_epoch.argtypes = [C.c_int64, C.c_int64, C.c_int64, f32p, v, i32p, f32p,
@@ -518,6 +521,19 @@ def transform_init(indices, directed, embedding):
     if lib().orc_transform_init(indices, directed, q, k, emb, emb.shape[1], out) != 0:
         raise FloatingPointError("transform initialization row has no positive memberships")
     return out
+
+
+def cross_entropy(head, tail, graph, weights, a, b):
+    """FP64 fuzzy-set cross entropy over the graph's positive edges. a and b
+    are the layout's f32 curve parameters, widened to f64 as the device does."""
+    a, b = float(np.float32(a)), float(np.float32(b))
+    head = np.ascontiguousarray(head, np.float32)
+    tail_arr = None if tail is None else np.ascontiguousarray(tail, np.float32)
+    return float(lib().orc_cross_entropy(
+        head, head.shape[0], None if tail_arr is None else tail_arr.ctypes.data, head.shape[1],
+        np.ascontiguousarray(graph["offsets"], np.uint32),
+        np.ascontiguousarray(graph["indices"], np.uint32),
+        np.ascontiguousarray(weights, np.float32), float(a), float(b)))
 
 
 def taus_next(state):

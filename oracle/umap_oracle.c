@@ -1781,3 +1781,34 @@ int orc_classic_optimize(int64_t h, int64_t n, int64_t d, float *head, const flo
     free(pruned); free(per); free(npos); free(nper); free(nneg); free(offsets); free(ids); free(rng);
     return 0;
 }
+
+
+/* ------------------------------------------------------------------------
+ * FP64 objective: sum_e -w ln(max(phi, 1e-4)) - (1 - w) ln(max(1 - phi,
+ * 1e-4)), phi = 1 / (1 + a q^b), in row-major edge order with Neumaier
+ * compensated summation.
+ * ------------------------------------------------------------------------ */
+double orc_cross_entropy(const float *head, int64_t h, const float *tail, int64_t d,
+                         const uint32_t *offsets, const uint32_t *indices, const float *weights,
+                         double a, double b) {
+    const float *y = tail ? tail : head;
+    double sum = 0.0, comp = 0.0;
+    for (int64_t i = 0; i < h; ++i) {
+        for (uint32_t e = offsets[i]; e < offsets[i + 1]; ++e) {
+            int64_t j = indices[e];
+            double q = 0.0;
+            for (int64_t c = 0; c < d; ++c) {
+                double diff = (double)head[i * d + c] - (double)y[j * d + c];
+                q += diff * diff;
+            }
+            double p = q > 0.0 ? a * pow(q, b) : 0.0;
+            double phi = 1.0 / (1.0 + p), rest = p / (1.0 + p), w = (double)weights[e];
+            double term = -w * log(phi > 1e-4 ? phi : 1e-4) - (1.0 - w) * log(rest > 1e-4 ? rest : 1e-4);
+            double t = sum + term;
+            if (fabs(sum) >= fabs(term)) comp += (sum - t) + term;
+            else comp += (term - t) + sum;
+            sum = t;
+        }
+    }
+    return sum + comp;
+}

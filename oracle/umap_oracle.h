@@ -205,6 +205,12 @@ int orc_classic_optimize(int64_t h, int64_t n, int64_t d, float *head, const flo
                          float learning_rate, float gamma, int32_t rate, uint64_t seed,
                          int shared, double *npos_out, double *nneg_out, int64_t *rng_out);
 
+/* FP64 fuzzy-set cross entropy over the positive edges (the build's
+ * objective; no reference counterpart), compensated summation in row order. */
+double orc_cross_entropy(const float *head, int64_t h, const float *tail, int64_t d,
+                         const uint32_t *offsets, const uint32_t *indices, const float *weights,
+                         double a, double b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -340,3 +340,35 @@ def test_vectorised_clock_replay_matches_advance_clock():
             _, npos[e], nneg[e] = O.advance_clock(npos[e], nneg[e], g["periods"][e],
                                                   g["negative_periods"][e], t)
     assert (npos == v["next_positive"]).all() and (nneg == v["next_negative"]).all()
+
+
+def test_oracle_cross_entropy_matches_numpy_fsum():
+    """The FP64 objective against an independent vectorised restatement
+    summed exactly (math.fsum), and its closed form on one edge."""
+    import math
+
+    import numpy as np
+
+    from oracle import oracle as O
+    from tests.gpu_helpers import random_graph
+
+    g = random_graph(400, 400, 7, 5, 50)
+    rng = np.random.default_rng(1)
+    head = rng.normal(0, 3, (400, 2)).astype(np.float32)
+    head[10] = head[int(g["indices"][g["offsets"][10]])]   # a coincident pair (clipped term)
+    w = rng.uniform(0.05, 1.0, len(g["indices"])).astype(np.float32)
+    a, b = float(np.float32(1.5769434)), float(np.float32(0.8950608))
+    rows = np.repeat(np.arange(400), np.diff(g["offsets"]))
+    diff = head[rows].astype(np.float64) - head[g["indices"]].astype(np.float64)
+    q = (diff ** 2).sum(1)
+    p = np.where(q > 0, a * np.power(q, b, where=q > 0, out=np.zeros_like(q)), 0.0)
+    phi, rest = 1 / (1 + p), p / (1 + p)
+    w64 = w.astype(np.float64)
+    terms = -w64 * np.log(np.maximum(phi, 1e-4)) - (1 - w64) * np.log(np.maximum(rest, 1e-4))
+    want = math.fsum(terms.tolist())
+    got = O.cross_entropy(head, None, g, w, a, b)
+    assert abs(got - want) <= 1e-12 * abs(want)
+    # one edge at unit distance with w = 1: -ln(1 / (1 + a))
+    one = {"offsets": np.array([0, 1, 1], np.uint32), "indices": np.array([1], np.uint32)}
+    pts = np.array([[0, 0], [1, 0]], np.float32)
+    assert abs(O.cross_entropy(pts, None, one, np.ones(1, np.float32), a, b) - math.log1p(a)) < 1e-14
