@@ -301,14 +301,14 @@ struct umaphip_plan {
         const float *reference = shared ? head_now : tail.ptr;
         const unsigned long long tag = static_cast<unsigned long long>(k) << 32;
         launch_project_keys(reference, n, d, dir, keys.ptr, nullptr, status.ptr, tag, stream);
-        launch_sort_pairs(sort_ws.ptr, sort_ws.count, keys.ptr, keys_sorted.ptr, iota.ptr,
-                          proj_order.ptr, n, stream);
         if (shared) {
-            if (d == 2)
-                launch_rank_and_table(proj_order.ptr, n, head_now, ranks.ptr, table.ptr, stream);
-            else
-                launch_inverse_ranks(proj_order.ptr, n, ranks.ptr, stream);
+            // the sort's last pass writes the ranks and (d = 2) the table
+            launch_sort_pairs_ranked(sort_ws.ptr, sort_ws.count, keys.ptr, keys_sorted.ptr, nullptr,
+                                     proj_order.ptr, n, ranks.ptr, d == 2 ? head_now : nullptr,
+                                     d == 2 ? table.ptr : nullptr, stream);
         } else {
+            launch_sort_pairs(sort_ws.ptr, sort_ws.count, keys.ptr, keys_sorted.ptr, nullptr,
+                              proj_order.ptr, n, stream);
             launch_values_from_keys(keys_sorted.ptr, n, sorted_vals.ptr, stream);
             launch_project_keys(head_now, h, d, dir, nullptr, query_proj.ptr, status.ptr, tag,
                                 stream);

@@ -6,8 +6,6 @@
 // as the reference's uncontracted arithmetic does.
 #include <hip/hip_runtime.h>
 
-#include <rocprim/device/device_radix_sort.hpp>
-
 #include <algorithm>
 #include <cstdint>
 
@@ -1120,6 +1118,156 @@ __global__ __launch_bounds__(kObjThreads) void sum_partials_kernel(const double 
     if (threadIdx.x == 0) *out = s_sum[0];
 }
 
+// ------------------------------------------------------------- radix sort
+// Stable LSD radix sort of (u32 key, i32 value) pairs, 8-bit digits, four
+// passes. Per pass: per-tile digit histograms (a tile is 2048 keys), a scan
+// that gives every (digit, tile) its output offset (digit-major, after all
+// smaller digits), and a scatter that ranks each key among the equal digits
+// of its tile in tile order -- lanes with the same digit are found with
+// ballots over the nine digit bits (eight plus "valid"), then counted
+// across the waves before them -- so equal keys keep their input order.
+// No kernel waits on another workgroup.
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 8;
+constexpr int kSortTile = kSortThreads * kSortItems;
+
+__global__ __launch_bounds__(kSortThreads) void radix_zero_kernel(uint32_t *x, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kSortThreads + threadIdx.x;
+    if (i < n) x[i] = 0u;
+}
+
+__global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const uint32_t *keys, int64_t n,
+                                                                 int shift, int64_t tiles,
+                                                                 uint32_t *hist,
+                                                                 uint32_t *totals) {
+    __shared__ uint32_t h[256];
+    __shared__ uint32_t t4[4][256];
+    const int tid = threadIdx.x;
+    h[tid] = 0u;
+    if (totals)
+        for (int d = 0; d < 4; ++d) t4[d][tid] = 0u;
+    __syncthreads();
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kSortTile;
+    for (int i = 0; i < kSortItems; ++i) {
+        const int64_t p = base + i * kSortThreads + tid;
+        if (p < n) {
+            const uint32_t k = keys[p];
+            atomicAdd(&h[(k >> shift) & 255u], 1u);
+            if (totals)
+                for (int d = 0; d < 4; ++d) atomicAdd(&t4[d][(k >> (8 * d)) & 255u], 1u);
+        }
+    }
+    __syncthreads();
+    hist[static_cast<int64_t>(tid) * tiles + blockIdx.x] = h[tid];
+    if (totals)
+        for (int d = 0; d < 4; ++d)
+            if (t4[d][tid]) atomicAdd(&totals[d * 256 + tid], t4[d][tid]);
+}
+
+__device__ inline uint32_t block_exclusive_scan_u32(uint32_t v, uint32_t *s_wave, uint32_t &total) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    uint32_t incl = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t up = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += up;
+    }
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    uint32_t before = 0;
+    total = 0;
+    for (int w = 0; w < kSortThreads / 64; ++w) {
+        if (w < wave) before += s_wave[w];
+        total += s_wave[w];
+    }
+    __syncthreads();
+    return before + incl - v;
+}
+
+// Workgroup d: offsets of digit d in every tile.
+__global__ __launch_bounds__(kSortThreads) void radix_scan_kernel(uint32_t *hist, int64_t tiles,
+                                                                 const uint32_t *totals) {
+    __shared__ uint32_t s_wave[kSortThreads / 64];
+    const int d = blockIdx.x, tid = threadIdx.x;
+    uint32_t all;
+    const uint32_t below = block_exclusive_scan_u32(totals[tid], s_wave, all);
+    __shared__ uint32_t s_base;
+    if (tid == d) s_base = below;
+    __syncthreads();
+    uint32_t carry = s_base;
+    uint32_t *row = hist + static_cast<int64_t>(d) * tiles;
+    for (int64_t c0 = 0; c0 < tiles; c0 += kSortThreads) {
+        const int64_t t = c0 + tid;
+        const uint32_t v = t < tiles ? row[t] : 0u;
+        uint32_t chunk;
+        const uint32_t ex = block_exclusive_scan_u32(v, s_wave, chunk);
+        if (t < tiles) row[t] = carry + ex;
+        carry += chunk;
+    }
+}
+
+__global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
+    const uint32_t *keys_in, const int32_t *vals_in, uint32_t *keys_out, int32_t *vals_out,
+    int64_t n, int shift, int64_t tiles, const uint32_t *offsets, int32_t *ranks_out,
+    const float *table_src, float *table_out) {
+    __shared__ uint32_t s_off[256];
+    __shared__ uint32_t s_wcnt[kSortThreads / 64][256];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kSortTile;
+    s_off[tid] = offsets[static_cast<int64_t>(tid) * tiles + blockIdx.x];
+    for (int w = 0; w < kSortThreads / 64; ++w) s_wcnt[w][tid] = 0u;
+    uint32_t k[kSortItems];
+    int32_t v[kSortItems];
+#pragma unroll
+    for (int i = 0; i < kSortItems; ++i) {
+        const int64_t p = base + i * kSortThreads + tid;
+        k[i] = 0u;
+        v[i] = 0;
+        if (p < n) {
+            k[i] = keys_in[p];
+            v[i] = vals_in ? vals_in[p] : static_cast<int32_t>(p);
+        }
+    }
+    __syncthreads();
+    const uint64_t below_lane = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int i = 0; i < kSortItems; ++i) {
+        const bool valid = base + i * kSortThreads + tid < n;
+        const uint32_t digit = valid ? ((k[i] >> shift) & 255u) : 256u;
+        uint64_t same = ~0ull;
+#pragma unroll
+        for (int bit = 0; bit < 9; ++bit) {
+            const bool set = (digit >> bit) & 1u;
+            const uint64_t ballot = __ballot(set);
+            same &= set ? ballot : ~ballot;
+        }
+        const uint32_t rank = static_cast<uint32_t>(__popcll(same & below_lane));
+        if (valid && rank == 0u) s_wcnt[wave][digit] = static_cast<uint32_t>(__popcll(same));
+        __syncthreads();
+        uint32_t pos = 0u;
+        if (valid) {
+            uint32_t before = 0u;
+            for (int w = 0; w < wave; ++w) before += s_wcnt[w][digit];
+            pos = s_off[digit] + before + rank;
+        }
+        __syncthreads();
+        uint32_t add = 0u;
+        for (int w = 0; w < kSortThreads / 64; ++w) {
+            add += s_wcnt[w][tid];
+            s_wcnt[w][tid] = 0u;
+        }
+        s_off[tid] += add;
+        __syncthreads();
+        if (valid) {
+            keys_out[pos] = k[i];
+            vals_out[pos] = v[i];
+            if (ranks_out) ranks_out[v[i]] = static_cast<int32_t>(pos);
+            if (table_out)
+                reinterpret_cast<float2 *>(table_out)[pos] =
+                    reinterpret_cast<const float2 *>(table_src)[v[i]];
+        }
+    }
+}
+
 // ------------------------------------------------------------ projection
 __device__ inline uint32_t radix_key(float v) {
     uint32_t bits = __float_as_uint(v);
@@ -1550,22 +1698,57 @@ void launch_iota(int32_t *out, int64_t n, hipStream_t s) {
     HIP_CHECK(hipGetLastError());
 }
 
+int64_t sort_tiles(int64_t n) { return (n + kSortTile - 1) / kSortTile; }
+
 size_t sort_pairs_workspace(int64_t n) {
-    size_t bytes = 0;
-    HIP_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, static_cast<const uint32_t *>(nullptr),
-                                        static_cast<uint32_t *>(nullptr),
-                                        static_cast<const int32_t *>(nullptr),
-                                        static_cast<int32_t *>(nullptr),
-                                        static_cast<unsigned int>(n), 0u, 32u));
-    return bytes;
+    auto up = [](size_t b) { return (b + 255) & ~static_cast<size_t>(255); };
+    return up(static_cast<size_t>(n) * 4) * 2 + up(static_cast<size_t>(256 * sort_tiles(n)) * 4) +
+           up(1024 * 4);
+}
+
+// Four passes: in -> tmp -> out -> tmp -> out. With ranks (and table_src /
+// table_out for d = 2), the last pass also writes ranks[value] = position and
+// table_out[position] = table_src[value].
+void launch_sort_pairs_ranked(void *ws, size_t ws_bytes, const uint32_t *keys_in,
+                              uint32_t *keys_out, const int32_t *vals_in, int32_t *vals_out,
+                              int64_t n, int32_t *ranks, const float *table_src, float *table_out,
+                              hipStream_t s) {
+    if (n == 0) return;
+    UMAPHIP_REQUIRE(n <= INT32_MAX, UMAPHIP_OVERFLOW, "sort size must fit i32");
+    UMAPHIP_REQUIRE(ws_bytes >= sort_pairs_workspace(n), UMAPHIP_CAPACITY, "sort workspace too small");
+    auto up = [](size_t b) { return (b + 255) & ~static_cast<size_t>(255); };
+    uint8_t *w = static_cast<uint8_t *>(ws);
+    uint32_t *tmp_k = reinterpret_cast<uint32_t *>(w);
+    int32_t *tmp_v = reinterpret_cast<int32_t *>(w + up(n * 4));
+    const int64_t tiles = sort_tiles(n);
+    uint32_t *hist = reinterpret_cast<uint32_t *>(w + 2 * up(n * 4));
+    uint32_t *totals = reinterpret_cast<uint32_t *>(w + 2 * up(n * 4) + up(256 * tiles * 4));
+    hipLaunchKernelGGL(radix_zero_kernel, dim3(4), dim3(kSortThreads), 0, s, totals, int64_t(1024));
+    const uint32_t *src_k = keys_in;
+    const int32_t *src_v = vals_in;
+    for (int pass = 0; pass < 4; ++pass) {
+        uint32_t *dst_k = (pass & 1) ? keys_out : tmp_k;
+        int32_t *dst_v = (pass & 1) ? vals_out : tmp_v;
+        const int shift = 8 * pass;
+        hipLaunchKernelGGL(radix_hist_kernel, dim3(static_cast<unsigned>(tiles)), dim3(kSortThreads),
+                           0, s, src_k, n, shift, tiles, hist, pass == 0 ? totals : nullptr);
+        hipLaunchKernelGGL(radix_scan_kernel, dim3(256), dim3(kSortThreads), 0, s, hist, tiles,
+                           totals + pass * 256);
+        const bool last = pass == 3;
+        hipLaunchKernelGGL(radix_scatter_kernel, dim3(static_cast<unsigned>(tiles)),
+                           dim3(kSortThreads), 0, s, src_k, src_v, dst_k, dst_v, n, shift, tiles,
+                           hist, last ? ranks : nullptr, last ? table_src : nullptr,
+                           last ? table_out : nullptr);
+        src_k = dst_k;
+        src_v = dst_v;
+    }
+    HIP_CHECK(hipGetLastError());
 }
 
 void launch_sort_pairs(void *ws, size_t ws_bytes, const uint32_t *keys_in, uint32_t *keys_out,
                        const int32_t *vals_in, int32_t *vals_out, int64_t n, hipStream_t s) {
-    if (n == 0) return;
-    size_t bytes = ws_bytes;
-    HIP_CHECK(rocprim::radix_sort_pairs(ws, bytes, keys_in, keys_out, vals_in, vals_out,
-                                        static_cast<unsigned int>(n), 0u, 32u, s));
+    launch_sort_pairs_ranked(ws, ws_bytes, keys_in, keys_out, vals_in, vals_out, n, nullptr,
+                             nullptr, nullptr, s);
 }
 
 void launch_values_from_keys(const uint32_t *keys, int64_t n, float *values, hipStream_t s) {

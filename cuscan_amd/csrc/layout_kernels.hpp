@@ -123,8 +123,16 @@ void launch_keys_from_values(const float *values, int64_t n, uint32_t *keys,
                              unsigned long long *status, hipStream_t s);
 void launch_iota(int32_t *out, int64_t n, hipStream_t s);
 size_t sort_pairs_workspace(int64_t n);
+// Stable ascending LSD radix sort of (key, value) pairs; vals_in may be null
+// (values = input positions). The _ranked form also writes, from its last
+// pass, ranks[value] = position and (table_src non-null, d = 2)
+// table_out[position] = table_src[value].
 void launch_sort_pairs(void *ws, size_t ws_bytes, const uint32_t *keys_in, uint32_t *keys_out,
                        const int32_t *vals_in, int32_t *vals_out, int64_t n, hipStream_t s);
+void launch_sort_pairs_ranked(void *ws, size_t ws_bytes, const uint32_t *keys_in,
+                              uint32_t *keys_out, const int32_t *vals_in, int32_t *vals_out,
+                              int64_t n, int32_t *ranks, const float *table_src, float *table_out,
+                              hipStream_t s);
 void launch_values_from_keys(const uint32_t *keys, int64_t n, float *values, hipStream_t s);
 void launch_inverse_ranks(const int32_t *order, int64_t n, int32_t *ranks, hipStream_t s);
 void launch_lower_bound(const float *sorted, int64_t n, const float *query, int64_t h,
