@@ -84,4 +84,8 @@ class LayoutBatch:
             self._h = None
 
     def __del__(self):
-        self.close()
+        # at interpreter shutdown the binding may already be torn down
+        try:
+            self.close()
+        except (TypeError, AttributeError):
+            pass

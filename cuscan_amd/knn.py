@@ -86,4 +86,8 @@ class KnnIndex:
             self._handle = None
 
     def __del__(self):
-        self.close()
+        # at interpreter shutdown the binding may already be torn down
+        try:
+            self.close()
+        except (TypeError, AttributeError):
+            pass

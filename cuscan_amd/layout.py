@@ -163,7 +163,11 @@ class LayoutPlan:
             self._handle = None
 
     def __del__(self):
-        self.close()
+        # at interpreter shutdown the binding may already be torn down
+        try:
+            self.close()
+        except (TypeError, AttributeError):
+            pass
 
 
 def orders_at_epoch(h, n, params: LayoutParams, *, shared: bool, epoch: int):

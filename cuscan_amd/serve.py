@@ -107,4 +107,8 @@ class TransformServer:
         self.close()
 
     def __del__(self):
-        self.close()
+        # at interpreter shutdown the binding may already be torn down
+        try:
+            self.close()
+        except (TypeError, AttributeError):
+            pass
