@@ -72,7 +72,9 @@ class ServerCounters(C.Structure):
         ("requests", C.c_int64), ("rows", C.c_int64), ("steps", C.c_int64),
         ("knn_batches", C.c_int64), ("mean_active", C.c_double), ("peak_active", C.c_int64),
         ("admission_waits", C.c_int64), ("memory_budget", C.c_int64),
-        ("memory_in_use", C.c_int64),
+        ("memory_in_use", C.c_int64), ("mean_queue_ms", C.c_double),
+        ("mean_knn_ms", C.c_double), ("mean_graph_ms", C.c_double),
+        ("mean_layout_ms", C.c_double),
     ]
 
 

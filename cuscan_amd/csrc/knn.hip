@@ -8,11 +8,15 @@
 //
 // Prefilter layout (gfx950, wave64): a workgroup of 4 waves owns 128 queries
 // (32 per wave, held for the whole scan as MFMA A-fragments in VGPRs) and
-// streams one slice of the references through LDS in chunks of 32 rows.
+// streams one slice of the references through a double-buffered LDS stage
+// in chunks of 16 rows: the next chunk is loaded straight from global
+// memory into LDS (global_load_lds) while the current one feeds the MFMAs.
 // Approximate distances ||r||^2 - 2 q.r are compared against each query's
-// running 64th-best threshold; survivors are appended to a per-(query,
-// slice) buffer that is compacted to its sorted best 64 whenever it could
-// overflow. Slices are merged per query, then reranked.
+// running 64th-best threshold (kept in registers); survivors are appended to
+// a per-(query, slice) buffer that is compacted to its sorted best 64 when
+// it could overflow. A wave owns the candidate state of its own 32 queries,
+// so the threshold epilogue needs no workgroup barrier. Slices are merged
+// per query, then reranked.
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -33,10 +37,9 @@ typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
 
 constexpr int kTQ = 128;      // queries per workgroup
-constexpr int kTR = 32;       // reference rows per LDS chunk
+constexpr int kTR = 16;       // reference rows per LDS chunk (one MFMA column block)
 constexpr int kCap = 128;     // candidate buffer per (query, slice)
 constexpr int kTop = 64;      // candidates kept per query
-constexpr int kPad = 8;       // LDS row padding (bf16 elements)
 
 __device__ inline uint16_t f32_to_bf16_rne(float f) {
     uint32_t u = __float_as_uint(f);
@@ -97,15 +100,34 @@ __device__ inline float compact_buffer(float *cd, int32_t *ci, int n, int lane) 
     return tau;
 }
 
+// Order-preserving u32 key of a float (the global threshold is kept as a
+// key so that atomicMin orders it); +inf maps to 0xFF800000.
+__device__ inline uint32_t float_key(float v) {
+    const uint32_t u = __float_as_uint(v);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ inline float key_float(uint32_t k) {
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
+}
+
+__global__ void fill_u32_kernel(uint32_t *p, int64_t n, uint32_t v) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 template <int KSTEPS>
 __global__ __launch_bounds__(256, 2) void knn_scan_kernel(
     const uint16_t *__restrict__ qb, int64_t nq, const uint16_t *__restrict__ rb,
     const float *__restrict__ rnorm, int64_t nr, int64_t per_slice, int slices,
     float *__restrict__ cand_d, int32_t *__restrict__ cand_i, int32_t *__restrict__ cand_n,
-    int exclude_self, int64_t query_offset) {
+    uint32_t *__restrict__ tau_key, int exclude_self, int64_t query_offset) {
     constexpr int DP = KSTEPS * 32;
-    constexpr int ROW = DP + kPad;
-    __shared__ uint16_t sB[kTR * ROW];
+    constexpr int SLOTS_ROW = DP / 8 + 1;       // 16-byte words per padded LDS row
+    constexpr int ROW = SLOTS_ROW * 8;          // bf16 per padded row
+    constexpr int INSTR = (kTR * SLOTS_ROW + 63) / 64;   // wave-wide 1 KB loads per chunk
+    constexpr int BUF = INSTR * 64 * 8;         // bf16 per buffer (whole 1 KB loads)
+    constexpr int kRefresh = 32;                // chunks between reads of the global thresholds
+    __shared__ __attribute__((aligned(16))) uint16_t sB[2 * BUF];
     __shared__ float s_tau[kTQ];
     __shared__ int s_cnt[kTQ];
 
@@ -115,6 +137,24 @@ __global__ __launch_bounds__(256, 2) void knn_scan_kernel(
     const int slice = blockIdx.y;
     const int64_t r_begin = static_cast<int64_t>(slice) * per_slice;
     const int64_t r_end = min<int64_t>(nr, r_begin + per_slice);
+
+    // One chunk into buffer `buf`: LDS word `slot` of the padded image holds
+    // row slot / SLOTS_ROW, word slot % SLOTS_ROW (the last word of a row and
+    // the slots past the chunk are padding; they load a harmless in-bounds
+    // word).
+    auto stage = [&](int buf, int64_t chunk) {
+        for (int j = wave; j < INSTR; j += 4) {
+            const int slot = j * 64 + lane;
+            const int row = slot / SLOTS_ROW, col = slot % SLOTS_ROW;
+            const int64_t ref = chunk + row;
+            const uint16_t *src = rb;
+            if (row < kTR && col < DP / 8 && ref < r_end) src = rb + ref * DP + col * 8;
+            __builtin_amdgcn_global_load_lds(
+                (const void __attribute__((address_space(1))) *)src,
+                (void __attribute__((address_space(3))) *)(sB + buf * BUF + j * 64 * 8), 16, 0, 0);
+        }
+    };
+    if (r_begin < r_end) stage(0, r_begin);
 
     // A fragments: lane holds row (lane % 16) and K elements 8*(lane/16)..+7
     // of each 32-wide K step, for two 16-query sub-blocks.
@@ -132,82 +172,117 @@ __global__ __launch_bounds__(256, 2) void knn_scan_kernel(
             }
         }
     }
-    if (tid < kTQ) {
-        s_tau[tid] = INFINITY;
-        s_cnt[tid] = 0;
+    if (lane < 32) {
+        s_tau[wave * 32 + lane] = INFINITY;
+        s_cnt[wave * 32 + lane] = 0;
     }
-    const int64_t buf_stride = static_cast<int64_t>(slices) * kCap;
+    // C layout: lane holds rows 4*(lane/16)+i (queries), column lane%16
+    // (reference). A candidate is kept when dt <= tau, tau being the lower
+    // of the slice's own 64th best and the query's best 64th best over all
+    // slices (both bound the query's final 64th best from above).
+    float tau_r[2][4];
+#pragma unroll
+    for (int qa = 0; qa < 2; ++qa)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) tau_r[qa][i] = INFINITY;
+    float rn_next = 0.0f;
+    if (r_begin + (lane & 15) < r_end) rn_next = rnorm[r_begin + (lane & 15)];
 
+    int buf = 0, since = 0;
     for (int64_t chunk = r_begin; chunk < r_end; chunk += kTR) {
-        __syncthreads();
-        // stage kTR reference rows (16-byte words)
-        constexpr int WORDS = kTR * (DP / 8);
-        for (int w = tid; w < WORDS; w += 256) {
-            const int rr = w / (DP / 8), cw = w % (DP / 8);
-            const int64_t ref = chunk + rr;
-            uint4 v = make_uint4(0, 0, 0, 0);
-            if (ref < r_end) v = *reinterpret_cast<const uint4 *>(rb + ref * DP + cw * 8);
-            *reinterpret_cast<uint4 *>(&sB[rr * ROW + cw * 8]) = v;
+        __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): this lane's loads of the current buffer
+        __syncthreads();                      // every lane's loads landed; the other buffer is free
+        if (++since == kRefresh) {
+            // thresholds other slices have published for this wave's queries
+            since = 0;
+            const int64_t q = q_block + wave * 32 + (lane & 31);
+            if (lane < 32 && q < nq) {
+                const float g = key_float(tau_key[q]);
+                if (g < s_tau[wave * 32 + lane]) s_tau[wave * 32 + lane] = g;
+            }
+#pragma unroll
+            for (int qa = 0; qa < 2; ++qa)
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    tau_r[qa][i] = s_tau[wave * 32 + qa * 16 + 4 * (lane >> 4) + i];
         }
-        __syncthreads();
-        floatx4 acc[2][2];
+        const float rn = rn_next;
+        if (chunk + kTR < r_end) {
+            stage(buf ^ 1, chunk + kTR);
+            const int64_t nref = chunk + kTR + (lane & 15);
+            rn_next = nref < r_end ? rnorm[nref] : 0.0f;
+        }
+        const uint16_t *B = sB + buf * BUF;
+        floatx4 acc[2] = {floatx4{0.0f, 0.0f, 0.0f, 0.0f}, floatx4{0.0f, 0.0f, 0.0f, 0.0f}};
+#pragma unroll
+        for (int ks = 0; ks < KSTEPS; ++ks) {
+            const uint4 raw = *reinterpret_cast<const uint4 *>(&B[(lane & 15) * ROW + ks * 32 + 8 * (lane >> 4)]);
+            const bf16x8 bfrag = __builtin_bit_cast(bf16x8, raw);
+#pragma unroll
+            for (int qa = 0; qa < 2; ++qa)
+                acc[qa] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[qa][ks], bfrag, acc[qa], 0, 0, 0);
+        }
+        const int64_t ref = chunk + (lane & 15);
+        bool any = false;
 #pragma unroll
         for (int qa = 0; qa < 2; ++qa)
 #pragma unroll
-            for (int rbk = 0; rbk < 2; ++rbk) acc[qa][rbk] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (int ks = 0; ks < KSTEPS; ++ks) {
-#pragma unroll
-            for (int rbk = 0; rbk < 2; ++rbk) {
-                const uint4 raw = *reinterpret_cast<const uint4 *>(
-                    &sB[(rbk * 16 + (lane & 15)) * ROW + ks * 32 + 8 * (lane >> 4)]);
-                const bf16x8 bfrag = __builtin_bit_cast(bf16x8, raw);
-#pragma unroll
-                for (int qa = 0; qa < 2; ++qa)
-                    acc[qa][rbk] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[qa][ks], bfrag,
-                                                                           acc[qa][rbk], 0, 0, 0);
-            }
-        }
-        // C layout: lane holds rows 4*(lane/16)+i (queries), column lane%16 (reference).
-#pragma unroll
-        for (int rbk = 0; rbk < 2; ++rbk) {
-            const int64_t ref = chunk + rbk * 16 + (lane & 15);
-            if (ref >= r_end) continue;
-            const float rn = rnorm[ref];
-#pragma unroll
-            for (int qa = 0; qa < 2; ++qa) {
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int ql = wave * 32 + qa * 16 + 4 * (lane >> 4) + i;
-                    const int64_t q = q_block + ql;
-                    if (q >= nq) continue;
-                    if (exclude_self && ref == query_offset + q) continue;
-                    const float dt = rn - 2.0f * acc[qa][rbk][i];
-                    if (dt < s_tau[ql]) {
-                        const int pos = atomicAdd(&s_cnt[ql], 1);
+            for (int i = 0; i < 4; ++i) {
+                const int ql = wave * 32 + qa * 16 + 4 * (lane >> 4) + i;
+                const int64_t q = q_block + ql;
+                const float dt = rn - 2.0f * acc[qa][i];
+                const bool keep = ref < r_end && q < nq && dt <= tau_r[qa][i] &&
+                                  !(exclude_self && ref == query_offset + q);
+                const uint64_t m = __ballot(keep);
+                if (m) {
+                    // the 16 lanes of a lane group share query ql; positions
+                    // by ballot prefix, count kept in the wave-private s_cnt
+                    const int base_n = s_cnt[ql];
+                    const uint32_t gm = static_cast<uint32_t>(m >> (16 * (lane >> 4))) & 0xFFFFu;
+                    if (keep) {
+                        const int pos = base_n + __popc(gm & ((1u << (lane & 15)) - 1u));
                         const int64_t base = (q * slices + slice) * kCap;
                         cand_d[base + pos] = dt;
                         cand_i[base + pos] = static_cast<int32_t>(ref);
                     }
+                    if ((lane & 15) == 0) s_cnt[ql] = base_n + __popc(gm);
+                    any = true;
                 }
             }
-        }
-        __syncthreads();
-        // compaction of buffers that could overflow on the next chunk
-        for (int ql = wave; ql < kTQ; ql += 4) {
-            const int n = s_cnt[ql];
-            const int64_t q = q_block + ql;
-            if (q >= nq || n <= kCap - kTR) continue;
-            const int64_t base = (q * slices + slice) * kCap;
-            const float tau = compact_buffer(cand_d + base, cand_i + base, n, lane);
-            if (lane == 0) {
-                s_tau[ql] = tau;
-                s_cnt[ql] = kTop;
+        if (any) {
+            // this wave's buffers that could overflow with the next chunk
+            const int n_l = lane < 32 ? s_cnt[wave * 32 + lane] : 0;
+            uint64_t full = __ballot(lane < 32 && n_l > kCap - kTR);
+            if (full) {
+                __threadfence_block();   // the wave's appends are visible to all its lanes
+                while (full) {
+                    const int l = __builtin_ctzll(full);
+                    full &= full - 1;
+                    const int ql = wave * 32 + l;
+                    const int64_t q = q_block + ql;
+                    const int n = __shfl(n_l, l, 64);
+                    const int64_t base = (q * slices + slice) * kCap;
+                    const float tau = compact_buffer(cand_d + base, cand_i + base, n, lane);
+                    if (lane == 0) {
+                        // publish; the query's bound is the lower of both
+                        const uint32_t old = atomicMin(&tau_key[q], float_key(tau));
+                        const float g = key_float(old);
+                        s_tau[ql] = g < tau ? g : tau;
+                        s_cnt[ql] = kTop;
+                    }
+                }
+#pragma unroll
+                for (int qa = 0; qa < 2; ++qa)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        tau_r[qa][i] = s_tau[wave * 32 + qa * 16 + 4 * (lane >> 4) + i];
             }
         }
+        buf ^= 1;
     }
-    __syncthreads();
-    for (int ql = wave; ql < kTQ; ql += 4) {
+    __threadfence_block();
+    for (int l = 0; l < 32; ++l) {
+        const int ql = wave * 32 + l;
         const int64_t q = q_block + ql;
         if (q >= nq) continue;
         const int n = s_cnt[ql];
@@ -215,7 +290,6 @@ __global__ __launch_bounds__(256, 2) void knn_scan_kernel(
         if (n > 0) compact_buffer(cand_d + base, cand_i + base, n, lane);
         if (lane == 0) cand_n[q * slices + slice] = n < kTop ? n : kTop;
     }
-    (void)buf_stride;
 }
 
 // Merge the sorted per-slice lists of each query into its best kTop
@@ -280,28 +354,39 @@ __global__ __launch_bounds__(64) void knn_merge_kernel(const float *cand_d, cons
 }
 
 // Exact rerank: one wave per query, one lane per candidate; the distance is
-// the reference's sequential f32 sum (no contraction in this build).
+// the reference's sequential f32 sum (no contraction in this build). The
+// candidate rows are staged through LDS 64 features at a time, so every
+// global load is a contiguous 256-byte segment of one row.
 __global__ __launch_bounds__(64) void knn_rerank_kernel(const float *query, int64_t nq,
                                                         const float *ref, int64_t nr, int64_t d,
                                                         const int32_t *top_i, const int32_t *top_n,
                                                         int k, int32_t *out_i, float *out_d) {
+    constexpr int FB = 64;
+    __shared__ float s_q[FB];
+    __shared__ float s_r[kTop][FB + 1];
     const int64_t q = blockIdx.x;
     const int lane = threadIdx.x;
     if (q >= nq) return;
     const int n = top_n[q];
-    int32_t id = INT32_MAX;
-    float dist = INFINITY;
-    if (lane < n) {
-        id = top_i[q * kTop + lane];
-        const float *qv = query + q * d;
-        const float *rv = ref + static_cast<int64_t>(id) * d;
-        float acc = 0.0f;
-        for (int64_t f = 0; f < d; ++f) {
-            const float delta = qv[f] - rv[f];
-            acc += delta * delta;
+    const int32_t id = lane < n ? top_i[q * kTop + lane] : INT32_MAX;
+    float acc = 0.0f;
+    for (int64_t f0 = 0; f0 < d; f0 += FB) {
+        const int fb = static_cast<int>(min<int64_t>(FB, d - f0));
+        __syncthreads();
+        s_q[lane] = lane < fb ? query[q * d + f0 + lane] : 0.0f;
+        for (int c = 0; c < n; ++c) {
+            const int64_t row = __shfl(id, c, 64);
+            s_r[c][lane] = lane < fb ? ref[row * d + f0 + lane] : 0.0f;
         }
-        dist = acc;
+        __syncthreads();
+        if (lane < n) {
+            for (int f = 0; f < fb; ++f) {
+                const float delta = s_q[f] - s_r[lane][f];
+                acc += delta * delta;
+            }
+        }
     }
+    const float dist = lane < n ? acc : INFINITY;
     int rank = 0;
     for (int f = 0; f < 64; ++f) {
         const float df = __shfl(dist, f, 64);
@@ -322,9 +407,9 @@ __global__ __launch_bounds__(64) void knn_rerank_kernel(const float *query, int6
 template <int KS>
 void launch_scan(dim3 grid, hipStream_t s, const uint16_t *qb, int64_t nq, const uint16_t *rb,
                  const float *rnorm, int64_t nr, int64_t per_slice, int slices, float *cd,
-                 int32_t *ci, int32_t *cn, int exclude_self, int64_t qoff) {
+                 int32_t *ci, int32_t *cn, uint32_t *tk, int exclude_self, int64_t qoff) {
     hipLaunchKernelGGL(knn_scan_kernel<KS>, grid, dim3(256), 0, s, qb, nq, rb, rnorm, nr, per_slice,
-                       slices, cd, ci, cn, exclude_self, qoff);
+                       slices, cd, ci, cn, tk, exclude_self, qoff);
 }
 
 int ksteps_for(int64_t d) {
@@ -381,6 +466,7 @@ void KnnIndex::reserve(int64_t q) {
     top_d_.alloc(static_cast<size_t>(q) * kTop);
     top_i_.alloc(static_cast<size_t>(q) * kTop);
     top_n_.alloc(static_cast<size_t>(q));
+    tau_key_.alloc(static_cast<size_t>(q));
     qbf_.alloc(static_cast<size_t>(q * dp_));
     reserved_q_ = q;
 }
@@ -400,14 +486,17 @@ void KnnIndex::query(const float *q_dev, int64_t nq, int k, bool exclude_self, i
     const int64_t per_slice = ((n_ + slices - 1) / slices + kTR - 1) / kTR * kTR;
     const dim3 grid(static_cast<unsigned>((nq + kTQ - 1) / kTQ), static_cast<unsigned>(slices));
     const int ex = exclude_self ? 1 : 0;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(static_cast<unsigned>((nq + 255) / 256)), dim3(256), 0, s,
+                       tau_key_.ptr, nq, 0xFF800000u);   // key of +inf
+    HIP_CHECK(hipGetLastError());
     switch (ks_) {
-        case 1: launch_scan<1>(grid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, per_slice, slices, cand_d_.ptr, cand_i_.ptr, cand_n_.ptr, ex, query_offset); break;
-        case 2: launch_scan<2>(grid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, per_slice, slices, cand_d_.ptr, cand_i_.ptr, cand_n_.ptr, ex, query_offset); break;
-        case 4: launch_scan<4>(grid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, per_slice, slices, cand_d_.ptr, cand_i_.ptr, cand_n_.ptr, ex, query_offset); break;
-        case 8: launch_scan<8>(grid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, per_slice, slices, cand_d_.ptr, cand_i_.ptr, cand_n_.ptr, ex, query_offset); break;
-        case 16: launch_scan<16>(grid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, per_slice, slices, cand_d_.ptr, cand_i_.ptr, cand_n_.ptr, ex, query_offset); break;
-        case 25: launch_scan<25>(grid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, per_slice, slices, cand_d_.ptr, cand_i_.ptr, cand_n_.ptr, ex, query_offset); break;
-        default: launch_scan<32>(grid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, per_slice, slices, cand_d_.ptr, cand_i_.ptr, cand_n_.ptr, ex, query_offset); break;
+        case 1: launch_scan<1>(grid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, per_slice, slices, cand_d_.ptr, cand_i_.ptr, cand_n_.ptr, tau_key_.ptr, ex, query_offset); break;
+        case 2: launch_scan<2>(grid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, per_slice, slices, cand_d_.ptr, cand_i_.ptr, cand_n_.ptr, tau_key_.ptr, ex, query_offset); break;
+        case 4: launch_scan<4>(grid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, per_slice, slices, cand_d_.ptr, cand_i_.ptr, cand_n_.ptr, tau_key_.ptr, ex, query_offset); break;
+        case 8: launch_scan<8>(grid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, per_slice, slices, cand_d_.ptr, cand_i_.ptr, cand_n_.ptr, tau_key_.ptr, ex, query_offset); break;
+        case 16: launch_scan<16>(grid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, per_slice, slices, cand_d_.ptr, cand_i_.ptr, cand_n_.ptr, tau_key_.ptr, ex, query_offset); break;
+        case 25: launch_scan<25>(grid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, per_slice, slices, cand_d_.ptr, cand_i_.ptr, cand_n_.ptr, tau_key_.ptr, ex, query_offset); break;
+        default: launch_scan<32>(grid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, per_slice, slices, cand_d_.ptr, cand_i_.ptr, cand_n_.ptr, tau_key_.ptr, ex, query_offset); break;
     }
     HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(knn_merge_kernel, dim3(static_cast<unsigned>(nq)), dim3(64), 0, s,

@@ -35,6 +35,7 @@ private:
     DeviceArray<float> ref_, rnorm_, cand_d_, top_d_;
     DeviceArray<uint16_t> ref_bf_, qbf_;
     DeviceArray<int32_t> cand_i_, cand_n_, top_i_, top_n_;
+    DeviceArray<uint32_t> tau_key_;   // per query: best published slice threshold (float key)
 };
 
 }  // namespace umaphip

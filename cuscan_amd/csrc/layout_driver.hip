@@ -667,10 +667,14 @@ umaphip_status umaphip_plan_create(int64_t h, int64_t n, int64_t d, const float 
             P->host_orders[0].alloc(n);
             P->host_orders[1].alloc(n);
         }
-        // Edge-balanced unit sweep: Adam, d = 2, large layouts, epochs that fit
-        // the 16-bit firing shadow, and negative periods equal to the
-        // reference formula (period / rate) * 1.5 for every edge.
-        bool unit_ok = !P->momentum && d == 2 && !P->small && E > 0 && params->epochs < 65535;
+        // Edge-balanced unit sweep: Adam fits, d = 2, large layouts, epochs that
+        // fit the 16-bit firing shadow, and negative periods equal to the
+        // reference formula (period / rate) * 1.5 for every edge. Transform
+        // rows have at most n_neighbors edges (no hub rows), so transforms
+        // keep the tile sweep, whose per-row sums the transform server
+        // reproduces bit for bit.
+        bool unit_ok = P->shared && !P->momentum && d == 2 && !P->small && E > 0 &&
+                       params->epochs < 65535;
         if (unit_ok && params->negative_sample_rate > 0) {
             const double rate = static_cast<double>(params->negative_sample_rate);
             for (int64_t e = 0; e < E && unit_ok; ++e)

@@ -289,11 +289,12 @@ __global__ void momentum_decay_kernel(int64_t h, int64_t d, float *vel, int64_t 
 //      same work whether its edge fired or not; each negative term derives
 //      its candidate rank e*(epoch+p+1) mod M independently.
 //   C. One lane per row adds its row's terms in edge order, i.e. the
-//      reference's per-term sequential accumulation order.
+//      reference's per-term sequential accumulation order. A chunk with
+//      more terms than LDS holds runs B and C in several windows, so the
+//      sums do not depend on the tile shape.
 // The fused epilogue applies the lazy Adam step into a second position
 // buffer (Jacobi double buffer). Deterministic; no global atomics.
 constexpr int kTileRows = 64;
-constexpr int kTermCap = 2048;
 
 __device__ inline float2 ld2(const float *p, int64_t row) {
     return reinterpret_cast<const float2 *>(p)[row];
@@ -384,7 +385,7 @@ __device__ inline void sweep_tile(const SweepArgs &A, int64_t tile) {
     __shared__ int32_t s_j[THREADS];
     __shared__ uint8_t s_row[THREADS];
     __shared__ uint32_t s_em[THREADS];
-    __shared__ int s_wave[WAVES], s_wave2[WAVES];
+    __shared__ int s_wave[WAVES];
 
     const int tid = threadIdx.x;
     const int64_t row0 = tile * ROWS;
@@ -508,38 +509,36 @@ __device__ inline void sweep_tile(const SweepArgs &A, int64_t tile) {
         s_row[tid] = static_cast<uint8_t>(r);
         int total;
         const int my_base = block_exclusive_scan<WAVES>(terms, tid, s_wave, total);
-        const bool term_parallel = total <= TCAP;
-        // Chunks with more terms than LDS holds fall back to one combined
-        // term per edge, computed by the edge's own lane.
-        int fb_total;
-        const int fb_base = block_exclusive_scan<WAVES>(terms > 0 ? 1 : 0, tid, s_wave2, fb_total);
-        (void)fb_total;
-        if (term_parallel) {
-            s_base[tid] = my_base;
-            for (int k = 0; k < terms; ++k) s_owner[my_base + k] = static_cast<uint8_t>(tid);
-            if (retry && terms == 2) s_terms[my_base + 1] = retry_sum;
-        } else {
-            s_base[tid] = fb_base;
-        }
-        if (tid == THREADS - 1) s_base[THREADS] = term_parallel ? total : fb_base + (terms > 0);
-        __syncthreads();
+        s_base[tid] = my_base;
+        if (tid == THREADS - 1) s_base[THREADS] = total;
 
-        // ---- B: force terms (two per thread in flight)
-        if (term_parallel) {
-            for (int item0 = tid; item0 < total; item0 += 2 * THREADS) {
+        // The chunk's terms are processed in windows of TCAP (almost always
+        // one): B computes a window's terms term-parallel, C adds them to
+        // their rows in term order. The per-row sum is therefore the
+        // sequential per-term sum whatever the tile shape or term count.
+        for (int w0 = 0; w0 < total; w0 += TCAP) {
+            const int wend = min(total, w0 + TCAP);
+            for (int k = max(my_base, w0); k < min(my_base + terms, wend); ++k)
+                s_owner[k - w0] = static_cast<uint8_t>(tid);
+            if (retry && terms == 2 && my_base + 1 >= w0 && my_base + 1 < wend)
+                s_terms[my_base + 1 - w0] = retry_sum;
+            __syncthreads();
+
+            // ---- B: force terms (two per thread in flight)
+            for (int item0 = w0 + tid; item0 < wend; item0 += 2 * THREADS) {
                 float2 y[2], xi[2];
                 int kk[2];
                 bool live[2], keep[2];
 #pragma unroll
                 for (int u = 0; u < 2; ++u) {
                     const int item = item0 + u * THREADS;
-                    live[u] = item < total;
+                    live[u] = item < wend;
                     keep[u] = live[u];
                     kk[u] = 0;
                     y[u] = make_float2(0.0f, 0.0f);
                     xi[u] = make_float2(0.0f, 0.0f);
                     if (!live[u]) continue;
-                    const int owner = s_owner[item];
+                    const int owner = s_owner[item - w0];
                     const int k = item - s_base[owner];
                     const int rr = s_row[owner];
                     kk[u] = k;
@@ -581,65 +580,26 @@ __device__ inline void sweep_tile(const SweepArgs &A, int64_t tile) {
                         const float c = repel_fast(q, A.a, A.b, A.gamma, s_neg);
                         term = make_float2(c * dx, c * dy);
                     }
-                    s_terms[item0 + u * THREADS] = term;
+                    s_terms[item0 + u * THREADS - w0] = term;
                 }
             }
-        } else if (terms > 0) {
-            // fallback: the lane's whole edge as one term
-            const int64_t i = s_rowid[r];
-            const int64_t seg = s_seg[r], seg_end = seg + (s_off[r + 1] - s_off[r]);
-            const float2 xi = s_pos[r];
-            float2 g = make_float2(0.0f, 0.0f);
-            const float2 y = ld2(A.tgt, static_cast<uint32_t>(s_j[tid]));
-            const float dx = xi.x - y.x, dy = xi.y - y.y;
-            const float q = dx * dx + dy * dy;
-            if (q > 0.0f) {
-                const float c = attract_fast(q, A.a, A.b, fm);
-                g = make_float2(c * dx, c * dy);
-            }
-            if (retry) {
-                g.x = g.x + retry_sum.x;
-                g.y = g.y + retry_sum.y;
-            } else if (count > 0) {
-                NegWalk w;
-                w.init(static_cast<uint64_t>(e), A.epoch, count, small, false, n64, A.range,
-                       0);
-                w.first = s_first[r];
-                uint64_t rank;
-                while (w.next(rank)) {
-                    float2 yn;
-                    if (A.exclude) {
-                        const int64_t jj = static_cast<uint32_t>(A.order[rank]);
-                        if (excluded(A.indices, seg, seg_end, i, jj, A.shared != 0)) continue;
-                        yn = A.table ? ld2(A.table, rank) : ld2(A.tgt, jj);
-                    } else {
-                        yn = A.table ? ld2(A.table, rank)
-                                     : ld2(A.tgt, static_cast<uint32_t>(A.order[rank]));
-                    }
-                    const float nx = xi.x - yn.x, ny = xi.y - yn.y;
-                    const float c = repel_fast(nx * nx + ny * ny, A.a, A.b, A.gamma, s_neg);
-                    g.x = g.x + c * nx;
-                    g.y = g.y + c * ny;
-                }
-            }
-            s_terms[fb_base] = g;
-        }
-        __syncthreads();
+            __syncthreads();
 
-        // ---- C: per-row sums in edge (and term) order
-        if (tid < rows) {
-            const int32_t lo = max(s_off[tid], base) - base;
-            const int32_t hi = min(s_off[tid + 1], base + THREADS) - base;
-            if (lo < hi) {
-                const int t_lo = s_base[lo], t_hi = s_base[hi];
-                for (int k = t_lo; k < t_hi; ++k) {
-                    const float2 c = s_terms[k];
-                    acc.x = acc.x + c.x;
-                    acc.y = acc.y + c.y;
+            // ---- C: per-row sums in edge (and term) order
+            if (tid < rows) {
+                const int32_t lo = max(s_off[tid], base) - base;
+                const int32_t hi = min(s_off[tid + 1], base + THREADS) - base;
+                if (lo < hi) {
+                    const int t_lo = max(s_base[lo], w0), t_hi = min(s_base[hi], wend);
+                    for (int k = t_lo; k < t_hi; ++k) {
+                        const float2 c = s_terms[k - w0];
+                        acc.x = acc.x + c.x;
+                        acc.y = acc.y + c.y;
+                    }
                 }
             }
+            __syncthreads();
         }
-        __syncthreads();
     }
 
     if (tid >= rows || !s_active[tid]) return;
@@ -693,6 +653,78 @@ __global__ __launch_bounds__(THREADS) void segment_sweep_kernel(const SweepSegme
     const int64_t tile = b - segs[lo].tile0;
     if (tile * ROWS >= A.h) return;
     sweep_tile<ROWS, THREADS>(A, tile);
+}
+
+// Transform layouts: workgroup b runs every epoch of tile b - tile0[s] of
+// the request s whose tile range holds b. Epoch t's arguments are those of
+// the per-epoch launch: the schedule, the Jacobi buffers t & 1 -> (t + 1) & 1
+// and the negative tables of the epoch's direction (layouts of at most 1024
+// rows: identity order at epoch 0, direction t - 1 after; larger layouts:
+// direction t, with each row's window rank the lower bound of its own
+// projection among the tail's sorted projections, as the per-epoch
+// projection and lower-bound launches compute it). A row's positions, Adam
+// moments, clocks and rank are read and written by the same lanes in every
+// epoch, so no cross-workgroup ordering is needed.
+constexpr int kTransformRows = 4;
+constexpr int kTransformThreads = 64;
+
+template <int ROWS, int THREADS>
+__global__ __launch_bounds__(THREADS) void transform_layout_kernel(const TransformSegment *segs,
+                                                                   int32_t n_segs,
+                                                                   TransformTables T) {
+    const int64_t b = blockIdx.x;
+    int lo = 0, hi = n_segs;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (segs[mid].tile0 <= b) lo = mid;
+        else hi = mid;
+    }
+    const TransformSegment &S = segs[lo];
+    SweepArgs A = S.args;
+    const int64_t tile = b - S.tile0;
+    if (tile * ROWS >= A.h) return;
+    float *buf[2] = {const_cast<float *>(A.pos), A.pos_out};
+    const bool small = A.h <= 1024;
+    const int64_t n = T.n;
+    const int tid = threadIdx.x;
+    const int64_t row = tile * ROWS + tid;
+    const bool own = tid < ROWS && row < A.h;
+    for (int32_t t = 0; t < T.epochs; ++t) {
+        A.pos = buf[t & 1];
+        A.pos_out = buf[(t + 1) & 1];
+        A.epoch = static_cast<uint32_t>(t);
+        A.alpha = T.alpha[t];
+        A.c1 = T.c1[t];
+        A.c2 = T.c2[t];
+        if (small) {
+            A.table = t == 0 ? T.tail : T.table + (t - 1) * n * 2;
+            A.order = t == 0 ? T.iota : T.order + (t - 1) * n;
+            A.ranks = nullptr;
+        } else {
+            const float *sorted = T.sorted + t * n;
+            A.table = T.table + t * n * 2;
+            A.order = T.order + t * n;
+            A.ranks = S.ranks;
+            if (own) {
+                const float2 p = ld2(A.pos, row);
+                float sum = 0.0f;
+                sum += p.x * T.dirs[2 * t];
+                sum += p.y * T.dirs[2 * t + 1];
+                if (!isfinite(sum))
+                    atomicMin(S.status, (static_cast<unsigned long long>(t) << 32) |
+                                            static_cast<unsigned long long>(row));
+                int64_t l = 0, h = n;
+                while (l < h) {
+                    const int64_t mid = l + (h - l) / 2;
+                    if (sorted[mid] < sum) l = mid + 1;
+                    else h = mid;
+                }
+                S.ranks[row] = static_cast<int32_t>(l < n - 1 ? l : n - 1);
+            }
+        }
+        sweep_tile<ROWS, THREADS>(A, tile);
+        __syncthreads();
+    }
 }
 
 // ------------------------------------------------- edge-balanced unit sweep
@@ -1613,6 +1645,17 @@ void launch_segment_sweep(const SweepSegment *segs, int32_t n_segs, int64_t tota
                                s, segs, n_segs);
             break;
     }
+    HIP_CHECK(hipGetLastError());
+}
+
+int64_t transform_tile_rows() { return kTransformRows; }
+
+void launch_transform_layouts(const TransformSegment *segs, int32_t n_segs, int64_t total_tiles,
+                              const TransformTables &tables, hipStream_t s) {
+    if (n_segs == 0 || total_tiles == 0 || tables.epochs == 0) return;
+    hipLaunchKernelGGL((transform_layout_kernel<kTransformRows, kTransformThreads>),
+                       dim3(static_cast<unsigned>(total_tiles)), dim3(kTransformThreads), 0, s, segs,
+                       n_segs, tables);
     HIP_CHECK(hipGetLastError());
 }
 

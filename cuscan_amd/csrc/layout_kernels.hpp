@@ -72,6 +72,34 @@ int64_t tile_rows(int shape = 0);
 void launch_segment_sweep(const SweepSegment *segs, int32_t n_segs, int64_t total_tiles,
                           hipStream_t s, int shape = 0);
 
+// Whole transform layouts in one launch. In a transform every row moves
+// alone against the fixed tail, and the tail's negative tables for each
+// epoch are fixed in advance, so a workgroup can run all epochs of its
+// tile without any grid-wide step. Segment s holds one request: `args`
+// carries its epoch-invariant arguments with pos = the initial positions
+// and pos_out = a second buffer (epoch t reads buffer t & 1); tiles of
+// transform_tile_rows() rows; segments in ascending tile0 order.
+struct TransformSegment {
+    SweepArgs args;
+    int64_t tile0;
+    int32_t *ranks;                   // [h] window ranks (h > 1024), rewritten each epoch
+    unsigned long long *status;       // min(epoch << 32 | row) over non-finite projections
+};
+struct TransformTables {
+    const float *alpha, *c1, *c2;     // [epochs] schedule and bias corrections
+    const int32_t *iota;              // identity order (h <= 1024, epoch 0)
+    const float *tail;                // [n, 2]
+    const int32_t *order;             // [epochs, n] projection order of each direction
+    const float *sorted;              // [epochs, n] sorted projections
+    const float *table;               // [epochs, n, 2] tail positions by rank
+    const float *dirs;                // [epochs, 2]
+    int64_t n;
+    int32_t epochs;
+};
+int64_t transform_tile_rows();
+void launch_transform_layouts(const TransformSegment *segs, int32_t n_segs, int64_t total_tiles,
+                              const TransformTables &tables, hipStream_t s);
+
 // Edge-balanced unit sweep for Adam layouts with H > 1024 and d = 2: one
 // wave per unit_edges() consecutive CSR edges; writes per-row gradients
 // (grad) and, for rows crossing a unit boundary, per-unit fragments.

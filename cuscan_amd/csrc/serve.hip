@@ -12,14 +12,14 @@
 //     on the request.
 // The server therefore keeps the references (BF16 + f32), the embedding and
 // the per-direction projection tables (order, sorted keys, tail positions by
-// rank) resident, and runs a step loop: each step advances every active
-// request by one epoch of its own (its own epoch index, schedule and
-// negative tables) in one batched tile-sweep launch. New requests are
-// admitted whenever HBM in a fixed arena and a kNN batch slot are free;
-// their kNN runs on a second stream while the step loop continues, and they
-// join at the next step. A request's rows are computed exactly as the
-// single-request transform computes them (same kernels, same per-row
-// summation order), so results do not depend on what else is in flight.
+// rank) resident. Requests are admitted whenever HBM in a fixed arena and a
+// kNN batch slot are free; a batch's kNN runs on its own stream, the host
+// then builds each request's graph, and the whole batch's transform layouts
+// (every epoch of every request) run in one launch on one of a few layout
+// streams, so batches overlap on the device and no host step loop paces the
+// epochs. A request's rows are computed exactly as the single-request
+// transform computes them (same tile sweep, same per-row summation order),
+// so results do not depend on what else is in flight.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -104,7 +104,8 @@ struct FifoArena {
 struct Request {
     int64_t ticket = 0, q = 0;
     std::vector<float> x;
-    double t_submit = 0, t_join = 0, t_done = 0;
+    // submitted, admitted to a kNN batch, kNN seen finished, layout launched, done
+    double t_submit = 0, t_admit = 0, t_knn = 0, t_join = 0, t_done = 0;
     // layout state (device, in the arena)
     size_t arena_off = 0;
     int64_t E = 0;
@@ -113,7 +114,7 @@ struct Request {
     double *per = nullptr, *npos = nullptr, *nper = nullptr, *nneg = nullptr;
     float *pos[2] = {nullptr, nullptr}, *m = nullptr, *v = nullptr, *proj = nullptr;
     unsigned long long *status = nullptr;
-    int epoch = 0, cur = 0;
+    int cur = 0;                 // buffer that holds the result
     hipEvent_t ev_final = nullptr;
     // result
     std::vector<float> out;
@@ -148,36 +149,40 @@ struct umaphip_server {
     DeviceArray<int32_t> iota, cache_order;
     DeviceArray<float> cache_sorted, cache_table;
     std::vector<float> alpha, c1, c2;
+    DeviceArray<float> sched;          // [3, epochs]: alpha, c1, c2
     // streams and staging
-    hipStream_t sl = nullptr, sk = nullptr;
+    static constexpr int kLayoutStreams = 3;
+    hipStream_t sk = nullptr, sl[kLayoutStreams] = {};
     int64_t max_batch = 0;
     DeviceArray<float> q_dev, d_dist;
     DeviceArray<int32_t> d_idx;
     std::unique_ptr<Pinned> q_pin, idx_pin, dist_pin, up_pin;
     size_t up_cap = 0;
     hipEvent_t ev_knn = nullptr, ev_up = nullptr;
-    static constexpr int kWindow = 8;
-    std::unique_ptr<Pinned> seg_pin[kWindow];
-    DeviceArray<SweepSegment> seg_dev[kWindow];
-    hipEvent_t ev_step[kWindow] = {};
-    int32_t seg_cap = 0;
-    int64_t step_index = 0;
+    // segment tables of in-flight layout launches (a ring; a slot is reused
+    // once its launch has finished)
+    static constexpr int kSlots = 8;
+    std::unique_ptr<Pinned> seg_pin[kSlots];
+    DeviceArray<TransformSegment> seg_dev[kSlots];
+    hipEvent_t ev_slot[kSlots] = {};
+    int64_t launch_index = 0;
     DeviceArray<uint8_t> arena_mem;
     FifoArena arena;
     // queues
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<std::shared_ptr<Request>> pending;
-    std::vector<std::shared_ptr<Request>> knn_batch, active;
-    std::deque<std::shared_ptr<Request>> finishing;
+    std::vector<std::shared_ptr<Request>> knn_batch;
+    std::deque<std::shared_ptr<Request>> finishing;   // admission order
     std::unordered_map<int64_t, std::shared_ptr<Request>> tickets;
     int64_t next_ticket = 1;
     bool stop = false;
     std::string fatal;
     std::thread worker;
     // statistics
-    int64_t st_requests = 0, st_rows = 0, st_steps = 0, st_knn_batches = 0, st_seg_sum = 0,
+    int64_t st_requests = 0, st_rows = 0, st_launches = 0, st_knn_batches = 0, st_seg_sum = 0,
             st_peak_active = 0, st_rejected_wait = 0;
+    double st_t_queue = 0, st_t_knn = 0, st_t_graph = 0, st_t_layout = 0;   // summed seconds
 
     ~umaphip_server() {
         {
@@ -186,17 +191,17 @@ struct umaphip_server {
         }
         cv_work.notify_all();
         if (worker.joinable()) worker.join();
-        if (sl) (void)hipStreamSynchronize(sl);
+        for (auto st : sl)
+            if (st) (void)hipStreamSynchronize(st);
         if (sk) (void)hipStreamSynchronize(sk);
-        for (auto &e : ev_step)
+        for (auto &e : ev_slot)
             if (e) (void)hipEventDestroy(e);
         if (ev_knn) (void)hipEventDestroy(ev_knn);
         if (ev_up) (void)hipEventDestroy(ev_up);
-        for (auto &r : active)
-            if (r->ev_final) (void)hipEventDestroy(r->ev_final);
         for (auto &r : finishing)
             if (r->ev_final) (void)hipEventDestroy(r->ev_final);
-        if (sl) (void)hipStreamDestroy(sl);
+        for (auto st : sl)
+            if (st) (void)hipStreamDestroy(st);
         if (sk) (void)hipStreamDestroy(sk);
     }
 
@@ -223,6 +228,7 @@ struct umaphip_server {
                     break;
                 }
                 r->arena_off = off;
+                r->t_admit = now_seconds();
                 rows += r->q;
                 batch.push_back(r);
                 pending.pop_front();
@@ -246,15 +252,19 @@ struct umaphip_server {
         ++st_knn_batches;
     }
 
-    // ---- host graph for a finished kNN batch, upload, join the step loop
+    // ---- host graphs of a finished kNN batch, upload, one layout launch
     void join_knn_batch() {
         HIP_CHECK(hipEventSynchronize(ev_up));  // previous uploads left the staging buffer
+        const int slot = static_cast<int>(launch_index % kSlots);
+        hipStream_t ls = sl[launch_index % kLayoutStreams];
+        HIP_CHECK(hipEventSynchronize(ev_slot[slot]));  // the slot's previous launch finished
+        const double t_knn = now_seconds();
         const int32_t *idx_all = static_cast<const int32_t *>(idx_pin->ptr);
         const float *dist_all = static_cast<const float *>(dist_pin->ptr);
         uint8_t *up = static_cast<uint8_t *>(up_pin->ptr);
         size_t up_used = 0;
         int64_t row = 0;
-        std::vector<std::shared_ptr<Request>> joined, failed;
+        std::vector<std::shared_ptr<Request>> joined;
         for (auto &r : knn_batch) {
             const int32_t *idx = idx_all + row * k;
             const float *dist = dist_all + row * k;
@@ -278,13 +288,11 @@ struct umaphip_server {
                                             init.data());
             if (st != UMAPHIP_OK) {
                 fail_request(r, st, umaphip_last_error());
-                failed.push_back(r);
                 continue;
             }
             if (epochs == 0) {
                 r->out = init;
                 r->host_ready = true;
-                failed.push_back(r);  // finished without a layout
                 continue;
             }
             std::vector<uint32_t> off(r->q + 1), ind(std::max<int64_t>(used, 1));
@@ -299,7 +307,6 @@ struct umaphip_server {
                                        &E);
             if (st != UMAPHIP_OK) {
                 fail_request(r, st, umaphip_last_error());
-                failed.push_back(r);
                 continue;
             }
             // carve the request's arena block
@@ -326,7 +333,7 @@ struct umaphip_server {
             r->status = reinterpret_cast<unsigned long long *>(take(8));
             auto stage = [&](void *dst, const void *src, size_t bytes) {
                 std::memcpy(up + up_used, src, bytes);
-                HIP_CHECK(hipMemcpyAsync(dst, up + up_used, bytes, hipMemcpyHostToDevice, sl));
+                HIP_CHECK(hipMemcpyAsync(dst, up + up_used, bytes, hipMemcpyHostToDevice, ls));
                 up_used += align256(bytes);
             };
             stage(r->offsets, off.data(), (r->q + 1) * 4);
@@ -336,116 +343,90 @@ struct umaphip_server {
             stage(r->nper, nper.data(), E * 8);
             stage(r->nneg, nneg.data(), E * 8);
             stage(r->pos[0], init.data(), r->q * d * 4);
-            HIP_CHECK(hipMemsetAsync(r->m, 0, r->q * d * 4, sl));
-            HIP_CHECK(hipMemsetAsync(r->v, 0, r->q * d * 4, sl));
-            launch_status_min_u64(r->status, ~0ull, sl);
-            r->epoch = 0;
-            r->cur = 0;
-            r->t_join = now_seconds();
+            HIP_CHECK(hipMemsetAsync(r->m, 0, r->q * d * 4, ls));
+            HIP_CHECK(hipMemsetAsync(r->v, 0, r->q * d * 4, ls));
+            launch_status_min_u64(r->status, ~0ull, ls);
+            r->cur = static_cast<int>(epochs & 1);
             joined.push_back(r);
         }
-        HIP_CHECK(hipEventRecord(ev_up, sl));
-        std::lock_guard<std::mutex> g(mu);
-        // failed requests still hold their arena block in FIFO order: they
-        // retire through the finishing queue behind earlier requests
-        for (auto &r : knn_batch) {
-            if (r->code != UMAPHIP_OK || r->host_ready) {
-                HIP_CHECK(hipEventCreateWithFlags(&r->ev_final, hipEventDisableTiming));
-                HIP_CHECK(hipEventRecord(r->ev_final, sl));
-                finishing.push_back(r);
-            } else {
-                active.push_back(r);
+        HIP_CHECK(hipEventRecord(ev_up, ls));
+        // one launch runs every epoch of every joined request
+        const int32_t ns = static_cast<int32_t>(joined.size());
+        if (ns > 0) {
+            TransformSegment *segs = static_cast<TransformSegment *>(seg_pin[slot]->ptr);
+            int64_t tiles = 0;
+            for (int32_t s = 0; s < ns; ++s) {
+                Request &r = *joined[s];
+                SweepArgs A{};
+                A.h = r.q; A.n = n;
+                A.pos = r.pos[0]; A.pos_out = r.pos[1];
+                A.tgt = tail.ptr;
+                A.offsets = r.offsets; A.indices = r.indices;
+                A.periods = r.per; A.next_positive = r.npos;
+                A.negative_periods = r.nper; A.next_negative = r.nneg;
+                A.m = r.m; A.v = r.v;
+                A.scale = scale;
+                A.a = a; A.b = b; A.gamma = gamma; A.range = range;
+                A.multiplier = 1.0f;
+                A.shared = 0; A.negatives = rate != 0; A.exclude = exclude; A.fused = 1;
+                segs[s].args = A;
+                segs[s].tile0 = tiles;
+                segs[s].ranks = r.ranks;
+                segs[s].status = r.status;
+                tiles += (r.q + transform_tile_rows() - 1) / transform_tile_rows();
             }
+            HIP_CHECK(hipMemcpyAsync(seg_dev[slot].ptr, segs, sizeof(TransformSegment) * ns,
+                                     hipMemcpyHostToDevice, ls));
+            TransformTables T{};
+            T.alpha = sched.ptr;
+            T.c1 = sched.ptr + epochs;
+            T.c2 = sched.ptr + 2 * epochs;
+            T.iota = iota.ptr;
+            T.tail = tail.ptr;
+            T.order = cache_order.ptr;
+            T.sorted = cache_sorted.ptr;
+            T.table = cache_table.ptr;
+            T.dirs = dirs.ptr;
+            T.n = n;
+            T.epochs = static_cast<int32_t>(epochs);
+            launch_transform_layouts(seg_dev[slot].ptr, ns, tiles, T, ls);
+            HIP_CHECK(hipEventRecord(ev_slot[slot], ls));
+            ++launch_index;
         }
-        st_peak_active = std::max<int64_t>(st_peak_active, static_cast<int64_t>(active.size()));
+        const double t_join = now_seconds();
+        for (auto &r : knn_batch) {
+            r->t_knn = t_knn;
+            r->t_join = t_join;
+            HIP_CHECK(hipEventCreateWithFlags(&r->ev_final, hipEventDisableTiming));
+            HIP_CHECK(hipEventRecord(r->ev_final, ls));
+        }
+        std::lock_guard<std::mutex> g(mu);
+        for (auto &r : knn_batch) finishing.push_back(r);
+        if (ns > 0) {
+            ++st_launches;
+            st_seg_sum += ns;
+        }
+        st_peak_active = std::max<int64_t>(st_peak_active, static_cast<int64_t>(finishing.size()));
         knn_batch.clear();
     }
 
-    // ---- one step: every active request advances one epoch
-    void step() {
-        const int slot = static_cast<int>(step_index % kWindow);
-        HIP_CHECK(hipEventSynchronize(ev_step[slot]));
-        const int32_t ns = static_cast<int32_t>(active.size());
-        if (ns > seg_cap) {
-            seg_cap = std::max<int32_t>(ns, 2 * seg_cap);
-            for (int w = 0; w < kWindow; ++w) {
-                HIP_CHECK(hipEventSynchronize(ev_step[w]));
-                seg_pin[w].reset(new Pinned(sizeof(SweepSegment) * seg_cap));
-                seg_dev[w].alloc(seg_cap);
-            }
-        }
-        SweepSegment *segs = static_cast<SweepSegment *>(seg_pin[slot]->ptr);
-        int64_t tiles = 0;
-        for (int32_t s = 0; s < ns; ++s) {
-            Request &r = *active[s];
-            const int t = r.epoch;
-            SweepArgs A{};
-            A.h = r.q; A.n = n;
-            A.pos = r.pos[r.cur]; A.pos_out = r.pos[1 - r.cur];
-            A.tgt = tail.ptr;
-            if (r.small) {
-                // small layouts: epoch 0 identity order, epoch t >= 1 the
-                // order of direction t - 1 (modern_runtime.rs:1128-1135)
-                A.table = t == 0 ? tail.ptr : cache_table.ptr + (t - 1) * n * 2;
-                A.order = t == 0 ? iota.ptr : cache_order.ptr + (t - 1) * n;
-                A.ranks = nullptr;
-            } else {
-                const float *dir = dirs.ptr + t * d;
-                launch_project_keys(r.pos[r.cur], r.q, d, dir, nullptr, r.proj, r.status,
-                                    static_cast<unsigned long long>(t) << 32, sl);
-                launch_lower_bound(cache_sorted.ptr + t * n, n, r.proj, r.q, r.ranks, sl);
-                A.table = cache_table.ptr + t * n * 2;
-                A.order = cache_order.ptr + t * n;
-                A.ranks = r.ranks;
-            }
-            A.offsets = r.offsets; A.indices = r.indices;
-            A.periods = r.per; A.next_positive = r.npos;
-            A.negative_periods = r.nper; A.next_negative = r.nneg;
-            A.m = r.m; A.v = r.v;
-            A.scale = scale;
-            A.epoch = static_cast<uint32_t>(t);
-            A.a = a; A.b = b; A.gamma = gamma; A.range = range;
-            A.multiplier = 1.0f; A.alpha = alpha[t]; A.c1 = c1[t]; A.c2 = c2[t];
-            A.shared = 0; A.negatives = rate != 0; A.exclude = exclude; A.fused = 1;
-            segs[s].args = A;
-            segs[s].tile0 = tiles;
-            tiles += (r.q + tile_rows() - 1) / tile_rows();
-        }
-        HIP_CHECK(hipMemcpyAsync(seg_dev[slot].ptr, segs, sizeof(SweepSegment) * ns,
-                                 hipMemcpyHostToDevice, sl));
-        launch_segment_sweep(seg_dev[slot].ptr, ns, tiles, sl);
-        HIP_CHECK(hipEventRecord(ev_step[slot], sl));
-        ++step_index;
-        ++st_steps;
-        st_seg_sum += ns;
-        std::vector<std::shared_ptr<Request>> still;
-        still.reserve(active.size());
-        for (auto &rp : active) {
-            rp->cur = 1 - rp->cur;
-            rp->epoch += 1;
-            if (rp->epoch == epochs) {
-                HIP_CHECK(hipEventCreateWithFlags(&rp->ev_final, hipEventDisableTiming));
-                HIP_CHECK(hipEventRecord(rp->ev_final, sl));
-                std::lock_guard<std::mutex> g(mu);
-                finishing.push_back(rp);
-            } else {
-                still.push_back(rp);
-            }
-        }
-        active.swap(still);
-    }
-
-    // ---- retire finished requests in FIFO order
+    // ---- retire every finished request (launches on different streams may
+    // finish out of admission order; the arena frees blocks by offset)
     bool retire() {
-        bool any = false;
-        for (;;) {
-            std::shared_ptr<Request> r;
-            {
-                std::lock_guard<std::mutex> g(mu);
-                if (finishing.empty()) break;
-                r = finishing.front();
+        std::vector<std::shared_ptr<Request>> ready;
+        {
+            std::lock_guard<std::mutex> g(mu);
+            for (auto it = finishing.begin(); it != finishing.end();) {
+                if (hipEventQuery((*it)->ev_final) == hipSuccess) {
+                    ready.push_back(*it);
+                    it = finishing.erase(it);
+                } else {
+                    ++it;
+                }
             }
-            if (hipEventQuery(r->ev_final) != hipSuccess) break;
+        }
+        if (ready.empty()) return false;
+        for (auto &r : ready) {
             (void)hipEventDestroy(r->ev_final);
             r->ev_final = nullptr;
             if (r->code == UMAPHIP_OK && !r->host_ready) {
@@ -458,17 +439,24 @@ struct umaphip_server {
                 for (float x : r->out) finite = finite && std::isfinite(x);
                 if (!finite) fail_request(r, UMAPHIP_NUMERICAL, "non-finite transform coordinates");
             }
-            std::lock_guard<std::mutex> g(mu);
-            r->t_done = now_seconds();
-            r->done = true;
-            finishing.pop_front();
-            arena.release(r->arena_off);
-            ++st_requests;
-            st_rows += r->q;
-            any = true;
         }
-        if (any) cv_done.notify_all();
-        return any;
+        {
+            std::lock_guard<std::mutex> g(mu);
+            const double t = now_seconds();
+            for (auto &r : ready) {
+                r->t_done = t;
+                st_t_queue += r->t_admit - r->t_submit;
+                st_t_knn += r->t_knn - r->t_admit;
+                st_t_graph += r->t_join - r->t_knn;
+                st_t_layout += r->t_done - r->t_join;
+                r->done = true;
+                arena.release(r->arena_off);
+                ++st_requests;
+                st_rows += r->q;
+            }
+        }
+        cv_done.notify_all();
+        return true;
     }
 
     void loop() {
@@ -479,7 +467,7 @@ struct umaphip_server {
                 {
                     std::unique_lock<std::mutex> g(mu);
                     if (stop) break;
-                    if (pending.empty() && active.empty() && knn_batch.empty() && finishing.empty())
+                    if (pending.empty() && knn_batch.empty() && finishing.empty())
                         cv_work.wait_for(g, std::chrono::milliseconds(5));
                     if (stop) break;
                 }
@@ -488,14 +476,8 @@ struct umaphip_server {
                     join_knn_batch();
                     busy = true;
                 }
-                if (!active.empty()) {
-                    step();
-                    busy = true;
-                }
                 busy = retire() || busy;
-                if (!busy && !finishing.empty())
-                    std::this_thread::sleep_for(std::chrono::microseconds(50));
-                else if (!busy && !knn_batch.empty())
+                if (!busy && (!finishing.empty() || !knn_batch.empty()))
                     std::this_thread::sleep_for(std::chrono::microseconds(20));
             }
         } catch (const Fail &) {
@@ -508,7 +490,6 @@ struct umaphip_server {
                     r->done = true;
                 }
             };
-            fail_all(active);
             fail_all(knn_batch);
             fail_all(pending);
             fail_all(finishing);
@@ -557,14 +538,25 @@ umaphip_status umaphip_server_create(const float *reference, int64_t n, int64_t 
         schedules(static_cast<int>(S->epochs), S->alpha0, false, false, S->alpha, mu);
         bias_corrections(static_cast<int>(S->epochs), S->c1, S->c2);
         S->max_batch = p.max_batch_rows;
-        HIP_CHECK(hipStreamCreateWithFlags(&S->sl, hipStreamNonBlocking));
+        for (auto &st : S->sl) HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         HIP_CHECK(hipStreamCreateWithFlags(&S->sk, hipStreamNonBlocking));
         HIP_CHECK(hipEventCreateWithFlags(&S->ev_knn, hipEventDisableTiming));
         HIP_CHECK(hipEventCreateWithFlags(&S->ev_up, hipEventDisableTiming));
-        HIP_CHECK(hipEventRecord(S->ev_up, S->sl));
-        for (auto &e : S->ev_step) {
-            HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-            HIP_CHECK(hipEventRecord(e, S->sl));
+        HIP_CHECK(hipEventRecord(S->ev_up, S->sl[0]));
+        for (int w = 0; w < umaphip_server::kSlots; ++w) {
+            HIP_CHECK(hipEventCreateWithFlags(&S->ev_slot[w], hipEventDisableTiming));
+            HIP_CHECK(hipEventRecord(S->ev_slot[w], S->sl[0]));
+            // a batch holds at most max_batch requests (each has a row)
+            S->seg_pin[w].reset(new Pinned(sizeof(TransformSegment) * S->max_batch));
+            S->seg_dev[w].alloc(S->max_batch);
+        }
+        if (S->epochs > 0) {
+            std::vector<float> sched(S->alpha);
+            sched.insert(sched.end(), S->c1.begin(), S->c1.end());
+            sched.insert(sched.end(), S->c2.begin(), S->c2.end());
+            S->sched.alloc(sched.size());
+            HIP_CHECK(hipMemcpy(S->sched.ptr, sched.data(), sched.size() * sizeof(float),
+                                hipMemcpyHostToDevice));
         }
         // resident model
         S->index.reset(new KnnIndex(reference, nullptr, n, features, S->max_batch, S->sk));
@@ -697,15 +689,20 @@ umaphip_status umaphip_server_stats(umaphip_server *server, umaphip_server_count
         std::lock_guard<std::mutex> g(server->mu);
         stats->requests = server->st_requests;
         stats->rows = server->st_rows;
-        stats->steps = server->st_steps;
+        stats->steps = server->st_launches;
         stats->knn_batches = server->st_knn_batches;
-        stats->mean_active = server->st_steps ? static_cast<double>(server->st_seg_sum) /
-                                                    static_cast<double>(server->st_steps)
-                                              : 0.0;
+        stats->mean_active = server->st_launches ? static_cast<double>(server->st_seg_sum) /
+                                                       static_cast<double>(server->st_launches)
+                                                 : 0.0;
         stats->peak_active = server->st_peak_active;
         stats->admission_waits = server->st_rejected_wait;
         stats->memory_budget = static_cast<int64_t>(server->arena.cap);
         stats->memory_in_use = static_cast<int64_t>(server->arena.in_use());
+        const double nreq = server->st_requests ? static_cast<double>(server->st_requests) : 1.0;
+        stats->mean_queue_ms = server->st_t_queue / nreq * 1e3;
+        stats->mean_knn_ms = server->st_t_knn / nreq * 1e3;
+        stats->mean_graph_ms = server->st_t_graph / nreq * 1e3;
+        stats->mean_layout_ms = server->st_t_layout / nreq * 1e3;
     });
 }
 

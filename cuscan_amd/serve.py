@@ -4,11 +4,10 @@
 embedding and the per-epoch projection tables of the embedding in HBM and
 serves ``UMAP.transform`` (crates/umap/src/standard_umap/estimator.rs:
 1367-2264) for many concurrent requests: requests are admitted while a
-fixed device arena has room, their kNN runs on its own stream, and every
-step advances each active request by one epoch in one batched launch. A
-request of up to 1024 rows gets exactly the coordinates ``model.transform``
-computes for it alone; larger requests differ only in the summation order
-of their rows' forces.
+fixed device arena has room, their kNN runs in batches on its own stream,
+and each batch's layouts (every epoch of every request) run in one launch.
+Every request gets exactly the coordinates ``model.transform`` computes for
+it alone.
 """
 from __future__ import annotations
 

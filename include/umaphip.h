@@ -358,10 +358,11 @@ umaphip_status umaphip_classic_optimize(int64_t h, int64_t n, int64_t d, float *
  * FittedUMAP::transform, crates/umap/src/standard_umap/estimator.rs:
  * 1367-2264, for many concurrent requests). The fitted references, the
  * embedding and the per-epoch projection tables of the embedding stay
- * resident; requests are admitted while a fixed device arena has room and
- * advance one epoch per batched step. A request's result equals what
- * UMAP.transform computes for it alone (same kernels and per-row order for
- * requests of up to 1024 rows).
+ * resident; requests are admitted while a fixed device arena has room, their
+ * kNN runs in batches, and each batch's transform layouts (all epochs of all
+ * its requests) run in one launch. A request's result equals what
+ * UMAP.transform computes for it alone, bit for bit (same tile sweep and
+ * per-row summation order).
  * ------------------------------------------------------------------------ */
 typedef struct umaphip_server umaphip_server;
 typedef struct {
@@ -383,9 +384,15 @@ typedef struct {
     int64_t memory_bytes;           /* admission arena; 0 = min(free/4, 1 GiB) */
 } umaphip_server_params;
 typedef struct {
-    int64_t requests, rows, steps, knn_batches;
-    double mean_active;             /* requests per step */
-    int64_t peak_active, admission_waits, memory_budget, memory_in_use;
+    int64_t requests, rows;
+    int64_t steps;                  /* layout launches (one per kNN batch with a layout) */
+    int64_t knn_batches;
+    double mean_active;             /* requests per layout launch */
+    int64_t peak_active;            /* most requests in flight after their kNN */
+    int64_t admission_waits, memory_budget, memory_in_use;
+    /* mean per-request stage times: waiting for admission, kNN batch (admit
+     * to kNN seen finished), host graph + upload + launch, layout + retire */
+    double mean_queue_ms, mean_knn_ms, mean_graph_ms, mean_layout_ms;
 } umaphip_server_counters;
 /* Builds the resident model (copies reference [n, features] and embedding
  * [n, d] from host memory) and starts the server thread. */

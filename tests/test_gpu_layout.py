@@ -292,8 +292,7 @@ DRIVER = [
     ("fit_adam_global_rate24_hubs", 3000, 3000, 2, True, "adam", 200_000, True, 10),
 ]
 # cases that must run the edge-balanced unit sweep
-UNIT_PATH = {"fit_adam_global_excl", "fit_adam_windows_adapt", "transform_large_window",
-             "fit_adam_global_hubs", "fit_adam_windows_hubs_excl", "fit_adam_global_rate24_hubs"}
+UNIT_PATH = {"fit_adam_global_excl", "fit_adam_windows_adapt", "fit_adam_global_hubs", "fit_adam_windows_hubs_excl", "fit_adam_global_rate24_hubs"}
 
 
 @pytest.mark.parametrize("case", DRIVER, ids=[c[0] for c in DRIVER])
