@@ -300,6 +300,14 @@ struct umaphip_plan {
         const float *dir = dirs.ptr + k * d;
         const float *reference = shared ? head_now : tail.ptr;
         const unsigned long long tag = static_cast<unsigned long long>(k) << 32;
+        if (shared && d == 2) {
+            // projection, its first histogram and the four sort passes; the
+            // last pass writes the ranks and the table
+            launch_project_sort_ranked(sort_ws.ptr, sort_ws.count, head_now, n, dir, status.ptr, tag,
+                                       keys.ptr, keys_sorted.ptr, proj_order.ptr, ranks.ptr,
+                                       table.ptr, stream);
+            return;
+        }
         launch_project_keys(reference, n, d, dir, keys.ptr, nullptr, status.ptr, tag, stream);
         if (shared) {
             // the sort's last pass writes the ranks and (d = 2) the table
@@ -662,6 +670,7 @@ umaphip_status umaphip_plan_create(int64_t h, int64_t n, int64_t d, const float 
             P->keys_sorted.alloc(n);
             P->proj_order.alloc(n);
             P->sort_ws.alloc(sort_pairs_workspace(n));
+            clear_sort_workspace(P->sort_ws.ptr, P->sort_ws.count, n, P->stream);
         }
         if (!P->hard) {
             P->host_orders[0].alloc(n);

@@ -1154,11 +1154,16 @@ __global__ __launch_bounds__(kObjThreads) void sum_partials_kernel(const double 
 // Stable LSD radix sort of (u32 key, i32 value) pairs, 8-bit digits, four
 // passes. Per pass: per-tile digit histograms (a tile is 2048 keys), a scan
 // that gives every (digit, tile) its output offset (digit-major, after all
-// smaller digits), and a scatter that ranks each key among the equal digits
-// of its tile in tile order -- lanes with the same digit are found with
-// ballots over the nine digit bits (eight plus "valid"), then counted
-// across the waves before them -- so equal keys keep their input order.
-// No kernel waits on another workgroup.
+// smaller digits), and a scatter. A tile's keys are taken wave-major (wave w
+// owns keys 512w..512w+511 of the tile, in rows of 64); the scatter ranks
+// each key among the equal digits before it -- lanes with the same digit are
+// found with ballots over the nine digit bits (eight plus "valid") and
+// counted into the wave's own digit counters row by row, then the four
+// waves' counters are prefixed -- so equal keys keep their input order. The
+// tile is then reordered in LDS by digit and written out in that order, so
+// consecutive lanes write consecutive positions of a digit's run. No kernel
+// waits on another workgroup. The digit totals of all passes are counted
+// once, before pass 0, into a workspace array that the last pass clears.
 constexpr int kSortThreads = 256;
 constexpr int kSortItems = 8;
 constexpr int kSortTile = kSortThreads * kSortItems;
@@ -1168,6 +1173,8 @@ __global__ __launch_bounds__(kSortThreads) void radix_zero_kernel(uint32_t *x, i
     if (i < n) x[i] = 0u;
 }
 
+// Per-tile histogram of one key byte (with totals: also the key-byte totals
+// of all four passes, accumulated into totals, which must be zero on entry).
 __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const uint32_t *keys, int64_t n,
                                                                  int shift, int64_t tiles,
                                                                  uint32_t *hist,
@@ -1240,30 +1247,41 @@ __global__ __launch_bounds__(kSortThreads) void radix_scan_kernel(uint32_t *hist
 __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     const uint32_t *keys_in, const int32_t *vals_in, uint32_t *keys_out, int32_t *vals_out,
     int64_t n, int shift, int64_t tiles, const uint32_t *offsets, int32_t *ranks_out,
-    const float *table_src, float *table_out) {
-    __shared__ uint32_t s_off[256];
-    __shared__ uint32_t s_wcnt[kSortThreads / 64][256];
+    const float *table_src, float *table_out, uint32_t *clear_totals) {
+    constexpr int WAVES = kSortThreads / 64;
+    constexpr int PER_WAVE = kSortTile / WAVES;
+    __shared__ uint32_t s_off[256];             // output position of the tile's run of each digit
+    __shared__ uint32_t s_start[256];           // start of each digit in the tile sorted by digit
+    __shared__ uint32_t s_wcnt[WAVES][256];     // per-wave digit counts, then their wave prefix
+    __shared__ uint32_t s_wave[WAVES];
+    __shared__ uint32_t s_key[kSortTile];
+    __shared__ int32_t s_val[kSortTile];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t base = static_cast<int64_t>(blockIdx.x) * kSortTile;
+    const int cnt = static_cast<int>(min<int64_t>(kSortTile, n - base));
+    if (clear_totals && blockIdx.x == 0)
+        for (int t = tid; t < 1024; t += kSortThreads) clear_totals[t] = 0u;
     s_off[tid] = offsets[static_cast<int64_t>(tid) * tiles + blockIdx.x];
-    for (int w = 0; w < kSortThreads / 64; ++w) s_wcnt[w][tid] = 0u;
+    for (int w = 0; w < WAVES; ++w) s_wcnt[w][tid] = 0u;
     uint32_t k[kSortItems];
     int32_t v[kSortItems];
 #pragma unroll
     for (int i = 0; i < kSortItems; ++i) {
-        const int64_t p = base + i * kSortThreads + tid;
+        const int local = wave * PER_WAVE + i * 64 + lane;
         k[i] = 0u;
         v[i] = 0;
-        if (p < n) {
-            k[i] = keys_in[p];
-            v[i] = vals_in ? vals_in[p] : static_cast<int32_t>(p);
+        if (local < cnt) {
+            k[i] = keys_in[base + local];
+            v[i] = vals_in ? vals_in[base + local] : static_cast<int32_t>(base + local);
         }
     }
     __syncthreads();
+    // rank of each key among the equal digits of its wave before it
     const uint64_t below_lane = (1ull << lane) - 1ull;
+    uint32_t lrank[kSortItems];
 #pragma unroll
     for (int i = 0; i < kSortItems; ++i) {
-        const bool valid = base + i * kSortThreads + tid < n;
+        const bool valid = wave * PER_WAVE + i * 64 + lane < cnt;
         const uint32_t digit = valid ? ((k[i] >> shift) & 255u) : 256u;
         uint64_t same = ~0ull;
 #pragma unroll
@@ -1272,30 +1290,46 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
             const uint64_t ballot = __ballot(set);
             same &= set ? ballot : ~ballot;
         }
-        const uint32_t rank = static_cast<uint32_t>(__popcll(same & below_lane));
-        if (valid && rank == 0u) s_wcnt[wave][digit] = static_cast<uint32_t>(__popcll(same));
-        __syncthreads();
-        uint32_t pos = 0u;
-        if (valid) {
-            uint32_t before = 0u;
-            for (int w = 0; w < wave; ++w) before += s_wcnt[w][digit];
-            pos = s_off[digit] + before + rank;
+        const uint32_t r = static_cast<uint32_t>(__popcll(same & below_lane));
+        const uint32_t before = valid ? s_wcnt[wave][digit] : 0u;
+        lrank[i] = before + r;
+        if (valid && r == 0u) s_wcnt[wave][digit] = before + static_cast<uint32_t>(__popcll(same));
+    }
+    __syncthreads();
+    // digit tid: wave prefix of its counts, and its start in the digit-sorted tile
+    uint32_t tot = 0u;
+    for (int w = 0; w < WAVES; ++w) {
+        const uint32_t c = s_wcnt[w][tid];
+        s_wcnt[w][tid] = tot;
+        tot += c;
+    }
+    uint32_t all;
+    s_start[tid] = block_exclusive_scan_u32(tot, s_wave, all);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kSortItems; ++i) {
+        if (wave * PER_WAVE + i * 64 + lane < cnt) {
+            const uint32_t digit = (k[i] >> shift) & 255u;
+            const uint32_t lp = s_start[digit] + s_wcnt[wave][digit] + lrank[i];
+            s_key[lp] = k[i];
+            s_val[lp] = v[i];
         }
-        __syncthreads();
-        uint32_t add = 0u;
-        for (int w = 0; w < kSortThreads / 64; ++w) {
-            add += s_wcnt[w][tid];
-            s_wcnt[w][tid] = 0u;
-        }
-        s_off[tid] += add;
-        __syncthreads();
-        if (valid) {
-            keys_out[pos] = k[i];
-            vals_out[pos] = v[i];
-            if (ranks_out) ranks_out[v[i]] = static_cast<int32_t>(pos);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kSortItems; ++i) {
+        const int idx = i * kSortThreads + tid;
+        if (idx < cnt) {
+            const uint32_t key = s_key[idx];
+            const int32_t val = s_val[idx];
+            const uint32_t digit = (key >> shift) & 255u;
+            const uint32_t pos = s_off[digit] + (static_cast<uint32_t>(idx) - s_start[digit]);
+            keys_out[pos] = key;
+            vals_out[pos] = val;
+            if (ranks_out) ranks_out[val] = static_cast<int32_t>(pos);
             if (table_out)
                 reinterpret_cast<float2 *>(table_out)[pos] =
-                    reinterpret_cast<const float2 *>(table_src)[v[i]];
+                    reinterpret_cast<const float2 *>(table_src)[val];
         }
     }
 }
@@ -1337,6 +1371,40 @@ __global__ void project_keys_kernel(const float *x, int64_t rows, int64_t d, con
     if (proj) proj[i] = sum;
     if (keys) keys[i] = radix_key(sum);
     if (!isfinite(sum)) atomicMin(status, tag | static_cast<unsigned long long>(i));
+}
+
+// Keys of the d = 2 projection x . dir for one sort tile (2048 rows): the
+// arithmetic of project_keys_kernel, plus the tile's histogram of the lowest
+// key byte and the key-byte totals of all four sort passes (totals must be
+// zero on entry).
+__global__ __launch_bounds__(kSortThreads) void project_keys_hist_kernel(
+    const float *x, int64_t rows, const float *dir, uint32_t *keys, unsigned long long *status,
+    unsigned long long tag, int64_t tiles, uint32_t *hist, uint32_t *totals) {
+    __shared__ uint32_t h[256];
+    __shared__ uint32_t t4[4][256];
+    const int tid = threadIdx.x;
+    h[tid] = 0u;
+    for (int d = 0; d < 4; ++d) t4[d][tid] = 0u;
+    __syncthreads();
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kSortTile;
+    for (int i = 0; i < kSortItems; ++i) {
+        const int64_t p = base + i * kSortThreads + tid;
+        if (p < rows) {
+            const float2 xy = reinterpret_cast<const float2 *>(x)[p];
+            float sum = 0.0f;
+            sum += xy.x * dir[0];
+            sum += xy.y * dir[1];
+            const uint32_t k = radix_key(sum);
+            keys[p] = k;
+            if (!isfinite(sum)) atomicMin(status, tag | static_cast<unsigned long long>(p));
+            atomicAdd(&h[k & 255u], 1u);
+            for (int d = 0; d < 4; ++d) atomicAdd(&t4[d][(k >> (8 * d)) & 255u], 1u);
+        }
+    }
+    __syncthreads();
+    hist[static_cast<int64_t>(tid) * tiles + blockIdx.x] = h[tid];
+    for (int d = 0; d < 4; ++d)
+        if (t4[d][tid]) atomicAdd(&totals[d * 256 + tid], t4[d][tid]);
 }
 
 __global__ void keys_from_values_kernel(const float *values, int64_t n, uint32_t *keys,
@@ -1749,43 +1817,85 @@ size_t sort_pairs_workspace(int64_t n) {
            up(1024 * 4);
 }
 
-// Four passes: in -> tmp -> out -> tmp -> out. With ranks (and table_src /
-// table_out for d = 2), the last pass also writes ranks[value] = position and
-// table_out[position] = table_src[value].
+namespace {
+struct SortWorkspace {
+    uint32_t *tmp_k;
+    int32_t *tmp_v;
+    uint32_t *hist, *totals;
+    int64_t tiles;
+};
+SortWorkspace sort_workspace(void *ws, size_t ws_bytes, int64_t n) {
+    UMAPHIP_REQUIRE(n <= INT32_MAX, UMAPHIP_OVERFLOW, "sort size must fit i32");
+    UMAPHIP_REQUIRE(ws_bytes >= sort_pairs_workspace(n), UMAPHIP_CAPACITY, "sort workspace too small");
+    auto up = [](size_t b) { return (b + 255) & ~static_cast<size_t>(255); };
+    uint8_t *w = static_cast<uint8_t *>(ws);
+    SortWorkspace W;
+    W.tiles = sort_tiles(n);
+    W.tmp_k = reinterpret_cast<uint32_t *>(w);
+    W.tmp_v = reinterpret_cast<int32_t *>(w + up(n * 4));
+    W.hist = reinterpret_cast<uint32_t *>(w + 2 * up(n * 4));
+    W.totals = reinterpret_cast<uint32_t *>(w + 2 * up(n * 4) + up(256 * W.tiles * 4));
+    return W;
+}
+
+// Passes 1..3 (pass 0 too unless its histogram is already in W.hist):
+// in -> tmp -> out -> tmp -> out; the last pass clears the totals.
+void sort_passes(const SortWorkspace &W, bool hist0_done, const uint32_t *keys_in,
+                 uint32_t *keys_out, const int32_t *vals_in, int32_t *vals_out, int64_t n,
+                 int32_t *ranks, const float *table_src, float *table_out, hipStream_t s) {
+    const unsigned tiles = static_cast<unsigned>(W.tiles);
+    const uint32_t *src_k = keys_in;
+    const int32_t *src_v = vals_in;
+    for (int pass = 0; pass < 4; ++pass) {
+        uint32_t *dst_k = (pass & 1) ? keys_out : W.tmp_k;
+        int32_t *dst_v = (pass & 1) ? vals_out : W.tmp_v;
+        const int shift = 8 * pass;
+        if (pass > 0 || !hist0_done)
+            hipLaunchKernelGGL(radix_hist_kernel, dim3(tiles), dim3(kSortThreads), 0, s, src_k, n,
+                               shift, W.tiles, W.hist, pass == 0 ? W.totals : nullptr);
+        hipLaunchKernelGGL(radix_scan_kernel, dim3(256), dim3(kSortThreads), 0, s, W.hist, W.tiles,
+                           W.totals + pass * 256);
+        const bool last = pass == 3;
+        hipLaunchKernelGGL(radix_scatter_kernel, dim3(tiles), dim3(kSortThreads), 0, s, src_k, src_v,
+                           dst_k, dst_v, n, shift, W.tiles, W.hist, last ? ranks : nullptr,
+                           last ? table_src : nullptr, last ? table_out : nullptr,
+                           last ? W.totals : nullptr);
+        src_k = dst_k;
+        src_v = dst_v;
+    }
+    HIP_CHECK(hipGetLastError());
+}
+}  // namespace
+
+// With ranks (and table_src / table_out for d = 2), the last pass also
+// writes ranks[value] = position and table_out[position] = table_src[value].
 void launch_sort_pairs_ranked(void *ws, size_t ws_bytes, const uint32_t *keys_in,
                               uint32_t *keys_out, const int32_t *vals_in, int32_t *vals_out,
                               int64_t n, int32_t *ranks, const float *table_src, float *table_out,
                               hipStream_t s) {
     if (n == 0) return;
-    UMAPHIP_REQUIRE(n <= INT32_MAX, UMAPHIP_OVERFLOW, "sort size must fit i32");
-    UMAPHIP_REQUIRE(ws_bytes >= sort_pairs_workspace(n), UMAPHIP_CAPACITY, "sort workspace too small");
-    auto up = [](size_t b) { return (b + 255) & ~static_cast<size_t>(255); };
-    uint8_t *w = static_cast<uint8_t *>(ws);
-    uint32_t *tmp_k = reinterpret_cast<uint32_t *>(w);
-    int32_t *tmp_v = reinterpret_cast<int32_t *>(w + up(n * 4));
-    const int64_t tiles = sort_tiles(n);
-    uint32_t *hist = reinterpret_cast<uint32_t *>(w + 2 * up(n * 4));
-    uint32_t *totals = reinterpret_cast<uint32_t *>(w + 2 * up(n * 4) + up(256 * tiles * 4));
-    hipLaunchKernelGGL(radix_zero_kernel, dim3(4), dim3(kSortThreads), 0, s, totals, int64_t(1024));
-    const uint32_t *src_k = keys_in;
-    const int32_t *src_v = vals_in;
-    for (int pass = 0; pass < 4; ++pass) {
-        uint32_t *dst_k = (pass & 1) ? keys_out : tmp_k;
-        int32_t *dst_v = (pass & 1) ? vals_out : tmp_v;
-        const int shift = 8 * pass;
-        hipLaunchKernelGGL(radix_hist_kernel, dim3(static_cast<unsigned>(tiles)), dim3(kSortThreads),
-                           0, s, src_k, n, shift, tiles, hist, pass == 0 ? totals : nullptr);
-        hipLaunchKernelGGL(radix_scan_kernel, dim3(256), dim3(kSortThreads), 0, s, hist, tiles,
-                           totals + pass * 256);
-        const bool last = pass == 3;
-        hipLaunchKernelGGL(radix_scatter_kernel, dim3(static_cast<unsigned>(tiles)),
-                           dim3(kSortThreads), 0, s, src_k, src_v, dst_k, dst_v, n, shift, tiles,
-                           hist, last ? ranks : nullptr, last ? table_src : nullptr,
-                           last ? table_out : nullptr);
-        src_k = dst_k;
-        src_v = dst_v;
-    }
+    const SortWorkspace W = sort_workspace(ws, ws_bytes, n);
+    hipLaunchKernelGGL(radix_zero_kernel, dim3(4), dim3(kSortThreads), 0, s, W.totals, int64_t(1024));
+    sort_passes(W, false, keys_in, keys_out, vals_in, vals_out, n, ranks, table_src, table_out, s);
+}
+
+void clear_sort_workspace(void *ws, size_t ws_bytes, int64_t n, hipStream_t s) {
+    if (n == 0) return;
+    const SortWorkspace W = sort_workspace(ws, ws_bytes, n);
+    hipLaunchKernelGGL(radix_zero_kernel, dim3(4), dim3(kSortThreads), 0, s, W.totals, int64_t(1024));
     HIP_CHECK(hipGetLastError());
+}
+
+void launch_project_sort_ranked(void *ws, size_t ws_bytes, const float *x, int64_t n,
+                                const float *dir, unsigned long long *status,
+                                unsigned long long tag, uint32_t *keys, uint32_t *keys_out,
+                                int32_t *vals_out, int32_t *ranks, float *table_out, hipStream_t s) {
+    if (n == 0) return;
+    const SortWorkspace W = sort_workspace(ws, ws_bytes, n);
+    hipLaunchKernelGGL(project_keys_hist_kernel, dim3(static_cast<unsigned>(W.tiles)),
+                       dim3(kSortThreads), 0, s, x, n, dir, keys, status, tag, W.tiles, W.hist,
+                       W.totals);
+    sort_passes(W, true, keys, keys_out, nullptr, vals_out, n, ranks, x, table_out, s);
 }
 
 void launch_sort_pairs(void *ws, size_t ws_bytes, const uint32_t *keys_in, uint32_t *keys_out,

@@ -161,6 +161,16 @@ void launch_sort_pairs_ranked(void *ws, size_t ws_bytes, const uint32_t *keys_in
                               uint32_t *keys_out, const int32_t *vals_in, int32_t *vals_out,
                               int64_t n, int32_t *ranks, const float *table_src, float *table_out,
                               hipStream_t s);
+// Clears the digit totals of a sort workspace; launch_project_sort_ranked
+// needs them clear on entry (every sort leaves them clear).
+void clear_sort_workspace(void *ws, size_t ws_bytes, int64_t n, hipStream_t s);
+// Fit refresh for d = 2: keys of x . dir (non-finite rows tracked in status as
+// launch_project_keys does), then the ranked sort of (key, row) with
+// ranks[row] = position and table_out[position] = x[row].
+void launch_project_sort_ranked(void *ws, size_t ws_bytes, const float *x, int64_t n,
+                                const float *dir, unsigned long long *status,
+                                unsigned long long tag, uint32_t *keys, uint32_t *keys_out,
+                                int32_t *vals_out, int32_t *ranks, float *table_out, hipStream_t s);
 void launch_values_from_keys(const uint32_t *keys, int64_t n, float *values, hipStream_t s);
 void launch_inverse_ranks(const int32_t *order, int64_t n, int32_t *ranks, hipStream_t s);
 void launch_lower_bound(const float *sorted, int64_t n, const float *query, int64_t h,
