@@ -29,6 +29,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -101,6 +102,87 @@ struct FifoArena {
     }
 };
 
+// Persistent helper threads for the per-request host graphs of a kNN batch:
+// run(n, f) calls f(0..n-1) on the helpers and the calling thread and
+// returns when every call has finished.
+class TaskPool {
+public:
+    explicit TaskPool(int threads) {
+        for (int t = 0; t < threads; ++t) workers_.emplace_back([this] { work(); });
+    }
+    ~TaskPool() {
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto &w : workers_) w.join();
+    }
+    void run(int64_t n, const std::function<void(int64_t)> &f) {
+        if (n <= 0) return;
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            job_ = &f;
+            n_ = n;
+            next_ = 0;
+            pending_ = n;
+            ++generation_;
+        }
+        cv_.notify_all();
+        drain();
+        std::unique_lock<std::mutex> g(mu_);
+        done_.wait(g, [&] { return pending_ == 0; });
+        job_ = nullptr;
+    }
+
+private:
+    void drain() {
+        for (;;) {
+            const std::function<void(int64_t)> *f;
+            int64_t i;
+            {
+                std::lock_guard<std::mutex> g(mu_);
+                if (!job_ || next_ >= n_) return;
+                f = job_;
+                i = next_++;
+            }
+            (*f)(i);
+            std::lock_guard<std::mutex> g(mu_);
+            if (--pending_ == 0) done_.notify_all();
+        }
+    }
+    void work() {
+        uint64_t seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> g(mu_);
+                cv_.wait(g, [&] { return stop_ || (generation_ != seen && job_ && next_ < n_); });
+                if (stop_) return;
+                seen = generation_;
+            }
+            drain();
+        }
+    }
+    std::vector<std::thread> workers_;
+    std::mutex mu_;
+    std::condition_variable cv_, done_;
+    const std::function<void(int64_t)> *job_ = nullptr;
+    int64_t n_ = 0, next_ = 0, pending_ = 0;
+    uint64_t generation_ = 0;
+    bool stop_ = false;
+};
+
+// One request's transform graph built on the host (fuzzy memberships,
+// initial placement, pruned compact CSR with clocks).
+struct HostGraph {
+    umaphip_status st = UMAPHIP_OK;
+    std::string error;
+    std::vector<float> init;
+    std::vector<uint32_t> off, ind;
+    std::vector<double> per, nper, npos, nneg;
+    int64_t E = 0;
+};
+
 struct Request {
     int64_t ticket = 0, q = 0;
     std::vector<float> x;
@@ -144,6 +226,7 @@ struct umaphip_server {
     uint32_t range = 0;
     float scale = 1;
     std::unique_ptr<KnnIndex> index;
+    std::unique_ptr<TaskPool> pool;   // host graph builders
     DeviceArray<float> tail, dirs;
     std::vector<float> tail_host, dirs_host;
     DeviceArray<int32_t> iota, cache_order;
@@ -252,6 +335,47 @@ struct umaphip_server {
         ++st_knn_batches;
     }
 
+    // ---- one request's host graph (runs on the task pool)
+    void build_graph(const Request &r, const int32_t *idx, const float *dist, HostGraph &G) {
+        const int64_t cap = 2 * r.q * k;
+        std::vector<int32_t> heads(cap), tails(cap);
+        std::vector<float> w(cap), rho(r.q), sigma(r.q), directed(r.q * k);
+        int64_t used = 0;
+        umaphip_status st = umaphip_fuzzy_graph(idx, dist, r.q, k, local_connectivity, 1.0f, mix, 1,
+                                                n, heads.data(), tails.data(), w.data(),
+                                                rho.data(), sigma.data(), directed.data(), &used);
+        bool any = false;
+        for (int64_t e = 0; e < used && !any; ++e) any = w[e] > 0.0f;
+        if (st == UMAPHIP_OK && !any) {
+            G.st = UMAPHIP_INVALID_INPUT;
+            G.error = "transform fuzzy graph is empty after positive-edge pruning";
+            return;
+        }
+        G.init.resize(r.q * d);
+        if (st == UMAPHIP_OK)
+            st = umaphip_transform_init(idx, directed.data(), r.q, k, tail_host.data(), n, d,
+                                        G.init.data());
+        if (st != UMAPHIP_OK || epochs == 0) {
+            G.st = st;
+            if (st != UMAPHIP_OK) G.error = umaphip_last_error();
+            return;
+        }
+        const size_t cap_e = static_cast<size_t>(std::max<int64_t>(used, 1));
+        G.off.resize(r.q + 1);
+        G.ind.resize(cap_e);
+        std::vector<float> cw(cap_e);
+        G.per.resize(cap_e);
+        G.nper.resize(cap_e);
+        G.npos.resize(cap_e);
+        G.nneg.resize(cap_e);
+        st = umaphip_compact_graph(r.q, n, used, heads.data(), tails.data(), w.data(),
+                                   static_cast<double>(epochs), static_cast<int32_t>(epochs),
+                                   static_cast<float>(rate), 1, G.off.data(), G.ind.data(), cw.data(),
+                                   G.per.data(), G.nper.data(), G.npos.data(), G.nneg.data(), &G.E);
+        G.st = st;
+        if (st != UMAPHIP_OK) G.error = umaphip_last_error();
+    }
+
     // ---- host graphs of a finished kNN batch, upload, one layout launch
     void join_knn_batch() {
         HIP_CHECK(hipEventSynchronize(ev_up));  // previous uploads left the staging buffer
@@ -265,50 +389,28 @@ struct umaphip_server {
         size_t up_used = 0;
         int64_t row = 0;
         std::vector<std::shared_ptr<Request>> joined;
-        for (auto &r : knn_batch) {
-            const int32_t *idx = idx_all + row * k;
-            const float *dist = dist_all + row * k;
-            row += r->q;
-            const int64_t cap = 2 * r->q * k;
-            std::vector<int32_t> heads(cap), tails(cap);
-            std::vector<float> w(cap), rho(r->q), sigma(r->q), directed(r->q * k);
-            int64_t used = 0;
-            umaphip_status st = umaphip_fuzzy_graph(idx, dist, r->q, k, local_connectivity, 1.0f,
-                                                    mix, 1, n, heads.data(), tails.data(), w.data(),
-                                                    rho.data(), sigma.data(), directed.data(), &used);
-            bool any = false;
-            for (int64_t e = 0; e < used && !any; ++e) any = w[e] > 0.0f;
-            if (st == UMAPHIP_OK && !any) {
-                st = UMAPHIP_INVALID_INPUT;
-                set_error("transform fuzzy graph is empty after positive-edge pruning");
-            }
-            std::vector<float> init(r->q * d);
-            if (st == UMAPHIP_OK)
-                st = umaphip_transform_init(idx, directed.data(), r->q, k, tail_host.data(), n, d,
-                                            init.data());
-            if (st != UMAPHIP_OK) {
-                fail_request(r, st, umaphip_last_error());
+        const int64_t nb = static_cast<int64_t>(knn_batch.size());
+        std::vector<int64_t> first_row(nb);
+        for (int64_t i = 0; i < nb; ++i) {
+            first_row[i] = row;
+            row += knn_batch[i]->q;
+        }
+        std::vector<HostGraph> built(nb);
+        pool->run(nb, [&](int64_t i) { build_graph(*knn_batch[i], idx_all + first_row[i] * k,
+                                                   dist_all + first_row[i] * k, built[i]); });
+        for (int64_t i = 0; i < nb; ++i) {
+            auto &r = knn_batch[i];
+            HostGraph &G = built[i];
+            if (G.st != UMAPHIP_OK) {
+                fail_request(r, G.st, G.error);
                 continue;
             }
             if (epochs == 0) {
-                r->out = init;
+                r->out = G.init;
                 r->host_ready = true;
                 continue;
             }
-            std::vector<uint32_t> off(r->q + 1), ind(std::max<int64_t>(used, 1));
-            std::vector<float> cw(std::max<int64_t>(used, 1));
-            std::vector<double> per(std::max<int64_t>(used, 1)), nper(per.size()), npos(per.size()),
-                nneg(per.size());
-            int64_t E = 0;
-            st = umaphip_compact_graph(r->q, n, used, heads.data(), tails.data(), w.data(),
-                                       static_cast<double>(epochs), static_cast<int32_t>(epochs),
-                                       static_cast<float>(rate), 1, off.data(), ind.data(),
-                                       cw.data(), per.data(), nper.data(), npos.data(), nneg.data(),
-                                       &E);
-            if (st != UMAPHIP_OK) {
-                fail_request(r, st, umaphip_last_error());
-                continue;
-            }
+            const int64_t E = G.E;
             // carve the request's arena block
             uint8_t *p = arena.base + r->arena_off;
             auto take = [&](size_t bytes) {
@@ -336,13 +438,13 @@ struct umaphip_server {
                 HIP_CHECK(hipMemcpyAsync(dst, up + up_used, bytes, hipMemcpyHostToDevice, ls));
                 up_used += align256(bytes);
             };
-            stage(r->offsets, off.data(), (r->q + 1) * 4);
-            stage(r->indices, ind.data(), E * 4);
-            stage(r->per, per.data(), E * 8);
-            stage(r->npos, npos.data(), E * 8);
-            stage(r->nper, nper.data(), E * 8);
-            stage(r->nneg, nneg.data(), E * 8);
-            stage(r->pos[0], init.data(), r->q * d * 4);
+            stage(r->offsets, G.off.data(), (r->q + 1) * 4);
+            stage(r->indices, G.ind.data(), E * 4);
+            stage(r->per, G.per.data(), E * 8);
+            stage(r->npos, G.npos.data(), E * 8);
+            stage(r->nper, G.nper.data(), E * 8);
+            stage(r->nneg, G.nneg.data(), E * 8);
+            stage(r->pos[0], G.init.data(), r->q * d * 4);
             HIP_CHECK(hipMemsetAsync(r->m, 0, r->q * d * 4, ls));
             HIP_CHECK(hipMemsetAsync(r->v, 0, r->q * d * 4, ls));
             launch_status_min_u64(r->status, ~0ull, ls);
@@ -538,6 +640,8 @@ umaphip_status umaphip_server_create(const float *reference, int64_t n, int64_t 
         schedules(static_cast<int>(S->epochs), S->alpha0, false, false, S->alpha, mu);
         bias_corrections(static_cast<int>(S->epochs), S->c1, S->c2);
         S->max_batch = p.max_batch_rows;
+        S->pool.reset(new TaskPool(static_cast<int>(
+            std::min<unsigned>(15, std::max<unsigned>(1, std::thread::hardware_concurrency()) - 1))));
         for (auto &st : S->sl) HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         HIP_CHECK(hipStreamCreateWithFlags(&S->sk, hipStreamNonBlocking));
         HIP_CHECK(hipEventCreateWithFlags(&S->ev_knn, hipEventDisableTiming));
