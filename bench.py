@@ -484,9 +484,24 @@ def main():
     if args.quality:
         from cuscan_amd.quality import trustworthiness_device
 
-        tw, pres = trustworthiness_device(x.cpu().numpy(), emb, k=15, sample=10_000)
+        xs = x.cpu().numpy()
+        tw, pres = trustworthiness_device(xs, emb, k=15, sample=10_000)
         out["config"]["trustworthiness@15"] = round(tw, 4)
         out["config"]["knn_preservation@15"] = round(pres, 4)
+        # the oracle's whole layout on the same graph and init, same sample
+        from oracle import oracle as O
+
+        t0 = time.time()
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        emb_cpu = O.optimize_compact(init, None, compact, epochs=args.epochs, a=a, b=b,
+                                     negative_selection_range=200_000, seed=42, threads=threads,
+                                     collect_stats=False)[0]
+        cpu_s = time.time() - t0
+        tw_c, pres_c = trustworthiness_device(xs, emb_cpu, k=15, sample=10_000)
+        out["config"]["oracle_trustworthiness@15"] = round(tw_c, 4)
+        out["config"]["oracle_knn_preservation@15"] = round(pres_c, 4)
+        out["config"]["oracle_layout_seconds"] = round(cpu_s, 1)
+        log(f"quality: device {tw:.4f}/{pres:.4f}, oracle {tw_c:.4f}/{pres_c:.4f} ({cpu_s:.0f}s)")
     if not args.no_batch:
         log("batched layouts ...")
         out["batched"] = batch_benchmark(args)
