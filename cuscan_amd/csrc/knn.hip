@@ -66,38 +66,58 @@ __global__ void to_bf16_kernel(const float *x, int64_t rows, int64_t d, int64_t 
     }
 }
 
-// Rank-based selection of the best kTop of n <= kCap entries of one
-// (query, slice) buffer by one wave; keeps them sorted in place and returns
-// the new threshold (the kTop-th distance, or +inf when fewer).
+// Sorts the n <= kCap = 128 entries of one (query, slice) buffer by
+// (distance, id) with a bitonic network (two entries per lane, entry e in
+// lane e % 64, register e / 64), keeps the best kTop = 64 sorted in place
+// and returns the new threshold (the kTop-th distance, or +inf when fewer).
+// Ids are unique, so the order is total.
 __device__ inline float compact_buffer(float *cd, int32_t *ci, int n, int lane) {
-    float d0 = lane < n ? cd[lane] : INFINITY;
-    int32_t i0 = lane < n ? ci[lane] : INT32_MAX;
-    float d1 = lane + 64 < n ? cd[lane + 64] : INFINITY;
-    int32_t i1 = lane + 64 < n ? ci[lane + 64] : INT32_MAX;
-    int r0 = 0, r1 = 0;
-    for (int f = 0; f < n; ++f) {
-        const int src = f & 63;
-        const float df = f < 64 ? __shfl(d0, src, 64) : __shfl(d1, src, 64);
-        const int32_t idf = f < 64 ? __shfl(i0, src, 64) : __shfl(i1, src, 64);
-        r0 += (df < d0 || (df == d0 && idf < i0)) ? 1 : 0;
-        r1 += (df < d1 || (df == d1 && idf < i1)) ? 1 : 0;
+    static_assert(kCap == 128 && kTop == 64, "two entries per lane");
+    float d[2];
+    int32_t id[2];
+    d[0] = lane < n ? cd[lane] : INFINITY;
+    id[0] = lane < n ? ci[lane] : INT32_MAX;
+    d[1] = lane + 64 < n ? cd[lane + 64] : INFINITY;
+    id[1] = lane + 64 < n ? ci[lane + 64] : INT32_MAX;
+#pragma unroll
+    for (int k = 2; k <= 128; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j == 64) {
+                // entries lane and lane + 64 (k = 128: ascending)
+                const bool gt = d[0] > d[1] || (d[0] == d[1] && id[0] > id[1]);
+                if (gt) {
+                    const float td = d[0];
+                    const int32_t ti = id[0];
+                    d[0] = d[1];
+                    id[0] = id[1];
+                    d[1] = td;
+                    id[1] = ti;
+                }
+                continue;
+            }
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                const float dp = __shfl_xor(d[r], j, 64);
+                const int32_t ip = __shfl_xor(id[r], j, 64);
+                const int e = lane + 64 * r;
+                const bool lower = (lane & j) == 0;     // e is the lower entry of its pair
+                const bool asc = (e & k) == 0;          // the pair sorts ascending
+                const bool mine_gt = d[r] > dp || (d[r] == dp && id[r] > ip);
+                // lower keeps the min when ascending, the max when descending
+                const bool take = lower == asc ? mine_gt : !mine_gt;
+                if (take) {
+                    d[r] = dp;
+                    id[r] = ip;
+                }
+            }
+        }
     }
     __builtin_amdgcn_wave_barrier();
-    if (lane < n && r0 < kTop) {
-        cd[r0] = d0;
-        ci[r0] = i0;
-    }
-    if (lane + 64 < n && r1 < kTop) {
-        cd[r1] = d1;
-        ci[r1] = i1;
-    }
-    float tau = INFINITY;
-    if (n >= kTop) {
-        if (r0 == kTop - 1 && lane < n) tau = d0;
-        if (r1 == kTop - 1 && lane + 64 < n) tau = d1;
-        for (int off = 32; off > 0; off >>= 1) tau = fminf(tau, __shfl_xor(tau, off, 64));
-    }
-    return tau;
+    cd[lane] = d[0];
+    ci[lane] = id[0];
+    const float kth = __shfl(d[0], kTop - 1, 64);
+    return n >= kTop ? kth : INFINITY;
 }
 
 // Order-preserving u32 key of a float (the global threshold is kept as a
