@@ -40,6 +40,7 @@ constexpr int kTQ = 128;      // queries per workgroup
 constexpr int kTR = 16;       // reference rows per LDS chunk (one MFMA column block)
 constexpr int kCap = 128;     // candidate buffer per (query, slice)
 constexpr int kTop = 64;      // candidates kept per query
+constexpr int kMaxSlices = 256;   // slices per query batch (KnnIndex::slices_for)
 
 __device__ inline uint16_t f32_to_bf16_rne(float f) {
     uint32_t u = __float_as_uint(f);
@@ -313,63 +314,79 @@ __global__ __launch_bounds__(256, 2) void knn_scan_kernel(
 }
 
 // Merge the sorted per-slice lists of each query into its best kTop
-// (sorted by distance, then id). One wave per query, LDS merge path.
+// (sorted by distance, then id). One wave per query; lane l holds entry l
+// of the running list A. Each slice list B is loaded reversed, so A followed
+// by B is bitonic: a half-cleaner keeps the 64 smallest and a bitonic merge
+// sorts them, all in registers. Every member of the query's best 64 is at
+// most its published threshold, so slices whose best candidate is above it
+// are skipped; the next listed slice is loaded while the current one merges.
 __global__ __launch_bounds__(64) void knn_merge_kernel(const float *cand_d, const int32_t *cand_i,
-                                                       const int32_t *cand_n, int64_t nq,
+                                                       const int32_t *cand_n,
+                                                       const uint32_t *tau_key, int64_t nq,
                                                        int slices, float *top_d, int32_t *top_i,
                                                        int32_t *top_n) {
-    __shared__ float sa_d[kTop], sb_d[kTop];
-    __shared__ int32_t sa_i[kTop], sb_i[kTop];
+    __shared__ int32_t s_list[kMaxSlices];
+    __shared__ int s_count;
     const int64_t q = blockIdx.x;
     const int lane = threadIdx.x;
     if (q >= nq) return;
-    int na = cand_n[q * slices];
-    {
-        const int64_t base = (q * slices) * kCap;
-        sa_d[lane] = lane < na ? cand_d[base + lane] : INFINITY;
-        sa_i[lane] = lane < na ? cand_i[base + lane] : INT32_MAX;
+    const float tau = key_float(tau_key[q]);
+    if (lane == 0) s_count = 0;
+    __syncthreads();
+    for (int s0 = 0; s0 < slices; s0 += 64) {
+        const int s = s0 + lane;
+        bool need = false;
+        if (s < slices) {
+            const int n = cand_n[q * slices + s];
+            need = n > 0 && cand_d[(q * slices + s) * kCap] <= tau;
+        }
+        const uint64_t m = __ballot(need);
+        if (need) s_list[s_count + __popcll(m & ((1ull << lane) - 1ull))] = s;
+        __syncthreads();
+        if (lane == 0) s_count += __popcll(m);
+        __syncthreads();
     }
-    for (int s = 1; s < slices; ++s) {
-        const int nb = cand_n[q * slices + s];
-        if (nb == 0) continue;
+    const int listed = s_count;
+    float ad = INFINITY;
+    int32_t ai = INT32_MAX;
+    int na = 0;
+    float next_d = INFINITY;
+    int32_t next_i = INT32_MAX;
+    int next_n = 0;
+    auto fetch = [&](int li) {
+        const int s = s_list[li];
+        next_n = cand_n[q * slices + s];
         const int64_t base = (q * slices + s) * kCap;
-        sb_d[lane] = lane < nb ? cand_d[base + lane] : INFINITY;
-        sb_i[lane] = lane < nb ? cand_i[base + lane] : INT32_MAX;
-        __syncthreads();
-        const float ad = sa_d[lane], bd = sb_d[lane];
-        const int32_t ai = sa_i[lane], bi = sb_i[lane];
-        // rank of A[lane] = lane + #{B < A[lane]}; rank of B[lane] = lane + #{A < B[lane]}
-        int lo = 0, hi = kTop;
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            const float md = sb_d[mid];
-            if (md < ad || (md == ad && sb_i[mid] < ai)) lo = mid + 1;
-            else hi = mid;
+        const int r = kTop - 1 - lane;   // reversed
+        next_d = r < next_n ? cand_d[base + r] : INFINITY;
+        next_i = r < next_n ? cand_i[base + r] : INT32_MAX;
+    };
+    if (listed > 0) fetch(0);
+    for (int li = 0; li < listed; ++li) {
+        const float bd = next_d;
+        const int32_t bi = next_i;
+        const int nb = next_n;
+        if (li + 1 < listed) fetch(li + 1);
+        // half-cleaner: entry l against entry 64 + l of the bitonic A ++ reverse(B)
+        if (bd < ad || (bd == ad && bi < ai)) {
+            ad = bd;
+            ai = bi;
         }
-        const int ra = lane + lo;
-        lo = 0;
-        hi = kTop;
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            const float md = sa_d[mid];
-            if (md < bd || (md == bd && sa_i[mid] < bi)) lo = mid + 1;
-            else hi = mid;
-        }
-        const int rbk = lane + lo;
-        __syncthreads();
-        if (ra < kTop) {
-            sa_d[ra] = ad;
-            sa_i[ra] = ai;
-        }
-        if (rbk < kTop) {
-            sa_d[rbk] = bd;
-            sa_i[rbk] = bi;
+#pragma unroll
+        for (int j = 32; j > 0; j >>= 1) {
+            const float dp = __shfl_xor(ad, j, 64);
+            const int32_t ip = __shfl_xor(ai, j, 64);
+            const bool mine_gt = ad > dp || (ad == dp && ai > ip);
+            const bool lower = (lane & j) == 0;
+            if (lower == mine_gt) {
+                ad = dp;
+                ai = ip;
+            }
         }
         na = min(kTop, na + nb);
-        __syncthreads();
     }
-    top_d[q * kTop + lane] = sa_d[lane];
-    top_i[q * kTop + lane] = sa_i[lane];
+    top_d[q * kTop + lane] = ad;
+    top_i[q * kTop + lane] = ai;
     if (lane == 0) top_n[q] = na;
 }
 
@@ -394,9 +411,18 @@ __global__ __launch_bounds__(64) void knn_rerank_kernel(const float *query, int6
         const int fb = static_cast<int>(min<int64_t>(FB, d - f0));
         __syncthreads();
         s_q[lane] = lane < fb ? query[q * d + f0 + lane] : 0.0f;
-        for (int c = 0; c < n; ++c) {
-            const int64_t row = __shfl(id, c, 64);
-            s_r[c][lane] = lane < fb ? ref[row * d + f0 + lane] : 0.0f;
+        for (int c0 = 0; c0 < n; c0 += 16) {
+            // sixteen row segments in flight, then into LDS
+            float v[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const int c = c0 + u;
+                const int64_t row = __shfl(id, c & 63, 64);
+                v[u] = (c < n && lane < fb) ? ref[row * d + f0 + lane] : 0.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < 16; ++u)
+                if (c0 + u < n) s_r[c0 + u][lane] = v[u];
         }
         __syncthreads();
         if (lane < n) {
@@ -520,8 +546,8 @@ void KnnIndex::query(const float *q_dev, int64_t nq, int k, bool exclude_self, i
     }
     HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(knn_merge_kernel, dim3(static_cast<unsigned>(nq)), dim3(64), 0, s,
-                       cand_d_.ptr, cand_i_.ptr, cand_n_.ptr, nq, slices, top_d_.ptr, top_i_.ptr,
-                       top_n_.ptr);
+                       cand_d_.ptr, cand_i_.ptr, cand_n_.ptr, tau_key_.ptr, nq, slices, top_d_.ptr,
+                       top_i_.ptr, top_n_.ptr);
     HIP_CHECK(hipGetLastError());
     if (cand_ids) {
         // prefilter lists only (best first), no rerank
