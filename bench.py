@@ -154,13 +154,13 @@ def serve_benchmark(x, emb, compact_index, a, b, args):
     # ---- plain per-request transform (the UMAP.transform call chain)
     params = L.LayoutParams(epochs=epochs, a=a, b=b, initial_alpha=0.25, seed=42)
 
+    emb_dev = torch.as_tensor(emb, device=x.device)
+
     def plain(xq):
         idx, dist = compact_index.query(xq, k)
-        g = G.fuzzy_graph(idx, dist, local_connectivity=0.0, bipartite=True, n_reference=len(emb))
-        init = G.transform_init(idx, g["directed"], emb)
-        cg = G.compact_graph(len(xq), len(emb), g["heads"], g["tails"], g["weights"], epochs=epochs,
-                             rate=5.0, prune_divisor=float(epochs), transform=True)
-        out, _, _, _ = L.optimize_compact(init, emb, cg, params, shared=False)
+        cg = G.transform_graph_device(idx, dist, emb_dev, local_connectivity=0.0, epochs=epochs,
+                                      rate=5.0)
+        out, _, _, _ = L.optimize_compact(cg["init"], emb, cg, params, shared=False)
         return out
 
     plain(req[-1])  # warm

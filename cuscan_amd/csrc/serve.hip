@@ -13,13 +13,14 @@
 // The server therefore keeps the references (BF16 + f32), the embedding and
 // the per-direction projection tables (order, sorted keys, tail positions by
 // rank) resident. Requests are admitted whenever HBM in a fixed arena and a
-// kNN batch slot are free; a batch's kNN runs on its own stream, the host
-// then builds each request's graph, and the whole batch's transform layouts
-// (every epoch of every request) run in one launch on one of a few layout
-// streams, so batches overlap on the device and no host step loop paces the
-// epochs. A request's rows are computed exactly as the single-request
-// transform computes them (same tile sweep, same per-row summation order),
-// so results do not depend on what else is in flight.
+// kNN batch slot are free; a batch's kNN runs on its own stream, then, on one
+// of a few layout streams, every request's graph is built on the device from
+// the kNN outputs (graph_device.hip) and the whole batch's transform layouts
+// (every epoch of every request) run in one launch, so batches overlap on
+// the device and the host only fills descriptor tables. A request's rows
+// are computed exactly as the single-request transform computes them (same
+// graph kernels, same tile sweep and per-row summation order), so results
+// do not depend on what else is in flight.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -29,7 +30,6 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
-#include <functional>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -38,6 +38,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "graph_device.hpp"
 #include "knn.hpp"
 #include "layout_host.hpp"
 #include "layout_kernels.hpp"
@@ -102,87 +103,6 @@ struct FifoArena {
     }
 };
 
-// Persistent helper threads for the per-request host graphs of a kNN batch:
-// run(n, f) calls f(0..n-1) on the helpers and the calling thread and
-// returns when every call has finished.
-class TaskPool {
-public:
-    explicit TaskPool(int threads) {
-        for (int t = 0; t < threads; ++t) workers_.emplace_back([this] { work(); });
-    }
-    ~TaskPool() {
-        {
-            std::lock_guard<std::mutex> g(mu_);
-            stop_ = true;
-        }
-        cv_.notify_all();
-        for (auto &w : workers_) w.join();
-    }
-    void run(int64_t n, const std::function<void(int64_t)> &f) {
-        if (n <= 0) return;
-        {
-            std::lock_guard<std::mutex> g(mu_);
-            job_ = &f;
-            n_ = n;
-            next_ = 0;
-            pending_ = n;
-            ++generation_;
-        }
-        cv_.notify_all();
-        drain();
-        std::unique_lock<std::mutex> g(mu_);
-        done_.wait(g, [&] { return pending_ == 0; });
-        job_ = nullptr;
-    }
-
-private:
-    void drain() {
-        for (;;) {
-            const std::function<void(int64_t)> *f;
-            int64_t i;
-            {
-                std::lock_guard<std::mutex> g(mu_);
-                if (!job_ || next_ >= n_) return;
-                f = job_;
-                i = next_++;
-            }
-            (*f)(i);
-            std::lock_guard<std::mutex> g(mu_);
-            if (--pending_ == 0) done_.notify_all();
-        }
-    }
-    void work() {
-        uint64_t seen = 0;
-        for (;;) {
-            {
-                std::unique_lock<std::mutex> g(mu_);
-                cv_.wait(g, [&] { return stop_ || (generation_ != seen && job_ && next_ < n_); });
-                if (stop_) return;
-                seen = generation_;
-            }
-            drain();
-        }
-    }
-    std::vector<std::thread> workers_;
-    std::mutex mu_;
-    std::condition_variable cv_, done_;
-    const std::function<void(int64_t)> *job_ = nullptr;
-    int64_t n_ = 0, next_ = 0, pending_ = 0;
-    uint64_t generation_ = 0;
-    bool stop_ = false;
-};
-
-// One request's transform graph built on the host (fuzzy memberships,
-// initial placement, pruned compact CSR with clocks).
-struct HostGraph {
-    umaphip_status st = UMAPHIP_OK;
-    std::string error;
-    std::vector<float> init;
-    std::vector<uint32_t> off, ind;
-    std::vector<double> per, nper, npos, nneg;
-    int64_t E = 0;
-};
-
 struct Request {
     int64_t ticket = 0, q = 0;
     std::vector<float> x;
@@ -199,8 +119,8 @@ struct Request {
     int cur = 0;                 // buffer that holds the result
     hipEvent_t ev_final = nullptr;
     // result
+    TGraphState *graph = nullptr;   // device: the request's graph state
     std::vector<float> out;
-    bool host_ready = false;   // out already holds the result (no layout ran)
     umaphip_status code = UMAPHIP_OK;
     std::string error;
     bool done = false;         // guarded by the server mutex
@@ -210,7 +130,8 @@ struct Request {
 size_t request_bytes(int64_t q, int64_t k) {
     const size_t E = static_cast<size_t>(q * k);
     return align256((q + 1) * 4) + align256(E * 4) + 4 * align256(E * 8) +
-           4 * align256(q * 2 * 4) + align256(q * 4) + align256(q * 4) + 256;
+           4 * align256(q * 2 * 4) + align256(q * 4) + align256(q * 4) + 256 +
+           align256(sizeof(TGraphState));
 }
 
 }  // namespace
@@ -226,7 +147,6 @@ struct umaphip_server {
     uint32_t range = 0;
     float scale = 1;
     std::unique_ptr<KnnIndex> index;
-    std::unique_ptr<TaskPool> pool;   // host graph builders
     DeviceArray<float> tail, dirs;
     std::vector<float> tail_host, dirs_host;
     DeviceArray<int32_t> iota, cache_order;
@@ -239,14 +159,17 @@ struct umaphip_server {
     int64_t max_batch = 0;
     DeviceArray<float> q_dev, d_dist;
     DeviceArray<int32_t> d_idx;
-    std::unique_ptr<Pinned> q_pin, idx_pin, dist_pin, up_pin;
-    size_t up_cap = 0;
-    hipEvent_t ev_knn = nullptr, ev_up = nullptr;
+    std::unique_ptr<Pinned> q_pin;
+    hipEvent_t ev_knn = nullptr, ev_graph = nullptr;   // kNN outputs written / read
+    TGraphParams gparams{};
+    DeviceArray<int32_t> g_tail, g_cnt;               // graph workspace of one batch
+    DeviceArray<float> g_w;
     // segment tables of in-flight layout launches (a ring; a slot is reused
     // once its launch has finished)
     static constexpr int kSlots = 8;
-    std::unique_ptr<Pinned> seg_pin[kSlots];
+    std::unique_ptr<Pinned> seg_pin[kSlots], gseg_pin[kSlots];
     DeviceArray<TransformSegment> seg_dev[kSlots];
+    DeviceArray<TGraphSegment> gseg_dev[kSlots];
     hipEvent_t ev_slot[kSlots] = {};
     int64_t launch_index = 0;
     DeviceArray<uint8_t> arena_mem;
@@ -280,7 +203,7 @@ struct umaphip_server {
         for (auto &e : ev_slot)
             if (e) (void)hipEventDestroy(e);
         if (ev_knn) (void)hipEventDestroy(ev_knn);
-        if (ev_up) (void)hipEventDestroy(ev_up);
+        if (ev_graph) (void)hipEventDestroy(ev_graph);
         for (auto &r : finishing)
             if (r->ev_final) (void)hipEventDestroy(r->ev_final);
         for (auto st : sl)
@@ -325,158 +248,94 @@ struct umaphip_server {
             row += r->q;
         }
         HIP_CHECK(hipMemcpyAsync(q_dev.ptr, qp, rows * D * sizeof(float), hipMemcpyHostToDevice, sk));
+        // the previous batch's graph kernels have read the kNN outputs
+        HIP_CHECK(hipStreamWaitEvent(sk, ev_graph, 0));
         index->query(q_dev.ptr, rows, static_cast<int>(k), false, 0, d_idx.ptr, d_dist.ptr, sk);
-        HIP_CHECK(hipMemcpyAsync(idx_pin->ptr, d_idx.ptr, rows * k * sizeof(int32_t),
-                                 hipMemcpyDeviceToHost, sk));
-        HIP_CHECK(hipMemcpyAsync(dist_pin->ptr, d_dist.ptr, rows * k * sizeof(float),
-                                 hipMemcpyDeviceToHost, sk));
         HIP_CHECK(hipEventRecord(ev_knn, sk));
         knn_batch = std::move(batch);
         ++st_knn_batches;
     }
 
-    // ---- one request's host graph (runs on the task pool)
-    void build_graph(const Request &r, const int32_t *idx, const float *dist, HostGraph &G) {
-        const int64_t cap = 2 * r.q * k;
-        std::vector<int32_t> heads(cap), tails(cap);
-        std::vector<float> w(cap), rho(r.q), sigma(r.q), directed(r.q * k);
-        int64_t used = 0;
-        umaphip_status st = umaphip_fuzzy_graph(idx, dist, r.q, k, local_connectivity, 1.0f, mix, 1,
-                                                n, heads.data(), tails.data(), w.data(),
-                                                rho.data(), sigma.data(), directed.data(), &used);
-        bool any = false;
-        for (int64_t e = 0; e < used && !any; ++e) any = w[e] > 0.0f;
-        if (st == UMAPHIP_OK && !any) {
-            G.st = UMAPHIP_INVALID_INPUT;
-            G.error = "transform fuzzy graph is empty after positive-edge pruning";
-            return;
-        }
-        G.init.resize(r.q * d);
-        if (st == UMAPHIP_OK)
-            st = umaphip_transform_init(idx, directed.data(), r.q, k, tail_host.data(), n, d,
-                                        G.init.data());
-        if (st != UMAPHIP_OK || epochs == 0) {
-            G.st = st;
-            if (st != UMAPHIP_OK) G.error = umaphip_last_error();
-            return;
-        }
-        const size_t cap_e = static_cast<size_t>(std::max<int64_t>(used, 1));
-        G.off.resize(r.q + 1);
-        G.ind.resize(cap_e);
-        std::vector<float> cw(cap_e);
-        G.per.resize(cap_e);
-        G.nper.resize(cap_e);
-        G.npos.resize(cap_e);
-        G.nneg.resize(cap_e);
-        st = umaphip_compact_graph(r.q, n, used, heads.data(), tails.data(), w.data(),
-                                   static_cast<double>(epochs), static_cast<int32_t>(epochs),
-                                   static_cast<float>(rate), 1, G.off.data(), G.ind.data(), cw.data(),
-                                   G.per.data(), G.nper.data(), G.npos.data(), G.nneg.data(), &G.E);
-        G.st = st;
-        if (st != UMAPHIP_OK) G.error = umaphip_last_error();
-    }
-
-    // ---- host graphs of a finished kNN batch, upload, one layout launch
+    // ---- device graphs and one layout launch for a finished kNN batch
     void join_knn_batch() {
-        HIP_CHECK(hipEventSynchronize(ev_up));  // previous uploads left the staging buffer
         const int slot = static_cast<int>(launch_index % kSlots);
         hipStream_t ls = sl[launch_index % kLayoutStreams];
         HIP_CHECK(hipEventSynchronize(ev_slot[slot]));  // the slot's previous launch finished
         const double t_knn = now_seconds();
-        const int32_t *idx_all = static_cast<const int32_t *>(idx_pin->ptr);
-        const float *dist_all = static_cast<const float *>(dist_pin->ptr);
-        uint8_t *up = static_cast<uint8_t *>(up_pin->ptr);
-        size_t up_used = 0;
-        int64_t row = 0;
-        std::vector<std::shared_ptr<Request>> joined;
-        const int64_t nb = static_cast<int64_t>(knn_batch.size());
-        std::vector<int64_t> first_row(nb);
-        for (int64_t i = 0; i < nb; ++i) {
-            first_row[i] = row;
-            row += knn_batch[i]->q;
-        }
-        std::vector<HostGraph> built(nb);
-        pool->run(nb, [&](int64_t i) { build_graph(*knn_batch[i], idx_all + first_row[i] * k,
-                                                   dist_all + first_row[i] * k, built[i]); });
-        for (int64_t i = 0; i < nb; ++i) {
-            auto &r = knn_batch[i];
-            HostGraph &G = built[i];
-            if (G.st != UMAPHIP_OK) {
-                fail_request(r, G.st, G.error);
-                continue;
-            }
-            if (epochs == 0) {
-                r->out = G.init;
-                r->host_ready = true;
-                continue;
-            }
-            const int64_t E = G.E;
+        const int32_t ns = static_cast<int32_t>(knn_batch.size());
+        TGraphSegment *gsegs = static_cast<TGraphSegment *>(gseg_pin[slot]->ptr);
+        TransformSegment *segs = static_cast<TransformSegment *>(seg_pin[slot]->ptr);
+        int64_t row = 0, tiles = 0;
+        for (int32_t i = 0; i < ns; ++i) {
+            Request &r = *knn_batch[i];
             // carve the request's arena block
-            uint8_t *p = arena.base + r->arena_off;
+            uint8_t *p = arena.base + r.arena_off;
             auto take = [&](size_t bytes) {
                 uint8_t *q = p;
                 p += align256(bytes);
                 return q;
             };
-            r->E = E;
-            r->small = r->q <= 1024;
-            r->offsets = reinterpret_cast<int32_t *>(take((r->q + 1) * 4));
-            r->indices = reinterpret_cast<int32_t *>(take(r->q * k * 4));
-            r->per = reinterpret_cast<double *>(take(r->q * k * 8));
-            r->npos = reinterpret_cast<double *>(take(r->q * k * 8));
-            r->nper = reinterpret_cast<double *>(take(r->q * k * 8));
-            r->nneg = reinterpret_cast<double *>(take(r->q * k * 8));
-            r->pos[0] = reinterpret_cast<float *>(take(r->q * d * 4));
-            r->pos[1] = reinterpret_cast<float *>(take(r->q * d * 4));
-            r->m = reinterpret_cast<float *>(take(r->q * d * 4));
-            r->v = reinterpret_cast<float *>(take(r->q * d * 4));
-            r->ranks = reinterpret_cast<int32_t *>(take(r->q * 4));
-            r->proj = reinterpret_cast<float *>(take(r->q * 4));
-            r->status = reinterpret_cast<unsigned long long *>(take(8));
-            auto stage = [&](void *dst, const void *src, size_t bytes) {
-                std::memcpy(up + up_used, src, bytes);
-                HIP_CHECK(hipMemcpyAsync(dst, up + up_used, bytes, hipMemcpyHostToDevice, ls));
-                up_used += align256(bytes);
-            };
-            stage(r->offsets, G.off.data(), (r->q + 1) * 4);
-            stage(r->indices, G.ind.data(), E * 4);
-            stage(r->per, G.per.data(), E * 8);
-            stage(r->npos, G.npos.data(), E * 8);
-            stage(r->nper, G.nper.data(), E * 8);
-            stage(r->nneg, G.nneg.data(), E * 8);
-            stage(r->pos[0], G.init.data(), r->q * d * 4);
-            HIP_CHECK(hipMemsetAsync(r->m, 0, r->q * d * 4, ls));
-            HIP_CHECK(hipMemsetAsync(r->v, 0, r->q * d * 4, ls));
-            launch_status_min_u64(r->status, ~0ull, ls);
-            r->cur = static_cast<int>(epochs & 1);
-            joined.push_back(r);
+            r.small = r.q <= 1024;
+            r.offsets = reinterpret_cast<int32_t *>(take((r.q + 1) * 4));
+            r.indices = reinterpret_cast<int32_t *>(take(r.q * k * 4));
+            r.per = reinterpret_cast<double *>(take(r.q * k * 8));
+            r.npos = reinterpret_cast<double *>(take(r.q * k * 8));
+            r.nper = reinterpret_cast<double *>(take(r.q * k * 8));
+            r.nneg = reinterpret_cast<double *>(take(r.q * k * 8));
+            r.pos[0] = reinterpret_cast<float *>(take(r.q * d * 4));
+            r.pos[1] = reinterpret_cast<float *>(take(r.q * d * 4));
+            r.m = reinterpret_cast<float *>(take(r.q * d * 4));
+            r.v = reinterpret_cast<float *>(take(r.q * d * 4));
+            r.ranks = reinterpret_cast<int32_t *>(take(r.q * 4));
+            r.proj = reinterpret_cast<float *>(take(r.q * 4));
+            r.status = reinterpret_cast<unsigned long long *>(take(8));
+            r.graph = reinterpret_cast<TGraphState *>(take(sizeof(TGraphState)));
+            r.cur = static_cast<int>(epochs & 1);
+            TGraphSegment G{};
+            G.row0 = row;
+            G.rows = r.q;
+            G.offsets = reinterpret_cast<uint32_t *>(r.offsets);
+            G.indices = reinterpret_cast<uint32_t *>(r.indices);
+            G.periods = r.per;
+            G.negative_periods = r.nper;
+            G.next_positive = r.npos;
+            G.next_negative = r.nneg;
+            G.init = r.pos[0];
+            G.state = r.graph;
+            gsegs[i] = G;
+            row += r.q;
+            SweepArgs A{};
+            A.h = r.q; A.n = n;
+            A.pos = r.pos[0]; A.pos_out = r.pos[1];
+            A.tgt = tail.ptr;
+            A.offsets = r.offsets; A.indices = r.indices;
+            A.periods = r.per; A.next_positive = r.npos;
+            A.negative_periods = r.nper; A.next_negative = r.nneg;
+            A.m = r.m; A.v = r.v;
+            A.scale = scale;
+            A.a = a; A.b = b; A.gamma = gamma; A.range = range;
+            A.multiplier = 1.0f;
+            A.shared = 0; A.negatives = rate != 0; A.exclude = exclude; A.fused = 1;
+            segs[i].args = A;
+            segs[i].tile0 = tiles;
+            segs[i].ranks = r.ranks;
+            segs[i].status = r.status;
+            tiles += (r.q + transform_tile_rows() - 1) / transform_tile_rows();
         }
-        HIP_CHECK(hipEventRecord(ev_up, ls));
-        // one launch runs every epoch of every joined request
-        const int32_t ns = static_cast<int32_t>(joined.size());
-        if (ns > 0) {
-            TransformSegment *segs = static_cast<TransformSegment *>(seg_pin[slot]->ptr);
-            int64_t tiles = 0;
-            for (int32_t s = 0; s < ns; ++s) {
-                Request &r = *joined[s];
-                SweepArgs A{};
-                A.h = r.q; A.n = n;
-                A.pos = r.pos[0]; A.pos_out = r.pos[1];
-                A.tgt = tail.ptr;
-                A.offsets = r.offsets; A.indices = r.indices;
-                A.periods = r.per; A.next_positive = r.npos;
-                A.negative_periods = r.nper; A.next_negative = r.nneg;
-                A.m = r.m; A.v = r.v;
-                A.scale = scale;
-                A.a = a; A.b = b; A.gamma = gamma; A.range = range;
-                A.multiplier = 1.0f;
-                A.shared = 0; A.negatives = rate != 0; A.exclude = exclude; A.fused = 1;
-                segs[s].args = A;
-                segs[s].tile0 = tiles;
-                segs[s].ranks = r.ranks;
-                segs[s].status = r.status;
-                tiles += (r.q + transform_tile_rows() - 1) / transform_tile_rows();
-            }
+        HIP_CHECK(hipMemcpyAsync(gseg_dev[slot].ptr, gsegs, sizeof(TGraphSegment) * ns,
+                                 hipMemcpyHostToDevice, ls));
+        HIP_CHECK(hipStreamWaitEvent(ls, ev_knn, 0));
+        launch_transform_graphs(gseg_dev[slot].ptr, ns, row, d_idx.ptr, d_dist.ptr, k, n, gparams,
+                                tail.ptr, d, static_cast<int32_t>(std::max<int64_t>(epochs, 1)),
+                                static_cast<float>(rate), g_tail.ptr, g_w.ptr, g_cnt.ptr, ls);
+        HIP_CHECK(hipEventRecord(ev_graph, ls));
+        for (int32_t i = 0; i < ns; ++i) {
+            Request &r = *knn_batch[i];
+            HIP_CHECK(hipMemsetAsync(r.m, 0, r.q * d * 4, ls));
+            HIP_CHECK(hipMemsetAsync(r.v, 0, r.q * d * 4, ls));
+            launch_status_min_u64(r.status, ~0ull, ls);
+        }
+        if (epochs > 0) {
             HIP_CHECK(hipMemcpyAsync(seg_dev[slot].ptr, segs, sizeof(TransformSegment) * ns,
                                      hipMemcpyHostToDevice, ls));
             TransformTables T{};
@@ -492,9 +351,9 @@ struct umaphip_server {
             T.n = n;
             T.epochs = static_cast<int32_t>(epochs);
             launch_transform_layouts(seg_dev[slot].ptr, ns, tiles, T, ls);
-            HIP_CHECK(hipEventRecord(ev_slot[slot], ls));
-            ++launch_index;
         }
+        HIP_CHECK(hipEventRecord(ev_slot[slot], ls));
+        ++launch_index;
         const double t_join = now_seconds();
         for (auto &r : knn_batch) {
             r->t_knn = t_knn;
@@ -504,10 +363,8 @@ struct umaphip_server {
         }
         std::lock_guard<std::mutex> g(mu);
         for (auto &r : knn_batch) finishing.push_back(r);
-        if (ns > 0) {
-            ++st_launches;
-            st_seg_sum += ns;
-        }
+        ++st_launches;
+        st_seg_sum += ns;
         st_peak_active = std::max<int64_t>(st_peak_active, static_cast<int64_t>(finishing.size()));
         knn_batch.clear();
     }
@@ -531,15 +388,21 @@ struct umaphip_server {
         for (auto &r : ready) {
             (void)hipEventDestroy(r->ev_final);
             r->ev_final = nullptr;
-            if (r->code == UMAPHIP_OK && !r->host_ready) {
-                r->out.resize(r->q * d);
-                unsigned long long status = ~0ull;
-                HIP_CHECK(hipMemcpy(r->out.data(), r->pos[r->cur], r->q * d * sizeof(float),
-                                    hipMemcpyDeviceToHost));
-                HIP_CHECK(hipMemcpy(&status, r->status, sizeof(status), hipMemcpyDeviceToHost));
-                bool finite = status == ~0ull;
-                for (float x : r->out) finite = finite && std::isfinite(x);
-                if (!finite) fail_request(r, UMAPHIP_NUMERICAL, "non-finite transform coordinates");
+            if (r->code == UMAPHIP_OK) {
+                TGraphState gs{};
+                HIP_CHECK(hipMemcpy(&gs, r->graph, sizeof(gs), hipMemcpyDeviceToHost));
+                if (gs.errors) {
+                    fail_request(r, transform_graph_status(gs.errors), transform_graph_error(gs.errors));
+                } else {
+                    r->out.resize(r->q * d);
+                    unsigned long long status = ~0ull;
+                    HIP_CHECK(hipMemcpy(r->out.data(), r->pos[r->cur], r->q * d * sizeof(float),
+                                        hipMemcpyDeviceToHost));
+                    HIP_CHECK(hipMemcpy(&status, r->status, sizeof(status), hipMemcpyDeviceToHost));
+                    bool finite = status == ~0ull;
+                    for (float x : r->out) finite = finite && std::isfinite(x);
+                    if (!finite) fail_request(r, UMAPHIP_NUMERICAL, "non-finite transform coordinates");
+                }
             }
         }
         {
@@ -640,19 +503,19 @@ umaphip_status umaphip_server_create(const float *reference, int64_t n, int64_t 
         schedules(static_cast<int>(S->epochs), S->alpha0, false, false, S->alpha, mu);
         bias_corrections(static_cast<int>(S->epochs), S->c1, S->c2);
         S->max_batch = p.max_batch_rows;
-        S->pool.reset(new TaskPool(static_cast<int>(
-            std::min<unsigned>(15, std::max<unsigned>(1, std::thread::hardware_concurrency()) - 1))));
         for (auto &st : S->sl) HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         HIP_CHECK(hipStreamCreateWithFlags(&S->sk, hipStreamNonBlocking));
         HIP_CHECK(hipEventCreateWithFlags(&S->ev_knn, hipEventDisableTiming));
-        HIP_CHECK(hipEventCreateWithFlags(&S->ev_up, hipEventDisableTiming));
-        HIP_CHECK(hipEventRecord(S->ev_up, S->sl[0]));
+        HIP_CHECK(hipEventCreateWithFlags(&S->ev_graph, hipEventDisableTiming));
+        HIP_CHECK(hipEventRecord(S->ev_graph, S->sl[0]));
         for (int w = 0; w < umaphip_server::kSlots; ++w) {
             HIP_CHECK(hipEventCreateWithFlags(&S->ev_slot[w], hipEventDisableTiming));
             HIP_CHECK(hipEventRecord(S->ev_slot[w], S->sl[0]));
             // a batch holds at most max_batch requests (each has a row)
             S->seg_pin[w].reset(new Pinned(sizeof(TransformSegment) * S->max_batch));
             S->seg_dev[w].alloc(S->max_batch);
+            S->gseg_pin[w].reset(new Pinned(sizeof(TGraphSegment) * S->max_batch));
+            S->gseg_dev[w].alloc(S->max_batch);
         }
         if (S->epochs > 0) {
             std::vector<float> sched(S->alpha);
@@ -706,12 +569,12 @@ umaphip_status umaphip_server_create(const float *reference, int64_t n, int64_t 
         S->d_idx.alloc(S->max_batch * S->k);
         S->d_dist.alloc(S->max_batch * S->k);
         S->q_pin.reset(new Pinned(S->max_batch * features * sizeof(float)));
-        S->idx_pin.reset(new Pinned(S->max_batch * S->k * sizeof(int32_t)));
-        S->dist_pin.reset(new Pinned(S->max_batch * S->k * sizeof(float)));
-        // per row: offsets, k indices, 4k clocks/periods, init; per request:
-        // seven 256-byte alignments
-        S->up_cap = static_cast<size_t>(S->max_batch) * (S->k * 36 + 16 + 7 * 256);
-        S->up_pin.reset(new Pinned(S->up_cap));
+        // device graph workspace of one batch and the graph settings
+        // (transform local connectivity max(lc - 1, 0), bandwidth 1)
+        S->g_tail.alloc(S->max_batch * S->k);
+        S->g_w.alloc(S->max_batch * S->k);
+        S->g_cnt.alloc(S->max_batch);
+        S->gparams = transform_graph_params(S->k, S->local_connectivity);
         // admission arena: explicit size, or a quarter of the free HBM up to 1 GiB
         size_t free_b = 0, total_b = 0;
         HIP_CHECK(hipMemGetInfo(&free_b, &total_b));

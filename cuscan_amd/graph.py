@@ -87,3 +87,44 @@ def transform_init(indices, directed, embedding):
     _lib.check(_lib.lib().umaphip_transform_init(_p(indices), _p(directed), q, k, _p(emb),
                                                  emb.shape[0], emb.shape[1], _p(out)))
     return out
+
+
+def transform_graph_device(indices, distances, embedding, *, local_connectivity=0.0, epochs,
+                           rate):
+    """The transform front-end on the device (bipartite memberships, initial
+    placement, pruning at max / epochs, compact CSR with clocks) for one
+    request. indices/distances: [q, k] from the kNN (host or device);
+    embedding: the fitted [n, d] (host or device). Returns the compact graph
+    dict (host arrays, as compact_graph) plus 'init' [q, d]."""
+    import torch
+
+    dev = torch.device("cuda:0")
+    idx = torch.as_tensor(np.ascontiguousarray(indices, np.int32) if not isinstance(indices, torch.Tensor)
+                          else indices, device=dev).to(torch.int32).contiguous()
+    dist = torch.as_tensor(np.ascontiguousarray(distances, np.float32)
+                           if not isinstance(distances, torch.Tensor) else distances,
+                           device=dev).to(torch.float32).contiguous()
+    emb = torch.as_tensor(np.ascontiguousarray(embedding, np.float32)
+                          if not isinstance(embedding, torch.Tensor) else embedding,
+                          device=dev).to(torch.float32).contiguous()
+    q, k = idx.shape
+    n, d = emb.shape
+    cap = max(q * k, 1)
+    offsets = torch.empty(q + 1, dtype=torch.int32, device=dev)
+    out_idx = torch.empty(cap, dtype=torch.int32, device=dev)
+    arrays = [torch.empty(cap, dtype=torch.float64, device=dev) for _ in range(4)]
+    init = torch.empty((q, d), dtype=torch.float32, device=dev)
+    n_edges = C.c_int64()
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    _lib.check(_lib.lib().umaphip_transform_graph(
+        C.c_void_p(idx.data_ptr()), C.c_void_p(dist.data_ptr()), q, k, C.c_void_p(emb.data_ptr()),
+        n, d, float(local_connectivity), int(epochs), float(rate),
+        C.c_void_p(offsets.data_ptr()), C.c_void_p(out_idx.data_ptr()),
+        *[C.c_void_p(a.data_ptr()) for a in arrays], C.c_void_p(init.data_ptr()),
+        C.byref(n_edges), C.c_void_p(stream)))
+    E = n_edges.value
+    host = lambda t: t.cpu().numpy()
+    return dict(offsets=host(offsets).view(np.uint32), indices=host(out_idx[:E]).view(np.uint32),
+                periods=host(arrays[0][:E]), negative_periods=host(arrays[1][:E]),
+                next_positive=host(arrays[2][:E]), next_negative=host(arrays[3][:E]),
+                init=host(init))

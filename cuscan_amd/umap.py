@@ -292,32 +292,49 @@ class UMAP:
             return np.zeros((q, self.embedding_.shape[1]), np.float32)
         k = min(int(self.n_neighbors), n)
         idx, dist = self._index.query(x, k)
-        g = G.fuzzy_graph(idx, dist, local_connectivity=max(self.local_connectivity - 1.0, 0.0),
-                          bandwidth=1.0, mix=self.set_op_mix_ratio, bipartite=True, n_reference=n)
+        epochs = int(self.n_epochs) // 3 if self.n_epochs is not None else (100 if q <= 10_000 else 30)
+        tseed = int(self.transform_seed)
+        lc = max(self.local_connectivity - 1.0, 0.0)
+        if self.optimizer != "compatibility":
+            # the device front-end (the transform server builds its graphs
+            # with the same kernels): memberships, placement, pruned CSR
+            compact = G.transform_graph_device(idx, dist, self._embedding_device(),
+                                               local_connectivity=lc, epochs=max(epochs, 1),
+                                               rate=self.negative_sample_rate)
+            if epochs == 0:
+                return compact["init"]
+            params = self._layout_params(epochs, self._a, self._b, self.learning_rate / 4.0, tseed)
+            emb, _, _, _ = L.optimize_compact(compact["init"], self.embedding_, compact, params,
+                                              shared=False)
+            return emb
+        g = G.fuzzy_graph(idx, dist, local_connectivity=lc, bandwidth=1.0,
+                          mix=self.set_op_mix_ratio, bipartite=True, n_reference=n)
         if len(g["weights"]) == 0 or not (g["weights"] > 0).any():
             raise ValueError("transform fuzzy graph is empty after positive-edge pruning")
         init = G.transform_init(idx, g["directed"], self.embedding_)
-        epochs = int(self.n_epochs) // 3 if self.n_epochs is not None else (100 if q <= 10_000 else 30)
         if epochs == 0:
             return init
-        tseed = int(self.transform_seed)
-        if self.optimizer == "compatibility":
-            from .classic import optimize_classic
+        from .classic import optimize_classic
 
-            # transform schedule: prepare_schedule(.., n_epochs, n_epochs, ..)
-            # (estimator.rs:2158-2166), learning rate / 4, transform_seed
-            return optimize_classic(init, self.embedding_, g["heads"], g["tails"], g["weights"],
-                                    n_epochs=epochs, default_epochs=epochs, a=self._a, b=self._b,
-                                    learning_rate=self.learning_rate / 4.0,
-                                    gamma=self.repulsion_strength,
-                                    negative_sample_rate=int(self.negative_sample_rate),
-                                    seed=tseed, shared=False)
-        compact = G.compact_graph(q, n, g["heads"], g["tails"], g["weights"], epochs=epochs,
-                                  rate=self.negative_sample_rate, prune_divisor=float(epochs),
-                                  transform=True)
-        params = self._layout_params(epochs, self._a, self._b, self.learning_rate / 4.0, tseed)
-        emb, _, _, _ = L.optimize_compact(init, self.embedding_, compact, params, shared=False)
-        return emb
+        # compatibility transform: prepare_schedule(.., n_epochs, n_epochs, ..)
+        # (estimator.rs:2158-2166), learning rate / 4, transform_seed
+        return optimize_classic(init, self.embedding_, g["heads"], g["tails"], g["weights"],
+                                n_epochs=epochs, default_epochs=epochs, a=self._a, b=self._b,
+                                learning_rate=self.learning_rate / 4.0,
+                                gamma=self.repulsion_strength,
+                                negative_sample_rate=int(self.negative_sample_rate),
+                                seed=tseed, shared=False)
+
+    def _embedding_device(self):
+        """The fitted embedding as a resident device tensor."""
+        import torch
+
+        cached = getattr(self, "_embedding_dev", None)
+        if cached is None or cached.shape != self.embedding_.shape:
+            cached = torch.as_tensor(np.ascontiguousarray(self.embedding_, np.float32),
+                                     device=torch.device("cuda:0"))
+            self._embedding_dev = cached
+        return cached
 
     def save(self, path, overwrite: bool = False) -> None:
         raise NotImplementedError("model serialisation (HDF5 schema v5) is not part of this build")

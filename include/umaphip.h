@@ -301,6 +301,25 @@ umaphip_status umaphip_random_init(int64_t n, int64_t d, uint64_t seed, float re
 umaphip_status umaphip_transform_init(const int32_t *indices, const float *directed, int64_t q,
                                       int64_t k, const float *embedding, int64_t n, int64_t d,
                                       float *out);
+/* The transform front-end on the device, for one request's q query rows:
+ * bipartite fuzzy memberships (fuzzy_graph with bipartite = true,
+ * kernels/standard_umap/graph.rs:117-433), the initial placement
+ * (estimator.rs:2385-2481), pruning below max / epochs (estimator.rs:
+ * 2753-2798) and compact_graph (crates/umap-core/src/modern_layout.rs:
+ * 188-208, 303-430), in the host steps' operation order with the device's
+ * expf. Device pointers: indices/distances [q, k] from the kNN, the fitted
+ * embedding [n, d]; outputs offsets [q + 1], indices/periods/clocks with
+ * capacity q * k, init [q, d]. *n_edges (host) receives the kept edges.
+ * local_connectivity is the transform's value (max(fit value - 1, 0)).
+ * Synchronises the stream. The transform server builds its graphs with the
+ * same kernels, so served and single transforms agree bit for bit. */
+umaphip_status umaphip_transform_graph(const int32_t *indices, const float *distances, int64_t q,
+                                       int64_t k, const float *embedding, int64_t n, int64_t d,
+                                       float local_connectivity, int32_t epochs, float rate,
+                                       uint32_t *offsets, uint32_t *out_indices, double *periods,
+                                       double *negative_periods, double *next_positive,
+                                       double *next_negative, float *init, int64_t *n_edges,
+                                       void *stream);
 
 /* ------------------------------------------------------------------------
  * Compatibility (classic) SGD layout, optimizer="compatibility".
