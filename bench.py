@@ -120,13 +120,12 @@ def build_workload(rows, dims, clusters, seed, k, epochs):
     t0 = time.time()
     indices = np.concatenate([np.arange(rows, dtype=np.int32)[:, None], idx], axis=1)
     distances = np.concatenate([np.zeros((rows, 1), np.float32), dist], axis=1)
-    g = G.fuzzy_graph(indices, distances)
-    compact = G.compact_graph(rows, rows, g["heads"], g["tails"], g["weights"], epochs=epochs,
-                              rate=5.0, prune_divisor=float(epochs))
+    compact = G.fit_graph_device(indices, distances, epochs=epochs, rate=5.0,
+                                 prune_divisor=float(epochs))
     a, b = G.fit_curve(0.1, 1.0)
     init = G.random_init(rows, 2, seed)
     graph_s = time.time() - t0
-    log(f"fuzzy graph + compaction: E={len(compact['indices'])} in {graph_s:.1f}s")
+    log(f"fuzzy graph + compaction (device): E={len(compact['indices'])} in {graph_s:.2f}s")
     return x, compact, init, a, b, knn_s, graph_s, index
 
 

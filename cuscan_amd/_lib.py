@@ -141,6 +141,9 @@ SIGNATURES = {
                                     C.POINTER(C.c_float)]),
     "umaphip_random_init": (C.c_int, [_i64, _i64, C.c_uint64, C.c_float, C.c_float, _i32, _vp]),
     "umaphip_transform_init": (C.c_int, [_vp, _vp, _i64, _i64, _vp, _i64, _i64, _vp]),
+    "umaphip_fit_graph": (C.c_int, [_vp, _vp, _i64, _i64, C.c_float, C.c_float, C.c_float, _i32,
+                                    C.c_float, C.c_double, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                    _vp]),
     "umaphip_transform_graph": (C.c_int, [_vp, _vp, _i64, _i64, _vp, _i64, _i64, C.c_float, _i32,
                                           C.c_float, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "umaphip_classic_epoch": (C.c_int, [_vp, _vp]),

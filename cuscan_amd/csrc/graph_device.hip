@@ -23,6 +23,7 @@
 
 #include "common.hpp"
 #include "graph_device.hpp"
+#include "layout_kernels.hpp"
 
 namespace umaphip {
 namespace {
@@ -312,6 +313,216 @@ __global__ __launch_bounds__(kRowThreads) void tg_csr_kernel(
     if (bad) atomicOr(&S.state->errors, kTGraphBadPeriod);
 }
 
+// ------------------------------------------------------------- fit graph
+// The fit's fuzzy simplicial set on the device: memberships of every row
+// (as above, self excluded), the fuzzy union of each unordered pair emitted
+// once (fuzzy_graph's evaluate_edge, graph.rs:252-337), pruning below
+// max / divisor, both directions of every kept pair sorted by (head, tail)
+// with two stable radix sorts, and the compact CSR with clocks.
+
+__global__ __launch_bounds__(kRowThreads) void fg_rows_kernel(const int32_t *idx, const float *dist,
+                                                             int64_t n, int64_t k, TGraphParams P,
+                                                             const TGraphState *state,
+                                                             float *directed, int32_t *errors) {
+    const int64_t row = static_cast<int64_t>(blockIdx.x) * kRowThreads + threadIdx.x;
+    if (row >= n) return;
+    const int kk = static_cast<int>(k);
+    const int32_t *ids = idx + row * k;
+    const float *dd = dist + row * k;
+    float rho, sigma;
+    smooth_row(dd, kk, state->mean, P, rho, sigma);
+    bool invalid = false;
+    for (int s = 0; s < kk; ++s) {
+        const int32_t index = ids[s];
+        const float delta = dd[s] - rho;
+        const float safe = sigma == 0.0f ? 1.0f : sigma;
+        const float clamped = delta < 0.0f ? 0.0f : delta;
+        float value = expf(-clamped / safe);
+        if (dd[s] <= rho || sigma == 0.0f) value = 1.0f;
+        if (index < 0 || index >= n || index == row) value = 0.0f;
+        directed[row * k + s] = value;
+        if (index < -1 || index >= n) invalid = true;
+    }
+    if (invalid) atomicOr(errors, kTGraphInvalidIds);
+}
+
+// Slot e = row * k + s: the union weight of the pair {row, tail} when this
+// slot is the pair's emitter (first occurrence in the row, and the smaller
+// id when both rows list each other), else 0.
+__global__ __launch_bounds__(kSegThreads) void fg_emit_kernel(const int32_t *idx,
+                                                             const float *directed, int64_t n,
+                                                             int64_t k, float mix, float *emit_w,
+                                                             uint32_t *wmax_bits) {
+    const int64_t e = static_cast<int64_t>(blockIdx.x) * kSegThreads + threadIdx.x;
+    if (e >= n * k) return;
+    const int64_t row = e / k;
+    const int slot = static_cast<int>(e - row * k);
+    const int kk = static_cast<int>(k);
+    float out = 0.0f;
+    const int32_t tail = idx[e];
+    if (tail >= 0 && tail < n && tail != row) {
+        bool first = true;
+        float forward = 0.0f;
+        for (int o = 0; o < kk; ++o)
+            if (idx[row * k + o] == tail) {
+                if (o < slot) first = false;
+                forward += directed[row * k + o];
+            }
+        if (first) {
+            float reverse = 0.0f;
+            bool has_reverse = false;
+            for (int rs = 0; rs < kk; ++rs)
+                if (idx[static_cast<int64_t>(tail) * k + rs] == static_cast<int32_t>(row)) {
+                    reverse += directed[static_cast<int64_t>(tail) * k + rs];
+                    has_reverse = true;
+                }
+            if (!has_reverse || row <= tail) {
+                const float inter = forward * reverse;
+                const float uni = (forward + reverse) - inter;
+                const float fuzzy = mix * uni + (1.0f - mix) * inter;
+                if (fuzzy > 0.0f) out = fuzzy;
+            }
+        }
+    }
+    emit_w[e] = out;
+    if (out > 0.0f) atomicMax(wmax_bits, __float_as_uint(out));
+}
+
+// Kept emissions: 1 where the weight is positive and not below the prune
+// threshold.
+__global__ __launch_bounds__(kSegThreads) void fg_keep_kernel(const float *emit_w, int64_t m,
+                                                             const uint32_t *wmax_bits,
+                                                             float divisor, uint32_t *keep) {
+    const int64_t e = static_cast<int64_t>(blockIdx.x) * kSegThreads + threadIdx.x;
+    if (e >= m) return;
+    const float w = emit_w[e];
+    const float threshold = divisor > 0.0f ? __uint_as_float(*wmax_bits) / divisor : 0.0f;
+    keep[e] = (w == 0.0f || w < threshold) ? 0u : 1u;
+}
+
+// Exclusive scan of m u32 values in three launches (block totals, a scan of
+// the totals in one workgroup, block-local scans plus offsets); no
+// workgroup waits on another.
+constexpr int kScanItems = 4;
+constexpr int kScanTile = kSegThreads * kScanItems;
+
+__global__ __launch_bounds__(kSegThreads) void scan_totals_kernel(const uint32_t *in, int64_t m,
+                                                                 uint32_t *totals) {
+    __shared__ uint32_t s_wave[kSegThreads / 64];
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile;
+    uint32_t v = 0;
+    for (int i = 0; i < kScanItems; ++i) {
+        const int64_t p = base + static_cast<int64_t>(threadIdx.x) * kScanItems + i;
+        if (p < m) v += in[p];
+    }
+    uint32_t total;
+    (void)seg_exclusive_scan(v, s_wave, total);
+    if (threadIdx.x == 0) totals[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kSegThreads) void scan_block_offsets_kernel(uint32_t *totals,
+                                                                        int64_t blocks,
+                                                                        uint32_t *grand) {
+    __shared__ uint32_t s_wave[kSegThreads / 64];
+    uint32_t carry = 0;
+    for (int64_t b0 = 0; b0 < blocks; b0 += kSegThreads) {
+        const int64_t b = b0 + threadIdx.x;
+        const uint32_t v = b < blocks ? totals[b] : 0u;
+        uint32_t chunk;
+        const uint32_t ex = seg_exclusive_scan(v, s_wave, chunk);
+        if (b < blocks) totals[b] = carry + ex;
+        carry += chunk;
+    }
+    if (threadIdx.x == 0) *grand = carry;
+}
+
+__global__ __launch_bounds__(kSegThreads) void scan_apply_kernel(const uint32_t *in, int64_t m,
+                                                                const uint32_t *offsets,
+                                                                uint32_t *out) {
+    __shared__ uint32_t s_wave[kSegThreads / 64];
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile;
+    uint32_t v[kScanItems], sum = 0;
+    for (int i = 0; i < kScanItems; ++i) {
+        const int64_t p = base + static_cast<int64_t>(threadIdx.x) * kScanItems + i;
+        v[i] = p < m ? in[p] : 0u;
+        sum += v[i];
+    }
+    uint32_t total;
+    uint32_t run = offsets[blockIdx.x] + seg_exclusive_scan(sum, s_wave, total);
+    for (int i = 0; i < kScanItems; ++i) {
+        const int64_t p = base + static_cast<int64_t>(threadIdx.x) * kScanItems + i;
+        if (p < m) out[p] = run;
+        run += v[i];
+    }
+}
+
+// Both directions of every kept pair: (min, max) at 2p and (max, min) at 2p + 1.
+__global__ __launch_bounds__(kSegThreads) void fg_pairs_kernel(const int32_t *idx,
+                                                              const float *emit_w,
+                                                              const uint32_t *keep,
+                                                              const uint32_t *pos, int64_t m,
+                                                              int64_t k, uint32_t *heads,
+                                                              uint32_t *tails, float *w) {
+    const int64_t e = static_cast<int64_t>(blockIdx.x) * kSegThreads + threadIdx.x;
+    if (e >= m || !keep[e]) return;
+    const uint32_t row = static_cast<uint32_t>(e / k);
+    const uint32_t tail = static_cast<uint32_t>(idx[e]);
+    const uint32_t lo = row < tail ? row : tail, hi = row < tail ? tail : row;
+    const int64_t p = 2 * static_cast<int64_t>(pos[e]);
+    heads[p] = lo;
+    tails[p] = hi;
+    heads[p + 1] = hi;
+    tails[p + 1] = lo;
+    w[p] = emit_w[e];
+    w[p + 1] = emit_w[e];
+}
+
+__global__ __launch_bounds__(kSegThreads) void gather_u32_kernel(const uint32_t *src,
+                                                                const int32_t *perm, int64_t m,
+                                                                uint32_t *dst) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kSegThreads + threadIdx.x;
+    if (i < m) dst[i] = src[perm[i]];
+}
+
+// Row offsets from the sorted heads: offsets[r] = first position with head >= r.
+__global__ __launch_bounds__(kSegThreads) void fg_offsets_kernel(const uint32_t *heads_sorted,
+                                                                int64_t m, int64_t n,
+                                                                uint32_t *offsets) {
+    const int64_t r = static_cast<int64_t>(blockIdx.x) * kSegThreads + threadIdx.x;
+    if (r > n) return;
+    int64_t lo = 0, hi = m;
+    while (lo < hi) {
+        const int64_t mid = lo + (hi - lo) / 2;
+        if (heads_sorted[mid] < static_cast<uint32_t>(r)) lo = mid + 1;
+        else hi = mid;
+    }
+    offsets[r] = static_cast<uint32_t>(lo);
+}
+
+// Column, weight, periods and clocks of edge i of the sorted CSR.
+__global__ __launch_bounds__(kSegThreads) void fg_csr_kernel(
+    const uint32_t *tails, const float *w, const int32_t *perm, int64_t m,
+    const uint32_t *wmax_bits, int32_t epochs, float rate, uint32_t *indices, float *weights,
+    double *periods, double *negative_periods, double *next_positive, double *next_negative,
+    int32_t *errors) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kSegThreads + threadIdx.x;
+    if (i >= m) return;
+    const int64_t src = perm[i];
+    const float wi = w[src];
+    const float wmax = __uint_as_float(*wmax_bits);
+    const float samples = static_cast<float>(epochs) * (wi / wmax);
+    const double period = static_cast<double>(epochs) / static_cast<double>(samples);
+    const double nperiod = rate == 0.0f ? INFINITY : (period / static_cast<double>(rate)) * 1.5;
+    if (!isfinite(period) || period <= 0.0 || (rate > 0.0f && (!isfinite(nperiod) || nperiod <= 0.0)))
+        atomicOr(errors, kTGraphBadPeriod);
+    indices[i] = tails[src];
+    weights[i] = wi;
+    periods[i] = period;
+    negative_periods[i] = nperiod;
+    next_positive[i] = period;
+    next_negative[i] = nperiod;
+}
+
 }  // namespace
 
 TGraphParams transform_graph_params(int64_t k, float local_connectivity) {
@@ -413,6 +624,96 @@ umaphip_status umaphip_transform_graph(const int32_t *indices, const float *dist
         *n_edges = host.edges;
         UMAPHIP_REQUIRE(host.errors == 0, transform_graph_status(host.errors), "%s",
                         transform_graph_error(host.errors));
+    });
+}
+
+}  // extern "C"
+
+extern "C" {
+
+umaphip_status umaphip_fit_graph(const int32_t *indices, const float *distances, int64_t n,
+                                 int64_t k, float local_connectivity, float bandwidth, float mix,
+                                 int32_t epochs, float rate, double prune_divisor, uint32_t *offsets,
+                                 uint32_t *out_indices, float *weights, double *periods,
+                                 double *negative_periods, double *next_positive,
+                                 double *next_negative, int64_t *n_edges, void *stream) {
+    return guarded([&] {
+        UMAPHIP_REQUIRE(indices && distances && offsets && out_indices && weights && periods &&
+                            negative_periods && next_positive && next_negative && n_edges,
+                        UMAPHIP_INVALID_INPUT, "fit graph arguments are null");
+        UMAPHIP_REQUIRE(n > 0 && k > 0 && k <= kMaxGraphK, UMAPHIP_INVALID_INPUT,
+                        "fit graph needs rows and 1..%d neighbours", kMaxGraphK);
+        UMAPHIP_REQUIRE(std::isfinite(local_connectivity) && local_connectivity >= 0.0f &&
+                            std::isfinite(bandwidth) && bandwidth > 0.0f && std::isfinite(mix) &&
+                            mix >= 0.0f && mix <= 1.0f && epochs > 0 && std::isfinite(rate) &&
+                            rate >= 0.0f,
+                        UMAPHIP_INVALID_CONFIG, "invalid fit graph settings");
+        UMAPHIP_REQUIRE(2 * n * k <= INT32_MAX, UMAPHIP_OVERFLOW, "fit graph COO indices");
+        hipStream_t s = as_stream(stream);
+        const int64_t m = n * k;
+        TGraphParams P = transform_graph_params(k, local_connectivity);
+        P.target = std::log2(static_cast<float>(k)) * bandwidth;
+        DeviceArray<TGraphState> state(1);
+        DeviceArray<TGraphSegment> seg(1);
+        TGraphSegment S{};
+        S.row0 = 0;
+        S.rows = n;
+        S.state = state.ptr;
+        HIP_CHECK(hipMemcpyAsync(seg.ptr, &S, sizeof(S), hipMemcpyHostToDevice, s));
+        DeviceArray<float> directed(m), emit_w(m);
+        DeviceArray<uint32_t> keep(m), pos(m);
+        const int64_t blocks = (m + kScanTile - 1) / kScanTile;
+        DeviceArray<uint32_t> totals(blocks), grand(1);
+        const unsigned gm = static_cast<unsigned>((m + kSegThreads - 1) / kSegThreads);
+        hipLaunchKernelGGL(tg_mean_kernel, dim3(1), dim3(kSegThreads), 0, s, seg.ptr, distances, k);
+        hipLaunchKernelGGL(fg_rows_kernel, dim3(static_cast<unsigned>((n + kRowThreads - 1) / kRowThreads)),
+                           dim3(kRowThreads), 0, s, indices, distances, n, k, P, state.ptr,
+                           directed.ptr, &state.ptr->errors);
+        hipLaunchKernelGGL(fg_emit_kernel, dim3(gm), dim3(kSegThreads), 0, s, indices, directed.ptr,
+                           n, k, mix, emit_w.ptr, &state.ptr->wmax_bits);
+        const float divisor = prune_divisor > 0.0 ? static_cast<float>(prune_divisor) : 0.0f;
+        hipLaunchKernelGGL(fg_keep_kernel, dim3(gm), dim3(kSegThreads), 0, s, emit_w.ptr, m,
+                           &state.ptr->wmax_bits, divisor, keep.ptr);
+        hipLaunchKernelGGL(scan_totals_kernel, dim3(static_cast<unsigned>(blocks)), dim3(kSegThreads),
+                           0, s, keep.ptr, m, totals.ptr);
+        hipLaunchKernelGGL(scan_block_offsets_kernel, dim3(1), dim3(kSegThreads), 0, s, totals.ptr,
+                           blocks, grand.ptr);
+        hipLaunchKernelGGL(scan_apply_kernel, dim3(static_cast<unsigned>(blocks)), dim3(kSegThreads), 0,
+                           s, keep.ptr, m, totals.ptr, pos.ptr);
+        HIP_CHECK(hipGetLastError());
+        uint32_t kept = 0;
+        TGraphState hs{};
+        HIP_CHECK(hipMemcpyAsync(&kept, grand.ptr, sizeof(kept), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipMemcpyAsync(&hs, state.ptr, sizeof(hs), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        UMAPHIP_REQUIRE(!(hs.errors & kTGraphInvalidIds), UMAPHIP_NUMERICAL,
+                        "fuzzy graph contains missing or invalid neighbor indices");
+        UMAPHIP_REQUIRE(kept > 0, UMAPHIP_INVALID_INPUT,
+                        "fuzzy graph is empty after positive-edge pruning");
+        const int64_t E = 2 * static_cast<int64_t>(kept);
+        DeviceArray<uint32_t> heads(E), tails(E), keys_sorted(E), keys2(E);
+        DeviceArray<float> w(E);
+        DeviceArray<int32_t> perm1(E), perm(E);
+        DeviceArray<uint8_t> ws(sort_pairs_workspace(E));
+        hipLaunchKernelGGL(fg_pairs_kernel, dim3(gm), dim3(kSegThreads), 0, s, indices, emit_w.ptr,
+                           keep.ptr, pos.ptr, m, k, heads.ptr, tails.ptr, w.ptr);
+        // (head, tail) order: stable by tail, then stable by head
+        launch_sort_pairs(ws.ptr, ws.count, tails.ptr, keys_sorted.ptr, nullptr, perm1.ptr, E, s);
+        const unsigned ge = static_cast<unsigned>((E + kSegThreads - 1) / kSegThreads);
+        hipLaunchKernelGGL(gather_u32_kernel, dim3(ge), dim3(kSegThreads), 0, s, heads.ptr, perm1.ptr,
+                           E, keys2.ptr);
+        launch_sort_pairs(ws.ptr, ws.count, keys2.ptr, keys_sorted.ptr, perm1.ptr, perm.ptr, E, s);
+        hipLaunchKernelGGL(fg_offsets_kernel, dim3(static_cast<unsigned>((n + 1 + kSegThreads - 1) / kSegThreads)),
+                           dim3(kSegThreads), 0, s, keys_sorted.ptr, E, n, offsets);
+        hipLaunchKernelGGL(fg_csr_kernel, dim3(ge), dim3(kSegThreads), 0, s, tails.ptr, w.ptr,
+                           perm.ptr, E, &state.ptr->wmax_bits, epochs, rate, out_indices, weights,
+                           periods, negative_periods, next_positive, next_negative,
+                           &state.ptr->errors);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipMemcpyAsync(&hs, state.ptr, sizeof(hs), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        UMAPHIP_REQUIRE(!(hs.errors & kTGraphBadPeriod), UMAPHIP_NUMERICAL, "invalid layout period");
+        *n_edges = E;
     });
 }
 

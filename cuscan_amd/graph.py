@@ -128,3 +128,37 @@ def transform_graph_device(indices, distances, embedding, *, local_connectivity=
                 periods=host(arrays[0][:E]), negative_periods=host(arrays[1][:E]),
                 next_positive=host(arrays[2][:E]), next_negative=host(arrays[3][:E]),
                 init=host(init))
+
+
+def fit_graph_device(indices, distances, *, local_connectivity=1.0, bandwidth=1.0, mix=1.0,
+                     epochs, rate, prune_divisor):
+    """The fit graph on the device: fuzzy union, pruning below max /
+    prune_divisor and the compact CSR with clocks (as fuzzy_graph followed by
+    compact_graph). indices/distances: [n, k] with self first (host or
+    device). Returns the compact graph dict (host arrays, as compact_graph)."""
+    import torch
+
+    dev = torch.device("cuda:0")
+    idx = (indices if isinstance(indices, torch.Tensor)
+           else torch.as_tensor(np.ascontiguousarray(indices, np.int32))).to(dev, torch.int32).contiguous()
+    dist = (distances if isinstance(distances, torch.Tensor)
+            else torch.as_tensor(np.ascontiguousarray(distances, np.float32))).to(dev, torch.float32).contiguous()
+    n, k = idx.shape
+    cap = max(2 * n * k, 1)
+    offsets = torch.empty(n + 1, dtype=torch.int32, device=dev)
+    out_idx = torch.empty(cap, dtype=torch.int32, device=dev)
+    w = torch.empty(cap, dtype=torch.float32, device=dev)
+    arrays = [torch.empty(cap, dtype=torch.float64, device=dev) for _ in range(4)]
+    n_edges = C.c_int64()
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    _lib.check(_lib.lib().umaphip_fit_graph(
+        C.c_void_p(idx.data_ptr()), C.c_void_p(dist.data_ptr()), n, k, float(local_connectivity),
+        float(bandwidth), float(mix), int(epochs), float(rate), float(prune_divisor),
+        C.c_void_p(offsets.data_ptr()), C.c_void_p(out_idx.data_ptr()), C.c_void_p(w.data_ptr()),
+        *[C.c_void_p(a.data_ptr()) for a in arrays], C.byref(n_edges), C.c_void_p(stream)))
+    E = n_edges.value
+    host = lambda t: t.cpu().numpy()
+    return dict(offsets=host(offsets).view(np.uint32), indices=host(out_idx[:E]).view(np.uint32),
+                weights=host(w[:E]), periods=host(arrays[0][:E]),
+                negative_periods=host(arrays[1][:E]), next_positive=host(arrays[2][:E]),
+                next_negative=host(arrays[3][:E]))

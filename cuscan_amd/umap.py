@@ -223,8 +223,6 @@ class UMAP:
         idx, dist = index.query(x, nonself, exclude_self=True)
         indices = np.concatenate([np.arange(n, dtype=np.int32)[:, None], idx], axis=1)
         distances = np.concatenate([np.zeros((n, 1), np.float32), dist], axis=1)
-        g = G.fuzzy_graph(indices, distances, local_connectivity=self.local_connectivity,
-                          bandwidth=1.0, mix=self.set_op_mix_ratio)
         default_epochs = 500 if n <= 10_000 else 200
         n_epochs = int(self.n_epochs) if self.n_epochs is not None else default_epochs
         divisor = float(n_epochs if n_epochs > 10 else default_epochs)
@@ -234,6 +232,8 @@ class UMAP:
         if self.optimizer == "compatibility":
             from .classic import optimize_classic
 
+            g = G.fuzzy_graph(indices, distances, local_connectivity=self.local_connectivity,
+                              bandwidth=1.0, mix=self.set_op_mix_ratio)
             emb = optimize_classic(init, None, g["heads"], g["tails"], g["weights"],
                                    n_epochs=n_epochs, default_epochs=default_epochs, a=a, b=b,
                                    learning_rate=self.learning_rate,
@@ -241,9 +241,11 @@ class UMAP:
                                    negative_sample_rate=int(self.negative_sample_rate), seed=seed,
                                    shared=True)
         else:
-            compact = G.compact_graph(n, n, g["heads"], g["tails"], g["weights"],
-                                      epochs=n_epochs, rate=self.negative_sample_rate,
-                                      prune_divisor=divisor)
+            # fuzzy union, pruning and the compact CSR on the device
+            compact = G.fit_graph_device(indices, distances,
+                                         local_connectivity=self.local_connectivity,
+                                         bandwidth=1.0, mix=self.set_op_mix_ratio, epochs=n_epochs,
+                                         rate=self.negative_sample_rate, prune_divisor=divisor)
             params = self._layout_params(n_epochs, a, b, self.learning_rate, seed)
             emb, _, _, self.layout_stats_ = L.optimize_compact(init, None, compact, params,
                                                                shared=True)

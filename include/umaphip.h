@@ -301,6 +301,20 @@ umaphip_status umaphip_random_init(int64_t n, int64_t d, uint64_t seed, float re
 umaphip_status umaphip_transform_init(const int32_t *indices, const float *directed, int64_t q,
                                       int64_t k, const float *embedding, int64_t n, int64_t d,
                                       float *out);
+/* The fit graph on the device: fuzzy_graph (symmetric fuzzy union,
+ * kernels/standard_umap/graph.rs:21-433) followed by pruning below
+ * max / prune_divisor (estimator.rs:531-560) and compact_graph
+ * (crates/umap-core/src/modern_layout.rs:188-208, 303-430), in the host
+ * steps' operation order with the device's expf. Device pointers: indices
+ * and distances [n, k] (self first, as the fit's kNN gives them); outputs
+ * offsets [n + 1] and indices/weights/periods/clocks with capacity 2 n k.
+ * *n_edges (host) receives the edge count. Synchronises the stream. */
+umaphip_status umaphip_fit_graph(const int32_t *indices, const float *distances, int64_t n,
+                                 int64_t k, float local_connectivity, float bandwidth, float mix,
+                                 int32_t epochs, float rate, double prune_divisor, uint32_t *offsets,
+                                 uint32_t *out_indices, float *weights, double *periods,
+                                 double *negative_periods, double *next_positive,
+                                 double *next_negative, int64_t *n_edges, void *stream);
 /* The transform front-end on the device, for one request's q query rows:
  * bipartite fuzzy memberships (fuzzy_graph with bipartite = true,
  * kernels/standard_umap/graph.rs:117-433), the initial placement

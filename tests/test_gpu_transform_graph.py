@@ -75,3 +75,49 @@ def test_device_transform_graph_errors():
     none = np.full_like(idx, -1)            # no neighbours at all
     with pytest.raises(ValueError, match="empty after positive-edge pruning"):
         G.transform_graph_device(none, dist, emb, epochs=10, rate=5.0)
+
+
+def fit_case(n, k, seed):
+    from cuscan_amd.knn import KnnIndex
+
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(-10, 10, (8, 6))
+    x = (c[rng.integers(0, 8, n)] + rng.normal(0, 1, (n, 6))).astype(np.float32)
+    idx, dist = KnnIndex(x, max_queries=n).query(x, k - 1, exclude_self=True)
+    indices = np.concatenate([np.arange(n, dtype=np.int32)[:, None], idx], axis=1)
+    distances = np.concatenate([np.zeros((n, 1), np.float32), dist], axis=1)
+    return indices, distances
+
+
+@pytest.mark.parametrize("n,k,lc,mix,epochs", [
+    (500, 15, 1.0, 1.0, 200),
+    (3000, 15, 1.0, 1.0, 200),
+    (2000, 10, 1.5, 0.7, 50),
+    (300, 64, 0.0, 1.0, 500),
+])
+def test_device_fit_graph_matches_host_chain(n, k, lc, mix, epochs):
+    from cuscan_amd import graph as G
+
+    print(f"[gpu] fit graph n={n} k={k}", flush=True)
+    indices, distances = fit_case(n, k, 7 + n)
+    g = G.fuzzy_graph(indices, distances, local_connectivity=lc, bandwidth=1.0, mix=mix)
+    want = G.compact_graph(n, n, g["heads"], g["tails"], g["weights"], epochs=epochs, rate=5.0,
+                           prune_divisor=float(epochs))
+    got = G.fit_graph_device(indices, distances, local_connectivity=lc, mix=mix, epochs=epochs,
+                             rate=5.0, prune_divisor=float(epochs))
+    assert (got["offsets"] == want["offsets"]).all()
+    assert (got["indices"] == want["indices"]).all()
+    # a membership's last-ulp difference moves the bisection's sigma, and a
+    # union weight carries both rows' memberships: 1e-4 relative
+    np.testing.assert_allclose(got["weights"], want["weights"], rtol=1e-4)
+    for key in ("periods", "negative_periods", "next_positive", "next_negative"):
+        np.testing.assert_allclose(got[key], want[key], rtol=1e-4, err_msg=key)
+
+
+def test_device_fit_graph_rejects_invalid_ids():
+    from cuscan_amd import graph as G
+
+    indices, distances = fit_case(200, 8, 3)
+    indices[5, 3] = 200
+    with pytest.raises(RuntimeError, match="invalid neighbor indices"):
+        G.fit_graph_device(indices, distances, epochs=100, rate=5.0, prune_divisor=100.0)
