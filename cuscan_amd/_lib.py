@@ -101,6 +101,28 @@ class LayoutStats(C.Structure):
     ]
 
 
+
+class RecursiveParams(C.Structure):
+    _fields_ = [
+        ("coarsening_ratio", C.c_int64), ("base_init_threshold", C.c_int64),
+        ("n_embedding_epochs", C.c_int64), ("hub_selection", C.c_int32),
+        ("anchor_sample_size", C.c_int64),
+    ]
+
+
+class RecursiveLevel(C.Structure):
+    _fields_ = [
+        ("parts", C.c_int64), ("epochs", C.c_int64), ("a", C.c_float), ("b", C.c_float),
+        ("good_initialization", C.c_int32), ("curve_a", C.c_double), ("curve_b", C.c_double),
+    ]
+
+
+class RecursiveStats(C.Structure):
+    _fields_ = [
+        ("depth", C.c_int32), ("rows", C.c_int64 * 32), ("edges", C.c_int64 * 32),
+        ("sweeps", C.c_int32 * 32), ("host_ms", C.c_double), ("layout_ms", C.c_double),
+    ]
+
 # Every exported symbol of include/umaphip.h, with its C signature.
 _vp = C.c_void_p
 _i64 = C.c_int64
@@ -146,6 +168,24 @@ SIGNATURES = {
                                     _vp]),
     "umaphip_transform_graph": (C.c_int, [_vp, _vp, _i64, _i64, _vp, _i64, _i64, C.c_float, _i32,
                                           C.c_float, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "umaphip_init_rescale": (C.c_int, [_vp, _i64, _i64, _vp, _vp, _vp, _i32, C.c_float,
+                                       C.c_float]),
+    "umaphip_recursive_init": (C.c_int, [_i64, _vp, _vp, _vp, _vp, _i64, _i64, C.c_float,
+                                         C.c_float, C.POINTER(RecursiveParams), C.c_uint64, _vp,
+                                         C.POINTER(RecursiveStats)]),
+    "umaphip_recursive_level_params": (C.c_int, [_i64, C.POINTER(RecursiveLevel), C.c_float,
+                                                 C.c_float, C.POINTER(RecursiveParams),
+                                                 C.POINTER(RecursiveLevel), C.POINTER(_i32)]),
+    "umaphip_recursive_hubs": (C.c_int, [_i64, _vp, _vp, _vp, _i64, _i32, C.c_uint64, _vp]),
+    "umaphip_recursive_propagate": (C.c_int, [_i64, _vp, _vp, _vp, _i32, _vp, C.POINTER(_i32)]),
+    "umaphip_recursive_outliers": (C.c_int, [_i64, _vp, _vp, C.c_uint64, _vp]),
+    "umaphip_recursive_remap": (C.c_int, [_vp, _i64, C.POINTER(_i64)]),
+    "umaphip_recursive_aggregate": (C.c_int, [_i64, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp]),
+    "umaphip_recursive_transition": (C.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "umaphip_recursive_anchor": (C.c_int, [_vp, _i64, _i64, _i64, _i64, C.c_uint64, _vp, _vp]),
+    "umaphip_recursive_base_case": (C.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _vp]),
+    "umaphip_recursive_prolongate": (C.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _vp]),
+    "umaphip_recursive_procrustes": (C.c_int, [_vp, _i64, _vp, _vp, _i64]),
     "umaphip_classic_epoch": (C.c_int, [_vp, _vp]),
     "umaphip_plan_set_objective": (C.c_int, [_vp, _vp, _i32]),
     "umaphip_plan_objective": (C.c_int, [_vp, _vp, _i64, _vp]),

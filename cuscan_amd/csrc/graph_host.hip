@@ -14,32 +14,11 @@
 #include <vector>
 
 #include "common.hpp"
+#include "host_parallel.hpp"
 #include "stdrng.hpp"
 
 namespace umaphip {
 namespace {
-
-template <class F>
-void parallel_for(int64_t n, F &&body, int64_t grain = 1024) {
-    const int64_t hw = std::max<unsigned>(1, std::thread::hardware_concurrency());
-    const int64_t workers = std::min<int64_t>(std::min<int64_t>(hw, 16), (n + grain - 1) / grain);
-    if (workers <= 1) {
-        for (int64_t i = 0; i < n; ++i) body(i);
-        return;
-    }
-    std::atomic<int64_t> next{0};
-    std::vector<std::thread> pool;
-    for (int64_t w = 0; w < workers; ++w)
-        pool.emplace_back([&] {
-            for (;;) {
-                const int64_t start = next.fetch_add(grain);
-                if (start >= n) break;
-                const int64_t stop = std::min(n, start + grain);
-                for (int64_t i = start; i < stop; ++i) body(i);
-            }
-        });
-    for (auto &t : pool) t.join();
-}
 
 // smooth_row (crates/umap/src/kernels/standard_umap/graph.rs:117-242).
 void smooth_row(const float *dist, int64_t k, float global_mean, uint32_t floor_conn,
@@ -442,52 +421,155 @@ umaphip_status umaphip_fit_curve(float min_dist, float spread, float *a_out, flo
     });
 }
 
-// random_embedding + isotropic/per-axis rescale (crates/umap-core/src/
-// init.rs:1519-1530, init_rescale.rs:77-203). mode: 0 isotropic, 1 per axis.
+namespace {
+
+// Order key of f32::total_cmp for non-NaN values (-0 before +0).
+inline int32_t total_key(float x) {
+    int32_t b;
+    std::memcpy(&b, &x, 4);
+    return b ^ static_cast<int32_t>(static_cast<uint32_t>(b >> 31) >> 1);
+}
+
+// init_rescale::rescale (crates/umap-core/src/init_rescale.rs:77-203):
+// centre each axis (sequential f32 sum), then scale by the isotropic
+// geometric mean of the 5-95% spans (mode 0), per axis (mode 1), or by the
+// weight-median edge length of the graph (mode 2; edges in CSR order).
+void rescale(float *v, int64_t n, int64_t d, const int64_t *offsets, const int32_t *indices,
+             const float *weights, int32_t mode, float repulsion, float scale_factor) {
+    auto finite = [](float x) { return std::isfinite(x); };
+    for (int64_t c = 0; c < d; ++c) {
+        float sum = 0.0f;
+        for (int64_t r = 0; r < n; ++r) {
+            UMAPHIP_REQUIRE(finite(v[r * d + c]), UMAPHIP_NUMERICAL, "non-finite initialization");
+            sum += v[r * d + c];
+            UMAPHIP_REQUIRE(finite(sum), UMAPHIP_NUMERICAL, "non-finite initialization");
+        }
+        const float mean = sum / static_cast<float>(n);
+        for (int64_t r = 0; r < n; ++r) {
+            v[r * d + c] -= mean;
+            UMAPHIP_REQUIRE(finite(v[r * d + c]), UMAPHIP_NUMERICAL, "non-finite initialization");
+        }
+    }
+    auto apply = [&](double multiplier) {
+        const float m = static_cast<float>(multiplier);
+        UMAPHIP_REQUIRE(finite(m), UMAPHIP_NUMERICAL, "non-finite initialization");
+        for (int64_t i = 0; i < n * d; ++i) {
+            v[i] *= m;
+            UMAPHIP_REQUIRE(finite(v[i]), UMAPHIP_NUMERICAL, "non-finite initialization");
+        }
+    };
+    if (mode == 2) {
+        const int64_t E = offsets ? offsets[n] : 0;
+        if (E == 0) return;
+        struct Rec {
+            float dist;
+            float w;
+            int64_t ordinal;
+        };
+        std::vector<Rec> recs(static_cast<size_t>(E));
+        std::atomic<int> bad{0};
+        parallel_for(n, [&](int64_t h) {
+            for (int64_t e = offsets[h]; e < offsets[h + 1]; ++e) {
+                const int64_t t = indices[e];
+                float sq = 0.0f;
+                for (int64_t c = 0; c < d; ++c) {
+                    const float diff = v[h * d + c] - v[t * d + c];
+                    sq += diff * diff;
+                }
+                const float dist = std::sqrt(sq);
+                if (!finite(dist)) bad.store(1);
+                recs[e] = {dist, weights[e], e};
+            }
+        }, 2048);
+        UMAPHIP_REQUIRE(!bad.load(), UMAPHIP_NUMERICAL, "non-finite initialization");
+        double total = 0.0;
+        for (int64_t e = 0; e < E; ++e) {
+            total += static_cast<double>(weights[e]);
+            UMAPHIP_REQUIRE(std::isfinite(total), UMAPHIP_NUMERICAL, "non-finite initialization");
+        }
+        std::sort(recs.begin(), recs.end(), [](const Rec &a, const Rec &b) {
+            const int32_t ka = total_key(a.dist), kb = total_key(b.dist);
+            return ka != kb ? ka < kb : a.ordinal < b.ordinal;
+        });
+        double cumulative = 0.0;
+        float median = 0.0f;
+        for (const Rec &r : recs) {
+            cumulative += static_cast<double>(r.w);
+            if (cumulative >= total / 2.0) {
+                median = r.dist;
+                break;
+            }
+        }
+        const double spread = median == 0.0f ? 1.0 : static_cast<double>(median);
+        apply(static_cast<double>(scale_factor) / spread);
+        return;
+    }
+    auto quantile = [](const std::vector<double> &s, double p) {
+        const double h = static_cast<double>(s.size() - 1) * p;
+        const size_t lo = static_cast<size_t>(std::floor(h)), hi = static_cast<size_t>(std::ceil(h));
+        return s[lo] + (s[hi] - s[lo]) * (h - static_cast<double>(lo));
+    };
+    std::vector<double> spans(static_cast<size_t>(d)), scratch(static_cast<size_t>(n));
+    for (int64_t c = 0; c < d; ++c) {
+        for (int64_t r = 0; r < n; ++r) scratch[r] = v[r * d + c];
+        std::sort(scratch.begin(), scratch.end());
+        const double span = quantile(scratch, 0.95) - quantile(scratch, 0.05);
+        spans[c] = span == 0.0 ? 1.0 : span;
+    }
+    const double target = std::log10(static_cast<double>(n)) * 3.0 *
+                          std::log1p(static_cast<double>(repulsion)) / 0.6931471805599453 *
+                          static_cast<double>(scale_factor);
+    if (mode == 0) {
+        double logs = 0.0;
+        for (int64_t c = 0; c < d; ++c) logs += std::log(spans[c]);
+        apply(target / std::exp(logs / static_cast<double>(d)));
+    } else {
+        for (int64_t c = 0; c < d; ++c) {
+            const float mult = static_cast<float>(target / spans[c]);
+            UMAPHIP_REQUIRE(finite(mult), UMAPHIP_NUMERICAL, "non-finite initialization");
+            for (int64_t r = 0; r < n; ++r) {
+                v[r * d + c] *= mult;
+                UMAPHIP_REQUIRE(finite(v[r * d + c]), UMAPHIP_NUMERICAL,
+                                "non-finite initialization");
+            }
+        }
+    }
+}
+
+}  // namespace
+
+// random_embedding (crates/umap-core/src/init.rs:1519-1530) then, for
+// modes 0 and 1, the isotropic or per-axis rescale; mode -1 returns the raw
+// U(-10, 10) draws (graph_edge callers rescale them with the graph).
 umaphip_status umaphip_random_init(int64_t n, int64_t d, uint64_t seed, float repulsion,
                                    float scale_factor, int32_t mode, float *out) {
     return guarded([&] {
-        UMAPHIP_REQUIRE(n > 0 && d > 0 && out, UMAPHIP_INVALID_INPUT, "invalid init arguments");
+        UMAPHIP_REQUIRE(n > 0 && d > 0 && out && mode >= -1 && mode <= 1, UMAPHIP_INVALID_INPUT,
+                        "invalid init arguments");
         if (n == 1) {
             std::fill(out, out + d, 0.0f);
             return;
         }
         StdRng rng(seed);
         for (int64_t i = 0; i < n * d; ++i) out[i] = rng.range_f32(-10.0f, 10.0f);
-        for (int64_t c = 0; c < d; ++c) {
-            float sum = 0.0f;
-            for (int64_t r = 0; r < n; ++r) sum += out[r * d + c];
-            const float mean = sum / static_cast<float>(n);
-            for (int64_t r = 0; r < n; ++r) out[r * d + c] -= mean;
-        }
-        auto quantile = [](const std::vector<double> &s, double p) {
-            const double h = static_cast<double>(s.size() - 1) * p;
-            const size_t lo = static_cast<size_t>(std::floor(h)), hi = static_cast<size_t>(std::ceil(h));
-            return s[lo] + (s[hi] - s[lo]) * (h - static_cast<double>(lo));
-        };
-        std::vector<double> spans(static_cast<size_t>(d)), scratch(static_cast<size_t>(n));
-        for (int64_t c = 0; c < d; ++c) {
-            for (int64_t r = 0; r < n; ++r) scratch[r] = out[r * d + c];
-            std::sort(scratch.begin(), scratch.end());
-            const double span = quantile(scratch, 0.95) - quantile(scratch, 0.05);
-            spans[c] = span == 0.0 ? 1.0 : span;
-        }
-        const double target = std::log10(static_cast<double>(n)) * 3.0 *
-                              std::log1p(static_cast<double>(repulsion)) / 0.6931471805599453 *
-                              static_cast<double>(scale_factor);
-        if (mode == 0) {
-            double logs = 0.0;
-            for (int64_t c = 0; c < d; ++c) logs += std::log(spans[c]);
-            const float mult = static_cast<float>(target / std::exp(logs / static_cast<double>(d)));
-            UMAPHIP_REQUIRE(std::isfinite(mult), UMAPHIP_NUMERICAL, "non-finite rescale");
-            for (int64_t i = 0; i < n * d; ++i) out[i] *= mult;
-        } else {
-            for (int64_t c = 0; c < d; ++c) {
-                const float mult = static_cast<float>(target / spans[c]);
-                UMAPHIP_REQUIRE(std::isfinite(mult), UMAPHIP_NUMERICAL, "non-finite rescale");
-                for (int64_t r = 0; r < n; ++r) out[r * d + c] *= mult;
-            }
-        }
+        if (mode >= 0) rescale(out, n, d, nullptr, nullptr, nullptr, mode, repulsion, scale_factor);
+    });
+}
+
+umaphip_status umaphip_init_rescale(float *values, int64_t n, int64_t d, const int64_t *offsets,
+                                    const int32_t *indices, const float *weights, int32_t mode,
+                                    float repulsion, float scale_factor) {
+    return guarded([&] {
+        UMAPHIP_REQUIRE(n > 0 && d > 0 && values && mode >= 0 && mode <= 2,
+                        UMAPHIP_INVALID_INPUT, "invalid rescale arguments");
+        UMAPHIP_REQUIRE(mode != 2 || (offsets && (offsets[n] == 0 || (indices && weights))),
+                        UMAPHIP_INVALID_INPUT, "graph_edge rescale needs the graph");
+        if (mode == 2)
+            for (int64_t h = 0; h < n; ++h)
+                for (int64_t e = offsets[h]; e < offsets[h + 1]; ++e)
+                    UMAPHIP_REQUIRE(indices[e] >= 0 && indices[e] < n, UMAPHIP_INVALID_INPUT,
+                                    "graph_edge rescale: invalid edge endpoint");
+        rescale(values, n, d, offsets, indices, weights, mode, repulsion, scale_factor);
     });
 }
 

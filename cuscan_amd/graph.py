@@ -72,7 +72,7 @@ def fit_curve(min_dist=0.1, spread=1.0):
 
 def random_init(n, d, seed, *, repulsion=1.0, scale_factor=1.0, mode="isotropic"):
     out = np.empty((n, d), np.float32)
-    m = {"isotropic": 0, "per_axis": 1}[mode]
+    m = {"isotropic": 0, "per_axis": 1, "raw": -1}[mode]
     _lib.check(_lib.lib().umaphip_random_init(n, d, int(seed) & ((1 << 64) - 1), float(repulsion),
                                               float(scale_factor), m, _p(out)))
     return out

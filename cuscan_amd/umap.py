@@ -4,9 +4,12 @@ Keyword arguments, validation messages, dtype rules and defaults follow the
 reference's Python surface and ``UMAPConfig`` (crates/umap-core/src/
 config.rs:37-66). Fit and transform run the HIP pipeline: exact kNN on the
 device (BF16 MFMA prefilter + FP32 rerank), the fuzzy graph and compaction
-on host threads, and the modern layout (Adam or momentum) on the device.
+on the device, the recursive initialisation (host coarsening, coarse
+layouts on the device), and the modern layout (Adam or momentum) on the
+device.
 
-Scope of this build: ``init="random"``; ``optimizer`` in {"adam",
+Scope of this build: ``init`` in {"recursive", "auto", "random"} (the
+spectral solver is not provided); ``optimizer`` in {"adam",
 "momentum", "compatibility"} (the last runs the classic SGD epoch on the
 device); neighbour search is always exact, which equals the
 reference with ``neighbor_search="exact"`` (its "approximate" NN-descent is
@@ -117,6 +120,7 @@ class UMAP:
         ratio = self.recursive_coarsening_ratio
         if not _is_int(ratio) or int(ratio) not in (2, 3, 4):
             raise ValueError("recursive_coarsening_ratio must be 2, 3, or 4")
+        self._recursive = self._recursive_config()
         if not isinstance(self.optimizer, str) or self.optimizer not in ("compatibility", "adam", "momentum"):
             raise ValueError("optimizer must be 'compatibility', 'adam', or 'momentum'")
         q = self.negative_selection_range
@@ -179,14 +183,62 @@ class UMAP:
             raise ValueError("backend='cpu' is not available: this build runs on HIP devices")
         if self.neighbor_search not in ("auto", "exact", "approximate"):
             raise ValueError("neighbor_search must be 'auto', 'exact', or 'approximate'")
-        if self.init != "random":
+        if self.init == "spectral":
             raise NotImplementedError(
-                f"init={self.init!r} (spectral/recursive initialisation) is not part of this "
-                "build; pass init='random'")
-        if self.init_rescale == "graph_edge":
-            raise NotImplementedError("init_rescale='graph_edge' is not part of this build")
+                "init='spectral' (the block Krylov spectral solver) is not part of this build; "
+                "pass init='recursive', 'auto' or 'random'")
         if self.unique:
             raise NotImplementedError("unique=True is not part of this build")
+
+    def _recursive_config(self) -> dict:
+        """recursive_init_kwds checks of python/cuscan/standard_umap.py:146-182."""
+        supported = ("base_init_threshold", "n_embedding_epochs", "hub_selection",
+                     "anchor_sample_size")
+        kwds = self.recursive_init_kwds
+        if kwds is not None and not isinstance(kwds, dict):
+            raise ValueError("recursive_init_kwds must be a dict or None")
+        kwds = {} if kwds is None else kwds
+        if any(key not in supported for key in kwds):
+            raise ValueError(f"recursive_init_kwds supports only: {', '.join(supported)}")
+        config = {
+            "base_init_threshold": kwds.get("base_init_threshold", 1024),
+            "n_embedding_epochs": kwds.get("n_embedding_epochs", 50),
+            "hub_selection": kwds.get("hub_selection", "degree"),
+            "anchor_sample_size": kwds.get("anchor_sample_size", 16384),
+        }
+        for key, minimum, maximum in (("base_init_threshold", 2, 2**31 - 1),
+                                      ("n_embedding_epochs", 1, np.iinfo(np.uintp).max),
+                                      ("anchor_sample_size", 1, 2**31 - 1)):
+            value = config[key]
+            if not _is_int(value) or not minimum <= int(value) <= maximum:
+                raise ValueError(f"{key} must be an integer in [{minimum}, {maximum}]")
+            config[key] = int(value)
+        if not isinstance(config["hub_selection"], str) or config["hub_selection"] not in (
+                "degree", "diverse", "random"):
+            raise ValueError("hub_selection must be 'degree', 'diverse', or 'random'")
+        return config
+
+    def _initialize(self, x, graph, n, d, a, b, seed):
+        """Initial coordinates (estimator.rs:577-870): the recursive
+        multilevel layout (init 'recursive'/'auto') or U(-10, 10) draws, then
+        the configured rescale. graph is the pruned positive fit graph CSR."""
+        from . import recursive as R
+
+        kw = dict(repulsion=self.repulsion_strength, scale_factor=self.init_scale_factor,
+                  offsets=graph["offsets"], indices=graph["indices"], weights=graph["weights"])
+        if self.init in ("recursive", "auto"):
+            cfg = R.params(int(self.recursive_coarsening_ratio), **self._recursive)
+            values, self.recursive_stats_ = R.recursive_init(
+                graph["offsets"], graph["indices"], graph["weights"], x, d, a, b, seed, cfg,
+                return_stats=True)
+            return R.init_rescale(values, self.init_rescale, **kw)
+        from . import graph as G
+
+        if self.init_rescale != "graph_edge":
+            return G.random_init(n, d, seed, repulsion=self.repulsion_strength,
+                                 scale_factor=self.init_scale_factor, mode=self.init_rescale)
+        values = G.random_init(n, d, seed, mode="raw")
+        return R.init_rescale(values, "graph_edge", **kw)
 
     def _device(self):
         import torch
@@ -227,8 +279,14 @@ class UMAP:
         n_epochs = int(self.n_epochs) if self.n_epochs is not None else default_epochs
         divisor = float(n_epochs if n_epochs > 10 else default_epochs)
         a, b = G.fit_curve(self.min_dist, self.spread)
-        init = G.random_init(n, d, seed, repulsion=self.repulsion_strength,
-                             scale_factor=self.init_scale_factor, mode=self.init_rescale)
+        # fuzzy union, pruning and the compact CSR on the device; the pruned
+        # positive graph also feeds the recursive and graph_edge inits
+        compact = G.fit_graph_device(indices, distances,
+                                     local_connectivity=self.local_connectivity,
+                                     bandwidth=1.0, mix=self.set_op_mix_ratio, epochs=n_epochs,
+                                     rate=self.negative_sample_rate, prune_divisor=divisor)
+        init = self._initialize(x, compact, n, d, a, b, seed)
+        good = self.init in ("recursive", "auto")
         if self.optimizer == "compatibility":
             from .classic import optimize_classic
 
@@ -241,19 +299,14 @@ class UMAP:
                                    negative_sample_rate=int(self.negative_sample_rate), seed=seed,
                                    shared=True)
         else:
-            # fuzzy union, pruning and the compact CSR on the device
-            compact = G.fit_graph_device(indices, distances,
-                                         local_connectivity=self.local_connectivity,
-                                         bandwidth=1.0, mix=self.set_op_mix_ratio, epochs=n_epochs,
-                                         rate=self.negative_sample_rate, prune_divisor=divisor)
-            params = self._layout_params(n_epochs, a, b, self.learning_rate, seed)
+            params = self._layout_params(n_epochs, a, b, self.learning_rate, seed, good=good)
             emb, _, _, self.layout_stats_ = L.optimize_compact(init, None, compact, params,
                                                                shared=True)
         self.embedding_ = emb
         self._index, self._a, self._b = index, a, b
         return self
 
-    def _layout_params(self, epochs, a, b, alpha, seed):
+    def _layout_params(self, epochs, a, b, alpha, seed, good=False):
         from .layout import LayoutParams
 
         return LayoutParams(
@@ -262,7 +315,7 @@ class UMAP:
             negative_selection_range=int(self.negative_selection_range),
             negative_sample_scale=None if self.negative_sample_scale is None else float(self.negative_sample_scale),
             adaptation_samples=int(self.negative_sample_scale_adaptation_samples),
-            exclude_graph_neighbors=bool(self.exclude_graph_neighbors), good_initialization=False,
+            exclude_graph_neighbors=bool(self.exclude_graph_neighbors), good_initialization=good,
             seed=seed)
 
     def fit_transform(self, X):

@@ -292,9 +292,18 @@ umaphip_status umaphip_compact_graph(int64_t rows, int64_t columns, int64_t n_ed
 umaphip_status umaphip_fit_curve(float min_dist, float spread, float *a, float *b);
 /* Random U(-10,10) initialisation from StdRng(seed) then centring and
  * isotropic (mode 0) or per-axis (mode 1) robust rescaling
- * (crates/umap-core/src/init.rs:1519-1530, init_rescale.rs:77-203). */
+ * (crates/umap-core/src/init.rs:1519-1530, init_rescale.rs:77-203); mode -1
+ * returns the raw draws (for umaphip_init_rescale with mode 2). */
 umaphip_status umaphip_random_init(int64_t n, int64_t d, uint64_t seed, float repulsion,
                                    float scale_factor, int32_t mode, float *out);
+/* init_rescale::rescale (crates/umap-core/src/init_rescale.rs:77-203) in
+ * place on values [n, d]: centre each axis, then scale isotropically
+ * (mode 0), per axis (mode 1), or so that the weight-median edge length of
+ * the graph is scale_factor (mode 2, graph_edge; the graph is a CSR with
+ * offsets [n + 1] and its edges in CSR order; ignored by modes 0 and 1). */
+umaphip_status umaphip_init_rescale(float *values, int64_t n, int64_t d, const int64_t *offsets,
+                                    const int32_t *indices, const float *weights, int32_t mode,
+                                    float repulsion, float scale_factor);
 /* Transform initial placement from query neighbours and their directed
  * memberships (crates/umap/src/standard_umap/estimator.rs:2385-2481).
  * embedding is the fitted [n, d]; out is [q, d]. */
@@ -334,6 +343,105 @@ umaphip_status umaphip_transform_graph(const int32_t *indices, const float *dist
                                        double *negative_periods, double *next_positive,
                                        double *next_negative, float *init, int64_t *n_edges,
                                        void *stream);
+
+/* ------------------------------------------------------------------------
+ * Recursive (multilevel) initialisation, init="recursive" or "auto"
+ * (crates/umap-core/src/recursive_init/mod.rs:48-320 with the coarse
+ * layouts of crates/umap/src/standard_umap/estimator.rs:2506-2589).
+ * Host arrays; the coarse layouts run on the current device.
+ * ------------------------------------------------------------------------ */
+/* RecursiveInitConfig (crates/umap-core/src/config.rs:382-430). */
+typedef struct {
+    int64_t coarsening_ratio;       /* 2, 3 or 4 */
+    int64_t base_init_threshold;    /* >= 2 */
+    int64_t n_embedding_epochs;     /* >= 1, epochs of the finest coarse layout */
+    int32_t hub_selection;          /* 0 degree, 1 diverse, 2 random */
+    int64_t anchor_sample_size;     /* >= 1 */
+} umaphip_recursive_params;
+
+/* LevelParams (crates/umap-core/src/recursive_init/mod.rs:494-561). */
+typedef struct {
+    int64_t parts;                  /* requested coarse parts */
+    int64_t epochs;                 /* coarse layout epochs */
+    float a, b;                     /* the curve to the 1/4 power per level */
+    int32_t good_initialization;
+    double curve_a, curve_b;
+} umaphip_recursive_level;
+
+typedef struct {
+    int32_t depth;                  /* levels visited, the base case included */
+    int64_t rows[32];               /* graph rows at depth 1, 2, ... */
+    int64_t edges[32];
+    int32_t sweeps[32];             /* label propagation sweeps (0 at the base) */
+    double host_ms;                 /* host graph and geometry steps */
+    double layout_ms;               /* coarse layouts, wall clock */
+} umaphip_recursive_stats;
+
+/* recursive_init::run (mod.rs:48-123): coordinates [n, d] before the final
+ * rescale (apply umaphip_init_rescale, mode as configured, for
+ * recursive_init::finish, mod.rs:27-45). The graph is the pruned positive
+ * fit graph as a square CSR with sorted columns and symmetric support
+ * (offsets [n + 1], indices and weights [offsets[n]]); data is the [n,
+ * features] training matrix; (a, b) the fitted curve; seed the fit's seed.
+ * INVALID_INPUT for a malformed or asymmetric graph, INVALID_CONFIG for
+ * out-of-range parameters. stats may be NULL. */
+umaphip_status umaphip_recursive_init(int64_t n, const int64_t *offsets, const int32_t *indices,
+                                      const float *weights, const float *data, int64_t features,
+                                      int64_t d, float a, float b,
+                                      const umaphip_recursive_params *params, uint64_t seed,
+                                      float *out, umaphip_recursive_stats *stats);
+/* The steps of the recursion, for tests and for callers that drive their
+ * own levels (each mirrors the public function named). */
+/* LevelParams::first (previous == NULL, curve (a, b)) or previous.next(n)
+ * (mod.rs:510-539); *has_level = 0 when n is a base case. */
+umaphip_status umaphip_recursive_level_params(int64_t n, const umaphip_recursive_level *previous,
+                                       float a, float b, const umaphip_recursive_params *params,
+                                       umaphip_recursive_level *out, int32_t *has_level);
+/* initialize_labels (graph.rs:140-197); labels [n], -1 for non-hubs. */
+umaphip_status umaphip_recursive_hubs(int64_t n, const int64_t *offsets, const int32_t *indices,
+                                      const float *weights, int64_t parts, int32_t method,
+                                      uint64_t seed, int32_t *labels);
+/* propagate_labels (graph.rs:199-253); labels in/out. */
+umaphip_status umaphip_recursive_propagate(int64_t n, const int64_t *offsets,
+                                           const int32_t *indices, const float *weights,
+                                           int32_t share, int32_t *labels, int32_t *sweeps);
+/* label_outliers (graph.rs:255-297); labels in/out. */
+umaphip_status umaphip_recursive_outliers(int64_t n, const int64_t *offsets,
+                                          const int32_t *indices, uint64_t seed,
+                                          int32_t *labels);
+/* remap_labels (graph.rs:299-314); labels in/out, *parts = part count. */
+umaphip_status umaphip_recursive_remap(int32_t *labels, int64_t n, int64_t *parts);
+/* aggregate_graph (graph.rs:380-460): summed coarse CSR, outputs
+ * out_offsets [parts + 1] and capacity offsets[n] for indices/weights. */
+umaphip_status umaphip_recursive_aggregate(int64_t n, const int64_t *offsets,
+                                           const int32_t *indices, const double *weights,
+                                           const int32_t *labels, int64_t parts,
+                                           int64_t *out_offsets, int32_t *out_indices,
+                                           double *out_weights);
+/* transition_graph (graph.rs:462-533): outputs out_offsets [n + 1] and
+ * capacity 2 * offsets[n] for indices/weights. */
+umaphip_status umaphip_recursive_transition(int64_t n, const int64_t *offsets,
+                                            const int32_t *indices, const double *weights,
+                                            int64_t *out_offsets, int32_t *out_indices,
+                                            float *out_weights);
+/* anchor_reference (geometry.rs:266-427) with the run's generator
+ * (StdRng(stream_seed(seed, 1, 0))): values [n, d], mask [n] (1 = sampled). */
+umaphip_status umaphip_recursive_anchor(const float *data, int64_t n, int64_t features,
+                                        int64_t d, int64_t sample_size, uint64_t seed,
+                                        float *values, uint8_t *mask);
+/* base_case (geometry.rs:460-539). */
+umaphip_status umaphip_recursive_base_case(int64_t n, const int64_t *offsets,
+                                           const int32_t *indices, const float *weights,
+                                           const float *anchor, const uint8_t *mask, int64_t d,
+                                           int64_t depth, float *out);
+/* prolongate (geometry.rs:541-582); coarse [parts, d]. */
+umaphip_status umaphip_recursive_prolongate(int64_t n, const int64_t *offsets,
+                                            const int32_t *indices, const float *weights,
+                                            const int32_t *labels, const float *coarse,
+                                            int64_t parts, int64_t d, float *out);
+/* procrustes_align (geometry.rs:584-690); layout [n, d] in place. */
+umaphip_status umaphip_recursive_procrustes(float *layout, int64_t n, const float *anchor,
+                                            const uint8_t *mask, int64_t d);
 
 /* ------------------------------------------------------------------------
  * Compatibility (classic) SGD layout, optimizer="compatibility".

@@ -43,7 +43,7 @@ def test_transform_before_fit_raises():
 
 
 def test_unsupported_inits_are_explicit():
-    with pytest.raises(NotImplementedError, match="init='random'"):
+    with pytest.raises(NotImplementedError, match="init='spectral'"):
         UMAP(init="spectral").fit(blobs(4))
 
 
@@ -53,3 +53,26 @@ def test_defaults_match_reference_config():
     assert (u.optimizer, u.negative_selection_range, u.negative_sample_rate) == ("adam", 200_000, 5)
     assert (u.negative_sample_scale_adaptation_samples, u.transform_seed) == (128, 42)
     assert u.exclude_graph_neighbors is False and u.negative_sample_scale is None
+
+
+@pytest.mark.parametrize(("kwds", "match"), [
+    ([("base_init_threshold", 8)], "recursive_init_kwds must be a dict"),
+    ({"levels": 3}, "recursive_init_kwds supports only"),
+    ({"base_init_threshold": 1}, "base_init_threshold"),
+    ({"base_init_threshold": True}, "base_init_threshold"),
+    ({"n_embedding_epochs": 0}, "n_embedding_epochs"),
+    ({"anchor_sample_size": 2**31}, "anchor_sample_size"),
+    ({"hub_selection": "pagerank"}, "hub_selection"),
+])
+def test_recursive_init_kwds_are_validated(kwds, match):
+    """recursive_init_kwds checks of python/cuscan/standard_umap.py:146-182."""
+    with pytest.raises(ValueError, match=match):
+        UMAP(recursive_init_kwds=kwds).fit(blobs(4))
+
+
+def test_auto_init_resolves_to_recursive():
+    assert UMAP().init == "auto"
+    u = UMAP(recursive_init_kwds={"base_init_threshold": 8, "hub_selection": "diverse"})
+    u._validate()
+    assert u._recursive == {"base_init_threshold": 8, "n_embedding_epochs": 50,
+                            "hub_selection": "diverse", "anchor_sample_size": 16384}
