@@ -129,6 +129,47 @@ def build_workload(rows, dims, clusters, seed, k, epochs):
     return x, compact, init, a, b, knn_s, graph_s, index
 
 
+def recursive_benchmark(x, compact, a, b, args):
+    """init='auto' on the benchmark graph (defaults: ratio 4, base 1024, 50
+    coarse epochs, degree hubs, 16,384-row PCA anchor), then isotropic
+    rescale; with --quality the 200-epoch fit from it (good_initialization)
+    and its trustworthiness on the same 10k sample."""
+    import torch
+
+    from cuscan_amd import layout as L
+    from cuscan_amd import recursive as R
+
+    xs = x.cpu().numpy()
+    t0 = time.perf_counter()
+    values, st = R.recursive_init(compact["offsets"], compact["indices"], compact["weights"], xs,
+                                  2, a, b, 42, R.params(), return_stats=True)
+    R.init_rescale(values, "isotropic", offsets=compact["offsets"], indices=compact["indices"],
+                   weights=compact["weights"])
+    secs = time.perf_counter() - t0
+    res = {"seconds": round(secs, 2), "host_ms": round(st["host_ms"], 1),
+           "coarse_layout_ms": round(st["layout_ms"], 1), "level_rows": st["rows"],
+           "level_edges": st["edges"], "propagation_sweeps": st["sweeps"]}
+    log(f"recursive init {secs:.2f}s (host {st['host_ms']:.0f} ms, coarse layouts "
+        f"{st['layout_ms']:.0f} ms), levels {st['rows']}")
+    if args.quality:
+        from cuscan_amd.quality import trustworthiness_device
+
+        params = L.LayoutParams(epochs=args.epochs, a=a, b=b, seed=42, good_initialization=True)
+        plan = L.LayoutPlan(values, None, compact, params, shared=True)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        plan.run()
+        emb, _ = plan.download()
+        res["layout_seconds"] = round(time.perf_counter() - t1, 3)
+        tw, pres = trustworthiness_device(xs, emb, k=15, sample=10_000)
+        res["trustworthiness@15"] = round(tw, 4)
+        res["knn_preservation@15"] = round(pres, 4)
+        tw0, _ = trustworthiness_device(xs, values, k=15, sample=10_000)
+        res["init_trustworthiness@15"] = round(tw0, 4)
+        log(f"recursive fit quality {tw:.4f}/{pres:.4f} (init alone {tw0:.4f})")
+    return res
+
+
 def serve_benchmark(x, emb, compact_index, a, b, args):
     """Transform serving on the resident 1M-row model: requests of
     round(lognormal(ln 64, 1)) rows clipped to 1..4096 from the same mixture
@@ -408,6 +449,9 @@ def main():
     ap.add_argument("--cpu-epochs", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quality", action="store_true", help="report trustworthiness@15 (10k sample)")
+    ap.add_argument("--recursive", action="store_true",
+                    help="also time the recursive initialisation (init='auto') on the same "
+                         "graph, and with --quality the fit that starts from it")
     ap.add_argument("--no-serve", action="store_true", help="skip the transform serving leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-layouts leg")
     ap.add_argument("--batch-graphs", type=int, default=256)
@@ -501,6 +545,9 @@ def main():
         out["config"]["oracle_knn_preservation@15"] = round(pres_c, 4)
         out["config"]["oracle_layout_seconds"] = round(cpu_s, 1)
         log(f"quality: device {tw:.4f}/{pres:.4f}, oracle {tw_c:.4f}/{pres_c:.4f} ({cpu_s:.0f}s)")
+    if args.recursive:
+        log("recursive initialisation ...")
+        out["recursive_init"] = recursive_benchmark(x, compact, a, b, args)
     if not args.no_batch:
         log("batched layouts ...")
         out["batched"] = batch_benchmark(args)

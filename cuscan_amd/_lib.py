@@ -121,6 +121,7 @@ class RecursiveStats(C.Structure):
     _fields_ = [
         ("depth", C.c_int32), ("rows", C.c_int64 * 32), ("edges", C.c_int64 * 32),
         ("sweeps", C.c_int32 * 32), ("host_ms", C.c_double), ("layout_ms", C.c_double),
+        ("stage_ms", C.c_double * 8),
     ]
 
 # Every exported symbol of include/umaphip.h, with its C signature.

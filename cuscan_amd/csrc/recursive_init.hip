@@ -211,17 +211,23 @@ std::vector<int32_t> initialize_labels(const CsrView &g, int64_t parts, int meth
                 static_cast<int32_t>(label);
         return labels;
     }
-    std::vector<int32_t> key(static_cast<size_t>(n));
+    // Rows by (degree total order, row): a stable LSD radix sort of the
+    // unsigned order keys, starting from row order.
+    std::vector<uint32_t> key(static_cast<size_t>(n));
     parallel_for(n, [&](int64_t r) {
         float sum = 0.0f;
         for (int64_t e = g.ptr[r]; e < g.ptr[r + 1]; ++e) sum += g.w[e];
-        key[r] = total_key(sum);
+        key[r] = static_cast<uint32_t>(total_key(sum)) ^ 0x80000000u;
     }, 4096);
-    std::vector<int32_t> order(static_cast<size_t>(n));
+    std::vector<int32_t> order(static_cast<size_t>(n)), tmp(static_cast<size_t>(n));
     std::iota(order.begin(), order.end(), 0);
-    std::sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
-        return key[x] != key[y] ? key[x] < key[y] : x < y;
-    });
+    for (int shift = 0; shift < 32; shift += 11) {
+        std::vector<int64_t> bucket(2049, 0);
+        for (int64_t i = 0; i < n; ++i) ++bucket[((key[order[i]] >> shift) & 2047u) + 1];
+        for (int b = 0; b < 2048; ++b) bucket[b + 1] += bucket[b];
+        for (int64_t i = 0; i < n; ++i) tmp[bucket[(key[order[i]] >> shift) & 2047u]++] = order[i];
+        order.swap(tmp);
+    }
     if (method == 0) {
         for (int64_t label = 0; label < parts; ++label)
             labels[order[n - parts + label]] = static_cast<int32_t>(label);
@@ -255,58 +261,66 @@ std::vector<int32_t> initialize_labels(const CsrView &g, int64_t parts, int meth
 // sweep), at most 100 sweeps. Returns the sweeps run.
 int propagate_labels(const CsrView &g, std::vector<int32_t> &labels, bool share) {
     const int64_t n = g.n;
-    auto unlabeled = [&] {
-        int64_t c = 0;
-        for (int64_t r = 0; r < n; ++r) c += labels[r] < 0;
-        return c;
-    };
-    int64_t previous = unlabeled();
+    // Only unlabelled rows change, so each sweep visits the rows still
+    // unlabelled (in row order) and copies back just those.
+    std::vector<int32_t> active;
+    for (int64_t r = 0; r < n; ++r)
+        if (labels[r] < 0) active.push_back(static_cast<int32_t>(r));
+    int64_t previous = static_cast<int64_t>(active.size());
+    int32_t top = -1;
+    for (int32_t l : labels) top = std::max(top, l);
+    const size_t label_slots = static_cast<size_t>(top) + 1;
     std::vector<int32_t> next(labels);
-    struct Vote {
-        int32_t label;
-        int64_t edge;
-        float w;
-    };
+    // Per row, votes are grouped by label with a stamped slot table: a label
+    // opens its group at its first edge, so groups come out in first-edge
+    // order and each group's f32 sum adds its votes in edge order, as the
+    // reference's sort by (label, edge) and then by first edge does.
     struct Group {
-        int64_t first;
         int32_t label;
         float sum;
     };
+    static std::atomic<uint64_t> generation{0};
     for (int sweep = 0; sweep < 100; ++sweep) {
-        parallel_for(n, [&](int64_t r) {
-            if (labels[r] >= 0) return;
-            thread_local std::vector<Vote> votes;
-            thread_local std::vector<Group> grouped;
-            votes.clear();
-            grouped.clear();
+        const uint64_t tag = (generation.fetch_add(1) + 1) << 32;
+        parallel_for(static_cast<int64_t>(active.size()), [&](int64_t a) {
+            const int64_t r = active[a];
+            thread_local std::vector<uint64_t> stamp;
+            thread_local std::vector<int32_t> slot;
+            thread_local std::vector<Group> groups;
+            if (stamp.size() < label_slots) {
+                stamp.resize(label_slots, 0);
+                slot.resize(label_slots, 0);
+            }
+            const uint64_t mark = tag | static_cast<uint64_t>(r);
+            groups.clear();
             double total = 0.0;
             for (int64_t e = g.ptr[r]; e < g.ptr[r + 1]; ++e) {
                 const int32_t label = labels[g.idx[e]];
                 total += static_cast<double>(g.w[e]);
-                if (label >= 0) votes.push_back({label, e, g.w[e]});
+                if (label < 0) continue;
+                if (stamp[label] != mark) {
+                    stamp[label] = mark;
+                    slot[label] = static_cast<int32_t>(groups.size());
+                    groups.push_back({label, g.w[e]});
+                } else {
+                    groups[slot[label]].sum += g.w[e];
+                }
             }
-            if (votes.empty()) return;
-            std::sort(votes.begin(), votes.end(), [](const Vote &x, const Vote &y) {
-                return x.label != y.label ? x.label < y.label : x.edge < y.edge;
-            });
-            for (size_t i = 0; i < votes.size();) {
-                Group grp{votes[i].edge, votes[i].label, votes[i].w};
-                ++i;
-                while (i < votes.size() && votes[i].label == grp.label) grp.sum += votes[i++].w;
-                grouped.push_back(grp);
-            }
-            std::sort(grouped.begin(), grouped.end(),
-                      [](const Group &x, const Group &y) { return x.first < y.first; });
             double maximum = share ? 0.1 * total : 1.0;
-            for (const Group &grp : grouped)
+            for (const Group &grp : groups)
                 if (static_cast<double>(grp.sum) >= maximum) {
                     maximum = static_cast<double>(grp.sum);
                     next[r] = grp.label;
                 }
-        }, 2048);
-        labels = next;
+        }, 1024);
+        size_t kept = 0;
+        for (const int32_t r : active) {
+            labels[r] = next[r];
+            if (labels[r] < 0) active[kept++] = r;
+        }
+        active.resize(kept);
         if (sweep % 5 == 0) {
-            const int64_t remaining = unlabeled();
+            const int64_t remaining = static_cast<int64_t>(active.size());
             if (remaining == 0 || remaining == previous) return sweep + 1;
             previous = remaining;
         }
@@ -347,20 +361,21 @@ void label_outliers(const CsrView &g, std::vector<int32_t> &labels, uint64_t see
 
 // remap_labels (graph.rs:299-314): used labels to 0.. in increasing order,
 // unlabelled rows after them in row order. Returns the part count.
+// The rank of a label among the used labels is a prefix count over a
+// presence table, which equals the reference's sort, dedup and binary search.
 int64_t remap_labels(std::vector<int32_t> &labels) {
-    std::vector<int32_t> unique;
+    int32_t top = -1;
+    for (int32_t l : labels) top = std::max(top, l);
+    std::vector<int32_t> rank(static_cast<size_t>(top) + 1, 0);
     for (int32_t l : labels)
-        if (l >= 0) unique.push_back(l);
-    std::sort(unique.begin(), unique.end());
-    unique.erase(std::unique(unique.begin(), unique.end()), unique.end());
-    int64_t next = static_cast<int64_t>(unique.size());
-    for (int32_t &l : labels) {
-        if (l < 0)
-            l = static_cast<int32_t>(next++);
-        else
-            l = static_cast<int32_t>(std::lower_bound(unique.begin(), unique.end(), l) -
-                                     unique.begin());
+        if (l >= 0) rank[l] = 1;
+    int64_t next = 0;
+    for (int32_t &r : rank) {
+        const int32_t used = r;
+        r = static_cast<int32_t>(next);
+        next += used;
     }
+    for (int32_t &l : labels) l = l < 0 ? static_cast<int32_t>(next++) : rank[l];
     return next;
 }
 
@@ -386,47 +401,87 @@ struct AggInput {
 };
 
 // aggregate_graph_counted (graph.rs:390-460): per part, the member rows'
-// edges by target part (ties in edge order), summed in f64; self loops
-// (edges inside a part) are kept.
+// edges by target part, summed in f64 in edge order; self loops (edges
+// inside a part) are kept. Members are visited in row order and each row's
+// edges in order, so edge order is visiting order: a sparse accumulator
+// (slot per target part, reset by a per-part stamp) adds each target's
+// weights in exactly the order of the reference's (target, edge) sort, and
+// only the distinct targets are sorted.
 Summed aggregate(int64_t n, const AggInput &in, const std::vector<int32_t> &labels,
                  int64_t parts) {
     (void)n;
     std::vector<int64_t> starts;
     std::vector<int32_t> members;
     grouped_members(labels, parts, starts, members);
-    std::vector<std::vector<std::pair<int32_t, double>>> rows(static_cast<size_t>(parts));
-    parallel_for(parts, [&](int64_t part) {
-        thread_local std::vector<std::pair<int32_t, int64_t>> entries;
-        entries.clear();
-        for (int64_t m = starts[part]; m < starts[part + 1]; ++m) {
-            const int32_t r = members[m];
-            for (int64_t e = in.ptr[r]; e < in.ptr[r + 1]; ++e)
-                entries.push_back({labels[in.idx[e]], e});
+    // Stamps carry the pass's generation in their high half, so slots left
+    // by an earlier pass or call on the same thread never match.
+    static std::atomic<uint64_t> generation{0};
+    thread_local std::vector<uint64_t> stamp;
+    thread_local std::vector<int64_t> slot;
+    auto prepare = [&] {
+        if (stamp.size() < static_cast<size_t>(parts)) {
+            stamp.resize(static_cast<size_t>(parts), 0);
+            slot.resize(static_cast<size_t>(parts), 0);
         }
-        std::sort(entries.begin(), entries.end());
-        auto &out = rows[part];
-        for (const auto &en : entries) {
-            const double w = in.weight(en.second);
-            if (!out.empty() && out.back().first == en.first)
-                out.back().second += w;
-            else
-                out.push_back({en.first, w});
-        }
-    }, 256);
+    };
     Summed s;
     s.n = parts;
     s.ptr.assign(static_cast<size_t>(parts + 1), 0);
-    for (int64_t p = 0; p < parts; ++p) s.ptr[p + 1] = s.ptr[p] + static_cast<int64_t>(rows[p].size());
+    // Pass 1: distinct target parts per part (count_coarse_edges).
+    const uint64_t count_tag = (generation.fetch_add(1) + 1) << 32;
+    parallel_for(parts, [&](int64_t part) {
+        prepare();
+        const uint64_t mark = count_tag | static_cast<uint64_t>(part);
+        int64_t c = 0;
+        for (int64_t m = starts[part]; m < starts[part + 1]; ++m) {
+            const int32_t r = members[m];
+            for (int64_t e = in.ptr[r]; e < in.ptr[r + 1]; ++e) {
+                const int32_t t = labels[in.idx[e]];
+                if (stamp[t] != mark) {
+                    stamp[t] = mark;
+                    ++c;
+                }
+            }
+        }
+        s.ptr[part + 1] = c;
+    }, 256);
+    for (int64_t p = 0; p < parts; ++p) s.ptr[p + 1] += s.ptr[p];
     s.idx.resize(static_cast<size_t>(s.ptr[parts]));
     s.w.resize(static_cast<size_t>(s.ptr[parts]));
-    parallel_for(parts, [&](int64_t p) {
-        int64_t o = s.ptr[p];
-        for (const auto &en : rows[p]) {
-            s.idx[o] = en.first;
-            s.w[o++] = en.second;
+    // Pass 2: sums in edge order, then the part's targets sorted.
+    const uint64_t fill_tag = (generation.fetch_add(1) + 1) << 32;
+    parallel_for(parts, [&](int64_t part) {
+        prepare();
+        thread_local std::vector<std::pair<int32_t, double>> row;
+        const uint64_t mark = fill_tag | static_cast<uint64_t>(part);
+        int64_t o = s.ptr[part];
+        for (int64_t m = starts[part]; m < starts[part + 1]; ++m) {
+            const int32_t r = members[m];
+            for (int64_t e = in.ptr[r]; e < in.ptr[r + 1]; ++e) {
+                const int32_t t = labels[in.idx[e]];
+                const double w = in.weight(e);
+                if (stamp[t] != mark) {
+                    stamp[t] = mark;
+                    slot[t] = o;
+                    s.idx[o] = t;
+                    s.w[o++] = w;
+                } else {
+                    s.w[slot[t]] += w;
+                }
+            }
         }
-        std::vector<std::pair<int32_t, double>>().swap(rows[p]);
-    }, 1024);
+        const int64_t b = s.ptr[part], en = s.ptr[part + 1];
+        row.clear();
+        for (int64_t q = b; q < en; ++q) row.push_back({s.idx[q], s.w[q]});
+        std::sort(row.begin(), row.end(),
+                  [](const std::pair<int32_t, double> &x, const std::pair<int32_t, double> &y) {
+                      return x.first < y.first;
+                  });
+        for (int64_t q = b; q < en; ++q) {
+            s.idx[q] = row[q - b].first;
+            s.w[q] = row[q - b].second;
+        }
+    }, 256);
     return s;
 }
 
@@ -447,24 +502,39 @@ Csr transition(const Summed &s) {
     auto valid = [&](int64_t r, int64_t e) {
         return totals[r] > 0.0 && s.idx[e] != r && s.w[e] != 0.0;
     };
-    // Incoming valid edges, by target, in source order.
+    // Incoming valid edges, by target, in source order: a counting sort
+    // over contiguous row chunks (chunk c's entries for a target precede
+    // chunk c + 1's), each chunk counting and placing its own edges.
+    const int64_t chunks =
+        std::min<int64_t>(host_workers(), std::max<int64_t>(1, s.ptr[n] / (1 << 16)));
+    std::vector<int64_t> cnt(static_cast<size_t>(chunks * n), 0);
+    parallel_blocks(n, chunks, [&](int64_t c, int64_t b, int64_t en) {
+        int64_t *cc = &cnt[c * n];
+        for (int64_t r = b; r < en; ++r)
+            for (int64_t e = s.ptr[r]; e < s.ptr[r + 1]; ++e)
+                if (valid(r, e)) ++cc[s.idx[e]];
+    });
     std::vector<int64_t> tptr(static_cast<size_t>(n + 1), 0);
-    for (int64_t r = 0; r < n; ++r)
-        for (int64_t e = s.ptr[r]; e < s.ptr[r + 1]; ++e)
-            if (valid(r, e)) ++tptr[s.idx[e] + 1];
-    for (int64_t r = 0; r < n; ++r) tptr[r + 1] += tptr[r];
+    for (int64_t t = 0, run = 0; t < n; ++t) {
+        for (int64_t c = 0; c < chunks; ++c) {
+            const int64_t k = cnt[c * n + t];
+            cnt[c * n + t] = run;
+            run += k;
+        }
+        tptr[t + 1] = run;
+    }
     std::vector<int32_t> tsrc(static_cast<size_t>(tptr[n]));
     std::vector<double> tw(static_cast<size_t>(tptr[n]));
-    {
-        std::vector<int64_t> cur(tptr.begin(), tptr.end() - 1);
-        for (int64_t r = 0; r < n; ++r)
+    parallel_blocks(n, chunks, [&](int64_t c, int64_t b, int64_t en) {
+        int64_t *cc = &cnt[c * n];
+        for (int64_t r = b; r < en; ++r)
             for (int64_t e = s.ptr[r]; e < s.ptr[r + 1]; ++e)
                 if (valid(r, e)) {
-                    const int64_t o = cur[s.idx[e]]++;
+                    const int64_t o = cc[s.idx[e]]++;
                     tsrc[o] = static_cast<int32_t>(r);
                     tw[o] = s.w[e] / totals[r];
                 }
-    }
+    });
     // Walk the merged support of row r: fn(col, forward, reverse).
     auto walk = [&](int64_t r, auto &&fn) {
         int64_t e = s.ptr[r], t = tptr[r];
@@ -1072,6 +1142,13 @@ struct Run {
         layout_ms += ms_since(t0);
     }
 
+    // Adds the time since t to host stage `stage` and restarts t.
+    void lap(int stage, Clock::time_point &t) {
+        const auto now = Clock::now();
+        if (stats) stats->stage_ms[stage] += std::chrono::duration<double, std::milli>(now - t).count();
+        t = now;
+    }
+
     void record(int64_t depth, int64_t rows, int64_t edges, int sweeps) {
         if (!stats || depth < 1 || depth > 32) return;
         stats->depth = std::max<int32_t>(stats->depth, static_cast<int32_t>(depth));
@@ -1083,22 +1160,30 @@ struct Run {
     // descend (mod.rs:125-320).
     std::vector<float> descend(const CsrView &g, const Summed *summed, const Anchor &anchor,
                                std::optional<Level> params, int64_t depth) {
+        auto t = Clock::now();
         if (!params) {
             record(depth, g.n, g.edges(), 0);
-            return base_case(g, anchor, depth);
+            std::vector<float> out = base_case(g, anchor, depth);
+            lap(7, t);
+            return out;
         }
         const Level L = *params;
         std::vector<int32_t> labels =
             initialize_labels(g, L.parts, P.hub_selection, stream_seed(seed, 2, depth));
+        lap(1, t);
         const int sweeps = propagate_labels(g, labels, depth > 1);
         record(depth, g.n, g.edges(), sweeps);
+        lap(2, t);
         label_outliers(g, labels, stream_seed(seed, 3, depth));
         const int64_t parts = remap_labels(labels);
+        lap(3, t);
         const AggInput in = summed ? AggInput{summed->ptr.data(), summed->idx.data(), nullptr,
                                               summed->w.data()}
                                    : AggInput{g.ptr, g.idx, g.w, nullptr};
         Summed coarse_summed = aggregate(g.n, in, labels, parts);
+        lap(4, t);
         Csr coarse_graph = transition(coarse_summed);
+        lap(5, t);
         std::vector<float> coarse;
         {
             const Anchor ca = coarse_anchor(anchor, labels, parts);
@@ -1107,6 +1192,7 @@ struct Run {
         }
         coarse_summed = Summed();
         if (!coarse_graph.w.empty()) layout(coarse_graph, coarse, L, stream_seed(seed, 4, depth));
+        t = Clock::now();
         UMAPHIP_REQUIRE(static_cast<int64_t>(coarse.size()) == parts * d, UMAPHIP_NUMERICAL,
                         "invalid recursive coarse layout output");
         for (float x : coarse)
@@ -1115,6 +1201,7 @@ struct Run {
         coarse_graph = Csr();
         std::vector<float> result = prolongate(g, labels, coarse.data(), d);
         procrustes_align(result.data(), g.n, anchor);
+        lap(6, t);
         return result;
     }
 };
@@ -1161,6 +1248,7 @@ umaphip_status umaphip_recursive_init(int64_t n, const int64_t *offsets, const i
         StdRng rng(stream_seed(seed, 1, 0));
         const Anchor anchor =
             anchor_reference(data, n, features, d, params->anchor_sample_size, rng);
+        if (stats) stats->stage_ms[0] = ms_since(t0);
         const std::vector<float> result =
             run.descend(g, nullptr, anchor, first_level(n, a, b, *params), 1);
         std::memcpy(out, result.data(), result.size() * sizeof(float));

@@ -16,6 +16,8 @@ from . import _lib
 
 HUB_SELECTION = {"degree": 0, "diverse": 1, "random": 2}
 RESCALE = {"isotropic": 0, "per_axis": 1, "graph_edge": 2}
+STAGES = ("anchor", "hubs", "propagation", "outliers", "aggregation", "transition",
+          "prolongation", "base_case")
 
 
 def _p(a: np.ndarray) -> int:
@@ -57,7 +59,8 @@ def recursive_init(offsets, indices, weights, data, d, a, b, seed, config=None,
     depth = stats.depth
     return out, dict(depth=depth, rows=list(stats.rows[:depth]), edges=list(stats.edges[:depth]),
                      sweeps=list(stats.sweeps[:depth]), host_ms=stats.host_ms,
-                     layout_ms=stats.layout_ms)
+                     layout_ms=stats.layout_ms,
+                     stage_ms=dict(zip(STAGES, (round(v, 2) for v in stats.stage_ms))))
 
 
 def init_rescale(values, mode, *, repulsion=1.0, scale_factor=1.0, offsets=None, indices=None,

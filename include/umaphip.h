@@ -375,6 +375,10 @@ typedef struct {
     int32_t sweeps[32];             /* label propagation sweeps (0 at the base) */
     double host_ms;                 /* host graph and geometry steps */
     double layout_ms;               /* coarse layouts, wall clock */
+    double stage_ms[8];             /* host time by step, summed over levels: anchor,
+                                       hubs, propagation, outliers and remap,
+                                       aggregation, transition, prolongation and
+                                       alignment, base case */
 } umaphip_recursive_stats;
 
 /* recursive_init::run (mod.rs:48-123): coordinates [n, d] before the final
