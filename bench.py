@@ -509,7 +509,8 @@ def main():
                 "kernel": "unit_sweep_kernel + unit_adam_kernel (edge pass + lazy Adam)",
                 "bytes_per_launch": int(bytes_per_launch), "avg_launch_ms": round(sweep_avg_ms, 4),
                 "firing_edges_per_epoch": F / args.epochs, "negatives_per_epoch": S / args.epochs,
-                "refresh_avg_ms": round(pstats["refresh_ms"] / max(pstats["refresh_launches"], 1), 4)}
+                "refresh_avg_ms": round(pstats["refresh_ms"] / max(pstats["refresh_launches"], 1), 4),
+                "rank_ordered_epochs": pstats["rank_ordered_epochs"]}
     out = {
         "metric": "layout edge-epochs/s at 1M rows", "value": value, "unit": "edge-epochs/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

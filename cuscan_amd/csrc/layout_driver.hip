@@ -249,6 +249,11 @@ struct umaphip_plan {
     DeviceArray<uint16_t> fire;
     DeviceArray<int32_t> unit_k, unit_frag, nz_rows, nz_off, row_split, split_off;
     DeviceArray<float> grad, frag;
+    // rank-ordered unit sweep (projection windows, every row with an edge)
+    bool ranked = false;
+    int64_t ranked_epochs = 0;
+    int32_t row_cap_id = 0, row_cap_ranked = 0;   // unit row-table capacity per order
+    DeviceArray<int32_t> pref, pref_blocks, rank_unit_k;
 
     std::vector<float> alpha, mu, c1, c2;
     float scale0 = 1.0f;
@@ -331,6 +336,7 @@ struct umaphip_plan {
 void umaphip_plan::run() {
     const int epochs = p.epochs;
     const bool adam = !momentum;
+    ranked_epochs = 0;
     // reset state
     HIP_CHECK(hipMemcpyAsync(pos[0].ptr, head0.ptr, head0.bytes(), hipMemcpyDeviceToDevice, stream));
     HIP_CHECK(hipMemcpyAsync(npos.ptr, npos0.ptr, npos0.bytes(), hipMemcpyDeviceToDevice, stream));
@@ -460,10 +466,25 @@ void umaphip_plan::run() {
                 U.grad = grad.ptr; U.frag = frag.ptr;
                 U.units = units; U.n_edges = E;
                 U.rate = static_cast<double>(p.negative_sample_rate);
+                // with this epoch's projection order, sweep the rows by rank
+                const bool by_rank = ranked && projection_ready;
+                U.row_cap = by_rank ? row_cap_ranked : row_cap_id;
+                if (by_rank) {
+                    launch_rank_prefix(order_now, offsets.ptr, n, static_cast<int32_t>(E),
+                                       pref.ptr, pref_blocks.ptr, rank_unit_k.ptr, stream);
+                    U.pref = pref.ptr;
+                    U.unit_k = rank_unit_k.ptr;
+                    ++ranked_epochs;
+                }
                 launch_unit_sweep(U, stream);
-                launch_unit_adam(h, pos[c].ptr, pos[1 - c].ptr, grad.ptr, row_split.ptr,
-                                 split_off.ptr, frag.ptr, m.ptr, v.ptr, alpha[t], c1[t], c2[t],
-                                 stream);
+                if (by_rank)
+                    launch_unit_adam_ranked(h, pos[c].ptr, pos[1 - c].ptr, grad.ptr, ranks.ptr,
+                                            pref.ptr, frag.ptr, m.ptr, v.ptr, alpha[t], c1[t],
+                                            c2[t], stream);
+                else
+                    launch_unit_adam(h, pos[c].ptr, pos[1 - c].ptr, grad.ptr, row_split.ptr,
+                                     split_off.ptr, frag.ptr, m.ptr, v.ptr, alpha[t], c1[t],
+                                     c2[t], stream);
             } else {
                 launch_edge_sweep(A, stream);
             }
@@ -709,7 +730,30 @@ umaphip_status umaphip_plan_create(int64_t h, int64_t n, int64_t d, const float 
             up(P->split_off, T.split_off);
             P->grad.alloc(static_cast<size_t>(2 * h));
             HIP_CHECK(hipMemset(P->grad.ptr, 0, P->grad.bytes()));
-            P->frag.alloc(static_cast<size_t>(2 * std::max<int64_t>(T.split_off.back(), 1)));
+            // Projection windows over a graph whose every row has an edge:
+            // the epochs with a projection order sweep rows by rank (fragment
+            // slots 2u and 2u + 1 of every unit).
+            P->ranked = P->hard && T.nz_rows.empty() && E < INT32_MAX;
+            // Row-table capacity: the rows of a unit in CSR order are known;
+            // in any rank order a unit of UE slots holds at most
+            // 2 + (UE - 2) / (smallest degree) rows (two partial rows).
+            const int64_t UE = unit_edges(), narrow = unit_narrow_rows();
+            int64_t id_rows = 0, min_deg = E;
+            for (int64_t u = 0; u < T.units; ++u)
+                id_rows = std::max<int64_t>(id_rows, T.unit_k[2 * u + 1] - T.unit_k[2 * u] + 1);
+            for (int64_t i = 0; i < h; ++i)
+                min_deg = std::min<int64_t>(min_deg, offsets[i + 1] - offsets[i]);
+            P->row_cap_id = static_cast<int32_t>(id_rows <= narrow ? narrow : UE);
+            P->row_cap_ranked = static_cast<int32_t>(
+                min_deg > 0 && 2 + (UE - 2) / min_deg <= narrow ? narrow : UE);
+            int64_t frag_slots = std::max<int64_t>(T.split_off.back(), 1);
+            if (P->ranked) {
+                frag_slots = std::max<int64_t>(frag_slots, 2 * T.units);
+                P->pref.alloc(static_cast<size_t>(n + 1));
+                P->pref_blocks.alloc(static_cast<size_t>(rank_prefix_blocks(n)));
+                P->rank_unit_k.alloc(static_cast<size_t>(2 * T.units));
+            }
+            P->frag.alloc(static_cast<size_t>(2 * frag_slots));
         }
         if (P->momentum) {
             P->source_dev[0].alloc(h);
@@ -855,6 +899,7 @@ umaphip_status umaphip_plan_download(umaphip_plan *plan, float *head, double *ne
             stats->refresh_ms = plan->ran ? plan->pairs_ms(plan->refresh_ev, plan->refresh_used) : 0.0;
             stats->refresh_launches = plan->refresh_used / 2;
             stats->edge_balanced = plan->unit_path ? 1 : 0;
+            stats->rank_ordered_epochs = plan->ran ? plan->ranked_epochs : 0;
         }
         UMAPHIP_REQUIRE(status == ~0ull, UMAPHIP_NUMERICAL,
                         "non-finite %s projection at row %llu (refresh %llu)",

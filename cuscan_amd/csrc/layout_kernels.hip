@@ -751,6 +751,7 @@ constexpr int kUnitEPL = 2;
 constexpr int kUnitEdges = 64 * kUnitEPL;
 constexpr int kUnitTermCap = kUnitEdges * 6;
 constexpr int kUnitGroups = 4;
+constexpr int kUnitNarrowRows = 32;
 
 __device__ inline uint16_t fire_shadow(double next_positive) {
     const double c = ceil(next_positive);
@@ -779,65 +780,95 @@ __device__ inline void segmented_accumulate(float2 term, int row, int lane, floa
     }
 }
 
+// RANKED: units run over the rows in the epoch's projection order (rank r
+// covers sequence slots [pref[r], pref[r+1])), so the heads a wave, and an
+// XCD, sweep together have neighbouring ranks and their negative windows
+// overlap in L2. Requires every row to have an edge.
+// RC: capacity of the unit row table; the plan picks 32 when no unit can
+// hold more rows, which keeps the workgroup under 4 KB of LDS (10 waves per
+// SIMD, the register limit) instead of about 8 KB (5 waves).
+template <bool RANKED, int RC>
 __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
     const SweepArgs &A = U.s;
     constexpr int EPL = kUnitEPL, UE = kUnitEdges, TCAP = kUnitTermCap, G = kUnitGroups;
-    __shared__ int32_t s_off[UE + 1];   // unit-local first edge of each unit row
-    __shared__ int32_t s_seg[UE + 1];   // CSR start of each unit row (+ end of the last)
-    __shared__ int32_t s_rowid[UE];
-    __shared__ float2 s_pos[UE];
-    __shared__ uint32_t s_first[UE];
-    __shared__ float2 s_acc[UE];
-    __shared__ uint16_t s_owner[TCAP];
-    __shared__ int32_t s_base[UE + 1];
+    __shared__ uint8_t s_off[RC + 1];   // unit-local first edge of each unit row
+    __shared__ int32_t s_seg[RC + 1];   // CSR start of each unit row (+ end of the last)
+    __shared__ int32_t s_rowid[RC];
+    __shared__ float2 s_pos[RC];
+    __shared__ uint32_t s_first[RC];
+    __shared__ float2 s_acc[RC];
+    __shared__ uint8_t s_owner[TCAP];   // edge slot of each term
+    __shared__ uint16_t s_base[UE + 1]; // first term of each edge slot (read when total <= TCAP)
     __shared__ int32_t s_j[UE];
     __shared__ uint8_t s_row[UE];
     __shared__ uint32_t s_em[UE];
-    __shared__ uint32_t s_sel0[UE];
-    static_assert(UE <= 256, "unit rows are indexed by uint8");
+    __shared__ int32_t s_pst[RANKED ? RC + 1 : 1];  // sequence start of each unit row
+    __shared__ int32_t s_end[RANKED ? RC + 1 : 1];  // CSR end of each unit row
+    static_assert(UE < 256 && RC <= UE, "edge slots and unit rows are indexed by uint8");
+    static_assert(TCAP < 65536, "term bases are 16-bit");
 
     const int lane = threadIdx.x;
-    const int64_t u = blockIdx.x;
+    // Workgroups are dealt to the 8 XCDs in turn; workgroup b takes unit
+    // (b mod 8) * xcd_chunk + b / 8, so each XCD (and its L2) sweeps one
+    // contiguous eighth of the rows.
+    const int64_t u = (blockIdx.x % 8) * U.xcd_chunk + blockIdx.x / 8;
+    if (u >= U.units) return;
     const int64_t e0 = u * UE;
     const int ue = static_cast<int>(min<int64_t>(UE, U.n_edges - e0));
     const uint32_t epoch = A.epoch;
-    // 1. unit rows and firing shadow
-    const int32_t k0 = U.unit_k[2 * u], k1 = U.unit_k[2 * u + 1];
-    uint32_t fire[EPL];
-#pragma unroll
-    for (int k = 0; k < EPL; ++k) {
-        const int v = k * 64 + lane;
-        fire[k] = v < ue ? U.fire[e0 + v] : 0xffffffffu;
-    }
-    // 2. firing records, then the row data
-    double r_npos[EPL], r_nn[EPL], r_per[EPL];
-    uint64_t r_w[EPL];
-#pragma unroll
-    for (int k = 0; k < EPL; ++k) {
-        r_npos[k] = 0.0; r_nn[k] = 0.0; r_per[k] = 0.0; r_w[k] = 0;
-        if (fire[k] <= epoch) {
-            const double4 rec = reinterpret_cast<const double4 *>(U.rec)[e0 + k * 64 + lane];
-            r_npos[k] = rec.x;
-            r_nn[k] = rec.y;
-            r_per[k] = rec.z;
-            r_w[k] = static_cast<uint64_t>(__double_as_longlong(rec.w));
-        }
-    }
-    const int nr = k1 - k0 + 1;
     const uint64_t n64 = static_cast<uint64_t>(A.n);
     const uint64_t modulus = A.range;
     const bool windowed = modulus < n64;
+    // 1. unit rows and firing shadow
+    // first and last row (ranked: rank) of the unit
+    const int32_t k0 = U.unit_k[2 * u], k1 = U.unit_k[2 * u + 1];
+    const int nr = k1 - k0 + 1;
+    uint32_t fire[EPL];
+    int64_t eidx[EPL];
+    double r_npos[EPL], r_nn[EPL], r_per[EPL];
+    uint64_t r_w[EPL];
+    if constexpr (!RANKED) {
+#pragma unroll
+        for (int k = 0; k < EPL; ++k) {
+            const int v = k * 64 + lane;
+            eidx[k] = e0 + v;
+            fire[k] = v < ue ? U.fire[e0 + v] : 0xffffffffu;
+        }
+        // 2. firing records, then the row data
+#pragma unroll
+        for (int k = 0; k < EPL; ++k) {
+            r_npos[k] = 0.0; r_nn[k] = 0.0; r_per[k] = 0.0; r_w[k] = 0;
+            if (fire[k] <= epoch) {
+                const double4 rec = reinterpret_cast<const double4 *>(U.rec)[eidx[k]];
+                r_npos[k] = rec.x;
+                r_nn[k] = rec.y;
+                r_per[k] = rec.z;
+                r_w[k] = static_cast<uint64_t>(__double_as_longlong(rec.w));
+            }
+        }
+    }
     for (int lr = lane; lr < nr; lr += 64) {
         const int32_t k = k0 + lr;
-        const int32_t row = U.nz_rows ? U.nz_rows[k] : k;
-        const int32_t seg = U.nz_off[k];
+        int32_t row, seg, pst;
+        if constexpr (RANKED) {
+            row = A.order[k];
+            seg = A.offsets[row];
+            pst = U.pref[k];
+            s_pst[lr] = pst;
+            s_end[lr] = seg + (U.pref[k + 1] - pst);
+        } else {
+            row = U.nz_rows ? U.nz_rows[k] : k;
+            seg = U.nz_off[k];
+            pst = seg;
+        }
         s_seg[lr] = seg;
-        s_off[lr] = static_cast<int32_t>(max<int64_t>(seg, e0) - e0);
+        s_off[lr] = static_cast<uint8_t>(max<int64_t>(pst, e0) - e0);
         s_rowid[lr] = row;
         s_pos[lr] = ld2(A.pos, row);
         uint64_t first = 0;
         if (windowed) {
-            const uint64_t rk = static_cast<uint32_t>(A.ranks[row]);
+            const uint64_t rk = RANKED ? static_cast<uint64_t>(k)
+                                       : static_cast<uint64_t>(static_cast<uint32_t>(A.ranks[row]));
             const uint64_t half = modulus / 2;
             if (rk > half) first = rk - half;
             if (first > n64 - modulus) first = n64 - modulus;
@@ -846,20 +877,52 @@ __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
         s_acc[lr] = make_float2(0.0f, 0.0f);
     }
     if (lane == 0) {
-        s_off[nr] = ue;
-        s_seg[nr] = U.nz_off[k1 + 1];
+        s_off[nr] = static_cast<uint8_t>(ue);
+        if constexpr (!RANKED) s_seg[nr] = U.nz_off[k1 + 1];
     }
     const float scale = A.scale_ptr ? *A.scale_ptr : A.scale;
     const double now = static_cast<double>(epoch);
     __syncthreads();
+    if constexpr (RANKED) {
+        // slots to CSR edges through the row table, then the records
+#pragma unroll
+        for (int k = 0; k < EPL; ++k) {
+            const int slot = k * 64 + lane;
+            eidx[k] = 0;
+            fire[k] = 0xffffffffu;
+            if (slot < ue) {
+                int lo = 0, hi = nr;
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (s_off[mid] <= slot) lo = mid;
+                    else hi = mid;
+                }
+                eidx[k] = s_seg[lo] + (e0 + slot - s_pst[lo]);
+                fire[k] = U.fire[eidx[k]];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < EPL; ++k) {
+            r_npos[k] = 0.0; r_nn[k] = 0.0; r_per[k] = 0.0; r_w[k] = 0;
+            if (fire[k] <= epoch) {
+                const double4 rec = reinterpret_cast<const double4 *>(U.rec)[eidx[k]];
+                r_npos[k] = rec.x;
+                r_nn[k] = rec.y;
+                r_per[k] = rec.z;
+                r_w[k] = static_cast<uint64_t>(__double_as_longlong(rec.w));
+            }
+        }
+    }
 
     // clocks (bit-exact f64, reference order of operations) and term counts
     int running = 0;
     int t_cnt[EPL], t_base[EPL];
+    uint32_t sel0r[EPL];   // e (epoch + 1) mod M of the lane's own edge slots
 #pragma unroll
     for (int k = 0; k < EPL; ++k) {
         const int slot = k * 64 + lane;
         int t = 0, r = 0;
+        sel0r[k] = 0u;
         if (slot < ue) {
             int lo = 0, hi = nr;
             while (hi - lo > 1) {
@@ -869,7 +932,7 @@ __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
             }
             r = lo;
             if (fire[k] <= epoch) {
-                const int64_t e = e0 + slot;
+                const int64_t e = eidx[k];
                 const double per = r_per[k], nn = r_nn[k];
                 const double npos2 = r_npos[k] + per;
                 int32_t count = 0;
@@ -886,9 +949,9 @@ __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
                 const uint32_t em = static_cast<uint32_t>(r_w[k] >> 32);
                 s_j[slot] = static_cast<int32_t>(static_cast<uint32_t>(r_w[k]));
                 s_em[slot] = em;
-                s_sel0[slot] = negs ? static_cast<uint32_t>(mod_u64(
-                                          static_cast<uint64_t>(em) * U.t1, modulus, U.inv_mod))
-                                    : 0u;
+                sel0r[k] = negs ? static_cast<uint32_t>(mod_u64(static_cast<uint64_t>(em) * U.t1,
+                                                                modulus, U.inv_mod))
+                                : 0u;
             }
         }
         s_row[slot] = static_cast<uint8_t>(r);
@@ -899,17 +962,17 @@ __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
         }
         t_cnt[k] = t;
         t_base[k] = running + incl - t;
-        s_base[slot] = t_base[k];
+        s_base[slot] = static_cast<uint16_t>(t_base[k]);
         running += __shfl(incl, 63, 64);
     }
     const int total = running;
-    if (lane == 0) s_base[UE] = total;
+    if (lane == 0) s_base[UE] = static_cast<uint16_t>(total);
     const bool term_parallel = total <= TCAP;
     if (term_parallel) {
 #pragma unroll
         for (int k = 0; k < EPL; ++k)
             for (int q = 0; q < t_cnt[k]; ++q)
-                s_owner[t_base[k] + q] = static_cast<uint16_t>(k * 64 + lane);
+                s_owner[t_base[k] + q] = static_cast<uint8_t>(k * 64 + lane);
     }
     __syncthreads();
 
@@ -931,8 +994,9 @@ __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
                     continue;
                 }
                 // e*(epoch+kq) mod M from e*(epoch+1) mod M by adding e mod M
-                uint64_t sel = static_cast<uint64_t>(s_sel0[owner]) +
-                               static_cast<uint64_t>(kq - 1) * s_em[owner];
+                const uint64_t em_o = s_em[owner];
+                uint64_t sel = mod_u64(em_o * U.t1, modulus, U.inv_mod) +
+                               static_cast<uint64_t>(kq - 1) * em_o;
                 if (kq <= 4) {
                     while (sel >= modulus) sel -= modulus;
                 } else {
@@ -942,8 +1006,8 @@ __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
                 const uint64_t rank = static_cast<uint64_t>(s_first[rr]) + sel;
                 if (A.exclude) {
                     const int64_t jj = static_cast<uint32_t>(A.order[rank]);
-                    keep[g] = !excluded(A.indices, s_seg[rr], s_seg[rr + 1], s_rowid[rr], jj,
-                                        A.shared != 0);
+                    keep[g] = !excluded(A.indices, s_seg[rr], RANKED ? s_end[rr] : s_seg[rr + 1],
+                                        s_rowid[rr], jj, A.shared != 0);
                     y[g] = A.table ? ld2(A.table, rank) : ld2(A.tgt, jj);
                 } else {
                     y[g] = A.table ? ld2(A.table, rank)
@@ -993,7 +1057,7 @@ __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
                     const float c = attract_fast(q, A.a, A.b, 1.0f);
                     g = make_float2(c * dx, c * dy);
                 }
-                uint64_t sel = s_sel0[slot];
+                uint64_t sel = sel0r[k];
                 const uint64_t em = s_em[slot];
                 for (int p = 0; p + 1 < t_cnt[k]; ++p) {
                     if (p > 0) {
@@ -1004,8 +1068,8 @@ __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
                     float2 yn;
                     if (A.exclude) {
                         const int64_t jj = static_cast<uint32_t>(A.order[rank]);
-                        if (excluded(A.indices, s_seg[rr], s_seg[rr + 1], s_rowid[rr], jj,
-                                     A.shared != 0))
+                        if (excluded(A.indices, s_seg[rr], RANKED ? s_end[rr] : s_seg[rr + 1],
+                                     s_rowid[rr], jj, A.shared != 0))
                             continue;
                         yn = A.table ? ld2(A.table, rank) : ld2(A.tgt, jj);
                     } else {
@@ -1025,8 +1089,16 @@ __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
     for (int lr = lane; lr < nr; lr += 64) {
         const float2 g = s_acc[lr];
         int slot = -1;
-        if (lr == 0) slot = U.unit_frag[2 * u];
-        if (lr == nr - 1 && slot < 0) slot = U.unit_frag[2 * u + 1];
+        if constexpr (RANKED) {
+            // a row begun in an earlier unit stores slot 2u; one that
+            // continues into the next unit (and began here) stores 2u + 1
+            if (lr == 0 && s_pst[0] < e0) slot = static_cast<int>(2 * u);
+            else if (lr == nr - 1 && s_off[lr] + (s_end[lr] - s_seg[lr]) > ue)
+                slot = static_cast<int>(2 * u + 1);
+        } else {
+            if (lr == 0) slot = U.unit_frag[2 * u];
+            if (lr == nr - 1 && slot < 0) slot = U.unit_frag[2 * u + 1];
+        }
         if (slot >= 0) reinterpret_cast<float2 *>(U.frag)[slot] = g;
         else reinterpret_cast<float2 *>(U.grad)[s_rowid[lr]] = g;
     }
@@ -1088,6 +1160,141 @@ __global__ void unit_adam_kernel(int64_t h, const float *pos, float *pos_out, co
     reinterpret_cast<float2 *>(m)[i] = mm;
     reinterpret_cast<float2 *>(v)[i] = vv;
     reinterpret_cast<float2 *>(pos_out)[i] = out;
+}
+
+// Lazy Adam for the rank-ordered sweep: row i (rank r) covers sequence slots
+// [pref[r], pref[r+1]); a row inside one unit has its gradient in grad,
+// otherwise its fragments are slot 2u_a + 1 of its first unit and slot 2u of
+// every later unit, added in unit order.
+__global__ void unit_adam_ranked_kernel(int64_t h, const float *pos, float *pos_out,
+                                        const float *grad, const int32_t *ranks,
+                                        const int32_t *pref, const float *frag, float *m, float *v,
+                                        float alpha, float c1, float c2) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= h) return;
+    const int32_t r = ranks[i];
+    const int64_t b = pref[r], en = pref[r + 1];
+    const int64_t ua = b / kUnitEdges, ub = (en - 1) / kUnitEdges;
+    float2 g;
+    if (ua == ub) {
+        g = ld2(grad, i);
+    } else {
+        g = make_float2(0.0f, 0.0f);
+        const float2 f0 = ld2(frag, 2 * ua + 1);
+        g.x = g.x + f0.x;
+        g.y = g.y + f0.y;
+        for (int64_t w = ua + 1; w <= ub; ++w) {
+            const float2 f = ld2(frag, 2 * w);
+            g.x = g.x + f.x;
+            g.y = g.y + f.y;
+        }
+    }
+    const float2 x = ld2(pos, i);
+    float2 mm = ld2(m, i), vv = ld2(v, i);
+    float2 out = x;
+    if (g.x != 0.0f) {
+        mm.x = 0.9f * mm.x + (1.0f - 0.9f) * g.x;
+        vv.x = 0.99f * vv.x + ((1.0f - 0.99f) * g.x) * g.x;
+        out.x = x.x + (alpha * (mm.x / c1)) / (sqrtf(vv.x / c2) + 0.0001f);
+    }
+    if (g.y != 0.0f) {
+        mm.y = 0.9f * mm.y + (1.0f - 0.9f) * g.y;
+        vv.y = 0.99f * vv.y + ((1.0f - 0.99f) * g.y) * g.y;
+        out.y = x.y + (alpha * (mm.y / c1)) / (sqrtf(vv.y / c2) + 0.0001f);
+    }
+    reinterpret_cast<float2 *>(m)[i] = mm;
+    reinterpret_cast<float2 *>(v)[i] = vv;
+    reinterpret_cast<float2 *>(pos_out)[i] = out;
+}
+
+// Ranks holding each unit's first and last slot, for the rank-ordered
+// sweep: rank r covers slots [pref[r], pref[r+1]) (every row has an edge, so
+// each slot has exactly one rank and every entry is written once).
+__global__ void rank_units_kernel(const int32_t *pref, int64_t n, int64_t n_edges,
+                                  int32_t *unit_k) {
+    const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int64_t b = pref[r], en = pref[r + 1];
+    const int64_t last_unit = (n_edges - 1) / kUnitEdges;
+    for (int64_t u = b / kUnitEdges; u <= (en - 1) / kUnitEdges; ++u) {
+        const int64_t first = u * kUnitEdges;
+        const int64_t last = u == last_unit ? n_edges - 1 : first + kUnitEdges - 1;
+        if (first >= b && first < en) unit_k[2 * u] = static_cast<int32_t>(r);
+        if (last >= b && last < en) unit_k[2 * u + 1] = static_cast<int32_t>(r);
+    }
+}
+
+// Exclusive prefix sum of the row degrees in projection order:
+// pref[r] = sum over ranks below r of degree(order[r']), pref[n] = E.
+// Pass 1 scans blocks of kPrefBlock ranks, pass 2 scans the block totals in
+// one workgroup, pass 3 adds them.
+constexpr int kPrefThreads = 256, kPrefPer = 4, kPrefBlock = kPrefThreads * kPrefPer;
+
+__device__ inline int32_t block_exclusive_scan(int32_t x, int32_t *s_warp, int32_t &total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int32_t incl = x;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int32_t up = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += up;
+    }
+    if (lane == 63) s_warp[wave] = incl;
+    __syncthreads();
+    int32_t before = 0;
+    total = 0;
+    for (int w = 0; w < kPrefThreads / 64; ++w) {
+        if (w < wave) before += s_warp[w];
+        total += s_warp[w];
+    }
+    __syncthreads();
+    return before + incl - x;
+}
+
+__global__ __launch_bounds__(kPrefThreads) void rank_prefix_kernel(const int32_t *order,
+                                                                   const int32_t *offsets,
+                                                                   int64_t n, int32_t *pref,
+                                                                   int32_t *block_sums) {
+    __shared__ int32_t s_warp[kPrefThreads / 64];
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kPrefBlock + threadIdx.x * kPrefPer;
+    int32_t deg[kPrefPer], sum = 0;
+#pragma unroll
+    for (int q = 0; q < kPrefPer; ++q) {
+        deg[q] = 0;
+        if (base + q < n) {
+            const int32_t row = order[base + q];
+            deg[q] = offsets[row + 1] - offsets[row];
+        }
+        sum += deg[q];
+    }
+    int32_t total;
+    int32_t run = block_exclusive_scan(sum, s_warp, total);
+#pragma unroll
+    for (int q = 0; q < kPrefPer; ++q) {
+        if (base + q < n) pref[base + q] = run;
+        run += deg[q];
+    }
+    if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kPrefThreads) void prefix_blocks_kernel(int32_t *block_sums,
+                                                                     int64_t nb) {
+    __shared__ int32_t s_warp[kPrefThreads / 64];
+    int32_t carry = 0;
+    for (int64_t b0 = 0; b0 < nb; b0 += kPrefThreads) {
+        const int64_t b = b0 + threadIdx.x;
+        const int32_t x = b < nb ? block_sums[b] : 0;
+        int32_t total;
+        const int32_t ex = block_exclusive_scan(x, s_warp, total);
+        if (b < nb) block_sums[b] = carry + ex;
+        carry += total;
+    }
+}
+
+__global__ __launch_bounds__(kPrefThreads) void prefix_add_kernel(int32_t *pref, int64_t n,
+                                                                  const int32_t *block_sums,
+                                                                  int32_t total_edges) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kPrefThreads + threadIdx.x;
+    if (i < n) pref[i] += block_sums[i / kPrefBlock];
+    if (i == 0) pref[n] = total_edges;
 }
 
 // ------------------------------------------------ FP64 cross-entropy objective
@@ -1679,13 +1886,50 @@ void launch_edge_sweep(const SweepArgs &a, hipStream_t s) {
 }
 
 int64_t unit_edges() { return kUnitEdges; }
+int64_t unit_narrow_rows() { return kUnitNarrowRows; }
 
 void launch_unit_sweep(UnitSweepArgs a, hipStream_t s) {
     if (a.units == 0) return;
     const uint64_t modulus = a.s.range;
     a.t1 = (static_cast<uint64_t>(a.s.epoch) + 1u) % modulus;
     a.inv_mod = 1.0 / static_cast<double>(modulus);
-    hipLaunchKernelGGL(unit_sweep_kernel, dim3(static_cast<unsigned>(a.units)), dim3(64), 0, s, a);
+    a.xcd_chunk = (a.units + 7) / 8;
+    const dim3 grid(static_cast<unsigned>(8 * a.xcd_chunk));
+    const bool narrow = a.row_cap == kUnitNarrowRows;
+    if (a.pref) {
+        if (narrow) hipLaunchKernelGGL((unit_sweep_kernel<true, kUnitNarrowRows>), grid, dim3(64), 0, s, a);
+        else hipLaunchKernelGGL((unit_sweep_kernel<true, kUnitEdges>), grid, dim3(64), 0, s, a);
+    } else {
+        if (narrow) hipLaunchKernelGGL((unit_sweep_kernel<false, kUnitNarrowRows>), grid, dim3(64), 0, s, a);
+        else hipLaunchKernelGGL((unit_sweep_kernel<false, kUnitEdges>), grid, dim3(64), 0, s, a);
+    }
+    HIP_CHECK(hipGetLastError());
+}
+
+int64_t rank_prefix_blocks(int64_t n) { return (n + kPrefBlock - 1) / kPrefBlock; }
+
+void launch_rank_prefix(const int32_t *order, const int32_t *offsets, int64_t n,
+                        int32_t total_edges, int32_t *pref, int32_t *block_sums,
+                        int32_t *unit_k, hipStream_t s) {
+    if (n == 0 || total_edges == 0) return;
+    const int64_t nb = rank_prefix_blocks(n);
+    hipLaunchKernelGGL(rank_prefix_kernel, dim3(static_cast<unsigned>(nb)), dim3(kPrefThreads), 0,
+                       s, order, offsets, n, pref, block_sums);
+    hipLaunchKernelGGL(prefix_blocks_kernel, dim3(1), dim3(kPrefThreads), 0, s, block_sums, nb);
+    hipLaunchKernelGGL(prefix_add_kernel, dim3(static_cast<unsigned>((n + kPrefThreads - 1) /
+                                                                     kPrefThreads)),
+                       dim3(kPrefThreads), 0, s, pref, n, block_sums, total_edges);
+    hipLaunchKernelGGL(rank_units_kernel, dim3(grid_for(n)), dim3(kThreads), 0, s, pref, n,
+                       static_cast<int64_t>(total_edges), unit_k);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_unit_adam_ranked(int64_t h, const float *pos, float *pos_out, const float *grad,
+                             const int32_t *ranks, const int32_t *pref, const float *frag,
+                             float *m, float *v, float alpha, float c1, float c2, hipStream_t s) {
+    if (h == 0) return;
+    hipLaunchKernelGGL(unit_adam_ranked_kernel, dim3(grid_for(h)), dim3(kThreads), 0, s, h, pos,
+                       pos_out, grad, ranks, pref, frag, m, v, alpha, c1, c2);
     HIP_CHECK(hipGetLastError());
 }
 

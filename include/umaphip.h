@@ -180,6 +180,8 @@ typedef struct {
     double refresh_ms;              /* sum over projection refreshes (profiling on) */
     int64_t refresh_launches;
     int64_t edge_balanced;          /* 1 when epochs ran the edge-balanced unit sweep */
+    int64_t rank_ordered_epochs;    /* epochs of the last run whose unit sweep ran the rows
+                                       in projection order */
 } umaphip_layout_stats;
 
 /* Host arrays in, host arrays out (allocates and frees device memory).

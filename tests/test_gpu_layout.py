@@ -290,9 +290,18 @@ DRIVER = [
     ("fit_adam_global_hubs", 3000, 3000, 2, True, "adam", 200_000, False, 16),
     ("fit_adam_windows_hubs_excl", 3000, 3000, 2, True, "adam", 700, True, 16),
     ("fit_adam_global_rate24_hubs", 3000, 3000, 2, True, "adam", 200_000, True, 10),
+    # rank-ordered unit sweep (windows, every row with an edge): hub rows
+    # spanning units, exclusion, and the per-edge path of a rate of 24
+    ("fit_adam_windows_hubs_ranked", 3000, 3000, 2, True, "adam", 700, True, 16),
+    ("fit_adam_windows_rate24_hubs_ranked", 3000, 3000, 2, True, "adam", 900, True, 10),
 ]
 # cases that must run the edge-balanced unit sweep
-UNIT_PATH = {"fit_adam_global_excl", "fit_adam_windows_adapt", "fit_adam_global_hubs", "fit_adam_windows_hubs_excl", "fit_adam_global_rate24_hubs"}
+UNIT_PATH = {"fit_adam_global_excl", "fit_adam_windows_adapt", "fit_adam_global_hubs",
+             "fit_adam_windows_hubs_excl", "fit_adam_global_rate24_hubs",
+             "fit_adam_windows_hubs_ranked", "fit_adam_windows_rate24_hubs_ranked"}
+# ... and, every epoch, with the rows in projection order
+RANKED_PATH = {"fit_adam_windows_adapt", "fit_adam_windows_hubs_ranked",
+               "fit_adam_windows_rate24_hubs_ranked"}
 
 
 @pytest.mark.parametrize("case", DRIVER, ids=[c[0] for c in DRIVER])
@@ -314,7 +323,8 @@ def test_driver_matches_oracle_loop(case):
     rng = np.random.default_rng(abs(hash(name)) % 2**32)
     rate = 24 if "rate24" in name else 5
     if "hubs" in name:
-        g = hub_graph(h, 6, 11, epochs, rate=float(rate))
+        isolated = () if "ranked" in name else (3, 4, 2999)
+        g = hub_graph(h, 6, 11, epochs, rate=float(rate), isolated=isolated)
     else:
         g = random_graph(h, n, 6, 11, epochs, symmetric=shared)
     head = rng.uniform(-10, 10, (h, d)).astype(np.float32)
@@ -333,6 +343,7 @@ def test_driver_matches_oracle_loop(case):
 
     (got, gp, gn, stats), (exp, ep, en, est) = both(epochs)
     assert stats["edge_balanced"] == (name in UNIT_PATH)
+    assert stats["rank_ordered_epochs"] == (epochs if name in RANKED_PATH else 0)
     assert (gp == ep).all() and (gn == en).all()
     assert stats["adaptations"] == est["adaptations"]
     assert stats["refreshes"] == est["refreshes"]
