@@ -253,7 +253,7 @@ struct umaphip_plan {
     bool ranked = false;
     int64_t ranked_epochs = 0;
     int32_t row_cap_id = 0, row_cap_ranked = 0;   // unit row-table capacity per order
-    DeviceArray<int32_t> pref, pref_blocks, rank_unit_k;
+    DeviceArray<int32_t> pref, pref_blocks, rank_unit_k, rank_seg, rank_deg;
 
     std::vector<float> alpha, mu, c1, c2;
     float scale0 = 1.0f;
@@ -471,8 +471,10 @@ void umaphip_plan::run() {
                 U.row_cap = by_rank ? row_cap_ranked : row_cap_id;
                 if (by_rank) {
                     launch_rank_prefix(order_now, offsets.ptr, n, static_cast<int32_t>(E),
-                                       pref.ptr, pref_blocks.ptr, rank_unit_k.ptr, stream);
+                                       pref.ptr, pref_blocks.ptr, rank_seg.ptr, rank_deg.ptr,
+                                       rank_unit_k.ptr, stream);
                     U.pref = pref.ptr;
+                    U.rank_seg = rank_seg.ptr;
                     U.unit_k = rank_unit_k.ptr;
                     ++ranked_epochs;
                 }
@@ -752,6 +754,8 @@ umaphip_status umaphip_plan_create(int64_t h, int64_t n, int64_t d, const float 
                 P->pref.alloc(static_cast<size_t>(n + 1));
                 P->pref_blocks.alloc(static_cast<size_t>(rank_prefix_blocks(n)));
                 P->rank_unit_k.alloc(static_cast<size_t>(2 * T.units));
+                P->rank_seg.alloc(static_cast<size_t>(n));
+                P->rank_deg.alloc(static_cast<size_t>(n));
             }
             P->frag.alloc(static_cast<size_t>(2 * frag_slots));
         }

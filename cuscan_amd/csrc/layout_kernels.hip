@@ -851,20 +851,22 @@ __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
         const int32_t k = k0 + lr;
         int32_t row, seg, pst;
         if constexpr (RANKED) {
+            // everything by rank: no load waits on the row id
             row = A.order[k];
-            seg = A.offsets[row];
+            seg = U.rank_seg[k];
             pst = U.pref[k];
             s_pst[lr] = pst;
             s_end[lr] = seg + (U.pref[k + 1] - pst);
+            s_pos[lr] = ld2(A.table, k);   // the position of order[k]
         } else {
             row = U.nz_rows ? U.nz_rows[k] : k;
             seg = U.nz_off[k];
             pst = seg;
+            s_pos[lr] = ld2(A.pos, row);
         }
         s_seg[lr] = seg;
         s_off[lr] = static_cast<uint8_t>(max<int64_t>(pst, e0) - e0);
         s_rowid[lr] = row;
-        s_pos[lr] = ld2(A.pos, row);
         uint64_t first = 0;
         if (windowed) {
             const uint64_t rk = RANKED ? static_cast<uint64_t>(k)
@@ -1207,23 +1209,6 @@ __global__ void unit_adam_ranked_kernel(int64_t h, const float *pos, float *pos_
     reinterpret_cast<float2 *>(pos_out)[i] = out;
 }
 
-// Ranks holding each unit's first and last slot, for the rank-ordered
-// sweep: rank r covers slots [pref[r], pref[r+1]) (every row has an edge, so
-// each slot has exactly one rank and every entry is written once).
-__global__ void rank_units_kernel(const int32_t *pref, int64_t n, int64_t n_edges,
-                                  int32_t *unit_k) {
-    const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    const int64_t b = pref[r], en = pref[r + 1];
-    const int64_t last_unit = (n_edges - 1) / kUnitEdges;
-    for (int64_t u = b / kUnitEdges; u <= (en - 1) / kUnitEdges; ++u) {
-        const int64_t first = u * kUnitEdges;
-        const int64_t last = u == last_unit ? n_edges - 1 : first + kUnitEdges - 1;
-        if (first >= b && first < en) unit_k[2 * u] = static_cast<int32_t>(r);
-        if (last >= b && last < en) unit_k[2 * u + 1] = static_cast<int32_t>(r);
-    }
-}
-
 // Exclusive prefix sum of the row degrees in projection order:
 // pref[r] = sum over ranks below r of degree(order[r']), pref[n] = E.
 // Pass 1 scans blocks of kPrefBlock ranks, pass 2 scans the block totals in
@@ -1252,7 +1237,9 @@ __device__ inline int32_t block_exclusive_scan(int32_t x, int32_t *s_warp, int32
 __global__ __launch_bounds__(kPrefThreads) void rank_prefix_kernel(const int32_t *order,
                                                                    const int32_t *offsets,
                                                                    int64_t n, int32_t *pref,
-                                                                   int32_t *block_sums) {
+                                                                   int32_t *block_sums,
+                                                                   int32_t *rank_seg,
+                                                                   int32_t *rank_deg) {
     __shared__ int32_t s_warp[kPrefThreads / 64];
     const int64_t base = static_cast<int64_t>(blockIdx.x) * kPrefBlock + threadIdx.x * kPrefPer;
     int32_t deg[kPrefPer], sum = 0;
@@ -1261,7 +1248,10 @@ __global__ __launch_bounds__(kPrefThreads) void rank_prefix_kernel(const int32_t
         deg[q] = 0;
         if (base + q < n) {
             const int32_t row = order[base + q];
-            deg[q] = offsets[row + 1] - offsets[row];
+            const int32_t b = offsets[row];
+            deg[q] = offsets[row + 1] - b;
+            rank_seg[base + q] = b;
+            rank_deg[base + q] = deg[q];
         }
         sum += deg[q];
     }
@@ -1289,12 +1279,27 @@ __global__ __launch_bounds__(kPrefThreads) void prefix_blocks_kernel(int32_t *bl
     }
 }
 
-__global__ __launch_bounds__(kPrefThreads) void prefix_add_kernel(int32_t *pref, int64_t n,
-                                                                  const int32_t *block_sums,
-                                                                  int32_t total_edges) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * kPrefThreads + threadIdx.x;
-    if (i < n) pref[i] += block_sums[i / kPrefBlock];
-    if (i == 0) pref[n] = total_edges;
+// Adds the block offsets and scatters each rank into the units whose first
+// or last slot it holds (rank r covers slots [pref[r], pref[r] + deg)).
+__global__ __launch_bounds__(kPrefThreads) void prefix_units_kernel(int32_t *pref, int64_t n,
+                                                                    const int32_t *block_sums,
+                                                                    const int32_t *rank_deg,
+                                                                    int32_t total_edges,
+                                                                    int32_t *unit_k) {
+    const int64_t r = static_cast<int64_t>(blockIdx.x) * kPrefThreads + threadIdx.x;
+    if (r == 0) pref[n] = total_edges;
+    if (r >= n) return;
+    const int64_t b = pref[r] + block_sums[r / kPrefBlock];
+    pref[r] = static_cast<int32_t>(b);
+    const int64_t en = b + rank_deg[r];
+    if (en == b) return;
+    const int64_t last_unit = (static_cast<int64_t>(total_edges) - 1) / kUnitEdges;
+    for (int64_t u = b / kUnitEdges; u <= (en - 1) / kUnitEdges; ++u) {
+        const int64_t first = u * kUnitEdges;
+        const int64_t last = u == last_unit ? total_edges - 1 : first + kUnitEdges - 1;
+        if (first >= b && first < en) unit_k[2 * u] = static_cast<int32_t>(r);
+        if (last >= b && last < en) unit_k[2 * u + 1] = static_cast<int32_t>(r);
+    }
 }
 
 // ------------------------------------------------ FP64 cross-entropy objective
@@ -1910,17 +1915,16 @@ int64_t rank_prefix_blocks(int64_t n) { return (n + kPrefBlock - 1) / kPrefBlock
 
 void launch_rank_prefix(const int32_t *order, const int32_t *offsets, int64_t n,
                         int32_t total_edges, int32_t *pref, int32_t *block_sums,
-                        int32_t *unit_k, hipStream_t s) {
+                        int32_t *rank_seg, int32_t *rank_deg, int32_t *unit_k, hipStream_t s) {
     if (n == 0 || total_edges == 0) return;
     const int64_t nb = rank_prefix_blocks(n);
     hipLaunchKernelGGL(rank_prefix_kernel, dim3(static_cast<unsigned>(nb)), dim3(kPrefThreads), 0,
-                       s, order, offsets, n, pref, block_sums);
+                       s, order, offsets, n, pref, block_sums, rank_seg, rank_deg);
     hipLaunchKernelGGL(prefix_blocks_kernel, dim3(1), dim3(kPrefThreads), 0, s, block_sums, nb);
-    hipLaunchKernelGGL(prefix_add_kernel, dim3(static_cast<unsigned>((n + kPrefThreads - 1) /
-                                                                     kPrefThreads)),
-                       dim3(kPrefThreads), 0, s, pref, n, block_sums, total_edges);
-    hipLaunchKernelGGL(rank_units_kernel, dim3(grid_for(n)), dim3(kThreads), 0, s, pref, n,
-                       static_cast<int64_t>(total_edges), unit_k);
+    hipLaunchKernelGGL(prefix_units_kernel, dim3(static_cast<unsigned>((n + kPrefThreads - 1) /
+                                                                       kPrefThreads)),
+                       dim3(kPrefThreads), 0, s, pref, n, block_sums, rank_deg, total_edges,
+                       unit_k);
     HIP_CHECK(hipGetLastError());
 }
 

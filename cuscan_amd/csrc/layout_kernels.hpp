@@ -113,6 +113,7 @@ struct UnitSweepArgs {
     const int32_t *nz_off;     // CSR start of each nonzero-degree row, then E
     float *grad, *frag;
     const int32_t *pref;       // rank-ordered sweep: [n + 1] sequence start of each rank, or null
+    const int32_t *rank_seg;   // rank-ordered sweep: [n] CSR start of order[r]
     int64_t units, n_edges;
     int64_t xcd_chunk;         // units per XCD slice (set by the launcher)
     int32_t row_cap;           // unit_narrow_rows() when no unit holds more rows, else unit_edges()
@@ -128,12 +129,13 @@ void launch_unit_pack(const double *npos, const double *nneg, const double *peri
                       uint16_t *fire, hipStream_t s);
 void launch_unit_unpack(const double *rec, int64_t E, double *npos, double *nneg, hipStream_t s);
 // Rank-ordered sweep helpers: pref[r] = edges of the ranks below r in the
-// order (block_sums: rank_prefix_blocks(n) ints) and unit_k [units][2], the
-// ranks holding each unit's first and last slot; and the matching Adam pass.
+// order (block_sums: rank_prefix_blocks(n) ints), rank_seg / rank_deg [n] the
+// CSR start and degree of order[r], and unit_k [units][2], the ranks holding
+// each unit's first and last slot; and the matching Adam pass.
 int64_t rank_prefix_blocks(int64_t n);
 void launch_rank_prefix(const int32_t *order, const int32_t *offsets, int64_t n,
                         int32_t total_edges, int32_t *pref, int32_t *block_sums,
-                        int32_t *unit_k, hipStream_t s);
+                        int32_t *rank_seg, int32_t *rank_deg, int32_t *unit_k, hipStream_t s);
 void launch_unit_adam_ranked(int64_t h, const float *pos, float *pos_out, const float *grad,
                              const int32_t *ranks, const int32_t *pref, const float *frag,
                              float *m, float *v, float alpha, float c1, float c2, hipStream_t s);
