@@ -308,9 +308,12 @@ struct umaphip_plan {
         if (shared && d == 2) {
             // projection, its first histogram and the four sort passes; the
             // last pass writes the ranks and the table
+            // (rank-ordered sweeps: also each rank's CSR start and degree)
             launch_project_sort_ranked(sort_ws.ptr, sort_ws.count, head_now, n, dir, status.ptr, tag,
                                        keys.ptr, keys_sorted.ptr, proj_order.ptr, ranks.ptr,
-                                       table.ptr, stream);
+                                       table.ptr, stream, ranked ? offsets.ptr : nullptr,
+                                       ranked ? rank_seg.ptr : nullptr,
+                                       ranked ? rank_deg.ptr : nullptr);
             return;
         }
         launch_project_keys(reference, n, d, dir, keys.ptr, nullptr, status.ptr, tag, stream);
@@ -470,9 +473,8 @@ void umaphip_plan::run() {
                 const bool by_rank = ranked && projection_ready;
                 U.row_cap = by_rank ? row_cap_ranked : row_cap_id;
                 if (by_rank) {
-                    launch_rank_prefix(order_now, offsets.ptr, n, static_cast<int32_t>(E),
-                                       pref.ptr, pref_blocks.ptr, rank_seg.ptr, rank_deg.ptr,
-                                       rank_unit_k.ptr, stream);
+                    launch_rank_prefix(rank_deg.ptr, n, static_cast<int32_t>(E), pref.ptr,
+                                       pref_blocks.ptr, rank_unit_k.ptr, stream);
                     U.pref = pref.ptr;
                     U.rank_seg = rank_seg.ptr;
                     U.unit_k = rank_unit_k.ptr;

@@ -1234,25 +1234,15 @@ __device__ inline int32_t block_exclusive_scan(int32_t x, int32_t *s_warp, int32
     return before + incl - x;
 }
 
-__global__ __launch_bounds__(kPrefThreads) void rank_prefix_kernel(const int32_t *order,
-                                                                   const int32_t *offsets,
+__global__ __launch_bounds__(kPrefThreads) void rank_prefix_kernel(const int32_t *rank_deg,
                                                                    int64_t n, int32_t *pref,
-                                                                   int32_t *block_sums,
-                                                                   int32_t *rank_seg,
-                                                                   int32_t *rank_deg) {
+                                                                   int32_t *block_sums) {
     __shared__ int32_t s_warp[kPrefThreads / 64];
     const int64_t base = static_cast<int64_t>(blockIdx.x) * kPrefBlock + threadIdx.x * kPrefPer;
     int32_t deg[kPrefPer], sum = 0;
 #pragma unroll
     for (int q = 0; q < kPrefPer; ++q) {
-        deg[q] = 0;
-        if (base + q < n) {
-            const int32_t row = order[base + q];
-            const int32_t b = offsets[row];
-            deg[q] = offsets[row + 1] - b;
-            rank_seg[base + q] = b;
-            rank_deg[base + q] = deg[q];
-        }
+        deg[q] = base + q < n ? rank_deg[base + q] : 0;
         sum += deg[q];
     }
     int32_t total;
@@ -1459,7 +1449,8 @@ __global__ __launch_bounds__(kSortThreads) void radix_scan_kernel(uint32_t *hist
 __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     const uint32_t *keys_in, const int32_t *vals_in, uint32_t *keys_out, int32_t *vals_out,
     int64_t n, int shift, int64_t tiles, const uint32_t *offsets, int32_t *ranks_out,
-    const float *table_src, float *table_out, uint32_t *clear_totals) {
+    const float *table_src, float *table_out, uint32_t *clear_totals, const int32_t *csr_off,
+    int32_t *seg_out, int32_t *deg_out) {
     constexpr int WAVES = kSortThreads / 64;
     constexpr int PER_WAVE = kSortTile / WAVES;
     __shared__ uint32_t s_off[256];             // output position of the tile's run of each digit
@@ -1542,6 +1533,12 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
             if (table_out)
                 reinterpret_cast<float2 *>(table_out)[pos] =
                     reinterpret_cast<const float2 *>(table_src)[val];
+            if (seg_out) {
+                // CSR start and degree of the row at this rank
+                const int32_t b = csr_off[val];
+                seg_out[pos] = b;
+                deg_out[pos] = csr_off[val + 1] - b;
+            }
         }
     }
 }
@@ -1913,13 +1910,12 @@ void launch_unit_sweep(UnitSweepArgs a, hipStream_t s) {
 
 int64_t rank_prefix_blocks(int64_t n) { return (n + kPrefBlock - 1) / kPrefBlock; }
 
-void launch_rank_prefix(const int32_t *order, const int32_t *offsets, int64_t n,
-                        int32_t total_edges, int32_t *pref, int32_t *block_sums,
-                        int32_t *rank_seg, int32_t *rank_deg, int32_t *unit_k, hipStream_t s) {
+void launch_rank_prefix(const int32_t *rank_deg, int64_t n, int32_t total_edges, int32_t *pref,
+                        int32_t *block_sums, int32_t *unit_k, hipStream_t s) {
     if (n == 0 || total_edges == 0) return;
     const int64_t nb = rank_prefix_blocks(n);
     hipLaunchKernelGGL(rank_prefix_kernel, dim3(static_cast<unsigned>(nb)), dim3(kPrefThreads), 0,
-                       s, order, offsets, n, pref, block_sums, rank_seg, rank_deg);
+                       s, rank_deg, n, pref, block_sums);
     hipLaunchKernelGGL(prefix_blocks_kernel, dim3(1), dim3(kPrefThreads), 0, s, block_sums, nb);
     hipLaunchKernelGGL(prefix_units_kernel, dim3(static_cast<unsigned>((n + kPrefThreads - 1) /
                                                                        kPrefThreads)),
@@ -2090,7 +2086,9 @@ SortWorkspace sort_workspace(void *ws, size_t ws_bytes, int64_t n) {
 // in -> tmp -> out -> tmp -> out; the last pass clears the totals.
 void sort_passes(const SortWorkspace &W, bool hist0_done, const uint32_t *keys_in,
                  uint32_t *keys_out, const int32_t *vals_in, int32_t *vals_out, int64_t n,
-                 int32_t *ranks, const float *table_src, float *table_out, hipStream_t s) {
+                 int32_t *ranks, const float *table_src, float *table_out, hipStream_t s,
+                 const int32_t *csr_off = nullptr, int32_t *seg_out = nullptr,
+                 int32_t *deg_out = nullptr) {
     const unsigned tiles = static_cast<unsigned>(W.tiles);
     const uint32_t *src_k = keys_in;
     const int32_t *src_v = vals_in;
@@ -2107,7 +2105,8 @@ void sort_passes(const SortWorkspace &W, bool hist0_done, const uint32_t *keys_i
         hipLaunchKernelGGL(radix_scatter_kernel, dim3(tiles), dim3(kSortThreads), 0, s, src_k, src_v,
                            dst_k, dst_v, n, shift, W.tiles, W.hist, last ? ranks : nullptr,
                            last ? table_src : nullptr, last ? table_out : nullptr,
-                           last ? W.totals : nullptr);
+                           last ? W.totals : nullptr, last ? csr_off : nullptr,
+                           last ? seg_out : nullptr, last ? deg_out : nullptr);
         src_k = dst_k;
         src_v = dst_v;
     }
@@ -2137,13 +2136,15 @@ void clear_sort_workspace(void *ws, size_t ws_bytes, int64_t n, hipStream_t s) {
 void launch_project_sort_ranked(void *ws, size_t ws_bytes, const float *x, int64_t n,
                                 const float *dir, unsigned long long *status,
                                 unsigned long long tag, uint32_t *keys, uint32_t *keys_out,
-                                int32_t *vals_out, int32_t *ranks, float *table_out, hipStream_t s) {
+                                int32_t *vals_out, int32_t *ranks, float *table_out, hipStream_t s,
+                                const int32_t *csr_off, int32_t *seg_out, int32_t *deg_out) {
     if (n == 0) return;
     const SortWorkspace W = sort_workspace(ws, ws_bytes, n);
     hipLaunchKernelGGL(project_keys_hist_kernel, dim3(static_cast<unsigned>(W.tiles)),
                        dim3(kSortThreads), 0, s, x, n, dir, keys, status, tag, W.tiles, W.hist,
                        W.totals);
-    sort_passes(W, true, keys, keys_out, nullptr, vals_out, n, ranks, x, table_out, s);
+    sort_passes(W, true, keys, keys_out, nullptr, vals_out, n, ranks, x, table_out, s, csr_off,
+                seg_out, deg_out);
 }
 
 void launch_sort_pairs(void *ws, size_t ws_bytes, const uint32_t *keys_in, uint32_t *keys_out,

@@ -128,14 +128,14 @@ void launch_unit_pack(const double *npos, const double *nneg, const double *peri
                       const int32_t *indices, int64_t E, uint64_t modulus, double *rec,
                       uint16_t *fire, hipStream_t s);
 void launch_unit_unpack(const double *rec, int64_t E, double *npos, double *nneg, hipStream_t s);
-// Rank-ordered sweep helpers: pref[r] = edges of the ranks below r in the
-// order (block_sums: rank_prefix_blocks(n) ints), rank_seg / rank_deg [n] the
-// CSR start and degree of order[r], and unit_k [units][2], the ranks holding
-// each unit's first and last slot; and the matching Adam pass.
+// Rank-ordered sweep helpers: from rank_deg [n] (the degree of order[r],
+// written with rank_seg by the projection sort's last pass), pref[r] = edges
+// of the ranks below r (block_sums: rank_prefix_blocks(n) ints) and unit_k
+// [units][2], the ranks holding each unit's first and last slot; and the
+// matching Adam pass.
 int64_t rank_prefix_blocks(int64_t n);
-void launch_rank_prefix(const int32_t *order, const int32_t *offsets, int64_t n,
-                        int32_t total_edges, int32_t *pref, int32_t *block_sums,
-                        int32_t *rank_seg, int32_t *rank_deg, int32_t *unit_k, hipStream_t s);
+void launch_rank_prefix(const int32_t *rank_deg, int64_t n, int32_t total_edges, int32_t *pref,
+                        int32_t *block_sums, int32_t *unit_k, hipStream_t s);
 void launch_unit_adam_ranked(int64_t h, const float *pos, float *pos_out, const float *grad,
                              const int32_t *ranks, const int32_t *pref, const float *frag,
                              float *m, float *v, float alpha, float c1, float c2, hipStream_t s);
@@ -183,10 +183,14 @@ void clear_sort_workspace(void *ws, size_t ws_bytes, int64_t n, hipStream_t s);
 // Fit refresh for d = 2: keys of x . dir (non-finite rows tracked in status as
 // launch_project_keys does), then the ranked sort of (key, row) with
 // ranks[row] = position and table_out[position] = x[row].
+// With csr_off (the CSR offsets of the n rows), the last pass also writes
+// seg_out[position] / deg_out[position], the CSR start and degree of the row.
 void launch_project_sort_ranked(void *ws, size_t ws_bytes, const float *x, int64_t n,
                                 const float *dir, unsigned long long *status,
                                 unsigned long long tag, uint32_t *keys, uint32_t *keys_out,
-                                int32_t *vals_out, int32_t *ranks, float *table_out, hipStream_t s);
+                                int32_t *vals_out, int32_t *ranks, float *table_out, hipStream_t s,
+                                const int32_t *csr_off = nullptr, int32_t *seg_out = nullptr,
+                                int32_t *deg_out = nullptr);
 void launch_values_from_keys(const uint32_t *keys, int64_t n, float *values, hipStream_t s);
 void launch_inverse_ranks(const int32_t *order, int64_t n, int32_t *ranks, hipStream_t s);
 void launch_lower_bound(const float *sorted, int64_t n, const float *query, int64_t h,
