@@ -148,7 +148,8 @@ def recursive_benchmark(x, compact, a, b, args):
     secs = time.perf_counter() - t0
     res = {"seconds": round(secs, 2), "host_ms": round(st["host_ms"], 1),
            "coarse_layout_ms": round(st["layout_ms"], 1), "level_rows": st["rows"],
-           "level_edges": st["edges"], "propagation_sweeps": st["sweeps"]}
+           "level_edges": st["edges"], "propagation_sweeps": st["sweeps"],
+           "host_stage_ms": st["stage_ms"]}
     log(f"recursive init {secs:.2f}s (host {st['host_ms']:.0f} ms, coarse layouts "
         f"{st['layout_ms']:.0f} ms), levels {st['rows']}")
     if args.quality:

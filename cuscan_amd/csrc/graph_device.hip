@@ -63,6 +63,38 @@ __global__ __launch_bounds__(kSegThreads) void tg_mean_kernel(const TGraphSegmen
     }
 }
 
+// The fit's mean distance over [0, m) in two deterministic levels: block b
+// sums the b-th contiguous chunk in f64 (fixed tree), then one workgroup adds
+// the block sums in order and writes the state.
+constexpr int kMeanBlocks = 1024;
+
+__global__ __launch_bounds__(kSegThreads) void fg_mean_partial_kernel(const float *dist, int64_t m,
+                                                                      double *partial) {
+    __shared__ double s_sum[kSegThreads];
+    const int tid = threadIdx.x;
+    const int64_t chunk = (m + kMeanBlocks - 1) / kMeanBlocks;
+    const int64_t b = static_cast<int64_t>(blockIdx.x) * chunk, e = min<int64_t>(m, b + chunk);
+    double acc = 0.0;
+    for (int64_t i = b + tid; i < e; i += kSegThreads) acc += static_cast<double>(dist[i]);
+    s_sum[tid] = acc;
+    __syncthreads();
+    for (int w = kSegThreads / 2; w > 0; w >>= 1) {
+        if (tid < w) s_sum[tid] = s_sum[tid] + s_sum[tid + w];
+        __syncthreads();
+    }
+    if (tid == 0) partial[blockIdx.x] = s_sum[0];
+}
+
+__global__ void fg_mean_final_kernel(const double *partial, int64_t m, TGraphState *state) {
+    if (threadIdx.x != 0) return;
+    double sum = 0.0;
+    for (int b = 0; b < kMeanBlocks; ++b) sum += partial[b];
+    state->mean = static_cast<float>(sum / static_cast<double>(m));
+    state->wmax_bits = 0u;
+    state->edges = 0;
+    state->errors = 0;
+}
+
 // smooth_row (graph.rs:117-242) on one row's k distances.
 __device__ inline void smooth_row(const float *dist, int k, float global_mean,
                                   const TGraphParams &P, float &rho_out, float &sigma_out) {
@@ -654,18 +686,16 @@ umaphip_status umaphip_fit_graph(const int32_t *indices, const float *distances,
         TGraphParams P = transform_graph_params(k, local_connectivity);
         P.target = std::log2(static_cast<float>(k)) * bandwidth;
         DeviceArray<TGraphState> state(1);
-        DeviceArray<TGraphSegment> seg(1);
-        TGraphSegment S{};
-        S.row0 = 0;
-        S.rows = n;
-        S.state = state.ptr;
-        HIP_CHECK(hipMemcpyAsync(seg.ptr, &S, sizeof(S), hipMemcpyHostToDevice, s));
         DeviceArray<float> directed(m), emit_w(m);
         DeviceArray<uint32_t> keep(m), pos(m);
         const int64_t blocks = (m + kScanTile - 1) / kScanTile;
         DeviceArray<uint32_t> totals(blocks), grand(1);
         const unsigned gm = static_cast<unsigned>((m + kSegThreads - 1) / kSegThreads);
-        hipLaunchKernelGGL(tg_mean_kernel, dim3(1), dim3(kSegThreads), 0, s, seg.ptr, distances, k);
+        DeviceArray<double> mean_partial(kMeanBlocks);
+        hipLaunchKernelGGL(fg_mean_partial_kernel, dim3(kMeanBlocks), dim3(kSegThreads), 0, s,
+                           distances, m, mean_partial.ptr);
+        hipLaunchKernelGGL(fg_mean_final_kernel, dim3(1), dim3(64), 0, s, mean_partial.ptr, m,
+                           state.ptr);
         hipLaunchKernelGGL(fg_rows_kernel, dim3(static_cast<unsigned>((n + kRowThreads - 1) / kRowThreads)),
                            dim3(kRowThreads), 0, s, indices, distances, n, k, P, state.ptr,
                            directed.ptr, &state.ptr->errors);

@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "host_parallel.hpp"
 #include "layout_host.hpp"
 #include "layout_kernels.hpp"
 #include "stdrng.hpp"
@@ -711,8 +712,11 @@ umaphip_status umaphip_plan_create(int64_t h, int64_t n, int64_t d, const float 
                        params->epochs < 65535;
         if (unit_ok && params->negative_sample_rate > 0) {
             const double rate = static_cast<double>(params->negative_sample_rate);
-            for (int64_t e = 0; e < E && unit_ok; ++e)
-                unit_ok = negative_periods[e] == (periods[e] / rate) * 1.5;
+            std::atomic<int> differs{0};
+            parallel_for(E, [&](int64_t e) {
+                if (negative_periods[e] != (periods[e] / rate) * 1.5) differs.store(1);
+            }, 1 << 16);
+            unit_ok = !differs.load();
         }
         if (unit_ok) {
             UnitTables T = build_unit_tables(offsets, h, E, unit_edges());

@@ -1081,15 +1081,15 @@ void procrustes_align(float *layout, int64_t n, const Anchor &A) {
 
 // ------------------------------------------------------------ recursion
 
-// edge_periods (crates/umap-core/src/modern_layout.rs:188-208).
-void edge_periods(float w, float wmax, int64_t epochs, float rate, double &period,
+// edge_periods (crates/umap-core/src/modern_layout.rs:188-208); false for an
+// invalid period (called from worker threads, so it does not throw).
+bool edge_periods(float w, float wmax, int64_t epochs, float rate, double &period,
                   double &negative) {
     const float samples = static_cast<float>(epochs) * (w / wmax);
     period = static_cast<double>(epochs) / static_cast<double>(samples);
     negative = rate == 0.0f ? INFINITY : (period / static_cast<double>(rate)) * 1.5;
-    UMAPHIP_REQUIRE(std::isfinite(period) && period > 0.0 &&
-                        (rate <= 0.0f || (std::isfinite(negative) && negative > 0.0)),
-                    UMAPHIP_NUMERICAL, "invalid layout period");
+    return std::isfinite(period) && period > 0.0 &&
+           (rate <= 0.0f || (std::isfinite(negative) && negative > 0.0));
 }
 
 struct Run {
@@ -1111,14 +1111,18 @@ struct Run {
                         "recursive coarse graph too large");
         std::vector<uint32_t> offsets(static_cast<size_t>(rows + 1)), indices(static_cast<size_t>(E));
         std::vector<double> periods(static_cast<size_t>(E)), negative(static_cast<size_t>(E));
+        std::vector<double> next_pos(static_cast<size_t>(E)), next_neg(static_cast<size_t>(E));
         for (int64_t r = 0; r <= rows; ++r) offsets[r] = static_cast<uint32_t>(g.ptr[r]);
         float wmax = 0.0f;
         for (float w : g.w) wmax = std::max(wmax, w);
-        for (int64_t e = 0; e < E; ++e) {
+        std::atomic<int> bad{0};
+        parallel_for(E, [&](int64_t e) {
             indices[e] = static_cast<uint32_t>(g.idx[e]);
-            edge_periods(g.w[e], wmax, L.epochs, 1.0f, periods[e], negative[e]);
-        }
-        std::vector<double> next_pos(periods), next_neg(negative);
+            if (!edge_periods(g.w[e], wmax, L.epochs, 1.0f, periods[e], negative[e])) bad.store(1);
+            next_pos[e] = periods[e];
+            next_neg[e] = negative[e];
+        }, 1 << 15);
+        UMAPHIP_REQUIRE(!bad.load(), UMAPHIP_NUMERICAL, "invalid layout period");
         UMAPHIP_REQUIRE(L.epochs <= INT32_MAX, UMAPHIP_OVERFLOW, "recursive layout epochs");
         umaphip_layout_params lp{};
         lp.optimizer = 0;
