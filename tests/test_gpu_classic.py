@@ -4,7 +4,7 @@ The contract is embedding quality (crates/umap/src/kernels/standard_umap/
 layout.rs:42-47): coordinates come from concurrent atomic updates. What is
 deterministic is pinned exactly: the f64 clocks and the per-head Tausworthe
 states after every epoch (umap-learn's recorded frames, and the oracle's
-whole runs). Whole runs are judged by trustworthiness@15 within 0.01 of the
+whole runs). Whole runs are checked by trustworthiness@15 within 0.01 of the
 oracle's serial run on the same graph and initialisation.
 """
 import numpy as np

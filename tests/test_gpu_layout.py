@@ -296,14 +296,18 @@ DRIVER = [
     # spanning units, exclusion, and the per-edge path of a rate of 24
     ("fit_adam_windows_hubs_ranked", 3000, 3000, 2, True, "adam", 700, True, 16),
     ("fit_adam_windows_rate24_hubs_ranked", 3000, 3000, 2, True, "adam", 900, True, 10),
+    # rows of degree < 5: the 128-row unit table, in rank order and in CSR order
+    ("fit_adam_windows_sparse_ranked", 3000, 3000, 2, True, "adam", 600, True, 14),
+    ("fit_adam_global_sparse", 3000, 3000, 2, True, "adam", 200_000, False, 14),
 ]
 # cases that must run the edge-balanced unit sweep
 UNIT_PATH = {"fit_adam_global_excl", "fit_adam_windows_adapt", "fit_adam_global_hubs",
              "fit_adam_windows_hubs_excl", "fit_adam_global_rate24_hubs",
-             "fit_adam_windows_hubs_ranked", "fit_adam_windows_rate24_hubs_ranked"}
+             "fit_adam_windows_hubs_ranked", "fit_adam_windows_rate24_hubs_ranked",
+             "fit_adam_windows_sparse_ranked", "fit_adam_global_sparse"}
 # ... and, every epoch, with the rows in projection order
 RANKED_PATH = {"fit_adam_windows_adapt", "fit_adam_windows_hubs_ranked",
-               "fit_adam_windows_rate24_hubs_ranked"}
+               "fit_adam_windows_rate24_hubs_ranked", "fit_adam_windows_sparse_ranked"}
 
 
 @pytest.mark.parametrize("case", DRIVER, ids=[c[0] for c in DRIVER])
@@ -316,7 +320,7 @@ def test_driver_matches_oracle_loop(case):
     coordinate difference can swap two neighbours of the sorted projection
     and change later negatives (the reference's own CPU and GPU backends
     differ the same way), so coordinates are compared over the first three
-    epochs (1e-5) and full runs are judged by embedding quality elsewhere.
+    epochs (1e-5) and full runs are checked by embedding quality elsewhere.
     """
     from cuscan_amd import layout
 
@@ -328,7 +332,8 @@ def test_driver_matches_oracle_loop(case):
         isolated = () if "ranked" in name else (3, 4, 2999)
         g = hub_graph(h, 6, 11, epochs, rate=float(rate), isolated=isolated)
     else:
-        g = random_graph(h, n, 6, 11, epochs, symmetric=shared)
+        k = 1 if "sparse" in name else 6
+        g = random_graph(h, n, k, 11, epochs, symmetric=shared)
     head = rng.uniform(-10, 10, (h, d)).astype(np.float32)
     tail = None if shared else rng.uniform(-10, 10, (n, d)).astype(np.float32)
     print(f"[gpu] driver case={name}", flush=True)

@@ -1,5 +1,5 @@
 """End-to-end UMAP fit/transform on the device against the CPU reference
-pipeline (oracle): full stochastic runs are judged by trustworthiness@15
+pipeline (oracle): full stochastic runs are checked by trustworthiness@15
 within 0.01 (the reference's acceptance margin for whole runs)."""
 import numpy as np
 import pytest
