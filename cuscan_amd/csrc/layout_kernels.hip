@@ -747,11 +747,11 @@ __global__ __launch_bounds__(THREADS) void transform_layout_kernel(const Transfo
 // plan takes this path only when every stored negative period equals it.
 // The firing shadow holds min(max(ceil(next_positive), 0), 65535): for an
 // integer epoch t, next_positive <= t exactly when the shadow is <= t.
-constexpr int kUnitEPL = 2;
+constexpr int kUnitEPL = 4;
 constexpr int kUnitEdges = 64 * kUnitEPL;
 constexpr int kUnitTermCap = kUnitEdges * 6;
 constexpr int kUnitGroups = 4;
-constexpr int kUnitNarrowRows = 32;
+constexpr int kUnitNarrowRows = 64;
 
 __device__ inline uint16_t fire_shadow(double next_positive) {
     const double c = ceil(next_positive);
@@ -784,14 +784,14 @@ __device__ inline void segmented_accumulate(float2 term, int row, int lane, floa
 // covers sequence slots [pref[r], pref[r+1])), so the heads a wave, and an
 // XCD, sweep together have neighbouring ranks and their negative windows
 // overlap in L2. Requires every row to have an edge.
-// RC: capacity of the unit row table; the plan picks 32 when no unit can
-// hold more rows, which keeps the workgroup under 4 KB of LDS (10 waves per
-// SIMD, the register limit) instead of about 8 KB (5 waves).
+// RC: capacity of the unit row table; the plan picks 64 when no unit can
+// hold more rows, which keeps the workgroup at about 7 KB of LDS (6 waves
+// per SIMD) instead of about 14 KB (3 waves).
 template <bool RANKED, int RC>
 __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
     const SweepArgs &A = U.s;
     constexpr int EPL = kUnitEPL, UE = kUnitEdges, TCAP = kUnitTermCap, G = kUnitGroups;
-    __shared__ uint8_t s_off[RC + 1];   // unit-local first edge of each unit row
+    __shared__ uint16_t s_off[RC + 1];  // unit-local first edge of each unit row
     __shared__ int32_t s_seg[RC + 1];   // CSR start of each unit row (+ end of the last)
     __shared__ int32_t s_rowid[RC];
     __shared__ float2 s_pos[RC];
@@ -804,7 +804,7 @@ __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
     __shared__ uint32_t s_em[UE];
     __shared__ int32_t s_pst[RANKED ? RC + 1 : 1];  // sequence start of each unit row
     __shared__ int32_t s_end[RANKED ? RC + 1 : 1];  // CSR end of each unit row
-    static_assert(UE < 256 && RC <= UE, "edge slots and unit rows are indexed by uint8");
+    static_assert(UE <= 256 && RC <= UE, "edge slots and unit rows are indexed by uint8");
     static_assert(TCAP < 65536, "term bases are 16-bit");
 
     const int lane = threadIdx.x;
@@ -865,7 +865,7 @@ __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
             s_pos[lr] = ld2(A.pos, row);
         }
         s_seg[lr] = seg;
-        s_off[lr] = static_cast<uint8_t>(max<int64_t>(pst, e0) - e0);
+        s_off[lr] = static_cast<uint16_t>(max<int64_t>(pst, e0) - e0);
         s_rowid[lr] = row;
         uint64_t first = 0;
         if (windowed) {
@@ -879,7 +879,7 @@ __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
         s_acc[lr] = make_float2(0.0f, 0.0f);
     }
     if (lane == 0) {
-        s_off[nr] = static_cast<uint8_t>(ue);
+        s_off[nr] = static_cast<uint16_t>(ue);
         if constexpr (!RANKED) s_seg[nr] = U.nz_off[k1 + 1];
     }
     const float scale = A.scale_ptr ? *A.scale_ptr : A.scale;
