@@ -406,12 +406,23 @@ def cpu_baseline(compact, init, a, b, epochs, sample_epochs, terms):
     mean_terms = sum(terms) / len(terms)
     ratio = mean_terms / sample_terms
     raw = E * sample_epochs / secs
+    # one host thread on the first few epochs, scaled the same way
+    one_epochs = 4
+    t0 = time.time()
+    O.optimize_compact(init, None, compact, epochs=epochs, a=a, b=b, negative_selection_range=200_000,
+                       seed=42, threads=1, stop_epoch=one_epochs, collect_stats=False)
+    one_secs = time.time() - t0
+    one_ratio = mean_terms / (sum(terms[:one_epochs]) / one_epochs)
+    one = E * one_epochs / one_secs / one_ratio
     return {"value": raw / ratio, "unit": "edge-epochs/s", "cores": threads, "kind": "port",
+            "one_thread_value": one,
             "sample": f"oracle C restatement (OpenMP, {threads} threads, {cpu_model()}, "
                       f"nproc {os.cpu_count()}) timed on epochs 0..{sample_epochs - 1} of the same "
                       f"200-epoch layout, graph and init ({secs:.1f}s, raw {raw:.3e} edge-epochs/s); "
                       f"extrapolated to the layout average by force terms per epoch "
-                      f"({mean_terms:.4g} average vs {sample_terms:.4g} in the sample)"}
+                      f"({mean_terms:.4g} average vs {sample_terms:.4g} in the sample); "
+                      f"one_thread_value: 1 thread on epochs 0..{one_epochs - 1} ({one_secs:.1f}s), "
+                      f"scaled the same way"}
 
 
 def copy_calibration(gib):
@@ -510,6 +521,9 @@ def main():
                 "kernel": "unit_sweep_kernel + unit_adam_kernel (edge pass + lazy Adam)",
                 "bytes_per_launch": int(bytes_per_launch), "avg_launch_ms": round(sweep_avg_ms, 4),
                 "firing_edges_per_epoch": F / args.epochs, "negatives_per_epoch": S / args.epochs,
+                # B_gather: the 8-byte position gathers (tails and negative table rows),
+                # reported beside the stream bytes and not in the HBM fraction
+                "gather_bytes_per_epoch": int(8 * (F + S) / args.epochs),
                 "refresh_avg_ms": round(pstats["refresh_ms"] / max(pstats["refresh_launches"], 1), 4),
                 "rank_ordered_epochs": pstats["rank_ordered_epochs"]}
     out = {
