@@ -472,6 +472,11 @@ void umaphip_plan::run() {
                 U.rate = static_cast<double>(p.negative_sample_rate);
                 // with this epoch's projection order, sweep the rows by rank
                 const bool by_rank = ranked && projection_ready;
+                // The rank-ordered sweep decides firing from next_positive in
+                // the record and leaves the firing shadow behind; the id-order
+                // sweep reads the shadow, so it cannot follow a ranked epoch.
+                UMAPHIP_REQUIRE(by_rank || ranked_epochs == 0, UMAPHIP_BACKEND,
+                                "unit sweep: id-order epoch after a rank-ordered epoch");
                 U.row_cap = by_rank ? row_cap_ranked : row_cap_id;
                 if (by_rank) {
                     launch_rank_prefix(rank_deg.ptr, n, static_cast<int32_t>(E), pref.ptr,

@@ -338,6 +338,21 @@ __device__ inline float repel_fast(float q, float a, float b, float gamma, float
     return g16 * tanh_pos(c * q * rq * rcpf(gamma)) * rq;
 }
 
+// Either force coefficient with one q^b and one reciprocal, for sweeps whose
+// waves mix attraction and repulsion terms: attraction
+// k_att q^b / (q (1 + a q^b)) with k_att = -2ab; repulsion as repel_fast with
+// k_rep = 2 s gamma b, inv_gamma = 1/gamma and g16 = min(gamma, 16). q > 0.
+__device__ inline float unit_coef(float q, bool att, float a, float b, float k_att, float k_rep,
+                                  float inv_gamma, float g16) {
+    const float pb = fast_pow(q, b);
+    const float r = rcpf((att ? q : 0.001f + q) * (1.0f + a * pb));
+    if (att) return k_att * pb * r;
+    const float c = k_rep * r;
+    if (!(c > 0.0f)) return 0.0f;
+    const float rq = __builtin_amdgcn_rsqf(q);
+    return g16 * tanh_pos(c * q * rq * inv_gamma) * rq;
+}
+
 // x mod m for x < 2^64, m < 2^32, through a double-precision quotient
 // estimate corrected by at most two steps.
 __device__ inline uint64_t mod_u64(uint64_t x, uint64_t m, double inv_m) {
@@ -746,10 +761,17 @@ __global__ __launch_bounds__(THREADS) void transform_layout_kernel(const Transfo
 // period is recomputed as (period / rate) * 1.5, the reference's formula; a
 // plan takes this path only when every stored negative period equals it.
 // The firing shadow holds min(max(ceil(next_positive), 0), 65535): for an
-// integer epoch t, next_positive <= t exactly when the shadow is <= t.
+// integer epoch t, next_positive <= t exactly when the shadow is <= t. The
+// id-order sweep reads it to load only the records of firing edges; the
+// rank-ordered sweep reads every record of its rows (a row's records are
+// contiguous) and tests next_positive <= t itself, which saves the
+// dependent shadow load.
+// Per-term cost is not what bounds these sweeps: each unit waits on a chain
+// of dependent loads (rows, records, gathers), so occupancy (registers and
+// LDS) and the number of round trips set the rate.
 constexpr int kUnitEPL = 4;
 constexpr int kUnitEdges = 64 * kUnitEPL;
-constexpr int kUnitTermCap = kUnitEdges * 6;
+constexpr int kUnitTermCap = kUnitEdges * 4;
 constexpr int kUnitGroups = 4;
 constexpr int kUnitNarrowRows = 64;
 
@@ -759,18 +781,37 @@ __device__ inline uint16_t fire_shadow(double next_positive) {
     return c > 0.0 ? static_cast<uint16_t>(c) : 0;
 }
 
+// One step of a wave-wide segmented scan through DPP: the value of the lane
+// CTRL selects (row_shr:n within 16-lane rows, row_bcast:15 / row_bcast:31
+// across them) for the rows ROWMASK enables, else 0.
+template <int CTRL, int ROWMASK>
+__device__ inline void seg_scan_step(float &sx, float &sy, int rid) {
+    const float ux = __int_as_float(
+        __builtin_amdgcn_update_dpp(0, __float_as_int(sx), CTRL, ROWMASK, 0xf, false));
+    const float uy = __int_as_float(
+        __builtin_amdgcn_update_dpp(0, __float_as_int(sy), CTRL, ROWMASK, 0xf, false));
+    const int ur = __builtin_amdgcn_update_dpp(0, rid, CTRL, ROWMASK, 0xf, false);
+    if (ur == rid) {
+        sx = sx + ux;
+        sy = sy + uy;
+    }
+}
+
 // Adds the segmented (by row) inclusive sums of one 64-term group into the
-// row accumulators; `row` is -1 for lanes without a term.
+// row accumulators; `row` is -1 for lanes without a term. Rows are
+// non-decreasing over the lanes that hold terms, so a partial sum ending at
+// a lower lane belongs to this lane's row exactly when that lane's row is
+// this lane's row. Lanes without a term carry zeros under row id 0; a
+// source lane outside the enabled rows reads as (0, 0, id 0).
 __device__ inline void segmented_accumulate(float2 term, int row, int lane, float2 *s_acc) {
     float sx = term.x, sy = term.y;
-    for (int o = 1; o < 64; o <<= 1) {
-        const float ux = __shfl_up(sx, o, 64), uy = __shfl_up(sy, o, 64);
-        const int urow = __shfl_up(row, o, 64);
-        if (lane >= o && urow == row) {
-            sx = sx + ux;
-            sy = sy + uy;
-        }
-    }
+    const int rid = row + 1;
+    seg_scan_step<0x111, 0xf>(sx, sy, rid);   // row_shr:1
+    seg_scan_step<0x112, 0xf>(sx, sy, rid);   // row_shr:2
+    seg_scan_step<0x114, 0xf>(sx, sy, rid);   // row_shr:4
+    seg_scan_step<0x118, 0xf>(sx, sy, rid);   // row_shr:8
+    seg_scan_step<0x142, 0xa>(sx, sy, rid);   // row_bcast:15 into rows 1 and 3
+    seg_scan_step<0x143, 0xc>(sx, sy, rid);   // row_bcast:31 into rows 2 and 3
     const int next = __shfl_down(row, 1, 64);
     if (row >= 0 && (lane == 63 || next != row)) {
         float2 a = s_acc[row];
@@ -795,7 +836,7 @@ __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
     __shared__ int32_t s_seg[RC + 1];   // CSR start of each unit row (+ end of the last)
     __shared__ int32_t s_rowid[RC];
     __shared__ float2 s_pos[RC];
-    __shared__ uint32_t s_first[RC];
+    __shared__ uint32_t s_first[RANKED ? 1 : RC];   // window start of each unit row
     __shared__ float2 s_acc[RC];
     __shared__ uint8_t s_owner[TCAP];   // edge slot of each term
     __shared__ uint16_t s_base[UE + 1]; // first term of each edge slot (read when total <= TCAP)
@@ -875,15 +916,31 @@ __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
             if (rk > half) first = rk - half;
             if (first > n64 - modulus) first = n64 - modulus;
         }
-        s_first[lr] = static_cast<uint32_t>(first);
+        if constexpr (!RANKED) s_first[lr] = static_cast<uint32_t>(first);
         s_acc[lr] = make_float2(0.0f, 0.0f);
     }
     if (lane == 0) {
         s_off[nr] = static_cast<uint16_t>(ue);
         if constexpr (!RANKED) s_seg[nr] = U.nz_off[k1 + 1];
     }
+    // window start of unit row rr (ranked: from its rank k0 + rr)
+    auto row_first = [&](int rr) -> uint64_t {
+        if constexpr (RANKED) {
+            if (!windowed) return 0;
+            const uint64_t rk = static_cast<uint64_t>(k0 + rr), half = modulus / 2;
+            const uint64_t first = rk > half ? rk - half : 0;
+            return first > n64 - modulus ? n64 - modulus : first;
+        } else {
+            return s_first[rr];
+        }
+    };
     const float scale = A.scale_ptr ? *A.scale_ptr : A.scale;
     const double now = static_cast<double>(epoch);
+    const bool repel_on = A.gamma > 0.0f;
+    const float k_att = -2.0f * A.a * A.b;
+    const float k_rep = scale * 2.0f * A.gamma * A.b;
+    const float inv_gamma = rcpf(A.gamma);
+    const float g16 = A.gamma < 16.0f ? A.gamma : 16.0f;
     __syncthreads();
     if constexpr (RANKED) {
         // slots to CSR edges through the row table, then the records
@@ -899,19 +956,20 @@ __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
                     if (s_off[mid] <= slot) lo = mid;
                     else hi = mid;
                 }
+                s_row[slot] = static_cast<uint8_t>(lo);   // read back by this lane only
                 eidx[k] = s_seg[lo] + (e0 + slot - s_pst[lo]);
-                fire[k] = U.fire[eidx[k]];
             }
         }
 #pragma unroll
         for (int k = 0; k < EPL; ++k) {
             r_npos[k] = 0.0; r_nn[k] = 0.0; r_per[k] = 0.0; r_w[k] = 0;
-            if (fire[k] <= epoch) {
+            if (k * 64 + lane < ue) {
                 const double4 rec = reinterpret_cast<const double4 *>(U.rec)[eidx[k]];
                 r_npos[k] = rec.x;
                 r_nn[k] = rec.y;
                 r_per[k] = rec.z;
                 r_w[k] = static_cast<uint64_t>(__double_as_longlong(rec.w));
+                fire[k] = rec.x <= now ? 0u : 0xffffffffu;
             }
         }
     }
@@ -926,13 +984,17 @@ __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
         int t = 0, r = 0;
         sel0r[k] = 0u;
         if (slot < ue) {
-            int lo = 0, hi = nr;
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (s_off[mid] <= slot) lo = mid;
-                else hi = mid;
+            if constexpr (RANKED) {
+                r = s_row[slot];
+            } else {
+                int lo = 0, hi = nr;
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (s_off[mid] <= slot) lo = mid;
+                    else hi = mid;
+                }
+                r = lo;
             }
-            r = lo;
             if (fire[k] <= epoch) {
                 const int64_t e = eidx[k];
                 const double per = r_per[k], nn = r_nn[k];
@@ -945,7 +1007,7 @@ __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
                     nn2 = nn + static_cast<double>(count) * np;
                 }
                 reinterpret_cast<double2 *>(U.rec)[2 * e] = make_double2(npos2, nn2);
-                U.fire[e] = fire_shadow(npos2);
+                if constexpr (!RANKED) U.fire[e] = fire_shadow(npos2);
                 const int negs = count > 0 ? count : 0;
                 t = 1 + negs;
                 const uint32_t em = static_cast<uint32_t>(r_w[k] >> 32);
@@ -1005,7 +1067,7 @@ __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
                     sel = mod_u64(sel, modulus, U.inv_mod);
                 }
                 const int rr = s_row[owner];
-                const uint64_t rank = static_cast<uint64_t>(s_first[rr]) + sel;
+                const uint64_t rank = row_first(rr) + sel;
                 if (A.exclude) {
                     const int64_t jj = static_cast<uint32_t>(A.order[rank]);
                     keep[g] = !excluded(A.indices, s_seg[rr], RANKED ? s_end[rr] : s_seg[rr + 1],
@@ -1029,13 +1091,9 @@ __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
                     const float2 xi = s_pos[row];
                     const float dx = xi.x - y[g].x, dy = xi.y - y[g].y;
                     const float q = dx * dx + dy * dy;
-                    if (kq == 0) {
-                        if (q > 0.0f) {
-                            const float c = attract_fast(q, A.a, A.b, 1.0f);
-                            term = make_float2(c * dx, c * dy);
-                        }
-                    } else if (keep[g]) {
-                        const float c = repel_fast(q, A.a, A.b, A.gamma, scale);
+                    const bool att = kq == 0;
+                    if (q > 0.0f && (att || (keep[g] && repel_on))) {
+                        const float c = unit_coef(q, att, A.a, A.b, k_att, k_rep, inv_gamma, g16);
                         term = make_float2(c * dx, c * dy);
                     }
                 }
@@ -1066,7 +1124,7 @@ __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
                         sel += em;
                         if (sel >= modulus) sel -= modulus;
                     }
-                    const uint64_t rank = static_cast<uint64_t>(s_first[rr]) + sel;
+                    const uint64_t rank = row_first(rr) + sel;
                     float2 yn;
                     if (A.exclude) {
                         const int64_t jj = static_cast<uint32_t>(A.order[rank]);
