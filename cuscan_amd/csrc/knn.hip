@@ -17,6 +17,17 @@
 // it could overflow. A wave owns the candidate state of its own 32 queries,
 // so the threshold epilogue needs no workgroup barrier. Slices are merged
 // per query, then reranked.
+//
+// Sample bound (batches with many slices): a slice that starts from an
+// infinite threshold keeps its first 128 references and sorts its buffer
+// about every 64 survivors until its threshold tightens, and a wave does
+// that one query at a time, so small batches spent most of the scan there.
+// Before such a scan, the approximate distances of every query to a strided
+// sample of 8192 references are computed with the scan's own MFMA
+// arithmetic, and each query's 64th best of them seeds its global
+// threshold: 64 real references lie at or below it, so it bounds the
+// query's final 64th best from above and no candidate of the exact top 64
+// is dropped. With it a slice of 4096 references keeps about 32.
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -25,6 +36,7 @@
 #include <cstring>
 #include <memory>
 #include <vector>
+#include <type_traits>
 
 #include "common.hpp"
 #include "knn.hpp"
@@ -41,6 +53,10 @@ constexpr int kTR = 16;       // reference rows per LDS chunk (one MFMA column b
 constexpr int kCap = 128;     // candidate buffer per (query, slice)
 constexpr int kTop = 64;      // candidates kept per query
 constexpr int kMaxSlices = 256;   // slices per query batch (KnnIndex::slices_for)
+constexpr int kSample = 8192;      // references in the sample bound
+constexpr int kSampleRefs = 128;   // sample references per workgroup of the sample distances
+constexpr int kSampleMinSlices = 32;   // batches with at least this many slices use the bound
+constexpr int64_t kSampleMinRefs = 8 * kSample;
 
 __device__ inline uint16_t f32_to_bf16_rne(float f) {
     uint32_t u = __float_as_uint(f);
@@ -136,8 +152,54 @@ __global__ void fill_u32_kernel(uint32_t *p, int64_t n, uint32_t v) {
     if (i < n) p[i] = v;
 }
 
-template <int KSTEPS>
-__global__ __launch_bounds__(256, 2) void knn_scan_kernel(
+// Register staging of one chunk (the LDS image word j * 64 + lane of the
+// padded rows, for this wave's j = wave, wave + 4, ...; past the slice a
+// harmless in-bounds word), and its store into an LDS buffer.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// s_waitcnt vmcnt(N) (expcnt and lgkmcnt not waited on).
+template <int N>
+__device__ inline void wait_vmcnt() {
+    static_assert(N >= 0 && N < 64, "vmcnt is 6 bits");
+    __builtin_amdgcn_s_waitcnt((N & 15) | (0x7 << 4) | (0xF << 8) | ((N >> 4) << 14));
+}
+
+template <int LW, int INSTR, int SLOTS_ROW, int DP>
+__device__ __forceinline__ void fetch_chunk(u32x4 (&dst)[LW], float &rn, const uint16_t *rb,
+                                            const float *rnorm, int64_t chunk, int64_t r_end,
+                                            int wave, int lane) {
+#pragma unroll
+    for (int t = 0; t < LW; ++t) {
+        const int j = wave + 4 * t;
+        const int sl = j * 64 + lane;
+        const int row = sl / SLOTS_ROW, col = sl % SLOTS_ROW;
+        const int64_t ref = chunk + row;
+        const uint16_t *src = rb;
+        if (j < INSTR && row < kTR && col < DP / 8 && ref < r_end) src = rb + ref * DP + col * 8;
+        dst[t] = *reinterpret_cast<const u32x4 *>(src);
+    }
+    rn = rnorm[min<int64_t>(chunk + (lane & 15), r_end - 1)];
+}
+
+template <int LW, int INSTR>
+__device__ __forceinline__ void put_chunk(const u32x4 (&src)[LW], float rn, uint16_t *buf,
+                                          float *rn_buf, int wave, int lane) {
+#pragma unroll
+    for (int t = 0; t < LW; ++t) {
+        const int j = wave + 4 * t;
+        if (j < INSTR) *reinterpret_cast<u32x4 *>(buf + (j * 64 + lane) * 8) = src[t];
+    }
+    if (wave == 0 && lane < 16) rn_buf[lane] = rn;
+}
+
+// SMALL: grids of at most one workgroup per CU (small batches). The
+// global_load_lds staging waits for each chunk before the next is computed
+// (the compiler cannot tell which LDS buffer an in-flight DMA writes, so it
+// waits for all of them before an LDS read); SMALL instead loads chunks
+// into registers two chunks ahead and writes them to LDS itself, so two
+// chunks stay in flight while one is computed.
+template <int KSTEPS, bool SMALL>
+__global__ __launch_bounds__(256, SMALL ? 1 : 2) void knn_scan_kernel(
     const uint16_t *__restrict__ qb, int64_t nq, const uint16_t *__restrict__ rb,
     const float *__restrict__ rnorm, int64_t nr, int64_t per_slice, int slices,
     float *__restrict__ cand_d, int32_t *__restrict__ cand_i, int32_t *__restrict__ cand_n,
@@ -149,6 +211,7 @@ __global__ __launch_bounds__(256, 2) void knn_scan_kernel(
     constexpr int BUF = INSTR * 64 * 8;         // bf16 per buffer (whole 1 KB loads)
     constexpr int kRefresh = 32;                // chunks between reads of the global thresholds
     __shared__ __attribute__((aligned(16))) uint16_t sB[2 * BUF];
+    __shared__ float s_rn[SMALL ? 2 : 1][16];   // SMALL: reference norms of each buffered chunk
     __shared__ float s_tau[kTQ];
     __shared__ int s_cnt[kTQ];
 
@@ -175,7 +238,14 @@ __global__ __launch_bounds__(256, 2) void knn_scan_kernel(
                 (void __attribute__((address_space(3))) *)(sB + buf * BUF + j * 64 * 8), 16, 0, 0);
         }
     };
-    if (r_begin < r_end) stage(0, r_begin);
+    // SMALL: chunk loads into registers (slot 0 or 1), and from there into an
+    // LDS buffer; past the slice they load harmless in-bounds words.
+    constexpr int LW = SMALL ? (INSTR + 3) / 4 : 1;   // staging loads per wave (some use LW - 1)
+    u32x4 pre0[LW], pre1[LW];
+    float rn0 = 0.0f, rn1 = 0.0f;
+    if constexpr (!SMALL) {
+        if (r_begin < r_end) stage(0, r_begin);
+    }
 
     // A fragments: lane holds row (lane % 16) and K elements 8*(lane/16)..+7
     // of each 32-wide K step, for two 16-query sub-blocks.
@@ -194,7 +264,9 @@ __global__ __launch_bounds__(256, 2) void knn_scan_kernel(
         }
     }
     if (lane < 32) {
-        s_tau[wave * 32 + lane] = INFINITY;
+        // the query's global threshold so far (+inf, or its sample bound)
+        const int64_t q = q_block + wave * 32 + lane;
+        s_tau[wave * 32 + lane] = q < nq ? key_float(tau_key[q]) : INFINITY;
         s_cnt[wave * 32 + lane] = 0;
     }
     // C layout: lane holds rows 4*(lane/16)+i (queries), column lane%16
@@ -205,14 +277,40 @@ __global__ __launch_bounds__(256, 2) void knn_scan_kernel(
 #pragma unroll
     for (int qa = 0; qa < 2; ++qa)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) tau_r[qa][i] = INFINITY;
+        for (int i = 0; i < 4; ++i) {
+            const int64_t q = q_block + wave * 32 + qa * 16 + 4 * (lane >> 4) + i;
+            tau_r[qa][i] = q < nq ? key_float(tau_key[q]) : INFINITY;
+        }
     float rn_next = 0.0f;
-    if (r_begin + (lane & 15) < r_end) rn_next = rnorm[r_begin + (lane & 15)];
+    if (!SMALL && r_begin + (lane & 15) < r_end) rn_next = rnorm[r_begin + (lane & 15)];
+    if constexpr (SMALL) {
+        // the query fragments and thresholds have landed before the chunk
+        // loads start, so no wait in the loop counts them
+        wait_vmcnt<0>();
+        if (r_begin < r_end) {
+            fetch_chunk<LW, INSTR, SLOTS_ROW, DP>(pre0, rn0, rb, rnorm, r_begin, r_end, wave, lane);
+            fetch_chunk<LW, INSTR, SLOTS_ROW, DP>(pre1, rn1, rb, rnorm, r_begin + kTR, r_end, wave, lane);
+            put_chunk<LW, INSTR>(pre0, rn0, sB, &s_rn[0][0], wave, lane);
+            fetch_chunk<LW, INSTR, SLOTS_ROW, DP>(pre0, rn0, rb, rnorm, r_begin + 2 * kTR, r_end, wave, lane);
+        }
+    }
 
-    int buf = 0, since = 0;
-    for (int64_t chunk = r_begin; chunk < r_end; chunk += kTR) {
-        __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): this lane's loads of the current buffer
-        __syncthreads();                      // every lane's loads landed; the other buffer is free
+    // One chunk: wait for its buffer, compute it against the queries and keep
+    // the survivors. SMALL passes the buffer as a constant so that each of
+    // the two unrolled copies stores and refills a fixed register set.
+    int since = 0;
+    auto scan_chunk = [&](int64_t chunk, auto bufc) {
+        const int buf = bufc;
+        if constexpr (SMALL) {
+            // chunk's LDS writes are done and the other buffer is free; the
+            // fences order LDS only, so the chunks in flight are not waited on
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+            __builtin_amdgcn_s_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+        } else {
+            __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): this lane's loads of the current buffer
+            __syncthreads();                      // every lane's loads landed; the other buffer is free
+        }
         if (++since == kRefresh) {
             // thresholds other slices have published for this wave's queries
             since = 0;
@@ -227,11 +325,31 @@ __global__ __launch_bounds__(256, 2) void knn_scan_kernel(
                 for (int i = 0; i < 4; ++i)
                     tau_r[qa][i] = s_tau[wave * 32 + qa * 16 + 4 * (lane >> 4) + i];
         }
-        const float rn = rn_next;
-        if (chunk + kTR < r_end) {
-            stage(buf ^ 1, chunk + kTR);
-            const int64_t nref = chunk + kTR + (lane & 15);
-            rn_next = nref < r_end ? rnorm[nref] : 0.0f;
+        float rn;
+        if constexpr (SMALL) {
+            rn = s_rn[buf][lane & 15];
+            // registers: chunk + kTR in slot buf ^ 1, chunk + 2 kTR in slot buf;
+            // chunk + kTR goes to LDS and chunk + 3 kTR into its slot. Done
+            // past the slice's end too (harmless words into a buffer nobody
+            // reads): a conditional refill would leave the compiler unsure
+            // which register set holds the newest loads, and it would then
+            // wait for all of them.
+            if constexpr (decltype(bufc)::value == 0) {
+                put_chunk<LW, INSTR>(pre1, rn1, sB + BUF, &s_rn[1][0], wave, lane);
+                fetch_chunk<LW, INSTR, SLOTS_ROW, DP>(pre1, rn1, rb, rnorm, chunk + 3 * kTR, r_end,
+                                                      wave, lane);
+            } else {
+                put_chunk<LW, INSTR>(pre0, rn0, sB, &s_rn[0][0], wave, lane);
+                fetch_chunk<LW, INSTR, SLOTS_ROW, DP>(pre0, rn0, rb, rnorm, chunk + 3 * kTR, r_end,
+                                                      wave, lane);
+            }
+        } else {
+            rn = rn_next;
+            if (chunk + kTR < r_end) {
+                stage(buf ^ 1, chunk + kTR);
+                const int64_t nref = chunk + kTR + (lane & 15);
+                rn_next = nref < r_end ? rnorm[nref] : 0.0f;
+            }
         }
         const uint16_t *B = sB + buf * BUF;
         floatx4 acc[2] = {floatx4{0.0f, 0.0f, 0.0f, 0.0f}, floatx4{0.0f, 0.0f, 0.0f, 0.0f}};
@@ -299,7 +417,21 @@ __global__ __launch_bounds__(256, 2) void knn_scan_kernel(
                         tau_r[qa][i] = s_tau[wave * 32 + qa * 16 + 4 * (lane >> 4) + i];
             }
         }
-        buf ^= 1;
+    };
+    if constexpr (SMALL) {
+        // pairs of chunks, then an odd last one (in buffer 0)
+        int64_t chunk = r_begin;
+        for (; chunk + kTR < r_end; chunk += 2 * kTR) {
+            scan_chunk(chunk, std::integral_constant<int, 0>{});
+            scan_chunk(chunk + kTR, std::integral_constant<int, 1>{});
+        }
+        if (chunk < r_end) scan_chunk(chunk, std::integral_constant<int, 0>{});
+    } else {
+        int b = 0;
+        for (int64_t chunk = r_begin; chunk < r_end; chunk += kTR) {
+            scan_chunk(chunk, b);
+            b ^= 1;
+        }
     }
     __threadfence_block();
     for (int l = 0; l < 32; ++l) {
@@ -310,6 +442,111 @@ __global__ __launch_bounds__(256, 2) void knn_scan_kernel(
         const int64_t base = (q * slices + slice) * kCap;
         if (n > 0) compact_buffer(cand_d + base, cand_i + base, n, lane);
         if (lane == 0) cand_n[q * slices + slice] = n < kTop ? n : kTop;
+    }
+}
+
+// Approximate distances of a block of 128 queries to kSampleRefs sample
+// references (sample entry i is reference i * nr / m), in chunks of 16:
+// the A fragments, B fragments, MFMA sequence over the K steps and
+// rn - 2 q.r of knn_scan_kernel, so each value equals the one the scan
+// computes for the same (query, reference). Excluded pairs (self) and
+// queries past nq get +inf.
+template <int KSTEPS>
+__global__ __launch_bounds__(256) void knn_sample_dist_kernel(
+    const uint16_t *__restrict__ qb, int64_t nq, const uint16_t *__restrict__ rb,
+    const float *__restrict__ rnorm, int64_t nr, int m, float *__restrict__ dist,
+    int exclude_self, int64_t query_offset) {
+    constexpr int DP = KSTEPS * 32;
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int64_t q_block = static_cast<int64_t>(blockIdx.x) * kTQ;
+    bf16x8 A[2][KSTEPS];
+#pragma unroll
+    for (int qa = 0; qa < 2; ++qa) {
+        const int64_t row = q_block + wave * 32 + qa * 16 + (lane & 15);
+#pragma unroll
+        for (int ks = 0; ks < KSTEPS; ++ks) {
+            if (row < nq) {
+                const uint4 raw = *reinterpret_cast<const uint4 *>(qb + row * DP + ks * 32 + 8 * (lane >> 4));
+                A[qa][ks] = __builtin_bit_cast(bf16x8, raw);
+            } else {
+                A[qa][ks] = __builtin_bit_cast(bf16x8, make_uint4(0, 0, 0, 0));
+            }
+        }
+    }
+    for (int c = 0; c < kSampleRefs / kTR; ++c) {
+        const int i = blockIdx.y * kSampleRefs + c * kTR + (lane & 15);   // sample entry
+        const int64_t ref = static_cast<int64_t>(i) * nr / m;
+        floatx4 acc[2] = {floatx4{0.0f, 0.0f, 0.0f, 0.0f}, floatx4{0.0f, 0.0f, 0.0f, 0.0f}};
+#pragma unroll
+        for (int ks = 0; ks < KSTEPS; ++ks) {
+            const uint4 raw = *reinterpret_cast<const uint4 *>(rb + ref * DP + ks * 32 + 8 * (lane >> 4));
+            const bf16x8 bfrag = __builtin_bit_cast(bf16x8, raw);
+#pragma unroll
+            for (int qa = 0; qa < 2; ++qa)
+                acc[qa] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[qa][ks], bfrag, acc[qa], 0, 0, 0);
+        }
+        const float rn = rnorm[ref];
+#pragma unroll
+        for (int qa = 0; qa < 2; ++qa)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t q = q_block + wave * 32 + qa * 16 + 4 * (lane >> 4) + r;
+                if (q >= nq) continue;
+                const float dt = rn - 2.0f * acc[qa][r];
+                const bool excluded = exclude_self && ref == query_offset + q;
+                dist[q * m + i] = excluded ? INFINITY : dt;
+            }
+    }
+}
+
+// Workgroup q: the kTop-th smallest of the query's m sample distances by a
+// most-significant-byte-first radix select over their order-preserving
+// keys, widened by 2^-20 of its magnitude, becomes the query's global
+// threshold (+inf when fewer than kTop sample distances are finite).
+__global__ __launch_bounds__(256) void knn_sample_select_kernel(const float *__restrict__ dist, int m,
+                                                                uint32_t *__restrict__ tau_key) {
+    __shared__ uint32_t s_hist[256];
+    __shared__ uint32_t s_wsum[4];
+    __shared__ uint32_t s_sel[2];   // chosen byte, rank left within it
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t q = blockIdx.x;
+    const float *row = dist + q * m;
+    uint32_t prefix = 0u, mask = 0u, want = kTop - 1;   // 0-based rank of the kTop-th smallest
+    for (int shift = 24; shift >= 0; shift -= 8) {
+        s_hist[tid] = 0u;
+        __syncthreads();
+        for (int i = tid; i < m; i += 256) {
+            const uint32_t k = float_key(row[i]);
+            if ((k & mask) == prefix) atomicAdd(&s_hist[(k >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        // the byte whose count range [below, below + count) holds rank want
+        const uint32_t c = s_hist[tid];
+        uint32_t incl = c;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t up = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += up;
+        }
+        if (lane == 63) s_wsum[wave] = incl;
+        __syncthreads();
+        for (int w = 0; w < wave; ++w) incl += s_wsum[w];
+        const uint32_t below = incl - c;
+        if (c > 0u && below <= want && want < incl) {
+            s_sel[0] = static_cast<uint32_t>(tid);
+            s_sel[1] = want - below;
+        }
+        __syncthreads();
+        prefix |= s_sel[0] << shift;
+        mask |= 255u << shift;
+        want = s_sel[1];
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const float kth = key_float(prefix);
+        // sample values equal the scan's; the widening only guards the bound
+        const float bound = m >= kTop && kth < INFINITY ? kth + fabsf(kth) * 0x1.0p-20f : INFINITY;
+        tau_key[q] = float_key(bound);
     }
 }
 
@@ -454,8 +691,20 @@ template <int KS>
 void launch_scan(dim3 grid, hipStream_t s, const uint16_t *qb, int64_t nq, const uint16_t *rb,
                  const float *rnorm, int64_t nr, int64_t per_slice, int slices, float *cd,
                  int32_t *ci, int32_t *cn, uint32_t *tk, int exclude_self, int64_t qoff) {
-    hipLaunchKernelGGL(knn_scan_kernel<KS>, grid, dim3(256), 0, s, qb, nq, rb, rnorm, nr, per_slice,
-                       slices, cd, ci, cn, tk, exclude_self, qoff);
+    if (static_cast<int64_t>(grid.x) * grid.y <= 256)
+        hipLaunchKernelGGL((knn_scan_kernel<KS, true>), grid, dim3(256), 0, s, qb, nq, rb, rnorm, nr,
+                           per_slice, slices, cd, ci, cn, tk, exclude_self, qoff);
+    else
+        hipLaunchKernelGGL((knn_scan_kernel<KS, false>), grid, dim3(256), 0, s, qb, nq, rb, rnorm, nr,
+                           per_slice, slices, cd, ci, cn, tk, exclude_self, qoff);
+}
+
+template <int KS>
+void launch_sample_dist(dim3 grid, hipStream_t s, const uint16_t *qb, int64_t nq, const uint16_t *rb,
+                        const float *rnorm, int64_t nr, int m, float *dist, int exclude_self,
+                        int64_t qoff) {
+    hipLaunchKernelGGL(knn_sample_dist_kernel<KS>, grid, dim3(256), 0, s, qb, nq, rb, rnorm, nr, m,
+                       dist, exclude_self, qoff);
 }
 
 int ksteps_for(int64_t d) {
@@ -489,6 +738,10 @@ KnnIndex::KnnIndex(const float *ref_host, const float *ref_device, int64_t n, in
     reserve(max_q);
 }
 
+bool KnnIndex::uses_sample(int64_t q) const {
+    return n_ >= kSampleMinRefs && slices_for(q) >= kSampleMinSlices;
+}
+
 int KnnIndex::slices_for(int64_t q) const {
     const int64_t blocks = (q + kTQ - 1) / kTQ;
     // Enough workgroups to fill the chip (256 CUs, two resident each), but
@@ -513,6 +766,11 @@ void KnnIndex::reserve(int64_t q) {
     top_i_.alloc(static_cast<size_t>(q) * kTop);
     top_n_.alloc(static_cast<size_t>(q));
     tau_key_.alloc(static_cast<size_t>(q));
+    // sample distances of the largest batch that uses the sample bound
+    int64_t sq = 0;
+    for (int64_t qq = q; qq >= 1 && sq == 0; --qq)
+        if (uses_sample(qq)) sq = qq;
+    if (sq > 0) sample_d_.alloc(static_cast<size_t>(sq) * kSample);
     qbf_.alloc(static_cast<size_t>(q * dp_));
     reserved_q_ = q;
 }
@@ -532,8 +790,25 @@ void KnnIndex::query(const float *q_dev, int64_t nq, int k, bool exclude_self, i
     const int64_t per_slice = ((n_ + slices - 1) / slices + kTR - 1) / kTR * kTR;
     const dim3 grid(static_cast<unsigned>((nq + kTQ - 1) / kTQ), static_cast<unsigned>(slices));
     const int ex = exclude_self ? 1 : 0;
-    hipLaunchKernelGGL(fill_u32_kernel, dim3(static_cast<unsigned>((nq + 255) / 256)), dim3(256), 0, s,
-                       tau_key_.ptr, nq, 0xFF800000u);   // key of +inf
+    if (uses_sample(nq)) {
+        // every query's global threshold starts at its sample bound
+        const dim3 sgrid(static_cast<unsigned>((nq + kTQ - 1) / kTQ), kSample / kSampleRefs);
+        switch (ks_) {
+            case 1: launch_sample_dist<1>(sgrid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, kSample, sample_d_.ptr, ex, query_offset); break;
+            case 2: launch_sample_dist<2>(sgrid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, kSample, sample_d_.ptr, ex, query_offset); break;
+            case 4: launch_sample_dist<4>(sgrid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, kSample, sample_d_.ptr, ex, query_offset); break;
+            case 8: launch_sample_dist<8>(sgrid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, kSample, sample_d_.ptr, ex, query_offset); break;
+            case 16: launch_sample_dist<16>(sgrid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, kSample, sample_d_.ptr, ex, query_offset); break;
+            case 25: launch_sample_dist<25>(sgrid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, kSample, sample_d_.ptr, ex, query_offset); break;
+            default: launch_sample_dist<32>(sgrid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, kSample, sample_d_.ptr, ex, query_offset); break;
+        }
+        HIP_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(knn_sample_select_kernel, dim3(static_cast<unsigned>(nq)), dim3(256), 0, s,
+                           sample_d_.ptr, kSample, tau_key_.ptr);
+    } else {
+        hipLaunchKernelGGL(fill_u32_kernel, dim3(static_cast<unsigned>((nq + 255) / 256)), dim3(256), 0,
+                           s, tau_key_.ptr, nq, 0xFF800000u);   // key of +inf
+    }
     HIP_CHECK(hipGetLastError());
     switch (ks_) {
         case 1: launch_scan<1>(grid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, per_slice, slices, cand_d_.ptr, cand_i_.ptr, cand_n_.ptr, tau_key_.ptr, ex, query_offset); break;

@@ -28,11 +28,14 @@ public:
     int64_t dims() const { return d_; }
     const float *reference() const { return ref_.ptr; }
     int slices_for(int64_t q) const;
+    // Whether a batch of q rows seeds its thresholds from the sample bound.
+    bool uses_sample(int64_t q) const;
 
 private:
     int64_t n_, d_, dp_ = 0, reserved_q_ = 0;
     int ks_ = 0;
     DeviceArray<float> ref_, rnorm_, cand_d_, top_d_;
+    DeviceArray<float> sample_d_;     // [q, 8192] sample-bound distances of a batch
     DeviceArray<uint16_t> ref_bf_, qbf_;
     DeviceArray<int32_t> cand_i_, cand_n_, top_i_, top_n_;
     DeviceArray<uint32_t> tau_key_;   // per query: best published slice threshold (float key)

@@ -82,3 +82,41 @@ def test_continuous_data_recall():
     assert recall >= 0.999
     same = (gi == ei).all(axis=1)
     assert (gd[same].view(np.uint32) == ed[same].view(np.uint32)).all()
+
+
+@pytest.mark.parametrize("q", [1, 64, 700])
+def test_sample_bound_bit_exact(q):
+    """150,000 references: batches of up to 1,792 rows take at least 32
+    slices, so their thresholds start from the 8,192-row sample bound.
+    Results stay bit-identical to the exact kernel, and the merged prefilter
+    lists equal those of a 2,200-row batch (29 slices, no bound): the bound
+    drops no candidate of the top 64."""
+    from cuscan_amd.knn import KnnIndex
+
+    ref = image_like(150_000, 64, 10, 11)
+    qry = image_like(2200, 64, 10, 12)
+    print(f"[gpu] knn sample bound q={q}", flush=True)
+    idx = KnnIndex(ref, max_queries=2200)
+    gi, gd = idx.query(qry[:q], 15)
+    ei, ed = O.exact_knn(qry[:q], ref, 15, threads=8)
+    assert (gi == ei).all() and (gd.view(np.uint32) == ed.view(np.uint32)).all()
+    qt = torch.from_numpy(qry).cuda()
+    small_ids, small_n = idx.candidates_device(qt[:q])
+    big_ids, big_n = idx.candidates_device(qt)
+    assert torch.equal(small_n, big_n[:q]) and torch.equal(small_ids, big_ids[:q])
+    assert (small_n == 64).all()
+    idx.close()
+
+
+def test_sample_bound_exclude_self_with_offset():
+    """Fit-graph style queries (rows of the reference set itself, excluding
+    self) through the sample bound, for a batch that starts at row 1,000."""
+    from cuscan_amd.knn import KnnIndex
+
+    x = image_like(150_000, 48, 8, 5)
+    idx = KnnIndex(x, max_queries=256)
+    gi, gd = idx.query(x[1000:1256], 14, exclude_self=True, query_offset=1000)
+    ei, ed = O.exact_knn(x[1000:1256], x, 14, exclude_self=True, query_offset=1000, threads=8)
+    assert (gi == ei).all() and (gd.view(np.uint32) == ed.view(np.uint32)).all()
+    assert not (gi == np.arange(1000, 1256)[:, None]).any()
+    idx.close()
