@@ -525,7 +525,8 @@ def main():
                 # reported beside the stream bytes and not in the HBM fraction
                 "gather_bytes_per_epoch": int(8 * (F + S) / args.epochs),
                 "refresh_avg_ms": round(pstats["refresh_ms"] / max(pstats["refresh_launches"], 1), 4),
-                "rank_ordered_epochs": pstats["rank_ordered_epochs"]}
+                "rank_ordered_epochs": pstats["rank_ordered_epochs"],
+                "compact_records": pstats["compact_records"]}
     out = {
         "metric": "layout edge-epochs/s at 1M rows", "value": value, "unit": "edge-epochs/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

@@ -98,6 +98,7 @@ class LayoutStats(C.Structure):
         ("sweep_ms", C.c_double), ("sweep_launches", C.c_int64),
         ("refresh_ms", C.c_double), ("refresh_launches", C.c_int64),
         ("edge_balanced", C.c_int64), ("rank_ordered_epochs", C.c_int64),
+        ("compact_records", C.c_int64),
     ]
 
 

@@ -245,6 +245,7 @@ struct umaphip_plan {
 
     // edge-balanced unit sweep (Adam, d = 2, H > 1024)
     bool unit_path = false;
+    bool compact = false;   // 24-byte records (rank-ordered plans whose periods allow it)
     int64_t units = 0;
     DeviceArray<double> rec;
     DeviceArray<uint16_t> fire;
@@ -346,8 +347,8 @@ void umaphip_plan::run() {
     HIP_CHECK(hipMemcpyAsync(npos.ptr, npos0.ptr, npos0.bytes(), hipMemcpyDeviceToDevice, stream));
     HIP_CHECK(hipMemcpyAsync(nneg.ptr, nneg0.ptr, nneg0.bytes(), hipMemcpyDeviceToDevice, stream));
     if (unit_path)
-        launch_unit_pack(npos0.ptr, nneg0.ptr, periods.ptr, indices.ptr, E, range, rec.ptr,
-                         fire.ptr, stream);
+        launch_unit_pack(npos0.ptr, nneg0.ptr, periods.ptr, indices.ptr, E, range, compact,
+                         static_cast<double>(p.epochs), rec.ptr, fire.ptr, stream);
     HIP_CHECK(hipMemsetAsync(updates.ptr, 0, updates.bytes(), stream));
     if (adam) {
         HIP_CHECK(hipMemsetAsync(m.ptr, 0, m.bytes(), stream));
@@ -465,6 +466,8 @@ void umaphip_plan::run() {
                 UnitSweepArgs U{};
                 U.s = A;
                 U.rec = rec.ptr; U.fire = fire.ptr;
+                U.compact = compact;
+                U.per_num = static_cast<double>(p.epochs);
                 U.unit_k = unit_k.ptr; U.unit_frag = unit_frag.ptr;
                 U.nz_rows = nz_rows.count ? nz_rows.ptr : nullptr; U.nz_off = nz_off.ptr;
                 U.grad = grad.ptr; U.frag = frag.ptr;
@@ -584,7 +587,7 @@ void umaphip_plan::run() {
         }
     }
     HIP_CHECK(hipEventRecord(ev_stop, stream));
-    if (unit_path) launch_unit_unpack(rec.ptr, E, npos.ptr, nneg.ptr, stream);
+    if (unit_path) launch_unit_unpack(rec.ptr, E, compact, npos.ptr, nneg.ptr, stream);
     cur = c;
     refresh_count = refreshes;
     ran = true;
@@ -727,7 +730,6 @@ umaphip_status umaphip_plan_create(int64_t h, int64_t n, int64_t d, const float 
             UnitTables T = build_unit_tables(offsets, h, E, unit_edges());
             P->unit_path = true;
             P->units = T.units;
-            P->rec.alloc(static_cast<size_t>(4 * E));
             P->fire.alloc(static_cast<size_t>(E));
             auto up = [](DeviceArray<int32_t> &dst, const std::vector<int32_t> &src) {
                 if (src.empty()) return;
@@ -747,6 +749,11 @@ umaphip_status umaphip_plan_create(int64_t h, int64_t n, int64_t d, const float 
             // the epochs with a projection order sweep rows by rank (fragment
             // slots 2u and 2u + 1 of every unit).
             P->ranked = P->hard && T.nz_rows.empty() && E < INT32_MAX;
+            // Rank-ordered plans keep 24-byte records when every period is
+            // rebuilt exactly from an f32 sample count.
+            P->compact = P->ranked &&
+                         periods_compactable(periods, E, static_cast<double>(params->epochs));
+            P->rec.alloc(static_cast<size_t>((P->compact ? 3 : 4) * E));
             // Row-table capacity: the rows of a unit in CSR order are known;
             // in any rank order a unit of UE slots holds at most
             // 2 + (UE - 2) / (smallest degree) rows (two partial rows).
@@ -915,6 +922,7 @@ umaphip_status umaphip_plan_download(umaphip_plan *plan, float *head, double *ne
             stats->refresh_launches = plan->refresh_used / 2;
             stats->edge_balanced = plan->unit_path ? 1 : 0;
             stats->rank_ordered_epochs = plan->ran ? plan->ranked_epochs : 0;
+            stats->compact_records = plan->compact ? 1 : 0;
         }
         UMAPHIP_REQUIRE(status == ~0ull, UMAPHIP_NUMERICAL,
                         "non-finite %s projection at row %llu (refresh %llu)",

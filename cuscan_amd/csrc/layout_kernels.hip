@@ -7,10 +7,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdint>
 
 #include "common.hpp"
 #include "layout_kernels.hpp"
+#include "host_parallel.hpp"
 
 namespace umaphip {
 
@@ -828,8 +830,11 @@ __device__ inline void segmented_accumulate(float2 term, int row, int lane, floa
 // RC: capacity of the unit row table; the plan picks 64 when no unit can
 // hold more rows, which keeps the workgroup at about 7 KB of LDS (6 waves
 // per SIMD) instead of about 14 KB (3 waves).
-template <bool RANKED, int RC>
+// COMPACT (rank-ordered only): 24-byte records, the period rebuilt as
+// per_num / samples and e mod M from the edge index.
+template <bool RANKED, int RC, bool COMPACT>
 __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
+    static_assert(RANKED || !COMPACT, "compact records are read by rank only");
     const SweepArgs &A = U.s;
     constexpr int EPL = kUnitEPL, UE = kUnitEdges, TCAP = kUnitTermCap, G = kUnitGroups;
     __shared__ uint16_t s_off[RC + 1];  // unit-local first edge of each unit row
@@ -964,12 +969,19 @@ __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
         for (int k = 0; k < EPL; ++k) {
             r_npos[k] = 0.0; r_nn[k] = 0.0; r_per[k] = 0.0; r_w[k] = 0;
             if (k * 64 + lane < ue) {
-                const double4 rec = reinterpret_cast<const double4 *>(U.rec)[eidx[k]];
-                r_npos[k] = rec.x;
-                r_nn[k] = rec.y;
-                r_per[k] = rec.z;
-                r_w[k] = static_cast<uint64_t>(__double_as_longlong(rec.w));
-                fire[k] = rec.x <= now ? 0u : 0xffffffffu;
+                if constexpr (COMPACT) {
+                    const double *rec = U.rec + 3 * eidx[k];
+                    r_npos[k] = rec[0];
+                    r_nn[k] = rec[1];
+                    r_w[k] = static_cast<uint64_t>(__double_as_longlong(rec[2]));
+                } else {
+                    const double4 rec = reinterpret_cast<const double4 *>(U.rec)[eidx[k]];
+                    r_npos[k] = rec.x;
+                    r_nn[k] = rec.y;
+                    r_per[k] = rec.z;
+                    r_w[k] = static_cast<uint64_t>(__double_as_longlong(rec.w));
+                }
+                fire[k] = r_npos[k] <= now ? 0u : 0xffffffffu;
             }
         }
     }
@@ -997,7 +1009,11 @@ __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
             }
             if (fire[k] <= epoch) {
                 const int64_t e = eidx[k];
-                const double per = r_per[k], nn = r_nn[k];
+                double per = r_per[k];
+                if constexpr (COMPACT)
+                    per = U.per_num / static_cast<double>(
+                                          __uint_as_float(static_cast<uint32_t>(r_w[k] >> 32)));
+                const double nn = r_nn[k];
                 const double npos2 = r_npos[k] + per;
                 int32_t count = 0;
                 double nn2 = nn;
@@ -1006,11 +1022,18 @@ __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
                     count = static_cast<int32_t>((now - nn) / np);
                     nn2 = nn + static_cast<double>(count) * np;
                 }
-                reinterpret_cast<double2 *>(U.rec)[2 * e] = make_double2(npos2, nn2);
+                if constexpr (COMPACT) {
+                    U.rec[3 * e] = npos2;
+                    U.rec[3 * e + 1] = nn2;
+                } else {
+                    reinterpret_cast<double2 *>(U.rec)[2 * e] = make_double2(npos2, nn2);
+                }
                 if constexpr (!RANKED) U.fire[e] = fire_shadow(npos2);
                 const int negs = count > 0 ? count : 0;
                 t = 1 + negs;
-                const uint32_t em = static_cast<uint32_t>(r_w[k] >> 32);
+                const uint32_t em =
+                    COMPACT ? static_cast<uint32_t>(mod_u64(static_cast<uint64_t>(e), modulus, U.inv_mod))
+                            : static_cast<uint32_t>(r_w[k] >> 32);
                 s_j[slot] = static_cast<int32_t>(static_cast<uint32_t>(r_w[k]));
                 s_em[slot] = em;
                 sel0r[k] = negs ? static_cast<uint32_t>(mod_u64(static_cast<uint64_t>(em) * U.t1,
@@ -1165,23 +1188,33 @@ __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
 }
 
 __global__ void unit_pack_kernel(const double *npos, const double *nneg, const double *periods,
-                                 const int32_t *indices, int64_t E, uint64_t modulus,
-                                 double *rec, uint16_t *fire) {
+                                 const int32_t *indices, int64_t E, uint64_t modulus, bool compact,
+                                 double per_num, double *rec, uint16_t *fire) {
     const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (e >= E) return;
-    const uint64_t em = static_cast<uint64_t>(e) % modulus;
-    const uint64_t w = (em << 32) | static_cast<uint32_t>(indices[e]);
-    reinterpret_cast<double4 *>(rec)[e] =
-        make_double4(npos[e], nneg[e], periods[e], __longlong_as_double(static_cast<long long>(w)));
+    if (compact) {
+        const float samples = static_cast<float>(per_num / periods[e]);
+        const uint64_t w = (static_cast<uint64_t>(__float_as_uint(samples)) << 32) |
+                           static_cast<uint32_t>(indices[e]);
+        rec[3 * e] = npos[e];
+        rec[3 * e + 1] = nneg[e];
+        rec[3 * e + 2] = __longlong_as_double(static_cast<long long>(w));
+    } else {
+        const uint64_t em = static_cast<uint64_t>(e) % modulus;
+        const uint64_t w = (em << 32) | static_cast<uint32_t>(indices[e]);
+        reinterpret_cast<double4 *>(rec)[e] = make_double4(
+            npos[e], nneg[e], periods[e], __longlong_as_double(static_cast<long long>(w)));
+    }
     fire[e] = fire_shadow(npos[e]);
 }
 
-__global__ void unit_unpack_kernel(const double *rec, int64_t E, double *npos, double *nneg) {
+__global__ void unit_unpack_kernel(const double *rec, int64_t E, bool compact, double *npos,
+                                   double *nneg) {
     const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (e >= E) return;
-    const double2 c = reinterpret_cast<const double2 *>(rec)[2 * e];
-    npos[e] = c.x;
-    nneg[e] = c.y;
+    const int64_t at = compact ? 3 * e : 4 * e;
+    npos[e] = rec[at];
+    nneg[e] = rec[at + 1];
 }
 
 // Lazy Adam over rows; rows split across units add their fragments in unit
@@ -1956,12 +1989,17 @@ void launch_unit_sweep(UnitSweepArgs a, hipStream_t s) {
     a.xcd_chunk = (a.units + 7) / 8;
     const dim3 grid(static_cast<unsigned>(8 * a.xcd_chunk));
     const bool narrow = a.row_cap == kUnitNarrowRows;
-    if (a.pref) {
-        if (narrow) hipLaunchKernelGGL((unit_sweep_kernel<true, kUnitNarrowRows>), grid, dim3(64), 0, s, a);
-        else hipLaunchKernelGGL((unit_sweep_kernel<true, kUnitEdges>), grid, dim3(64), 0, s, a);
+    UMAPHIP_REQUIRE(a.pref || !a.compact, UMAPHIP_BACKEND,
+                    "unit sweep: compact records need the rank-ordered sweep");
+    if (a.pref && a.compact) {
+        if (narrow) hipLaunchKernelGGL((unit_sweep_kernel<true, kUnitNarrowRows, true>), grid, dim3(64), 0, s, a);
+        else hipLaunchKernelGGL((unit_sweep_kernel<true, kUnitEdges, true>), grid, dim3(64), 0, s, a);
+    } else if (a.pref) {
+        if (narrow) hipLaunchKernelGGL((unit_sweep_kernel<true, kUnitNarrowRows, false>), grid, dim3(64), 0, s, a);
+        else hipLaunchKernelGGL((unit_sweep_kernel<true, kUnitEdges, false>), grid, dim3(64), 0, s, a);
     } else {
-        if (narrow) hipLaunchKernelGGL((unit_sweep_kernel<false, kUnitNarrowRows>), grid, dim3(64), 0, s, a);
-        else hipLaunchKernelGGL((unit_sweep_kernel<false, kUnitEdges>), grid, dim3(64), 0, s, a);
+        if (narrow) hipLaunchKernelGGL((unit_sweep_kernel<false, kUnitNarrowRows, false>), grid, dim3(64), 0, s, a);
+        else hipLaunchKernelGGL((unit_sweep_kernel<false, kUnitEdges, false>), grid, dim3(64), 0, s, a);
     }
     HIP_CHECK(hipGetLastError());
 }
@@ -2047,19 +2085,29 @@ void launch_cross_entropy(const float *head, int64_t h, const float *tail, int64
 }
 
 void launch_unit_pack(const double *npos, const double *nneg, const double *periods,
-                      const int32_t *indices, int64_t E, uint64_t modulus, double *rec,
-                      uint16_t *fire, hipStream_t s) {
+                      const int32_t *indices, int64_t E, uint64_t modulus, bool compact,
+                      double per_num, double *rec, uint16_t *fire, hipStream_t s) {
     if (E == 0) return;
     hipLaunchKernelGGL(unit_pack_kernel, dim3(grid_for(E)), dim3(kThreads), 0, s, npos, nneg,
-                       periods, indices, E, modulus, rec, fire);
+                       periods, indices, E, modulus, compact, per_num, rec, fire);
     HIP_CHECK(hipGetLastError());
 }
 
-void launch_unit_unpack(const double *rec, int64_t E, double *npos, double *nneg, hipStream_t s) {
+void launch_unit_unpack(const double *rec, int64_t E, bool compact, double *npos, double *nneg,
+                        hipStream_t s) {
     if (E == 0) return;
-    hipLaunchKernelGGL(unit_unpack_kernel, dim3(grid_for(E)), dim3(kThreads), 0, s, rec, E, npos,
-                       nneg);
+    hipLaunchKernelGGL(unit_unpack_kernel, dim3(grid_for(E)), dim3(kThreads), 0, s, rec, E, compact,
+                       npos, nneg);
     HIP_CHECK(hipGetLastError());
+}
+
+bool periods_compactable(const double *periods, int64_t E, double per_num) {
+    std::atomic<int> differs{0};
+    parallel_for(E, [&](int64_t e) {
+        const float samples = static_cast<float>(per_num / periods[e]);
+        if (per_num / static_cast<double>(samples) != periods[e]) differs.store(1);
+    }, 1 << 16);
+    return !differs.load();
 }
 
 void launch_unit_adam(int64_t h, const float *pos, float *pos_out, const float *grad,

@@ -105,7 +105,8 @@ void launch_transform_layouts(const TransformSegment *segs, int32_t n_segs, int6
 // (grad) and, for rows crossing a unit boundary, per-unit fragments.
 struct UnitSweepArgs {
     SweepArgs s;               // positions, negatives and parameters (pos_out, m, v unused)
-    double *rec;               // [E] records {next_positive, next_negative, period, j | e mod M << 32}
+    double *rec;               // [E] records {next_positive, next_negative, period, j | e mod M << 32},
+                               // compact: {next_positive, next_negative, j | samples (f32 bits) << 32}
     uint16_t *fire;            // [E] firing shadow
     const int32_t *unit_k;     // [units][2] first/last nonzero-degree row index of each unit
     const int32_t *unit_frag;  // [units][2] fragment slot of the unit's first/last row, or -1
@@ -120,14 +121,21 @@ struct UnitSweepArgs {
     double rate;               // negative_sample_rate
     uint64_t t1;               // (epoch + 1) mod M (set by the launcher)
     double inv_mod;            // 1 / M (set by the launcher)
+    bool compact;              // 24-byte records (rank-ordered sweep only)
+    double per_num;            // compact: period = per_num / samples
 };
 int64_t unit_edges();
 int64_t unit_narrow_rows();
 void launch_unit_sweep(UnitSweepArgs a, hipStream_t s);
 void launch_unit_pack(const double *npos, const double *nneg, const double *periods,
-                      const int32_t *indices, int64_t E, uint64_t modulus, double *rec,
+                      const int32_t *indices, int64_t E, uint64_t modulus, bool compact,
+                      double per_num, double *rec,
                       uint16_t *fire, hipStream_t s);
-void launch_unit_unpack(const double *rec, int64_t E, double *npos, double *nneg, hipStream_t s);
+void launch_unit_unpack(const double *rec, int64_t E, bool compact, double *npos, double *nneg,
+                        hipStream_t s);
+// Whether every period equals per_num / float(per_num / period) exactly, so
+// that 24-byte records (f32 samples instead of the f64 period) reproduce it.
+bool periods_compactable(const double *periods, int64_t E, double per_num);
 // Rank-ordered sweep helpers: from rank_deg [n] (the degree of order[r],
 // written with rank_seg by the projection sort's last pass), pref[r] = edges
 // of the ranks below r (block_sums: rank_prefix_blocks(n) ints) and unit_k

@@ -182,6 +182,8 @@ typedef struct {
     int64_t edge_balanced;          /* 1 when epochs ran the edge-balanced unit sweep */
     int64_t rank_ordered_epochs;    /* epochs of the last run whose unit sweep ran the rows
                                        in projection order */
+    int64_t compact_records;        /* 1 when the rank-ordered sweep kept 24-byte edge records
+                                       (every period rebuilt exactly from an f32 sample count) */
 } umaphip_layout_stats;
 
 /* Host arrays in, host arrays out (allocates and frees device memory).

@@ -351,6 +351,8 @@ def test_driver_matches_oracle_loop(case):
     (got, gp, gn, stats), (exp, ep, en, est) = both(epochs)
     assert stats["edge_balanced"] == (name in UNIT_PATH)
     assert stats["rank_ordered_epochs"] == (epochs if name in RANKED_PATH else 0)
+    # the graph's periods come from the same epoch count: 24-byte records
+    assert stats["compact_records"] == (name in RANKED_PATH)
     assert (gp == ep).all() and (gn == en).all()
     assert stats["adaptations"] == est["adaptations"]
     assert stats["refreshes"] == est["refreshes"]
