@@ -57,6 +57,7 @@ constexpr int kSample = 8192;      // references in the sample bound
 constexpr int kSampleRefs = 128;   // sample references per workgroup of the sample distances
 constexpr int kSampleMinSlices = 32;   // batches with at least this many slices use the bound
 constexpr int64_t kSampleMinRefs = 8 * kSample;
+constexpr int64_t kWideRerankMaxQ = 8192;   // batches up to this many rows rerank with 4 waves
 
 __device__ inline uint16_t f32_to_bf16_rne(float f) {
     uint32_t u = __float_as_uint(f);
@@ -687,6 +688,72 @@ __global__ __launch_bounds__(64) void knn_rerank_kernel(const float *query, int6
     }
 }
 
+// knn_rerank_kernel for small batches, where each query's chain of staging
+// rounds (13 blocks of 64 features, 4 rounds of 16 rows each) was the cost:
+// four waves load the 64 candidate rows of a 128-feature block at once (32
+// loads in flight per lane), and wave 0 sums them in the same sequential
+// feature order, so distances are bit-identical to knn_rerank_kernel's.
+__global__ __launch_bounds__(256) void knn_rerank_wide_kernel(const float *query, int64_t nq,
+                                                              const float *ref, int64_t nr,
+                                                              int64_t d, const int32_t *top_i,
+                                                              const int32_t *top_n, int k,
+                                                              int32_t *out_i, float *out_d) {
+    constexpr int FB = 128;
+    __shared__ float s_q[FB];
+    __shared__ float s_r[kTop][FB + 1];
+    const int64_t q = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (q >= nq) return;
+    const int n = top_n[q];
+    const int32_t id = lane < n ? top_i[q * kTop + lane] : INT32_MAX;
+    float acc = 0.0f;
+    for (int64_t f0 = 0; f0 < d; f0 += FB) {
+        const int fb = static_cast<int>(min<int64_t>(FB, d - f0));
+        __syncthreads();
+        if (tid < FB) s_q[tid] = tid < fb ? query[q * d + f0 + tid] : 0.0f;
+        // wave w: candidates 16w .. 16w + 15, features lane and lane + 64
+        float v[16][2];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int c = 16 * wave + u;
+            const int64_t row = __shfl(id, c, 64);
+            v[u][0] = (c < n && lane < fb) ? ref[row * d + f0 + lane] : 0.0f;
+            v[u][1] = (c < n && lane + 64 < fb) ? ref[row * d + f0 + 64 + lane] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int c = 16 * wave + u;
+            if (c < n) {
+                s_r[c][lane] = v[u][0];
+                s_r[c][lane + 64] = v[u][1];
+            }
+        }
+        __syncthreads();
+        if (wave == 0 && lane < n) {
+            for (int f = 0; f < fb; ++f) {
+                const float delta = s_q[f] - s_r[lane][f];
+                acc += delta * delta;
+            }
+        }
+    }
+    if (wave != 0) return;
+    const float dist = lane < n ? acc : INFINITY;
+    int rank = 0;
+    for (int f = 0; f < 64; ++f) {
+        const float df = __shfl(dist, f, 64);
+        const int32_t idf = __shfl(id, f, 64);
+        rank += (df < dist || (df == dist && idf < id)) ? 1 : 0;
+    }
+    if (lane < n && rank < k) {
+        out_i[q * k + rank] = id;
+        out_d[q * k + rank] = sqrtf(dist);
+    }
+    if (lane >= n && lane < k) {
+        out_i[q * k + lane] = static_cast<int32_t>(nr);
+        out_d[q * k + lane] = sqrtf(FLT_MAX);
+    }
+}
+
 template <int KS>
 void launch_scan(dim3 grid, hipStream_t s, const uint16_t *qb, int64_t nq, const uint16_t *rb,
                  const float *rnorm, int64_t nr, int64_t per_slice, int slices, float *cd,
@@ -832,9 +899,16 @@ void KnnIndex::query(const float *q_dev, int64_t nq, int k, bool exclude_self, i
                                  hipMemcpyDeviceToDevice, s));
         return;
     }
-    hipLaunchKernelGGL(knn_rerank_kernel, dim3(static_cast<unsigned>(nq)), dim3(64), 0, s,
-                       q_f32_for_rerank ? q_f32_for_rerank : q_dev, nq, ref_.ptr, n_, d_,
-                       top_i_.ptr, top_n_.ptr, k, idx_out, dist_out);
+    // small batches: one wide workgroup per query (its staging rounds are
+    // the latency); large batches: one wave per query
+    if (nq <= kWideRerankMaxQ)
+        hipLaunchKernelGGL(knn_rerank_wide_kernel, dim3(static_cast<unsigned>(nq)), dim3(256), 0, s,
+                           q_f32_for_rerank ? q_f32_for_rerank : q_dev, nq, ref_.ptr, n_, d_,
+                           top_i_.ptr, top_n_.ptr, k, idx_out, dist_out);
+    else
+        hipLaunchKernelGGL(knn_rerank_kernel, dim3(static_cast<unsigned>(nq)), dim3(64), 0, s,
+                           q_f32_for_rerank ? q_f32_for_rerank : q_dev, nq, ref_.ptr, n_, d_,
+                           top_i_.ptr, top_n_.ptr, k, idx_out, dist_out);
     HIP_CHECK(hipGetLastError());
 }
 
