@@ -57,7 +57,7 @@ constexpr int kSample = 8192;      // references in the sample bound
 constexpr int kSampleRefs = 128;   // sample references per workgroup of the sample distances
 constexpr int kSampleMinSlices = 32;   // batches with at least this many slices use the bound
 constexpr int64_t kSampleMinRefs = 8 * kSample;
-constexpr int64_t kWideRerankMaxQ = 8192;   // batches up to this many rows rerank with 4 waves
+constexpr int64_t kWideRerankMaxQ = 8192;   // batches up to this many rows merge and rerank with 4 waves
 
 __device__ inline uint16_t f32_to_bf16_rne(float f) {
     uint32_t u = __float_as_uint(f);
@@ -628,6 +628,90 @@ __global__ __launch_bounds__(64) void knn_merge_kernel(const float *cand_d, cons
     if (lane == 0) top_n[q] = na;
 }
 
+// knn_merge_kernel for small batches: four waves each merge every fourth
+// listed slice, then wave 0 merges the four lists. The best kTop under the
+// total order (distance, id) does not depend on the order of the merges, so
+// the result equals knn_merge_kernel's.
+__global__ __launch_bounds__(256) void knn_merge_wide_kernel(const float *cand_d,
+                                                             const int32_t *cand_i,
+                                                             const int32_t *cand_n,
+                                                             const uint32_t *tau_key, int64_t nq,
+                                                             int slices, float *top_d,
+                                                             int32_t *top_i, int32_t *top_n) {
+    __shared__ int32_t s_list[kMaxSlices];
+    __shared__ int s_count;
+    __shared__ float s_d[4][kTop];
+    __shared__ int32_t s_i[4][kTop];
+    __shared__ int s_na[4];
+    const int64_t q = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (q >= nq) return;
+    const float tau = key_float(tau_key[q]);
+    if (tid == 0) s_count = 0;
+    __syncthreads();
+    if (tid < slices) {
+        // list order is free: the merged result does not depend on it
+        const int n = cand_n[q * slices + tid];
+        if (n > 0 && cand_d[(q * slices + tid) * kCap] <= tau) s_list[atomicAdd(&s_count, 1)] = tid;
+    }
+    __syncthreads();
+    const int listed = s_count;
+    float ad = INFINITY;
+    int32_t ai = INT32_MAX;
+    int na = 0;
+    // merges sorted list B (entry 63 - lane in this lane) into the wave's list
+    auto merge_in = [&](float bd, int32_t bi) {
+        if (bd < ad || (bd == ad && bi < ai)) {
+            ad = bd;
+            ai = bi;
+        }
+#pragma unroll
+        for (int j = 32; j > 0; j >>= 1) {
+            const float dp = __shfl_xor(ad, j, 64);
+            const int32_t ip = __shfl_xor(ai, j, 64);
+            const bool mine_gt = ad > dp || (ad == dp && ai > ip);
+            const bool lower = (lane & j) == 0;
+            if (lower == mine_gt) {
+                ad = dp;
+                ai = ip;
+            }
+        }
+    };
+    float next_d = INFINITY;
+    int32_t next_i = INT32_MAX;
+    int next_n = 0;
+    auto fetch = [&](int li) {
+        const int sl = s_list[li];
+        next_n = cand_n[q * slices + sl];
+        const int64_t base = (q * slices + sl) * kCap;
+        const int r = kTop - 1 - lane;   // reversed
+        next_d = r < next_n ? cand_d[base + r] : INFINITY;
+        next_i = r < next_n ? cand_i[base + r] : INT32_MAX;
+    };
+    if (wave < listed) fetch(wave);
+    for (int li = wave; li < listed; li += 4) {
+        const float bd = next_d;
+        const int32_t bi = next_i;
+        const int nb = next_n;
+        if (li + 4 < listed) fetch(li + 4);
+        merge_in(bd, bi);
+        na = min(kTop, na + nb);
+    }
+    s_d[wave][lane] = ad;
+    s_i[wave][lane] = ai;
+    if (lane == 0) s_na[wave] = na;
+    __syncthreads();
+    if (wave != 0) return;
+#pragma unroll
+    for (int w = 1; w < 4; ++w) {
+        merge_in(s_d[w][kTop - 1 - lane], s_i[w][kTop - 1 - lane]);
+        na = min(kTop, na + s_na[w]);
+    }
+    top_d[q * kTop + lane] = ad;
+    top_i[q * kTop + lane] = ai;
+    if (lane == 0) top_n[q] = na;
+}
+
 // Exact rerank: one wave per query, one lane per candidate; the distance is
 // the reference's sequential f32 sum (no contraction in this build). The
 // candidate rows are staged through LDS 64 features at a time, so every
@@ -887,9 +971,14 @@ void KnnIndex::query(const float *q_dev, int64_t nq, int k, bool exclude_self, i
         default: launch_scan<32>(grid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, per_slice, slices, cand_d_.ptr, cand_i_.ptr, cand_n_.ptr, tau_key_.ptr, ex, query_offset); break;
     }
     HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(knn_merge_kernel, dim3(static_cast<unsigned>(nq)), dim3(64), 0, s,
-                       cand_d_.ptr, cand_i_.ptr, cand_n_.ptr, tau_key_.ptr, nq, slices, top_d_.ptr,
-                       top_i_.ptr, top_n_.ptr);
+    if (nq <= kWideRerankMaxQ)
+        hipLaunchKernelGGL(knn_merge_wide_kernel, dim3(static_cast<unsigned>(nq)), dim3(256), 0, s,
+                           cand_d_.ptr, cand_i_.ptr, cand_n_.ptr, tau_key_.ptr, nq, slices,
+                           top_d_.ptr, top_i_.ptr, top_n_.ptr);
+    else
+        hipLaunchKernelGGL(knn_merge_kernel, dim3(static_cast<unsigned>(nq)), dim3(64), 0, s,
+                           cand_d_.ptr, cand_i_.ptr, cand_n_.ptr, tau_key_.ptr, nq, slices,
+                           top_d_.ptr, top_i_.ptr, top_n_.ptr);
     HIP_CHECK(hipGetLastError());
     if (cand_ids) {
         // prefilter lists only (best first), no rerank
