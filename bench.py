@@ -18,6 +18,12 @@ summary. The CPU baseline times the oracle (C restatement, OpenMP over
 rows) on the same graph and init for the first epochs of the same layout
 and extrapolates by force terms per epoch.
 
+Side legs (each skippable): `knn_batches` times the exact kNN of query
+batches of 1 to 16,384 rows against the 1M references and prices each
+against its roofline (HBM for the BF16 stream below about 310 rows, MFMA
+above); `batched` runs 256 independent layouts as one hipGraph launch;
+`serving` runs the transform server under Poisson arrivals.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--rows N]
 """
 from __future__ import annotations
@@ -169,6 +175,41 @@ def recursive_benchmark(x, compact, a, b, args):
         res["init_trustworthiness@15"] = round(tw0, 4)
         log(f"recursive fit quality {tw:.4f}/{pres:.4f} (init alone {tw0:.4f})")
     return res
+
+
+def knn_batch_benchmark(index, args, sizes=(1, 64, 512, 2048, 4096, 16384), reps=5):
+    """Exact k = 15 kNN of query batches (rows of the same mixture, row seed
+    43) against the resident 1M references: the BF16 prefilter streams the
+    [N, 800] BF16 copy once per batch (HBM bound below about 310 rows, where
+    2 q FLOP per streamed byte meets the 2.5 PFLOP/s / 8 TB/s ridge), then
+    the merge and the FP32 rerank. Mean of `reps` timed calls after two
+    warm-up calls."""
+    import torch
+
+    n, d = index.n, index.d
+    dp = (d + 31) // 32 * 32
+    stream_bytes = n * dp * 2 + n * 4            # BF16 rows + norms
+    xq = make_data(max(sizes), args.dims, 10, 42, torch.device("cuda:0"), row_seed=43)
+    out = []
+    for q in sizes:
+        qs = xq[:q].contiguous()
+        for _ in range(2):
+            index.query_device(qs, 15)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            index.query_device(qs, 15)
+        torch.cuda.synchronize()
+        t = (time.perf_counter() - t0) / reps
+        flops = 2.0 * q * n * dp
+        hbm = q < 310
+        achieved = stream_bytes / t / 1e9 if hbm else flops / t / 1e12
+        peak = 8000.0 if hbm else 2500.0
+        out.append({"rows": q, "ms": round(t * 1e3, 3), "bound": "hbm" if hbm else "mfma",
+                    "achieved": round(achieved, 1), "unit": "GB/s" if hbm else "TFLOP/s",
+                    "peak": peak, "frac": round(achieved / peak, 4)})
+        log(f"knn batch {q}: {t * 1e3:.3f} ms, {achieved:.1f} {'GB/s' if hbm else 'TFLOP/s'}")
+    return out
 
 
 def serve_benchmark(x, emb, compact_index, a, b, args):
@@ -466,6 +507,7 @@ def main():
                          "graph, and with --quality the fit that starts from it")
     ap.add_argument("--no-serve", action="store_true", help="skip the transform serving leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-layouts leg")
+    ap.add_argument("--no-knn", action="store_true", help="skip the kNN batch-size leg")
     ap.add_argument("--batch-graphs", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--serve-requests", type=int, default=400)
@@ -565,6 +607,9 @@ def main():
     if args.recursive:
         log("recursive initialisation ...")
         out["recursive_init"] = recursive_benchmark(x, compact, a, b, args)
+    if not args.no_knn and args.rows >= 16384:
+        log("kNN batches ...")
+        out["knn_batches"] = knn_batch_benchmark(index, args)
     if not args.no_batch:
         log("batched layouts ...")
         out["batched"] = batch_benchmark(args)
