@@ -57,7 +57,7 @@ constexpr int kSample = 8192;      // references in the sample bound
 constexpr int kSampleRefs = 128;   // sample references per workgroup of the sample distances
 constexpr int kSampleMinSlices = 32;   // batches with at least this many slices use the bound
 constexpr int64_t kSampleMinRefs = 8 * kSample;
-constexpr int64_t kWideRerankMaxQ = 8192;   // batches up to this many rows merge and rerank with 4 waves
+constexpr int64_t kWideMergeMaxQ = 8192;   // batches up to this many rows merge with 4 waves
 
 __device__ inline uint16_t f32_to_bf16_rne(float f) {
     uint32_t u = __float_as_uint(f);
@@ -712,72 +712,13 @@ __global__ __launch_bounds__(256) void knn_merge_wide_kernel(const float *cand_d
     if (lane == 0) top_n[q] = na;
 }
 
-// Exact rerank: one wave per query, one lane per candidate; the distance is
-// the reference's sequential f32 sum (no contraction in this build). The
-// candidate rows are staged through LDS 64 features at a time, so every
-// global load is a contiguous 256-byte segment of one row.
-__global__ __launch_bounds__(64) void knn_rerank_kernel(const float *query, int64_t nq,
-                                                        const float *ref, int64_t nr, int64_t d,
-                                                        const int32_t *top_i, const int32_t *top_n,
-                                                        int k, int32_t *out_i, float *out_d) {
-    constexpr int FB = 64;
-    __shared__ float s_q[FB];
-    __shared__ float s_r[kTop][FB + 1];
-    const int64_t q = blockIdx.x;
-    const int lane = threadIdx.x;
-    if (q >= nq) return;
-    const int n = top_n[q];
-    const int32_t id = lane < n ? top_i[q * kTop + lane] : INT32_MAX;
-    float acc = 0.0f;
-    for (int64_t f0 = 0; f0 < d; f0 += FB) {
-        const int fb = static_cast<int>(min<int64_t>(FB, d - f0));
-        __syncthreads();
-        s_q[lane] = lane < fb ? query[q * d + f0 + lane] : 0.0f;
-        for (int c0 = 0; c0 < n; c0 += 16) {
-            // sixteen row segments in flight, then into LDS
-            float v[16];
-#pragma unroll
-            for (int u = 0; u < 16; ++u) {
-                const int c = c0 + u;
-                const int64_t row = __shfl(id, c & 63, 64);
-                v[u] = (c < n && lane < fb) ? ref[row * d + f0 + lane] : 0.0f;
-            }
-#pragma unroll
-            for (int u = 0; u < 16; ++u)
-                if (c0 + u < n) s_r[c0 + u][lane] = v[u];
-        }
-        __syncthreads();
-        if (lane < n) {
-            for (int f = 0; f < fb; ++f) {
-                const float delta = s_q[f] - s_r[lane][f];
-                acc += delta * delta;
-            }
-        }
-    }
-    const float dist = lane < n ? acc : INFINITY;
-    int rank = 0;
-    for (int f = 0; f < 64; ++f) {
-        const float df = __shfl(dist, f, 64);
-        const int32_t idf = __shfl(id, f, 64);
-        rank += (df < dist || (df == dist && idf < id)) ? 1 : 0;
-    }
-    if (lane < n && rank < k) {
-        out_i[q * k + rank] = id;
-        out_d[q * k + rank] = sqrtf(dist);
-    }
-    if (lane >= n && lane < k) {
-        // fewer candidates than k: sentinels fill the tail (rank n.. k-1)
-        out_i[q * k + lane] = static_cast<int32_t>(nr);
-        out_d[q * k + lane] = sqrtf(FLT_MAX);
-    }
-}
-
-// knn_rerank_kernel for small batches, where each query's chain of staging
-// rounds (13 blocks of 64 features, 4 rounds of 16 rows each) was the cost:
-// four waves load the 64 candidate rows of a 128-feature block at once (32
-// loads in flight per lane), and wave 0 sums them in the same sequential
-// feature order, so distances are bit-identical to knn_rerank_kernel's.
-__global__ __launch_bounds__(256) void knn_rerank_wide_kernel(const float *query, int64_t nq,
+// Exact rerank: one workgroup of four waves per query, one lane of wave 0
+// per candidate; the distance is the reference's sequential f32 sum over the
+// features (no contraction in this build). The four waves stage a
+// 128-feature block of all 64 candidate rows through LDS at once (32 loads
+// in flight per lane, every load part of a contiguous row segment), then
+// wave 0 adds the block in feature order.
+__global__ __launch_bounds__(256) void knn_rerank_kernel(const float *query, int64_t nq,
                                                               const float *ref, int64_t nr,
                                                               int64_t d, const int32_t *top_i,
                                                               const int32_t *top_n, int k,
@@ -971,7 +912,7 @@ void KnnIndex::query(const float *q_dev, int64_t nq, int k, bool exclude_self, i
         default: launch_scan<32>(grid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, per_slice, slices, cand_d_.ptr, cand_i_.ptr, cand_n_.ptr, tau_key_.ptr, ex, query_offset); break;
     }
     HIP_CHECK(hipGetLastError());
-    if (nq <= kWideRerankMaxQ)
+    if (nq <= kWideMergeMaxQ)
         hipLaunchKernelGGL(knn_merge_wide_kernel, dim3(static_cast<unsigned>(nq)), dim3(256), 0, s,
                            cand_d_.ptr, cand_i_.ptr, cand_n_.ptr, tau_key_.ptr, nq, slices,
                            top_d_.ptr, top_i_.ptr, top_n_.ptr);
@@ -988,16 +929,12 @@ void KnnIndex::query(const float *q_dev, int64_t nq, int k, bool exclude_self, i
                                  hipMemcpyDeviceToDevice, s));
         return;
     }
-    // small batches: one wide workgroup per query (its staging rounds are
-    // the latency); large batches: one wave per query
-    if (nq <= kWideRerankMaxQ)
-        hipLaunchKernelGGL(knn_rerank_wide_kernel, dim3(static_cast<unsigned>(nq)), dim3(256), 0, s,
-                           q_f32_for_rerank ? q_f32_for_rerank : q_dev, nq, ref_.ptr, n_, d_,
-                           top_i_.ptr, top_n_.ptr, k, idx_out, dist_out);
-    else
-        hipLaunchKernelGGL(knn_rerank_kernel, dim3(static_cast<unsigned>(nq)), dim3(64), 0, s,
-                           q_f32_for_rerank ? q_f32_for_rerank : q_dev, nq, ref_.ptr, n_, d_,
-                           top_i_.ptr, top_n_.ptr, k, idx_out, dist_out);
+    // one four-wave workgroup per query: its staging rounds are the
+    // latency at small batches, and it also has more loads in flight per CU
+    // at large ones (87 ns per query with one wave at 16,384 rows)
+    hipLaunchKernelGGL(knn_rerank_kernel, dim3(static_cast<unsigned>(nq)), dim3(256), 0, s,
+                       q_f32_for_rerank ? q_f32_for_rerank : q_dev, nq, ref_.ptr, n_, d_,
+                       top_i_.ptr, top_n_.ptr, k, idx_out, dist_out);
     HIP_CHECK(hipGetLastError());
 }
 
