@@ -151,6 +151,9 @@ SIGNATURES = {
     "umaphip_plan_download": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(LayoutStats)]),
     "umaphip_plan_stream": (_vp, [_vp]),
     "umaphip_plan_set_profiling": (C.c_int, [_vp, _i32]),
+    "umaphip_plan_set_record_format": (C.c_int, [_vp, _i32]),
+    "umaphip_plan_set_trace": (C.c_int, [_vp, _i32, _i32]),
+    "umaphip_plan_trace": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "umaphip_plan_destroy": (None, [_vp]),
     "umaphip_knn_index_create": (C.c_int, [_vp, _i32, _i64, _i64, _i64, C.POINTER(_vp)]),
     "umaphip_knn_query": (C.c_int, [_vp, _vp, _i64, _i32, _i32, _i64, _vp, _vp, _vp]),

@@ -246,6 +246,7 @@ struct umaphip_plan {
     // edge-balanced unit sweep (Adam, d = 2, H > 1024)
     bool unit_path = false;
     bool compact = false;   // 24-byte records (rank-ordered plans whose periods allow it)
+    bool compactable = false;  // every period is rebuilt exactly from its f32 sample count
     int64_t units = 0;
     DeviceArray<double> rec;
     DeviceArray<uint16_t> fire;
@@ -256,6 +257,16 @@ struct umaphip_plan {
     int64_t ranked_epochs = 0;
     int32_t row_cap_id = 0, row_cap_ranked = 0;   // unit row-table capacity per order
     DeviceArray<int32_t> pref, pref_blocks, rank_unit_k, rank_seg, rank_deg;
+
+    // optional trace of one unit-sweep epoch: the state it starts from
+    // (positions, negative order, ranks, scale), each row's assembled
+    // gradient and every negative drawn per edge
+    int32_t trace_epoch = -1, trace_cap = 0;
+    bool traced = false;
+    int32_t trace_info[3] = {0, 0, 0};   // rank-ordered, compact records, row-table capacity
+    DeviceArray<float> tr_head, tr_grad, tr_scale;
+    DeviceArray<int32_t> tr_order, tr_ranks, tr_count, tr_cand, tr_misc;  // misc: overflow, paths[2]
+    DeviceArray<uint8_t> tr_kept;
 
     std::vector<float> alpha, mu, c1, c2;
     float scale0 = 1.0f;
@@ -364,6 +375,11 @@ void umaphip_plan::run() {
     sweep_used = 0;
     refresh_used = 0;
     obj_count = 0;
+    traced = false;
+    if (trace_epoch >= 0) {
+        HIP_CHECK(hipMemsetAsync(tr_count.ptr, 0xFF, tr_count.bytes(), stream));   // -1: no fire
+        HIP_CHECK(hipMemsetAsync(tr_misc.ptr, 0, tr_misc.bytes(), stream));
+    }
     HIP_CHECK(hipEventRecord(ev_start, stream));
     OrderReplay orders(h, n, p.seed, alias, hard, momentum);
     const bool upload_negative = !hard;  // global mode reads host orders
@@ -489,15 +505,35 @@ void umaphip_plan::run() {
                     U.unit_k = rank_unit_k.ptr;
                     ++ranked_epochs;
                 }
+                const bool tracing = t == trace_epoch;
+                if (tracing) {
+                    const hipMemcpyKind dd = hipMemcpyDeviceToDevice;
+                    HIP_CHECK(hipMemcpyAsync(tr_head.ptr, pos[c].ptr, tr_head.bytes(), dd, stream));
+                    HIP_CHECK(hipMemcpyAsync(tr_order.ptr, order_now, tr_order.bytes(), dd, stream));
+                    HIP_CHECK(hipMemcpyAsync(tr_ranks.ptr, hard ? ranks.ptr : iota.ptr,
+                                             tr_ranks.bytes(), dd, stream));
+                    HIP_CHECK(hipMemcpyAsync(tr_scale.ptr, scale_dev.ptr, sizeof(float), dd, stream));
+                    U.tr_count = tr_count.ptr;
+                    U.tr_cand = tr_cand.ptr;
+                    U.tr_kept = tr_kept.ptr;
+                    U.tr_cap = trace_cap;
+                    U.tr_overflow = tr_misc.ptr;
+                    U.tr_paths = tr_misc.ptr + 1;
+                    trace_info[0] = by_rank ? 1 : 0;
+                    trace_info[1] = compact ? 1 : 0;
+                    trace_info[2] = U.row_cap;
+                    traced = true;
+                }
+                float *grad_out = tracing ? tr_grad.ptr : nullptr;
                 launch_unit_sweep(U, stream);
                 if (by_rank)
                     launch_unit_adam_ranked(h, pos[c].ptr, pos[1 - c].ptr, grad.ptr, ranks.ptr,
                                             pref.ptr, frag.ptr, m.ptr, v.ptr, alpha[t], c1[t],
-                                            c2[t], stream);
+                                            c2[t], stream, grad_out);
                 else
                     launch_unit_adam(h, pos[c].ptr, pos[1 - c].ptr, grad.ptr, row_split.ptr,
                                      split_off.ptr, frag.ptr, m.ptr, v.ptr, alpha[t], c1[t],
-                                     c2[t], stream);
+                                     c2[t], stream, grad_out);
             } else {
                 launch_edge_sweep(A, stream);
             }
@@ -751,8 +787,9 @@ umaphip_status umaphip_plan_create(int64_t h, int64_t n, int64_t d, const float 
             P->ranked = P->hard && T.nz_rows.empty() && E < INT32_MAX;
             // Rank-ordered plans keep 24-byte records when every period is
             // rebuilt exactly from an f32 sample count.
-            P->compact = P->ranked &&
-                         periods_compactable(periods, E, static_cast<double>(params->epochs));
+            P->compactable = P->ranked &&
+                             periods_compactable(periods, E, static_cast<double>(params->epochs));
+            P->compact = P->compactable;
             P->rec.alloc(static_cast<size_t>((P->compact ? 3 : 4) * E));
             // Row-table capacity: the rows of a unit in CSR order are known;
             // in any rank order a unit of UE slots holds at most
@@ -876,6 +913,86 @@ umaphip_status umaphip_cross_entropy(const float *head, int64_t h, const float *
         (void)n;
         launch_cross_entropy(head, h, tail ? tail : head, d, offsets, indices, weights, a, b,
                              static_cast<double *>(workspace), out, as_stream(stream));
+    });
+}
+
+umaphip_status umaphip_plan_set_record_format(umaphip_plan *plan, int32_t compact) {
+    return guarded([&] {
+        UMAPHIP_REQUIRE(plan != nullptr, UMAPHIP_INVALID_INPUT, "plan is null");
+        UMAPHIP_REQUIRE(plan->unit_path, UMAPHIP_INVALID_CONFIG,
+                        "edge records exist only in plans that use the edge-balanced sweep");
+        const bool want = compact != 0;
+        UMAPHIP_REQUIRE(!want || plan->compactable, UMAPHIP_INVALID_CONFIG,
+                        "24-byte records need a rank-ordered plan whose periods are rebuilt "
+                        "exactly from f32 sample counts");
+        if (want == plan->compact) return;
+        HIP_CHECK(hipStreamSynchronize(plan->stream));
+        plan->compact = want;
+        plan->rec.alloc(static_cast<size_t>((want ? 3 : 4) * plan->E));
+    });
+}
+
+umaphip_status umaphip_plan_set_trace(umaphip_plan *plan, int32_t epoch, int32_t negative_cap) {
+    return guarded([&] {
+        UMAPHIP_REQUIRE(plan != nullptr, UMAPHIP_INVALID_INPUT, "plan is null");
+        if (epoch < 0) {
+            plan->trace_epoch = -1;
+            return;
+        }
+        UMAPHIP_REQUIRE(plan->unit_path, UMAPHIP_INVALID_CONFIG,
+                        "the epoch trace covers the edge-balanced sweep (Adam, d = 2, more than "
+                        "1024 rows)");
+        UMAPHIP_REQUIRE(epoch < plan->p.epochs, UMAPHIP_INVALID_CONFIG,
+                        "trace epoch %d is not below the %d epochs", epoch, plan->p.epochs);
+        UMAPHIP_REQUIRE(negative_cap >= 1 && negative_cap <= 1024, UMAPHIP_INVALID_CONFIG,
+                        "negative capacity must be in 1..=1024");
+        const int64_t h = plan->h, n = plan->n, E = plan->E;
+        const size_t slots = static_cast<size_t>(E) * static_cast<size_t>(negative_cap);
+        HIP_CHECK(hipStreamSynchronize(plan->stream));
+        plan->tr_head.alloc(static_cast<size_t>(2 * h));
+        plan->tr_grad.alloc(static_cast<size_t>(2 * h));
+        plan->tr_scale.alloc(1);
+        plan->tr_order.alloc(static_cast<size_t>(n));
+        plan->tr_ranks.alloc(static_cast<size_t>(h));
+        plan->tr_count.alloc(static_cast<size_t>(E));
+        plan->tr_cand.alloc(slots);
+        plan->tr_kept.alloc(slots);
+        plan->tr_misc.alloc(3);
+        plan->trace_epoch = epoch;
+        plan->trace_cap = negative_cap;
+    });
+}
+
+umaphip_status umaphip_plan_trace(umaphip_plan *plan, float *head_before, int32_t *order,
+                                  int32_t *ranks, float *scale, float *gradient,
+                                  int32_t *negative_count, int32_t *negative_candidates,
+                                  uint8_t *negative_kept, int32_t *info) {
+    return guarded([&] {
+        UMAPHIP_REQUIRE(plan != nullptr && info != nullptr, UMAPHIP_INVALID_INPUT,
+                        "plan trace arguments are null");
+        HIP_CHECK(hipStreamSynchronize(plan->stream));
+        UMAPHIP_REQUIRE(plan->traced, UMAPHIP_INVALID_CONFIG,
+                        "no traced epoch: set a trace epoch, then run the plan");
+        int32_t misc[3] = {0, 0, 0};
+        HIP_CHECK(hipMemcpy(misc, plan->tr_misc.ptr, sizeof(misc), hipMemcpyDeviceToHost));
+        UMAPHIP_REQUIRE(misc[0] == 0, UMAPHIP_CAPACITY,
+                        "an edge drew more than %d negatives in the traced epoch", plan->trace_cap);
+        info[0] = plan->trace_info[0];
+        info[1] = plan->trace_info[1];
+        info[2] = plan->trace_info[2];
+        info[3] = misc[1];
+        info[4] = misc[2];
+        auto down = [](void *dst, const void *src, size_t bytes) {
+            if (dst && bytes) HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+        };
+        down(head_before, plan->tr_head.ptr, plan->tr_head.bytes());
+        down(order, plan->tr_order.ptr, plan->tr_order.bytes());
+        down(ranks, plan->tr_ranks.ptr, plan->tr_ranks.bytes());
+        down(scale, plan->tr_scale.ptr, sizeof(float));
+        down(gradient, plan->tr_grad.ptr, plan->tr_grad.bytes());
+        down(negative_count, plan->tr_count.ptr, plan->tr_count.bytes());
+        down(negative_candidates, plan->tr_cand.ptr, plan->tr_cand.bytes());
+        down(negative_kept, plan->tr_kept.ptr, plan->tr_kept.bytes());
     });
 }
 

@@ -832,7 +832,9 @@ __device__ inline void segmented_accumulate(float2 term, int row, int lane, floa
 // per SIMD) instead of about 14 KB (3 waves).
 // COMPACT (rank-ordered only): 24-byte records, the period rebuilt as
 // per_num / samples and e mod M from the edge index.
-template <bool RANKED, int RC, bool COMPACT>
+// TRACE: also records every drawn negative (candidate id and whether it
+// repels) per CSR edge, with the same selection code (tests).
+template <bool RANKED, int RC, bool COMPACT, bool TRACE>
 __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
     static_assert(RANKED || !COMPACT, "compact records are read by rank only");
     const SweepArgs &A = U.s;
@@ -1030,6 +1032,10 @@ __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
                 }
                 if constexpr (!RANKED) U.fire[e] = fire_shadow(npos2);
                 const int negs = count > 0 ? count : 0;
+                if constexpr (TRACE) {
+                    U.tr_count[e] = negs;
+                    if (negs > U.tr_cap) atomicOr(U.tr_overflow, 1);
+                }
                 t = 1 + negs;
                 const uint32_t em =
                     COMPACT ? static_cast<uint32_t>(mod_u64(static_cast<uint64_t>(e), modulus, U.inv_mod))
@@ -1060,6 +1066,9 @@ __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
         for (int k = 0; k < EPL; ++k)
             for (int q = 0; q < t_cnt[k]; ++q)
                 s_owner[t_base[k] + q] = static_cast<uint8_t>(k * 64 + lane);
+    }
+    if constexpr (TRACE) {
+        if (lane == 0) atomicAdd(U.tr_paths + (term_parallel ? 0 : 1), 1);
     }
     __syncthreads();
 
@@ -1099,6 +1108,14 @@ __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
                 } else {
                     y[g] = A.table ? ld2(A.table, rank)
                                    : ld2(A.tgt, static_cast<uint32_t>(A.order[rank]));
+                }
+                if constexpr (TRACE) {
+                    const int64_t e = RANKED ? s_seg[rr] + (e0 + owner - s_pst[rr]) : e0 + owner;
+                    if (kq <= U.tr_cap) {
+                        const int64_t at = e * U.tr_cap + (kq - 1);
+                        U.tr_cand[at] = A.order[rank];
+                        U.tr_kept[at] = keep[g] ? 1 : 0;
+                    }
                 }
             }
 #pragma unroll
@@ -1149,6 +1166,18 @@ __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
                     }
                     const uint64_t rank = row_first(rr) + sel;
                     float2 yn;
+                    if constexpr (TRACE) {
+                        if (p < U.tr_cap) {
+                            const int64_t at = eidx[k] * U.tr_cap + p;
+                            const int64_t jt = static_cast<uint32_t>(A.order[rank]);
+                            U.tr_cand[at] = static_cast<int32_t>(jt);
+                            U.tr_kept[at] =
+                                A.exclude && excluded(A.indices, s_seg[rr],
+                                                      RANKED ? s_end[rr] : s_seg[rr + 1],
+                                                      s_rowid[rr], jt, A.shared != 0)
+                                    ? 0 : 1;
+                        }
+                    }
                     if (A.exclude) {
                         const int64_t jj = static_cast<uint32_t>(A.order[rank]);
                         if (excluded(A.indices, s_seg[rr], RANKED ? s_end[rr] : s_seg[rr + 1],
@@ -1222,7 +1251,7 @@ __global__ void unit_unpack_kernel(const double *rec, int64_t E, bool compact, d
 __global__ void unit_adam_kernel(int64_t h, const float *pos, float *pos_out, const float *grad,
                                  const int32_t *row_split, const int32_t *split_off,
                                  const float *frag, float *m, float *v, float alpha, float c1,
-                                 float c2) {
+                                 float c2, float *grad_out) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= h) return;
     const int32_t s = row_split[i];
@@ -1237,6 +1266,7 @@ __global__ void unit_adam_kernel(int64_t h, const float *pos, float *pos_out, co
             g.y = g.y + f.y;
         }
     }
+    if (grad_out) reinterpret_cast<float2 *>(grad_out)[i] = g;
     const float2 x = ld2(pos, i);
     float2 mm = ld2(m, i), vv = ld2(v, i);
     float2 out = x;
@@ -1262,7 +1292,7 @@ __global__ void unit_adam_kernel(int64_t h, const float *pos, float *pos_out, co
 __global__ void unit_adam_ranked_kernel(int64_t h, const float *pos, float *pos_out,
                                         const float *grad, const int32_t *ranks,
                                         const int32_t *pref, const float *frag, float *m, float *v,
-                                        float alpha, float c1, float c2) {
+                                        float alpha, float c1, float c2, float *grad_out) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= h) return;
     const int32_t r = ranks[i];
@@ -1282,6 +1312,7 @@ __global__ void unit_adam_ranked_kernel(int64_t h, const float *pos, float *pos_
             g.y = g.y + f.y;
         }
     }
+    if (grad_out) reinterpret_cast<float2 *>(grad_out)[i] = g;
     const float2 x = ld2(pos, i);
     float2 mm = ld2(m, i), vv = ld2(v, i);
     float2 out = x;
@@ -1981,6 +2012,20 @@ void launch_edge_sweep(const SweepArgs &a, hipStream_t s) {
 int64_t unit_edges() { return kUnitEdges; }
 int64_t unit_narrow_rows() { return kUnitNarrowRows; }
 
+template <bool TRACE>
+void launch_unit_sweep_shape(const UnitSweepArgs &a, dim3 grid, bool narrow, hipStream_t s) {
+    if (a.pref && a.compact) {
+        if (narrow) hipLaunchKernelGGL((unit_sweep_kernel<true, kUnitNarrowRows, true, TRACE>), grid, dim3(64), 0, s, a);
+        else hipLaunchKernelGGL((unit_sweep_kernel<true, kUnitEdges, true, TRACE>), grid, dim3(64), 0, s, a);
+    } else if (a.pref) {
+        if (narrow) hipLaunchKernelGGL((unit_sweep_kernel<true, kUnitNarrowRows, false, TRACE>), grid, dim3(64), 0, s, a);
+        else hipLaunchKernelGGL((unit_sweep_kernel<true, kUnitEdges, false, TRACE>), grid, dim3(64), 0, s, a);
+    } else {
+        if (narrow) hipLaunchKernelGGL((unit_sweep_kernel<false, kUnitNarrowRows, false, TRACE>), grid, dim3(64), 0, s, a);
+        else hipLaunchKernelGGL((unit_sweep_kernel<false, kUnitEdges, false, TRACE>), grid, dim3(64), 0, s, a);
+    }
+}
+
 void launch_unit_sweep(UnitSweepArgs a, hipStream_t s) {
     if (a.units == 0) return;
     const uint64_t modulus = a.s.range;
@@ -1991,15 +2036,12 @@ void launch_unit_sweep(UnitSweepArgs a, hipStream_t s) {
     const bool narrow = a.row_cap == kUnitNarrowRows;
     UMAPHIP_REQUIRE(a.pref || !a.compact, UMAPHIP_BACKEND,
                     "unit sweep: compact records need the rank-ordered sweep");
-    if (a.pref && a.compact) {
-        if (narrow) hipLaunchKernelGGL((unit_sweep_kernel<true, kUnitNarrowRows, true>), grid, dim3(64), 0, s, a);
-        else hipLaunchKernelGGL((unit_sweep_kernel<true, kUnitEdges, true>), grid, dim3(64), 0, s, a);
-    } else if (a.pref) {
-        if (narrow) hipLaunchKernelGGL((unit_sweep_kernel<true, kUnitNarrowRows, false>), grid, dim3(64), 0, s, a);
-        else hipLaunchKernelGGL((unit_sweep_kernel<true, kUnitEdges, false>), grid, dim3(64), 0, s, a);
+    if (a.tr_count) {
+        UMAPHIP_REQUIRE(a.tr_cap > 0 && a.tr_cand && a.tr_kept && a.tr_overflow && a.tr_paths,
+                        UMAPHIP_INVALID_INPUT, "unit sweep trace buffers are null");
+        launch_unit_sweep_shape<true>(a, grid, narrow, s);
     } else {
-        if (narrow) hipLaunchKernelGGL((unit_sweep_kernel<false, kUnitNarrowRows, false>), grid, dim3(64), 0, s, a);
-        else hipLaunchKernelGGL((unit_sweep_kernel<false, kUnitEdges, false>), grid, dim3(64), 0, s, a);
+        launch_unit_sweep_shape<false>(a, grid, narrow, s);
     }
     HIP_CHECK(hipGetLastError());
 }
@@ -2022,10 +2064,11 @@ void launch_rank_prefix(const int32_t *rank_deg, int64_t n, int32_t total_edges,
 
 void launch_unit_adam_ranked(int64_t h, const float *pos, float *pos_out, const float *grad,
                              const int32_t *ranks, const int32_t *pref, const float *frag,
-                             float *m, float *v, float alpha, float c1, float c2, hipStream_t s) {
+                             float *m, float *v, float alpha, float c1, float c2, hipStream_t s,
+                             float *grad_out) {
     if (h == 0) return;
     hipLaunchKernelGGL(unit_adam_ranked_kernel, dim3(grid_for(h)), dim3(kThreads), 0, s, h, pos,
-                       pos_out, grad, ranks, pref, frag, m, v, alpha, c1, c2);
+                       pos_out, grad, ranks, pref, frag, m, v, alpha, c1, c2, grad_out);
     HIP_CHECK(hipGetLastError());
 }
 
@@ -2112,10 +2155,11 @@ bool periods_compactable(const double *periods, int64_t E, double per_num) {
 
 void launch_unit_adam(int64_t h, const float *pos, float *pos_out, const float *grad,
                       const int32_t *row_split, const int32_t *split_off, const float *frag,
-                      float *m, float *v, float alpha, float c1, float c2, hipStream_t s) {
+                      float *m, float *v, float alpha, float c1, float c2, hipStream_t s,
+                      float *grad_out) {
     if (h == 0) return;
     hipLaunchKernelGGL(unit_adam_kernel, dim3(grid_for(h)), dim3(kThreads), 0, s, h, pos, pos_out,
-                       grad, row_split, split_off, frag, m, v, alpha, c1, c2);
+                       grad, row_split, split_off, frag, m, v, alpha, c1, c2, grad_out);
     HIP_CHECK(hipGetLastError());
 }
 

@@ -123,6 +123,16 @@ struct UnitSweepArgs {
     double inv_mod;            // 1 / M (set by the launcher)
     bool compact;              // 24-byte records (rank-ordered sweep only)
     double per_num;            // compact: period = per_num / samples
+    // Epoch trace (null tr_count: off). Per CSR edge: tr_count[e] = the
+    // negatives drawn this epoch (left untouched when the edge does not
+    // fire), tr_cand[e * tr_cap + p] = candidate id of sample p and
+    // tr_kept[...] = 1 unless exclusion dropped it; tr_overflow is set when
+    // an edge draws more than tr_cap; tr_paths[0] / [1] count the units that
+    // took the term-parallel / per-edge path.
+    int32_t *tr_count, *tr_cand;
+    uint8_t *tr_kept;
+    int32_t tr_cap;
+    int32_t *tr_overflow, *tr_paths;
 };
 int64_t unit_edges();
 int64_t unit_narrow_rows();
@@ -144,12 +154,16 @@ bool periods_compactable(const double *periods, int64_t E, double per_num);
 int64_t rank_prefix_blocks(int64_t n);
 void launch_rank_prefix(const int32_t *rank_deg, int64_t n, int32_t total_edges, int32_t *pref,
                         int32_t *block_sums, int32_t *unit_k, hipStream_t s);
+// grad_out (may be null) receives each row's assembled gradient, the value
+// the Adam step consumes.
 void launch_unit_adam_ranked(int64_t h, const float *pos, float *pos_out, const float *grad,
                              const int32_t *ranks, const int32_t *pref, const float *frag,
-                             float *m, float *v, float alpha, float c1, float c2, hipStream_t s);
+                             float *m, float *v, float alpha, float c1, float c2, hipStream_t s,
+                             float *grad_out = nullptr);
 void launch_unit_adam(int64_t h, const float *pos, float *pos_out, const float *grad,
                       const int32_t *row_split, const int32_t *split_off, const float *frag,
-                      float *m, float *v, float alpha, float c1, float c2, hipStream_t s);
+                      float *m, float *v, float alpha, float c1, float c2, hipStream_t s,
+                      float *grad_out = nullptr);
 
 // FP64 fuzzy-set cross entropy over the positive edges (doubles of workspace:
 // cross_entropy_workspace(h)); *out is a device double.

@@ -136,6 +136,37 @@ class LayoutPlan:
                                                          C.byref(count)))
         return out
 
+    def set_record_format(self, compact: bool) -> None:
+        """24-byte (compact) or 32-byte edge records for the edge-balanced sweep."""
+        _lib.check(_lib.lib().umaphip_plan_set_record_format(self._handle, int(bool(compact))))
+
+    def set_trace(self, epoch: int, negative_cap: int = 16) -> None:
+        """Record one epoch of the edge-balanced sweep on later runs (epoch < 0: off)."""
+        self._trace_cap = int(negative_cap)
+        _lib.check(_lib.lib().umaphip_plan_set_trace(self._handle, int(epoch), int(negative_cap)))
+
+    def trace(self) -> dict:
+        """The last run's trace of the epoch chosen by ``set_trace``: the
+        state the sweep started from, each row's assembled gradient and the
+        negatives every firing edge drew (count -1: the edge did not fire)."""
+        cap = self._trace_cap
+        out = dict(head=np.zeros((self.h, 2), np.float32), order=np.zeros(self.n, np.int32),
+                   ranks=np.zeros(self.h, np.int32), scale=np.zeros(1, np.float32),
+                   gradient=np.zeros((self.h, 2), np.float32),
+                   count=np.zeros(self.n_edges, np.int32),
+                   candidates=np.zeros((self.n_edges, cap), np.int32),
+                   kept=np.zeros((self.n_edges, cap), np.uint8))
+        info = np.zeros(5, np.int32)
+        _lib.check(_lib.lib().umaphip_plan_trace(
+            self._handle, _ptr(out["head"]), _ptr(out["order"]), _ptr(out["ranks"]),
+            _ptr(out["scale"]), _ptr(out["gradient"]), _ptr(out["count"]),
+            _ptr(out["candidates"]), _ptr(out["kept"]), _ptr(info)))
+        out["scale"] = float(out["scale"][0])
+        out.update(rank_ordered=bool(info[0]), compact_records=bool(info[1]),
+                   row_cap=int(info[2]), term_parallel_units=int(info[3]),
+                   per_edge_units=int(info[4]))
+        return out
+
     def set_profiling(self, enabled: bool = True) -> None:
         _lib.check(_lib.lib().umaphip_plan_set_profiling(self._handle, int(bool(enabled))))
 

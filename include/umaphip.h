@@ -226,6 +226,37 @@ umaphip_status umaphip_plan_download(umaphip_plan *plan, float *head, double *ne
 /* Records device events around every fused edge sweep and projection
  * refresh of later runs (per-kernel durations in the stats). */
 umaphip_status umaphip_plan_set_profiling(umaphip_plan *plan, int32_t enabled);
+/* Edge-record format of a plan on the edge-balanced sweep (no reference
+ * counterpart; the reference keeps its clocks as four f64 arrays):
+ * compact = 1 selects 24-byte records {next_positive, next_negative,
+ * column | f32 sample count} (rank-ordered plans whose every period equals
+ * epochs / f64(f32(epochs / period))), 0 selects 32-byte records that keep
+ * the f64 period. Results are identical; plans default to 24 bytes when
+ * allowed. */
+umaphip_status umaphip_plan_set_record_format(umaphip_plan *plan, int32_t compact);
+/* Epoch trace, a test hook with no reference counterpart: it stands where
+ * the reference's runtime observers watch one epoch's state
+ * (EPOCH_OBSERVER / FORCE_OBSERVER, crates/umap/src/standard_umap/
+ * modern_runtime.rs:56-59, 1552-1939). For plans on the edge-balanced sweep
+ * only. With epoch >= 0, later runs record, at that epoch, the state the
+ * sweep starts from and what it produced; epoch < 0 turns it off.
+ * negative_cap bounds the negatives recorded per edge. */
+umaphip_status umaphip_plan_set_trace(umaphip_plan *plan, int32_t epoch, int32_t negative_cap);
+/* Synchronises and copies the last run's trace (NULL outputs are skipped):
+ * head_before [h*2] positions at the start of the epoch; order [n] the
+ * negative order in force; ranks [h] the window ranks (identity when
+ * negatives are global); scale [1] the repulsion scale in force;
+ * gradient [h*2] each row's assembled gradient, the value the lazy Adam
+ * step consumes; negative_count [E] the negatives drawn by each edge, -1
+ * when it did not fire; negative_candidates / negative_kept [E*cap] the
+ * candidate id of each draw and 1 unless exclusion dropped it. info [5]:
+ * rank-ordered sweep, 24-byte records, unit row-table capacity, units on
+ * the term-parallel path, units on the per-edge path. UMAPHIP_CAPACITY
+ * when an edge drew more than negative_cap. */
+umaphip_status umaphip_plan_trace(umaphip_plan *plan, float *head_before, int32_t *order,
+                                  int32_t *ranks, float *scale, float *gradient,
+                                  int32_t *negative_count, int32_t *negative_candidates,
+                                  uint8_t *negative_kept, int32_t *info);
 /* The plan's stream (hipStream_t as void*). */
 void *umaphip_plan_stream(umaphip_plan *plan);
 void umaphip_plan_destroy(umaphip_plan *plan);

@@ -128,6 +128,8 @@ def lib():
                                          C.c_float, C.c_int]
         L.orc_gradient_block64.argtypes = [C.POINTER(EpochArgs), C.c_int64, C.c_int64,
                                            C.c_double, f64p]
+        L.orc_gradient_block64_mag.argtypes = [C.POINTER(EpochArgs), C.c_int64, C.c_int64,
+                                               C.c_double, f64p, f64p]
         L.orc_adam_apply.argtypes = [C.POINTER(EpochArgs), C.c_int]
         L.orc_momentum_apply_block.argtypes = [C.POINTER(EpochArgs), C.c_int64, C.c_int64]
         L.orc_momentum_decay.argtypes = [C.POINTER(EpochArgs)]
@@ -330,6 +332,13 @@ class Epoch:
         g = np.zeros(self.head.shape, np.float64)
         lib().orc_gradient_block64(C.byref(self.args), start, end, multiplier, g)
         return g
+
+    def gradient_block64_mag(self, start, end, multiplier):
+        """(FP64 gradient, per-row sum of the terms' Euclidean norms)."""
+        g = np.zeros(self.head.shape, np.float64)
+        mag = np.zeros(self.head.shape[0], np.float64)
+        lib().orc_gradient_block64_mag(C.byref(self.args), start, end, multiplier, g, mag)
+        return g, mag
 
     def adam_apply(self, threads=1):
         lib().orc_adam_apply(C.byref(self.args), threads)

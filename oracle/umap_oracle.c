@@ -637,8 +637,18 @@ void orc_gradient_block(const orc_epoch_args *A, int64_t start, int64_t end, flo
 }
 
 /* The same gradient computed in float64 from the same f32 inputs. */
-void orc_gradient_block64(const orc_epoch_args *A, int64_t start, int64_t end,
-                          double multiplier, double *g) {
+/* Euclidean norm of one term coeff * (head_i - target_j), d components. */
+static double term_norm64(const orc_epoch_args *A, int64_t i, int64_t j, double coeff) {
+    double s = 0.0;
+    for (int64_t c = 0; c < A->d; ++c) {
+        double t = coeff * ((double)head_at(A, i, c) - (double)target_at(A, j, c));
+        s += t * t;
+    }
+    return sqrt(s);
+}
+
+static void gradient64(const orc_epoch_args *A, int64_t start, int64_t end, double multiplier,
+                       double *g, double *mag) {
     int small = A->h <= 1024;
     double a = A->a, b = A->b, gamma = A->gamma;
     int64_t d = A->d;
@@ -658,6 +668,7 @@ void orc_gradient_block64(const orc_epoch_args *A, int64_t start, int64_t end,
                                (1.0 + a * pow(q, b));
                 for (int64_t c = 0; c < d; ++c)
                     g[i * d + c] += coeff * ((double)head_at(A, i, c) - (double)target_at(A, j, c));
+                if (mag) mag[i] += term_norm64(A, i, j, coeff);
             }
             A->next_positive[e] = A->next_positive[e] + A->periods[e];
             if (A->negatives) {
@@ -686,6 +697,7 @@ void orc_gradient_block64(const orc_epoch_args *A, int64_t start, int64_t end,
                         for (int64_t c = 0; c < d; ++c)
                             g[i * d + c] +=
                                 f * ((double)head_at(A, i, c) - (double)target_at(A, jj, c));
+                        if (mag) mag[i] += term_norm64(A, i, jj, f);
                     }
                 }
                 double product = (double)count * A->negative_periods[e];
@@ -693,6 +705,16 @@ void orc_gradient_block64(const orc_epoch_args *A, int64_t start, int64_t end,
             }
         }
     }
+}
+
+void orc_gradient_block64(const orc_epoch_args *A, int64_t start, int64_t end,
+                          double multiplier, double *g) {
+    gradient64(A, start, end, multiplier, g, NULL);
+}
+
+void orc_gradient_block64_mag(const orc_epoch_args *A, int64_t start, int64_t end,
+                              double multiplier, double *g, double *magnitude) {
+    gradient64(A, start, end, multiplier, g, magnitude);
 }
 
 /* adam_kernel (kernels/standard_umap/modern_layout.rs:495-523). */

@@ -101,6 +101,12 @@ void orc_gradient_block(const orc_epoch_args *args, int64_t start, int64_t end,
  * f32 path does. */
 void orc_gradient_block64(const orc_epoch_args *args, int64_t start, int64_t end,
                           double multiplier, double *grad64);
+/* The same, also adding to magnitude[i] (h entries, zero-initialised by the
+ * caller) the Euclidean norm of every term of row i: the row's sum of term
+ * magnitudes, which bounds how well any arithmetic that rounds each term
+ * can resolve the row's sum (the condition of the sum). */
+void orc_gradient_block64_mag(const orc_epoch_args *args, int64_t start, int64_t end,
+                              double multiplier, double *grad64, double *magnitude);
 void orc_adam_apply(const orc_epoch_args *args, int threads);
 void orc_momentum_apply_block(const orc_epoch_args *args, int64_t start, int64_t end);
 void orc_momentum_decay(const orc_epoch_args *args);

@@ -63,6 +63,23 @@ def row_relative_error(g, g64):
     return float((np.linalg.norm(g - g64, axis=1)[keep] / n64[keep]).max())
 
 
+def term_relative_error(g, g64, magnitude, rel=1e-5):
+    """max over rows of (||g - g64|| - rel * ||g64||) / magnitude, with
+    magnitude the row's sum of term norms (FP64 oracle): how far each row is
+    from its FP64 value beyond `rel` of it, in units of the row's terms. A
+    sum whose every term is within eps relative of its exact value, added
+    without loss, stays within eps here whatever cancellation the row has."""
+    g = np.asarray(g, np.float64)
+    err = np.linalg.norm(g - g64, axis=1) - rel * np.linalg.norm(g64, axis=1)
+    mag = np.asarray(magnitude, np.float64)
+    if not np.any(mag > 0):
+        return 0.0 if not np.any(g) else float("inf")
+    if np.any((mag == 0) & (err > 0)):
+        return float("inf")
+    keep = mag > 0
+    return float(max((err[keep] / mag[keep]).max(), 0.0))
+
+
 def corrections(epoch):
     c1 = float(np.float32(1.0 - float(np.float32(0.9)) ** (epoch + 1)))
     c2 = float(np.float32(1.0 - float(np.float32(0.99)) ** (epoch + 1)))
