@@ -159,6 +159,8 @@ SIGNATURES = {
     "umaphip_knn_query": (C.c_int, [_vp, _vp, _i64, _i32, _i32, _i64, _vp, _vp, _vp]),
     "umaphip_knn_index_destroy": (None, [_vp]),
     "umaphip_knn_candidates": (C.c_int, [_vp, _vp, _i64, _vp, _vp, _vp]),
+    "umaphip_knn_set_rescan": (C.c_int, [_vp, _i32]),
+    "umaphip_knn_certificate_stats": (C.c_int, [_vp, _vp, _vp, _vp]),
     "umaphip_fuzzy_graph": (C.c_int, [_vp, _vp, _i64, _i64, C.c_float, C.c_float, C.c_float, _i32,
                                       _i64, _vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(_i64)]),
     "umaphip_compact_graph": (C.c_int, [_i64, _i64, _i64, _vp, _vp, _vp, C.c_double, _i32,

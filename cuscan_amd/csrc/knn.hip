@@ -28,6 +28,17 @@
 // threshold: 64 real references lie at or below it, so it bounds the
 // query's final 64th best from above and no candidate of the exact top 64
 // is dropped. With it a slice of 4096 references keeps about 32.
+//
+// Centring and the certificate: the BF16 copies hold x - s for a per-feature
+// shift s near the references' mean (distances do not change), so the
+// rounding error is relative to the spread of the data, not to its offset.
+// After the rerank, each query's result is certified: a bound on the BF16
+// rounding (measured per row) and on the f32 accumulation of the
+// approximate distances turns the 64th approximate distance into a lower
+// bound on the exact distance of every reference left out of the 64; when
+// that bound exceeds the reranked k-th distance, no left-out reference can
+// enter the top k. Queries the bound does not certify are rescanned exactly
+// in FP32 over all references, in the reference's summation order.
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -58,6 +69,9 @@ constexpr int kSampleRefs = 128;   // sample references per workgroup of the sam
 constexpr int kSampleMinSlices = 32;   // batches with at least this many slices use the bound
 constexpr int64_t kSampleMinRefs = 8 * kSample;
 constexpr int64_t kWideMergeMaxQ = 8192;   // batches up to this many rows merge with 4 waves
+constexpr int kMomentRows = 2048;          // rows per workgroup of the column moments
+constexpr int kExactQB = 4;                // queries per workgroup of the exact rescan
+constexpr int64_t kExactRound = 1024;      // uncertified queries per rescan round
 
 __device__ inline uint16_t f32_to_bf16_rne(float f) {
     uint32_t u = __float_as_uint(f);
@@ -67,21 +81,96 @@ __device__ inline uint16_t f32_to_bf16_rne(float f) {
 
 __device__ inline float bf16_to_f32(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
 
-__global__ void to_bf16_kernel(const float *x, int64_t rows, int64_t d, int64_t dp, uint16_t *out,
-                               float *norms) {
+// BF16 rows of x - shift, padded to dp features; norms[r] = the f32 sum of
+// the squared BF16 values, err[r] (may be null) = an upper bound on the
+// Euclidean norm of (BF16 row) - (x_r - shift), the row's total rounding.
+__global__ void to_bf16_kernel(const float *x, int64_t rows, int64_t d, int64_t dp,
+                               const float *shift, uint16_t *out, float *norms, float *err) {
     const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.y + threadIdx.y;
     if (r >= rows) return;
     float acc = 0.0f;
+    double esq = 0.0;
     for (int64_t c = threadIdx.x; c < dp; c += blockDim.x) {
-        const uint16_t h = c < d ? f32_to_bf16_rne(x[r * d + c]) : 0;
+        uint16_t h = 0;
+        if (c < d) {
+            const float xv = x[r * d + c], sv = shift[c];
+            h = f32_to_bf16_rne(xv - sv);
+            const double e = static_cast<double>(bf16_to_f32(h)) -
+                             (static_cast<double>(xv) - static_cast<double>(sv));
+            esq += e * e;
+        }
         out[r * dp + c] = h;
         const float v = bf16_to_f32(h);
         acc += v * v;
     }
-    if (norms) {
-        for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
-        if (threadIdx.x == 0) norms[r] = acc;
+    for (int off = 32; off > 0; off >>= 1) {
+        acc += __shfl_xor(acc, off, 64);
+        esq += __shfl_xor(esq, off, 64);
     }
+    if (threadIdx.x == 0) {
+        if (norms) norms[r] = acc;
+        if (err) err[r] = static_cast<float>(sqrt(esq) * (1.0 + 0x1.0p-20));
+    }
+}
+
+// Per-feature f64 sums of x and x^2 over kMomentRows rows per workgroup:
+// partial[(b * 2 + 0) * d + c] and partial[(b * 2 + 1) * d + c].
+__global__ __launch_bounds__(256) void column_moments_kernel(const float *x, int64_t n, int64_t d,
+                                                             double *partial) {
+    const int64_t b = blockIdx.x;
+    const int64_t r0 = b * kMomentRows, r1 = min<int64_t>(n, r0 + kMomentRows);
+    for (int64_t c = threadIdx.x; c < d; c += blockDim.x) {
+        double s1 = 0.0, s2 = 0.0;
+        for (int64_t r = r0; r < r1; ++r) {
+            const double v = x[r * d + c];
+            s1 += v;
+            s2 += v * v;
+        }
+        partial[(b * 2) * d + c] = s1;
+        partial[(b * 2 + 1) * d + c] = s2;
+    }
+}
+
+// The per-feature shift: the mean, rounded to a multiple of the largest
+// power of two not above the feature's standard deviation, so it is exact
+// in f32 and data on an integer grid keep their grid (0..255 data stay
+// integers within +-255, exact in BF16). Partial sums are added in a fixed
+// order, so the shift does not depend on scheduling.
+__global__ void shift_kernel(const double *partial, int64_t blocks, int64_t n, int64_t d,
+                             float *shift) {
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= d) return;
+    double s1 = 0.0, s2 = 0.0;
+    for (int64_t b = 0; b < blocks; ++b) {
+        s1 += partial[(b * 2) * d + c];
+        s2 += partial[(b * 2 + 1) * d + c];
+    }
+    const double mean = s1 / static_cast<double>(n);
+    const double var = fmax(s2 / static_cast<double>(n) - mean * mean, 0.0);
+    const double sd = sqrt(var);
+    double v = static_cast<double>(static_cast<float>(mean));
+    if (sd > 0.0 && isfinite(sd) && isfinite(mean)) {
+        const double step = exp2(floor(log2(sd)));
+        v = nearbyint(mean / step) * step;
+    }
+    shift[c] = isfinite(v) ? static_cast<float>(v) : 0.0f;
+}
+
+// eps_key = the order-preserving key of max over rows of err / ||BF16 row||:
+// every reference row's rounding is at most eps times its BF16 norm (a row
+// with rounding but zero BF16 norm makes eps infinite).
+__global__ void rounding_ratio_kernel(const float *norms, const float *err, int64_t n, double g,
+                                      uint32_t *eps_key) {
+    const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    float ratio = 0.0f;
+    if (r < n && err[r] > 0.0f) {
+        const double nr = static_cast<double>(norms[r]) * (1.0 - g);
+        ratio = nr > 0.0 ? static_cast<float>(static_cast<double>(err[r]) / sqrt(nr) *
+                                              (1.0 + 0x1.0p-20))
+                         : INFINITY;
+    }
+    for (int off = 32; off > 0; off >>= 1) ratio = fmaxf(ratio, __shfl_xor(ratio, off, 64));
+    if ((threadIdx.x & 63) == 0 && ratio > 0.0f) atomicMax(eps_key, __float_as_uint(ratio));
 }
 
 // Sorts the n <= kCap = 128 entries of one (query, slice) buffer by
@@ -722,7 +811,8 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(const float *query, int
                                                               const float *ref, int64_t nr,
                                                               int64_t d, const int32_t *top_i,
                                                               const int32_t *top_n, int k,
-                                                              int32_t *out_i, float *out_d) {
+                                                              int32_t *out_i, float *out_d,
+                                                              float *kth_d2) {
     constexpr int FB = 128;
     __shared__ float s_q[FB];
     __shared__ float s_r[kTop][FB + 1];
@@ -772,11 +862,210 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(const float *query, int
     if (lane < n && rank < k) {
         out_i[q * k + rank] = id;
         out_d[q * k + rank] = sqrtf(dist);
+        if (rank == k - 1) kth_d2[q] = dist;
     }
+    if (lane == 0 && n < k) kth_d2[q] = INFINITY;
     if (lane >= n && lane < k) {
         out_i[q * k + lane] = static_cast<int32_t>(nr);
         out_d[q * k + lane] = sqrtf(FLT_MAX);
     }
+}
+
+// Certificate of one query's reranked top k (one lane per query).
+// g bounds the relative error of an f32 sum of up to dp products or
+// squares, doubled (2 dp u / (1 - dp u), u = 2^-24). With x~ / y~ the BF16
+// rows (x - s) / (y - s), every reference left out of the 64 has an
+// approximate distance dt >= T, the 64th. |dt - (|y~|^2 - 2 x~.y~)| <=
+// c (|y~|^2 + 2 |x~| |y~|) with c = g + 2u, and |y~| <= |x~| + t for
+// t = |x~ - y~|, so t >= t_lo, the positive root of
+// (1 + c) t^2 + 4 c X t = X2l + T - 3 c X^2 (X, X2l: upper and lower bounds
+// of |x~| and |x~|^2). The rounding of the rows is at most err(x) and
+// eps |y~| <= eps (X + t), so |x - y| >= t_lo (1 - eps) - err(x) - eps X = L,
+// and the reference's f32 distance of y is at least L^2 (1 - g). When that
+// exceeds the k-th reranked distance, no left-out reference can enter the
+// top k (ties included). Uncertified queries are appended to fail_list.
+__global__ void knn_certify_kernel(const float *top_d, const int32_t *top_n, const float *qnorm,
+                                   const float *qerr, const uint32_t *eps_key,
+                                   const float *kth_d2, int64_t nq, double g, int force,
+                                   int32_t *fail_list, int32_t *fail_count) {
+    const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    bool ok = false;
+    if (!force) {
+        if (top_n[q] < kTop) {
+            ok = true;   // every eligible reference was reranked
+        } else {
+            const double u = 0x1.0p-24, c = g + 2.0 * u;
+            const double T = top_d[q * kTop + kTop - 1];
+            const double qn = qnorm[q];
+            const double x2l = qn * (1.0 - g), xu = sqrt(qn * (1.0 + g));
+            const double K = x2l + T - 3.0 * c * xu * xu;
+            double tlo = 0.0;
+            if (K > 0.0)
+                tlo = (sqrt(16.0 * c * c * xu * xu + 4.0 * (1.0 + c) * K) - 4.0 * c * xu) /
+                      (2.0 * (1.0 + c));
+            const double eps = __uint_as_float(*eps_key);
+            const double L = tlo * (1.0 - eps) - static_cast<double>(qerr[q]) - eps * xu;
+            const double kth = kth_d2[q];
+            ok = eps < 1.0 && L > 0.0 && L * L * (1.0 - g) * (1.0 - 0x1.0p-40) > kth;
+        }
+    }
+    if (!ok) fail_list[atomicAdd(fail_count, 1)] = static_cast<int32_t>(q);
+}
+
+// Exact rescan of the uncertified queries fail_list[base + slot] for slots
+// of this round (kExactQB per workgroup; blockIdx.y = a slice of the
+// references). Each lane owns one reference of a 64-row tile and adds the
+// squared f32 differences in feature order, the reference's sequential
+// distance (feature blocks staged through LDS). Candidates at or below the
+// query's reranked k-th distance, an upper bound on its exact k-th, are
+// kept by (distance, id); the slice's best 64 go to ex_d / ex_i
+// [(slot * slices + slice) * 64] with their count in ex_n.
+__global__ __launch_bounds__(64) void knn_exact_scan_kernel(
+    const float *__restrict__ query, const float *__restrict__ ref, int64_t nr, int64_t d,
+    const int32_t *fail_list, const int32_t *fail_count, int64_t base, int64_t per_slice,
+    int slices, const float *kth_d2, int exclude_self, int64_t query_offset, float *ex_d,
+    int32_t *ex_i, int32_t *ex_n) {
+    constexpr int FB = 64;
+    __shared__ float s_q[kExactQB][FB];
+    __shared__ float s_r[64][FB + 1];
+    __shared__ float s_cd[kExactQB][kCap];
+    __shared__ int32_t s_ci[kExactQB][kCap];
+    const int lane = threadIdx.x;
+    const int64_t f0 = base + static_cast<int64_t>(blockIdx.x) * kExactQB;
+    const int64_t count = *fail_count;
+    if (f0 >= count) return;
+    const int slice = blockIdx.y;
+    const int64_t r_begin = static_cast<int64_t>(slice) * per_slice;
+    const int64_t r_end = min<int64_t>(nr, r_begin + per_slice);
+    int64_t qid[kExactQB];
+    float tau[kExactQB];
+    int cnt[kExactQB];
+#pragma unroll
+    for (int b = 0; b < kExactQB; ++b) {
+        qid[b] = f0 + b < count ? fail_list[f0 + b] : -1;
+        tau[b] = qid[b] >= 0 ? kth_d2[qid[b]] : -INFINITY;
+        cnt[b] = 0;
+    }
+    for (int64_t t0 = r_begin; t0 < r_end; t0 += 64) {
+        float acc[kExactQB];
+#pragma unroll
+        for (int b = 0; b < kExactQB; ++b) acc[b] = 0.0f;
+        for (int64_t c0 = 0; c0 < d; c0 += FB) {
+            const int fb = static_cast<int>(min<int64_t>(FB, d - c0));
+            __syncthreads();
+            for (int rr = 0; rr < 64; ++rr) {
+                const int64_t row = t0 + rr;
+                s_r[rr][lane] = (row < r_end && lane < fb) ? ref[row * d + c0 + lane] : 0.0f;
+            }
+#pragma unroll
+            for (int b = 0; b < kExactQB; ++b)
+                s_q[b][lane] = (qid[b] >= 0 && lane < fb) ? query[qid[b] * d + c0 + lane] : 0.0f;
+            __syncthreads();
+            for (int f = 0; f < fb; ++f) {
+                const float rv = s_r[lane][f];
+#pragma unroll
+                for (int b = 0; b < kExactQB; ++b) {
+                    const float delta = s_q[b][f] - rv;
+                    acc[b] += delta * delta;
+                }
+            }
+        }
+        const int64_t mine = t0 + lane;
+#pragma unroll
+        for (int b = 0; b < kExactQB; ++b) {
+            const bool keep = qid[b] >= 0 && mine < r_end && acc[b] <= tau[b] &&
+                              !(exclude_self && mine == query_offset + qid[b]);
+            const uint64_t m = __ballot(keep);
+            if (!m) continue;
+            if (keep) {
+                const int pos = cnt[b] + __popcll(m & ((1ull << lane) - 1ull));
+                s_cd[b][pos] = acc[b];
+                s_ci[b][pos] = static_cast<int32_t>(mine);
+            }
+            cnt[b] += __popcll(m);
+            if (cnt[b] > kCap - 64) {
+                // keep the best 64 so the next tile fits
+                __syncthreads();
+                const float t = compact_buffer(s_cd[b], s_ci[b], cnt[b], lane);
+                __syncthreads();
+                cnt[b] = kTop;
+                tau[b] = fminf(tau[b], t);
+            }
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < kExactQB; ++b) {
+        if (qid[b] < 0) continue;
+        const int n = cnt[b];
+        __syncthreads();
+        if (n > 0) compact_buffer(s_cd[b], s_ci[b], n, lane);
+        __syncthreads();
+        const int64_t cell = (f0 + b - base) * slices + slice;
+        const int kept = n < kTop ? n : kTop;
+        if (lane < kept) {
+            ex_d[cell * kTop + lane] = s_cd[b][lane];
+            ex_i[cell * kTop + lane] = s_ci[b][lane];
+        }
+        if (lane == 0) ex_n[cell] = kept;
+    }
+}
+
+// Merges sorted list B (entry 63 - lane in this lane) into the sorted list
+// A held one entry per lane (half-cleaner of the bitonic A ++ reverse(B),
+// then a bitonic merge), keeping the best 64 by (distance, id).
+__device__ inline void merge_reversed(float &ad, int32_t &ai, float bd, int32_t bi, int lane) {
+    if (bd < ad || (bd == ad && bi < ai)) {
+        ad = bd;
+        ai = bi;
+    }
+#pragma unroll
+    for (int j = 32; j > 0; j >>= 1) {
+        const float dp = __shfl_xor(ad, j, 64);
+        const int32_t ip = __shfl_xor(ai, j, 64);
+        const bool mine_gt = ad > dp || (ad == dp && ai > ip);
+        const bool lower = (lane & j) == 0;
+        if (lower == mine_gt) {
+            ad = dp;
+            ai = ip;
+        }
+    }
+}
+
+// One wave per rescanned query of the round: merges its slices' lists and
+// overwrites its k results (square roots; sentinels past the references).
+__global__ __launch_bounds__(64) void knn_exact_merge_kernel(
+    const float *ex_d, const int32_t *ex_i, const int32_t *ex_n, const int32_t *fail_list,
+    const int32_t *fail_count, int64_t base, int slices, int k, int64_t nr, int32_t *out_i,
+    float *out_d) {
+    const int64_t slot = blockIdx.x;
+    if (base + slot >= *fail_count) return;
+    const int64_t q = fail_list[base + slot];
+    const int lane = threadIdx.x;
+    float ad = INFINITY;
+    int32_t ai = INT32_MAX;
+    int na = 0;
+    for (int sl = 0; sl < slices; ++sl) {
+        const int64_t cell = slot * slices + sl;
+        const int nb = ex_n[cell];
+        if (nb == 0) continue;
+        const int r = kTop - 1 - lane;
+        merge_reversed(ad, ai, r < nb ? ex_d[cell * kTop + r] : INFINITY,
+                       r < nb ? ex_i[cell * kTop + r] : INT32_MAX, lane);
+        na = min(kTop, na + nb);
+    }
+    if (lane < k) {
+        out_i[q * k + lane] = lane < na ? ai : static_cast<int32_t>(nr);
+        out_d[q * k + lane] = lane < na ? sqrtf(ad) : sqrtf(FLT_MAX);
+    }
+}
+
+__global__ void zero_i32_kernel(int32_t *p) { *p = 0; }
+
+// 2 n u / (1 - n u) for f32 sums of n terms, doubled.
+double sum_bound(int64_t n) {
+    const double nu = static_cast<double>(n) * 0x1.0p-24;
+    return 2.0 * nu / (1.0 - nu);
 }
 
 template <int KS>
@@ -822,11 +1111,33 @@ KnnIndex::KnnIndex(const float *ref_host, const float *ref_device, int64_t n, in
         HIP_CHECK(hipMemcpyAsync(ref_.ptr, ref_device, n * d * sizeof(float), hipMemcpyDeviceToDevice, s));
     else
         HIP_CHECK(hipMemcpyAsync(ref_.ptr, ref_host, n * d * sizeof(float), hipMemcpyHostToDevice, s));
+    // the per-feature shift from the references' moments
+    shift_.alloc(static_cast<size_t>(d));
+    {
+        const int64_t blocks = (n + kMomentRows - 1) / kMomentRows;
+        DeviceArray<double> partial(static_cast<size_t>(2 * blocks * d));
+        hipLaunchKernelGGL(column_moments_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256),
+                           0, s, ref_.ptr, n, d, partial.ptr);
+        hipLaunchKernelGGL(shift_kernel, dim3(static_cast<unsigned>((d + 255) / 256)), dim3(256),
+                           0, s, partial.ptr, blocks, n, d, shift_.ptr);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipStreamSynchronize(s));   // before the partials are freed
+    }
     ref_bf_.alloc(static_cast<size_t>(n * dp_));
     rnorm_.alloc(static_cast<size_t>(n));
-    hipLaunchKernelGGL(to_bf16_kernel, dim3(static_cast<unsigned>((n + 3) / 4)), dim3(64, 4), 0, s,
-                       ref_.ptr, n, d, dp_, ref_bf_.ptr, rnorm_.ptr);
-    HIP_CHECK(hipGetLastError());
+    eps_key_.alloc(1);
+    {
+        DeviceArray<float> rerr(static_cast<size_t>(n));
+        hipLaunchKernelGGL(to_bf16_kernel, dim3(static_cast<unsigned>((n + 3) / 4)), dim3(64, 4), 0,
+                           s, ref_.ptr, n, d, dp_, shift_.ptr, ref_bf_.ptr, rnorm_.ptr, rerr.ptr);
+        hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, eps_key_.ptr, 1, 0u);
+        hipLaunchKernelGGL(rounding_ratio_kernel, dim3(static_cast<unsigned>((n + 255) / 256)),
+                           dim3(256), 0, s, rnorm_.ptr, rerr.ptr, n, sum_bound(dp_), eps_key_.ptr);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipStreamSynchronize(s));   // before the row errors are freed
+    }
+    ex_slices_ = static_cast<int>(std::min<int64_t>(64, std::max<int64_t>(1, (n + 8191) / 8192)));
+    fail_count_.alloc(1);
     reserve(max_q);
 }
 
@@ -864,6 +1175,14 @@ void KnnIndex::reserve(int64_t q) {
         if (uses_sample(qq)) sq = qq;
     if (sq > 0) sample_d_.alloc(static_cast<size_t>(sq) * kSample);
     qbf_.alloc(static_cast<size_t>(q * dp_));
+    qnorm_.alloc(static_cast<size_t>(q));
+    qerr_.alloc(static_cast<size_t>(q));
+    kth_.alloc(static_cast<size_t>(q));
+    fail_list_.alloc(static_cast<size_t>(q));
+    const int64_t round = std::min<int64_t>(q, kExactRound);
+    ex_d_.alloc(static_cast<size_t>(round * ex_slices_ * kTop));
+    ex_i_.alloc(static_cast<size_t>(round * ex_slices_ * kTop));
+    ex_n_.alloc(static_cast<size_t>(round * ex_slices_));
     reserved_q_ = q;
 }
 
@@ -876,7 +1195,7 @@ void KnnIndex::query(const float *q_dev, int64_t nq, int k, bool exclude_self, i
                     "query batch %lld exceeds the reserved %lld rows", (long long)nq,
                     (long long)reserved_q_);
     hipLaunchKernelGGL(to_bf16_kernel, dim3(static_cast<unsigned>((nq + 3) / 4)), dim3(64, 4), 0, s,
-                       q_dev, nq, d_, dp_, qbf_.ptr, static_cast<float *>(nullptr));
+                       q_dev, nq, d_, dp_, shift_.ptr, qbf_.ptr, qnorm_.ptr, qerr_.ptr);
     HIP_CHECK(hipGetLastError());
     const int slices = slices_for(nq);
     const int64_t per_slice = ((n_ + slices - 1) / slices + kTR - 1) / kTR * kTR;
@@ -932,10 +1251,54 @@ void KnnIndex::query(const float *q_dev, int64_t nq, int k, bool exclude_self, i
     // one four-wave workgroup per query: its staging rounds are the
     // latency at small batches, and it also has more loads in flight per CU
     // at large ones (87 ns per query with one wave at 16,384 rows)
+    const float *q_exact = q_f32_for_rerank ? q_f32_for_rerank : q_dev;
     hipLaunchKernelGGL(knn_rerank_kernel, dim3(static_cast<unsigned>(nq)), dim3(256), 0, s,
-                       q_f32_for_rerank ? q_f32_for_rerank : q_dev, nq, ref_.ptr, n_, d_,
-                       top_i_.ptr, top_n_.ptr, k, idx_out, dist_out);
+                       q_exact, nq, ref_.ptr, n_, d_, top_i_.ptr, top_n_.ptr, k, idx_out, dist_out,
+                       kth_.ptr);
     HIP_CHECK(hipGetLastError());
+    // certificate, then the exact rescan of the queries it leaves open
+    hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(1), 0, s, fail_count_.ptr);
+    hipLaunchKernelGGL(knn_certify_kernel, dim3(static_cast<unsigned>((nq + 255) / 256)), dim3(256),
+                       0, s, top_d_.ptr, top_n_.ptr, qnorm_.ptr, qerr_.ptr, eps_key_.ptr, kth_.ptr,
+                       nq, sum_bound(dp_), force_rescan_ ? 1 : 0, fail_list_.ptr, fail_count_.ptr);
+    HIP_CHECK(hipGetLastError());
+    const int64_t round = std::min<int64_t>(nq, kExactRound);
+    int64_t open = nq;
+    if (nq > kWideMergeMaxQ) {
+        // large batches (not latency bound) launch only the rounds needed
+        int32_t host = 0;
+        HIP_CHECK(hipMemcpyAsync(&host, fail_count_.ptr, sizeof(host), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        open = host;
+    }
+    const int64_t ex_per_slice = ((n_ + ex_slices_ - 1) / ex_slices_ + 63) / 64 * 64;
+    for (int64_t base = 0; base < open; base += round) {
+        const dim3 sgrid(static_cast<unsigned>((round + kExactQB - 1) / kExactQB),
+                         static_cast<unsigned>(ex_slices_));
+        hipLaunchKernelGGL(knn_exact_scan_kernel, sgrid, dim3(64), 0, s, q_exact, ref_.ptr, n_, d_,
+                           fail_list_.ptr, fail_count_.ptr, base, ex_per_slice, ex_slices_, kth_.ptr,
+                           exclude_self ? 1 : 0, query_offset, ex_d_.ptr, ex_i_.ptr, ex_n_.ptr);
+        hipLaunchKernelGGL(knn_exact_merge_kernel, dim3(static_cast<unsigned>(round)), dim3(64), 0,
+                           s, ex_d_.ptr, ex_i_.ptr, ex_n_.ptr, fail_list_.ptr, fail_count_.ptr, base,
+                           ex_slices_, k, n_, idx_out, dist_out);
+        HIP_CHECK(hipGetLastError());
+    }
+}
+
+int64_t KnnIndex::rescans(hipStream_t s) const {
+    int32_t host = 0;
+    HIP_CHECK(hipMemcpyAsync(&host, fail_count_.ptr, sizeof(host), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    return host;
+}
+
+float KnnIndex::rounding_ratio(hipStream_t s) const {
+    uint32_t host = 0;
+    HIP_CHECK(hipMemcpyAsync(&host, eps_key_.ptr, sizeof(host), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    float v;
+    std::memcpy(&v, &host, sizeof(v));
+    return v;
 }
 
 }  // namespace umaphip
@@ -984,6 +1347,22 @@ umaphip_status umaphip_knn_candidates(umaphip_knn_index *index, const float *que
                         "invalid kNN candidate arguments");
         index->impl->query(queries, q, 1, false, 0, nullptr, nullptr, as_stream(stream), nullptr,
                            candidates, counts);
+    });
+}
+
+umaphip_status umaphip_knn_set_rescan(umaphip_knn_index *index, int32_t every_query) {
+    return guarded([&] {
+        UMAPHIP_REQUIRE(index && index->impl, UMAPHIP_INVALID_INPUT, "kNN index is null");
+        index->impl->set_force_rescan(every_query != 0);
+    });
+}
+
+umaphip_status umaphip_knn_certificate_stats(umaphip_knn_index *index, void *stream,
+                                             int64_t *rescanned, float *rounding_ratio) {
+    return guarded([&] {
+        UMAPHIP_REQUIRE(index && index->impl, UMAPHIP_INVALID_INPUT, "kNN index is null");
+        if (rescanned) *rescanned = index->impl->rescans(as_stream(stream));
+        if (rounding_ratio) *rounding_ratio = index->impl->rounding_ratio(as_stream(stream));
     });
 }
 

@@ -30,10 +30,25 @@ public:
     int slices_for(int64_t q) const;
     // Whether a batch of q rows seeds its thresholds from the sample bound.
     bool uses_sample(int64_t q) const;
+    // Queries of the last query() that the certificate left open and the
+    // exact FP32 rescan answered (synchronises s).
+    int64_t rescans(hipStream_t s) const;
+    // eps: every reference row's BF16 rounding is at most eps times its
+    // BF16 norm (0 when the shifted references are exact in BF16).
+    float rounding_ratio(hipStream_t s) const;
+    // Tests: rescan every query exactly (the certificate is not consulted).
+    void set_force_rescan(bool on) { force_rescan_ = on; }
 
 private:
     int64_t n_, d_, dp_ = 0, reserved_q_ = 0;
-    int ks_ = 0;
+    int ks_ = 0, ex_slices_ = 1;
+    bool force_rescan_ = false;
+    DeviceArray<float> shift_;        // [d] per-feature shift subtracted before BF16
+    DeviceArray<uint32_t> eps_key_;   // reference rounding ratio (float bits)
+    DeviceArray<float> qnorm_, qerr_, kth_;   // per query: |x~|^2, rounding, k-th exact d^2
+    DeviceArray<int32_t> fail_list_, fail_count_;
+    DeviceArray<float> ex_d_;         // exact rescan: per (query slot, slice) best 64
+    DeviceArray<int32_t> ex_i_, ex_n_;
     DeviceArray<float> ref_, rnorm_, cand_d_, top_d_;
     DeviceArray<float> sample_d_;     // [q, 8192] sample-bound distances of a batch
     DeviceArray<uint16_t> ref_bf_, qbf_;

@@ -80,6 +80,22 @@ class KnnIndex:
             out_d[s:s + batch] = d.cpu().numpy()
         return out_i, out_d
 
+    def set_rescan(self, every_query: bool) -> None:
+        """Rescan every query exactly in FP32 (tests); False restores the
+        per-query certificate, which rescans only the queries it leaves open."""
+        _lib.check(_lib.lib().umaphip_knn_set_rescan(self._handle, int(bool(every_query))))
+
+    def certificate_stats(self):
+        """(queries of the last query call rescanned exactly, largest BF16
+        rounding of a reference row relative to its norm)."""
+        import torch
+
+        n = C.c_int64()
+        eps = C.c_float()
+        _lib.check(_lib.lib().umaphip_knn_certificate_stats(
+            self._handle, torch.cuda.current_stream().cuda_stream, C.byref(n), C.byref(eps)))
+        return int(n.value), float(eps.value)
+
     def close(self):
         if getattr(self, "_handle", None):
             _lib.lib().umaphip_knn_index_destroy(self._handle)

@@ -276,7 +276,9 @@ umaphip_status umaphip_orders_at_epoch(int64_t h, int64_t n, const umaphip_layou
  * MFMA distance prefilter with FP32 accumulation keeps 64 candidates per
  * query; an exact FP32 rerank then reproduces the reference's sequential
  * squared-distance sum, lower-index tie-break, square root and sentinels
- * (id = n, distance = sqrt(f32::MAX)). k <= 64, features <= 1024.
+ * (id = n, distance = sqrt(f32::MAX)); a certificate per query, with an
+ * exact FP32 rescan where it fails, makes the result the reference's for
+ * any data. k <= 64, features <= 1024.
  * ------------------------------------------------------------------------ */
 typedef struct umaphip_knn_index umaphip_knn_index;
 /* Copies n x d f32 references (host or device pointer) into a resident
@@ -295,6 +297,20 @@ umaphip_status umaphip_knn_query(umaphip_knn_index *index, const float *queries,
  * prefilter's recall. */
 umaphip_status umaphip_knn_candidates(umaphip_knn_index *index, const float *queries, int64_t q,
                                       int32_t *candidates, int32_t *counts, void *stream);
+/* Exactness certificate (no reference counterpart; it keeps the prefilter
+ * from changing the reference's results). Each query's top k is certified
+ * against a bound on the BF16 rounding (the index subtracts a per-feature
+ * shift near the references' mean before rounding) and the f32
+ * accumulation of the approximate distances; a query the bound does not
+ * certify is rescanned exactly in FP32 over every reference. every_query =
+ * 1 rescans every query (tests), 0 restores the certificate. */
+umaphip_status umaphip_knn_set_rescan(umaphip_knn_index *index, int32_t every_query);
+/* Synchronises stream; rescanned (may be NULL) = queries of the last
+ * umaphip_knn_query that were rescanned exactly; rounding_ratio (may be
+ * NULL) = the largest BF16 rounding of a reference row relative to its norm
+ * (0 when the shifted references are exact in BF16). */
+umaphip_status umaphip_knn_certificate_stats(umaphip_knn_index *index, void *stream,
+                                             int64_t *rescanned, float *rounding_ratio);
 void umaphip_knn_index_destroy(umaphip_knn_index *index);
 
 /* ------------------------------------------------------------------------
