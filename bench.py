@@ -15,14 +15,19 @@ bytes of each epoch, divided by their average duration per epoch measured
 with device events on the plan's stream; `traffic` is the measured HBM
 bytes per epoch of the same kernels from the committed rocprofv3 counter
 summary. The CPU baseline times the oracle (C restatement, OpenMP over
-rows) on the same graph and init for the first epochs of the same layout
-and extrapolates by force terms per epoch.
+rows) on the whole 200-epoch layout from the same graph and init;
+`quality` compares trustworthiness@15 and kNN preservation of the device
+layout and of that oracle layout on a fixed 10,000-row sample (the gate:
+within 0.01).
 
 Side legs (each skippable): `knn_batches` times the exact kNN of query
 batches of 1 to 16,384 rows against the 1M references and prices each
 against its roofline (HBM for the BF16 stream below about 310 rows, MFMA
 above); `batched` runs 256 independent layouts as one hipGraph launch;
-`serving` runs the transform server under Poisson arrivals.
+`serving` runs the transform server under Poisson arrivals; `fit_2k` times
+the whole CPU fit of the 2,000 x 10 blob data (and the device fit), with
+trustworthiness; `layout_100k` times the 100,000 x 50 global-negative
+layout with exclusion on the device and in the oracle.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--rows N]
 """
@@ -430,24 +435,141 @@ def cpu_model():
     return "unknown"
 
 
-def cpu_baseline(compact, init, a, b, epochs, sample_epochs, terms):
-    """Times the oracle (C restatement, OpenMP over rows) on epochs
-    0..sample_epochs-1 of the same layout and extrapolates to the whole
-    layout by the exact per-epoch force-term counts (its cost scales with
-    the firing edges and negatives drawn, which are data independent)."""
+def host_threads():
+    return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+
+def oracle_layout(compact, init, a, b, epochs):
+    """The oracle's whole layout (all host threads) from the same graph and
+    init as the device plan: (embedding, seconds, threads)."""
     from oracle import oracle as O
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads = host_threads()
     t0 = time.time()
-    O.optimize_compact(init, None, compact, epochs=epochs, a=a, b=b, negative_selection_range=200_000,
-                       seed=42, threads=threads, stop_epoch=sample_epochs, collect_stats=False)
-    secs = time.time() - t0
+    emb = O.optimize_compact(init, None, compact, epochs=epochs, a=a, b=b,
+                             negative_selection_range=200_000, seed=42, threads=threads,
+                             collect_stats=False)[0]
+    return emb, time.time() - t0, threads
+
+
+def quality_leg(x, emb, emb_oracle):
+    """trustworthiness@15 and kNN preservation@15 of the device and oracle
+    layouts on the same fixed 10,000-row sample (comparison/umap06/
+    quality.py semantics)."""
+    from cuscan_amd.quality import trustworthiness_device
+
+    xs = x.cpu().numpy()
+    tw, pres = trustworthiness_device(xs, emb, k=15, sample=10_000)
+    tw_c, pres_c = trustworthiness_device(xs, emb_oracle, k=15, sample=10_000)
+    log(f"quality: device {tw:.4f}/{pres:.4f}, oracle {tw_c:.4f}/{pres_c:.4f}")
+    return {"trustworthiness@15": round(tw, 4), "oracle_trustworthiness@15": round(tw_c, 4),
+            "delta": round(tw - tw_c, 4), "knn_preservation@15": round(pres, 4),
+            "oracle_knn_preservation@15": round(pres_c, 4), "sample_rows": 10_000,
+            "gate": "|trustworthiness - oracle| <= 0.01", "pass": abs(tw - tw_c) <= 0.01}
+
+
+def fit_2k_leg():
+    """2,000 x 10 rows from 10 isotropic Gaussian blobs (centres U[-10, 10]^10,
+    sigma 1, 200 rows each, numpy seed 42); UMAP(n_neighbors=15,
+    n_components=2, optimizer=adam, init=random, 200 epochs,
+    negative_sample_rate=5): the whole CPU fit of the oracle (kNN, graph,
+    layout) on all host threads, and the device fit of the same data."""
+    import torch
+
+    from cuscan_amd.quality import trustworthiness
+    from cuscan_amd.umap import UMAP
+    from oracle import oracle as O
+
+    rng = np.random.default_rng(42)
+    centers = rng.uniform(-10, 10, (10, 10))
+    x = np.concatenate([centers[c] + rng.normal(0, 1.0, (200, 10)) for c in range(10)])
+    x = x.astype(np.float32)
+    threads = host_threads()
+    t0 = time.time()
+    emb_c = O.fit(x, n_neighbors=15, n_epochs=200, seed=42, threads=threads)
+    cpu_s = time.time() - t0
+    model = UMAP(n_neighbors=15, n_components=2, optimizer="adam", init="random", n_epochs=200,
+                 negative_sample_rate=5, random_state=42)
+    model.fit(x)   # warm
+    torch.cuda.synchronize()
+    t0 = time.time()
+    emb_d = UMAP(n_neighbors=15, n_components=2, optimizer="adam", init="random", n_epochs=200,
+                 negative_sample_rate=5, random_state=42).fit_transform(x)
+    dev_s = time.time() - t0
+    tw_c, _ = trustworthiness(x, emb_c, k=15)
+    tw_d, _ = trustworthiness(x, emb_d, k=15)
+    log(f"2k fit: cpu {cpu_s:.3f}s tw {tw_c:.4f}; device {dev_s:.3f}s tw {tw_d:.4f}")
+    return {"workload": "2000x10 blobs (10 clusters, sigma 1, seed 42), n_neighbors=15, adam, "
+                        "init=random, 200 epochs, negative_sample_rate=5",
+            "cpu_fit_seconds": round(cpu_s, 3), "cpu_trustworthiness@15": round(tw_c, 4),
+            "cpu_cores": threads, "device_fit_seconds": round(dev_s, 3),
+            "device_trustworthiness@15": round(tw_d, 4)}
+
+
+def layout_100k_leg(steps=5):
+    """100,000 x 50 rows of 50 clusters (centres U[-10, 10]^50, sigma
+    U[0.5, 2], seed 42), exact k=15 fuzzy graph, 200 Adam epochs, global
+    negatives with exclude_graph_neighbors: edge-epochs/s of the device plan
+    (mean of `steps` layouts) and of the oracle's whole layout."""
+    import torch
+
+    from cuscan_amd import graph as G
+    from cuscan_amd import layout as L
+    from cuscan_amd.knn import KnnIndex
+    from oracle import oracle as O
+
+    rng = np.random.default_rng(42)
+    centers = rng.uniform(-10, 10, (50, 50))
+    sig = rng.uniform(0.5, 2.0, 50)
+    lab = rng.integers(0, 50, 100_000)
+    x = (centers[lab] + rng.normal(0, 1, (100_000, 50)) * sig[lab, None]).astype(np.float32)
+    n = len(x)
+    index = KnnIndex(torch.from_numpy(x).cuda(), max_queries=n)
+    idx, dist = index.query(x, 14, exclude_self=True)
+    index.close()
+    g = G.fit_graph_device(np.concatenate([np.arange(n, dtype=np.int32)[:, None], idx], 1),
+                           np.concatenate([np.zeros((n, 1), np.float32), dist], 1), epochs=200,
+                           rate=5.0, prune_divisor=200.0)
+    a, b = G.fit_curve(0.1, 1.0)
+    head = G.random_init(n, 2, 42)
+    E = len(g["indices"])
+    params = L.LayoutParams(epochs=200, a=a, b=b, exclude_graph_neighbors=True)
+    plan = L.LayoutPlan(head, None, g, params, shared=True)
+    plan.run()
+    plan.download()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plan.run()
+    plan.download()
+    dev_s = (time.perf_counter() - t0) / steps
+    plan.close()
+    threads = host_threads()
+    t0 = time.time()
+    O.optimize_compact(head, None, g, epochs=200, a=a, b=b, exclude=True, threads=threads,
+                       collect_stats=False)
+    cpu_s = time.time() - t0
+    log(f"100k layout: device {dev_s * 1e3:.1f} ms ({E * 200 / dev_s:.3e} edge-epochs/s), "
+        f"oracle {cpu_s:.2f}s ({E * 200 / cpu_s:.3e})")
+    return {"workload": "100000x50, 50 clusters, exact k=15 fuzzy graph, 200 Adam epochs, global "
+                        "negatives, exclude_graph_neighbors", "edges": E,
+            "device_ms": round(dev_s * 1e3, 2), "device_value": E * 200 / dev_s,
+            "cpu_seconds": round(cpu_s, 2), "cpu_value": E * 200 / cpu_s, "cpu_cores": threads,
+            "unit": "edge-epochs/s", "speedup": round(cpu_s / dev_s, 1)}
+
+
+def cpu_baseline(compact, init, a, b, epochs, terms, oracle_run):
+    """The oracle's whole layout (C restatement, OpenMP over rows, all host
+    threads) on the same graph and init: E * epochs / seconds. The
+    one-thread figure times epochs 0..3 on one thread and scales them to the
+    layout average by the exact per-epoch force-term counts (the oracle's
+    cost follows the firing edges and negatives drawn, which are data
+    independent)."""
+    from oracle import oracle as O
+
+    _, secs, threads = oracle_run
     E = len(compact["indices"])
-    sample_terms = sum(terms[:sample_epochs]) / sample_epochs
     mean_terms = sum(terms) / len(terms)
-    ratio = mean_terms / sample_terms
-    raw = E * sample_epochs / secs
-    # one host thread on the first few epochs, scaled the same way
     one_epochs = 4
     t0 = time.time()
     O.optimize_compact(init, None, compact, epochs=epochs, a=a, b=b, negative_selection_range=200_000,
@@ -455,15 +577,13 @@ def cpu_baseline(compact, init, a, b, epochs, sample_epochs, terms):
     one_secs = time.time() - t0
     one_ratio = mean_terms / (sum(terms[:one_epochs]) / one_epochs)
     one = E * one_epochs / one_secs / one_ratio
-    return {"value": raw / ratio, "unit": "edge-epochs/s", "cores": threads, "kind": "port",
+    return {"value": E * epochs / secs, "unit": "edge-epochs/s", "cores": threads, "kind": "port",
             "one_thread_value": one,
             "sample": f"oracle C restatement (OpenMP, {threads} threads, {cpu_model()}, "
-                      f"nproc {os.cpu_count()}) timed on epochs 0..{sample_epochs - 1} of the same "
-                      f"200-epoch layout, graph and init ({secs:.1f}s, raw {raw:.3e} edge-epochs/s); "
-                      f"extrapolated to the layout average by force terms per epoch "
-                      f"({mean_terms:.4g} average vs {sample_terms:.4g} in the sample); "
-                      f"one_thread_value: 1 thread on epochs 0..{one_epochs - 1} ({one_secs:.1f}s), "
-                      f"scaled the same way"}
+                      f"nproc {os.cpu_count()}): the whole {epochs}-epoch layout from the same "
+                      f"graph and init ({secs:.1f}s); one_thread_value: 1 thread on epochs "
+                      f"0..{one_epochs - 1} ({one_secs:.1f}s), scaled to the layout average by "
+                      f"force terms per epoch"}
 
 
 def copy_calibration(gib):
@@ -499,9 +619,11 @@ def main():
     ap.add_argument("--rows", type=int, default=1_000_000)
     ap.add_argument("--dims", type=int, default=784)
     ap.add_argument("--epochs", type=int, default=200)
-    ap.add_argument("--cpu-epochs", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--quality", action="store_true", help="report trustworthiness@15 (10k sample)")
+    ap.add_argument("--no-quality", action="store_true",
+                    help="skip the trustworthiness@15 comparison with the oracle's layout")
+    ap.add_argument("--no-small", action="store_true",
+                    help="skip the 2,000 x 10 fit and 100,000 x 50 layout legs")
     ap.add_argument("--recursive", action="store_true",
                     help="also time the recursive initialisation (init='auto') on the same "
                          "graph, and with --quality the fit that starts from it")
@@ -516,6 +638,7 @@ def main():
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only run a plain device copy of GIB GiB (counter calibration)")
     args = ap.parse_args()
+    args.quality = not args.no_quality
     if args.copy_calibration > 0:
         copy_calibration(args.copy_calibration)
         return
@@ -583,27 +706,13 @@ def main():
                    "final_scale": round(stats["final_scale"], 6)},
         "roofline": roofline,
     }
+    oracle_run = None
+    if args.quality or not args.no_cpu:
+        log("oracle layout (whole) ...")
+        oracle_run = oracle_layout(compact, init, a, b, args.epochs)
+        log(f"oracle layout {oracle_run[1]:.1f}s on {oracle_run[2]} threads")
     if args.quality:
-        from cuscan_amd.quality import trustworthiness_device
-
-        xs = x.cpu().numpy()
-        tw, pres = trustworthiness_device(xs, emb, k=15, sample=10_000)
-        out["config"]["trustworthiness@15"] = round(tw, 4)
-        out["config"]["knn_preservation@15"] = round(pres, 4)
-        # the oracle's whole layout on the same graph and init, same sample
-        from oracle import oracle as O
-
-        t0 = time.time()
-        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        emb_cpu = O.optimize_compact(init, None, compact, epochs=args.epochs, a=a, b=b,
-                                     negative_selection_range=200_000, seed=42, threads=threads,
-                                     collect_stats=False)[0]
-        cpu_s = time.time() - t0
-        tw_c, pres_c = trustworthiness_device(xs, emb_cpu, k=15, sample=10_000)
-        out["config"]["oracle_trustworthiness@15"] = round(tw_c, 4)
-        out["config"]["oracle_knn_preservation@15"] = round(pres_c, 4)
-        out["config"]["oracle_layout_seconds"] = round(cpu_s, 1)
-        log(f"quality: device {tw:.4f}/{pres:.4f}, oracle {tw_c:.4f}/{pres_c:.4f} ({cpu_s:.0f}s)")
+        out["quality"] = quality_leg(x, emb, oracle_run[0])
     if args.recursive:
         log("recursive initialisation ...")
         out["recursive_init"] = recursive_benchmark(x, compact, a, b, args)
@@ -616,9 +725,14 @@ def main():
     if not args.no_serve:
         log("transform serving ...")
         out["serving"] = serve_benchmark(x, emb, index, a, b, args)
+    if not args.no_small:
+        log("2k fit ...")
+        out["fit_2k"] = fit_2k_leg()
+        log("100k layout ...")
+        out["layout_100k"] = layout_100k_leg()
     if not args.no_cpu:
         log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(compact, init, a, b, args.epochs, args.cpu_epochs, terms)
+        out["cpu_baseline"] = cpu_baseline(compact, init, a, b, args.epochs, terms, oracle_run)
     print(json.dumps(out), flush=True)
 
 

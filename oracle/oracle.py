@@ -63,7 +63,8 @@ class LayoutParams(C.Structure):
         ("negative_sample_rate", C.c_int64), ("negative_selection_range", C.c_int64),
         ("has_scale", C.c_int), ("scale", C.c_float),
         ("adaptation_samples", C.c_int64), ("exclude", C.c_int), ("good", C.c_int),
-        ("seed", C.c_uint64),
+        ("seed", C.c_uint64), ("obj_every", C.c_int), ("obj_weights", C.c_void_p),
+        ("obj_out", C.c_void_p),
     ]
 
 
@@ -441,9 +442,11 @@ def optimize_compact(head, tail, graph, *, epochs, a, b, repulsion=1.0, initial_
                      optimizer="adam", negative_sample_rate=5, negative_selection_range=200_000,
                      negative_sample_scale=None, adaptation_samples=128, exclude=False,
                      good=False, seed=42, shared=True, threads=None, stop_epoch=0,
-                     collect_stats=True):
+                     collect_stats=True, objective_weights=None, objective_every=0):
     """Run the restated epoch loop in place on copies; returns (head, clocks, stats).
-    collect_stats replays each epoch's negatives once more to count them."""
+    collect_stats replays each epoch's negatives once more to count them.
+    With objective_every > 0, stats["objective"] holds the FP64 cross entropy
+    after every objective_every-th epoch and after the last."""
     head = np.array(head, np.float32, order="C", copy=True)
     h, d = head.shape
     tail_arr = None if shared else np.ascontiguousarray(tail, np.float32)
@@ -462,6 +465,13 @@ def optimize_compact(head, tail, graph, *, epochs, a, b, repulsion=1.0, initial_
     P.exclude = int(exclude)
     P.good = int(good)
     P.seed = seed
+    obj_w = obj_out = None
+    if objective_every > 0:
+        obj_w = np.ascontiguousarray(objective_weights, np.float32)
+        obj_out = np.zeros(epochs // objective_every + 1, np.float64)
+        P.obj_every = objective_every
+        P.obj_weights = obj_w.ctypes.data
+        P.obj_out = obj_out.ctypes.data
     S = LayoutStats()
     rc = lib().orc_optimize_compact(
         h, n, d, head, None if tail_arr is None else tail_arr.ctypes.data,
@@ -474,6 +484,9 @@ def optimize_compact(head, tail, graph, *, epochs, a, b, repulsion=1.0, initial_
     if rc != 0:
         raise RuntimeError(f"oracle layout failed ({rc})")
     stats = {f: getattr(S, f) for f, _ in LayoutStats._fields_}
+    if obj_out is not None:
+        last = min(stop_epoch, epochs) if stop_epoch > 0 else epochs
+        stats["objective"] = obj_out[:last // objective_every + (last % objective_every != 0)]
     return head, npos, nneg, stats
 
 

@@ -1100,6 +1100,7 @@ int orc_optimize_compact(int64_t h, int64_t n, int64_t d, float *head, const flo
     if (!alias)
         for (int64_t i = 0; i < n; ++i) negative[i] = (int32_t)i;
     int negative_live = !alias;
+    int64_t obj_count = 0;
     orc_rng order_rng;
     orc_rng_seed(&order_rng, P->seed);
 
@@ -1193,6 +1194,10 @@ int orc_optimize_compact(int64_t h, int64_t n, int64_t d, float *head, const flo
             stats->negative_samples += draws;
         }
         orc_modern_epoch(&A, threads);
+        if (P->obj_every > 0 && P->obj_out &&
+            ((epoch + 1) % P->obj_every == 0 || epoch + 1 == epochs))
+            P->obj_out[obj_count++] = orc_cross_entropy(head, h, shared ? NULL : tail, d, offsets,
+                                                        indices, P->obj_weights, P->a, P->b);
         if (hard && epoch == 0) negative_live = 0;
         /* prepare_next: the source order first, then the live negative order,
          * from the same generator (modern_orders.rs:544-631). */

@@ -157,6 +157,13 @@ typedef struct {
     int exclude;
     int good;
     uint64_t seed;
+    /* objective trace (the build's FP64 cross entropy, no reference
+     * counterpart): with obj_every > 0, after every obj_every-th epoch and
+     * after the last one, obj_out[k++] = orc_cross_entropy of the embedding
+     * over the positive edges with obj_weights [E] */
+    int obj_every;
+    const float *obj_weights;
+    double *obj_out;
 } orc_layout_params;
 
 typedef struct {
