@@ -14,9 +14,9 @@ same graph and initial layout through what a run must preserve:
 - the FP64 fuzzy cross entropy of the device trajectory every 10 epochs
   against the oracle's (seed 42), measured against how far the oracle's own
   trajectories with seeds 43 and 44 stand from it: the device run must be
-  no further from it than those are (OBJ_SPREAD times the spread; the sensitive
-  check: on this mixture the preservation is near 0 for any layout that
-  separates the clusters);
+  no further from it than those are (the sensitive check: on this mixture
+  the preservation is near 0 for any layout that separates the clusters);
+  the first point, before the runs diverge, within 1e-3;
 - the adapted repulsion scale at the end, within SCALE_TOL relative.
 """
 import numpy as np
