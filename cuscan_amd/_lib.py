@@ -138,6 +138,7 @@ SIGNATURES = {
     "umaphip_project": (C.c_int, [_vp, _i64, _i64, _i64, _i64, _vp, _vp, _vp]),
     "umaphip_argsort_workspace_bytes": (C.c_int, [_i64, C.POINTER(C.c_size_t)]),
     "umaphip_stable_argsort_f32": (C.c_int, [_vp, _i64, _vp, _vp, _vp, _vp, C.c_size_t, _vp]),
+    "umaphip_stable_argsort_f32_noalloc": (C.c_int, [_vp, _i64, _vp, _vp, _vp, _vp]),
     "umaphip_projection_ranks": (C.c_int, [_vp, _vp, _i64, _vp, _i64, _vp, _i32, _vp]),
     "umaphip_sample_resultants": (C.c_int, [_vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i64,
                                             C.c_uint32, C.c_float, C.c_float, C.c_float, _i32,

@@ -16,6 +16,7 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1236,6 +1237,73 @@ umaphip_status umaphip_stable_argsort_f32(const float *values, int64_t n, int32_
         launch_sort_pairs(scratch, workspace_bytes - 3 * up(n * 4), keys, keys_sorted, iota, order,
                           n, s);
         launch_values_from_keys(keys_sorted, n, sorted, s);
+    });
+}
+
+}  // extern "C"
+
+// Scratch of the allocation-free argsort: the workspace of
+// umaphip_stable_argsort_f32 for up to kNoAllocSortKeys keys, as a
+// file-scope device array that every device holds until the process ends.
+// Host code never uses its name as an address: each device's copy is
+// resolved once with hipGetSymbolAddress and kept per device.
+namespace {
+constexpr int64_t kNoAllocSortKeys = int64_t(1) << 20;
+constexpr size_t kNoAllocAlign = 256;
+constexpr size_t align_up(size_t x) { return (x + kNoAllocAlign - 1) / kNoAllocAlign * kNoAllocAlign; }
+// argsort workspace: keys, sorted keys, iota, then the radix sort's own
+// (ping-pong keys and values, per-tile digit histograms, digit totals)
+constexpr size_t kSortTileKeys = 2048;
+constexpr size_t kNoAllocScratchBytes =
+    3 * align_up(kNoAllocSortKeys * 4) + 2 * align_up(kNoAllocSortKeys * 4) +
+    align_up(256 * ((kNoAllocSortKeys + kSortTileKeys - 1) / kSortTileKeys) * 4) +
+    align_up(1024 * 4);
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void *g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+// This device's copy of `scratch` (resolved on the first call on a device,
+// with that device current).
+void *scratch_for_current_device() {
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    UMAPHIP_REQUIRE(dev >= 0 && dev < kMaxDevices, UMAPHIP_BACKEND, "device id %d out of range",
+                    dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        const hipError_t rc = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        UMAPHIP_REQUIRE(rc == hipSuccess && ptr != nullptr, UMAPHIP_BACKEND,
+                        "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(rc));
+        g_scratch_ptr[dev] = ptr;
+    }
+    return g_scratch_ptr[dev];
+}
+}  // namespace
+
+extern "C" {
+
+umaphip_status umaphip_stable_argsort_f32_noalloc(const float *values, int64_t n, int32_t *order,
+                                                  float *sorted, int64_t *status_out,
+                                                  void *stream) {
+    return guarded([&] {
+        UMAPHIP_REQUIRE(n >= 0, UMAPHIP_INVALID_INPUT, "invalid argsort size");
+        UMAPHIP_REQUIRE(n <= kNoAllocSortKeys, UMAPHIP_CAPACITY,
+                        "the allocation-free argsort holds at most %lld keys (got %lld); use "
+                        "umaphip_stable_argsort_f32 with a workspace",
+                        (long long)kNoAllocSortKeys, (long long)n);
+        size_t need = 0;
+        if (umaphip_argsort_workspace_bytes(n, &need) != UMAPHIP_OK) throw Fail{UMAPHIP_INVALID_INPUT};
+        UMAPHIP_REQUIRE(need <= kNoAllocScratchBytes, UMAPHIP_BACKEND,
+                        "argsort scratch too small (%zu < %zu)", kNoAllocScratchBytes, need);
+        void *ws = scratch_for_current_device();
+        const umaphip_status st = umaphip_stable_argsort_f32(values, n, order, sorted, status_out,
+                                                             ws, kNoAllocScratchBytes, stream);
+        if (st != UMAPHIP_OK) throw Fail{st};
     });
 }
 

@@ -132,6 +132,18 @@ umaphip_status umaphip_argsort_workspace_bytes(int64_t n, size_t *bytes);
 umaphip_status umaphip_stable_argsort_f32(const float *values, int64_t n, int32_t *order,
                                           float *sorted, int64_t *status_out, void *workspace,
                                           size_t workspace_bytes, void *stream);
+/* umaphip_stable_argsort_f32 without a workspace argument and without any
+ * allocation. No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid allocation). Its workspace is a
+ * static device array of about 20.5 MB per device, held until the process
+ * ends, so n is at most 1,048,576 keys (UMAPHIP_CAPACITY beyond that).
+ * Results equal umaphip_stable_argsort_f32's. Every call on a device
+ * shares that device's array: calls that overlap on one device (on
+ * different streams or host threads) race and must be serialised. */
+umaphip_status umaphip_stable_argsort_f32_noalloc(const float *values, int64_t n, int32_t *order,
+                                                  float *sorted, int64_t *status_out,
+                                                  void *stream);
 /* shared: ranks[order[j]] = j (n entries). Otherwise ranks[i] =
  * min(lower_bound(sorted, query[i]), n - 1) for h queries. */
 umaphip_status umaphip_projection_ranks(const int32_t *order, const float *sorted, int64_t n,
