@@ -1,5 +1,7 @@
-"""Batched layouts: many independent modern Adam fits in one launch per epoch,
-with the epoch loop captured once as a hipGraph and a tile-shape autotuner.
+"""Batched layouts: many independent modern Adam fits in one persistent
+cooperative launch that runs every epoch (grid barrier between epochs), or in
+one launch per epoch with the epoch loop captured once as a hipGraph, with a
+tile-shape autotuner.
 
 Each graph follows optimize_compact (crates/umap/src/standard_umap/
 modern_runtime.rs:1011-1378) in global negative mode with its own seed;
@@ -18,9 +20,11 @@ SHAPES = ("64 rows x 256 lanes", "32 rows x 128 lanes", "16 rows x 64 lanes")
 
 
 class LayoutBatch:
-    def __init__(self, graphs, heads, params: LayoutParams, seeds):
+    def __init__(self, graphs, heads, params: LayoutParams, seeds, persistent: bool = False):
         """graphs: compact graphs (dicts as from graph.compact_graph);
-        heads: initial [rows, 2] embeddings; seeds: one layout seed each."""
+        heads: initial [rows, 2] embeddings; seeds: one layout seed each;
+        persistent: one cooperative launch for all epochs (else one launch
+        per epoch)."""
         if not (len(graphs) == len(heads) == len(seeds)) or not graphs:
             raise ValueError("graphs, heads and seeds must be non-empty and equally long")
         rows = np.array([len(g["offsets"]) - 1 for g in graphs], np.int64)
@@ -46,6 +50,12 @@ class LayoutBatch:
             a["seeds"].ctypes.data, C.byref(handle)))
         self._h = handle
         self.shape = 0
+        self.set_persistent(persistent)
+
+    def set_persistent(self, persistent: bool) -> None:
+        """Launch form of later runs and tuning (same results either way)."""
+        _lib.check(_lib.lib().umaphip_batch_set_persistent(self._h, int(bool(persistent))))
+        self.persistent = bool(persistent)
 
     def tune(self, probe_epochs: int = 20):
         """Times each tile shape on the first epochs; keeps the fastest.

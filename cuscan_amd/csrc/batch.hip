@@ -7,11 +7,16 @@
 // known before the first epoch: the per-epoch negative orders come from the
 // graph's own StdRng(seed) shuffles (modern_orders.rs:359-680) and are
 // generated on host threads up front, and each epoch's arguments (schedule
-// value, bias corrections, order, Jacobi buffers) are fixed. Every epoch is
-// one segment-sweep launch in which each graph is a segment of tiles; a
-// graph's rows are computed exactly as a launch of that graph alone would
-// compute them. The tile shape (rows per tile x lanes) is chosen by timing
-// the candidate shapes on the first epochs (umaphip_batch_tune).
+// value, bias corrections, order, Jacobi buffers) are fixed. In each epoch
+// every graph is a segment of tiles; a graph's rows are computed exactly as
+// a launch of that graph alone would compute them. Two launch forms give
+// the same bits: one segment-sweep launch per epoch (replayed from a captured
+// hipGraph), or one persistent cooperative launch that runs every epoch with
+// a grid barrier between epochs. The cooperative launch is issued directly,
+// never captured: a graph node replayed without the cooperative launch's
+// set-up would have no grid barrier to meet at. The tile shape (rows per
+// tile x lanes) is chosen by timing the candidate shapes on the first epochs
+// in the chosen form (umaphip_batch_tune).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -43,11 +48,13 @@ struct umaphip_batch {
     DeviceArray<int32_t> offsets, indices, orders;
     DeviceArray<double> periods, negperiods, npos, nneg, npos0, nneg0;
     DeviceArray<SweepSegment> segs;
+    DeviceArray<int32_t> counters;      // persistent launch: tile counters per epoch and range
     int shape = 0;
     int64_t tiles = 0;
     bool built = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    bool persistent = false;            // one cooperative launch for every epoch
     float last_ms = 0.0f;
     bool ran = false;
 
@@ -111,6 +118,7 @@ struct umaphip_batch {
             }
         }
         segs.alloc(std::max<size_t>(host.size(), 1));
+        counters.alloc(static_cast<size_t>(std::max(epochs, 1)) * 8);
         if (!host.empty())
             HIP_CHECK(hipMemcpy(segs.ptr, host.data(), host.size() * sizeof(SweepSegment),
                                 hipMemcpyHostToDevice));
@@ -128,10 +136,16 @@ struct umaphip_batch {
     }
 
     void enqueue_epochs(int count) {
+        if (persistent) {
+            HIP_CHECK(launch_persistent_segments(segs.ptr, G, tiles, count, counters.ptr, stream,
+                                                 shape));
+            return;
+        }
         for (int t = 0; t < count; ++t)
             launch_segment_sweep(segs.ptr + static_cast<size_t>(t) * G, G, tiles, stream, shape);
     }
 
+    // the per-epoch launches of every epoch as one graph
     void capture() {
         if (exec) return;
         HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
@@ -140,12 +154,13 @@ struct umaphip_batch {
         HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
     }
 
+    // persistent: one cooperative launch whatever use_graph says
     void run(bool use_graph) {
         if (!built) build(shape);
-        if (use_graph) capture();
+        if (use_graph && !persistent) capture();
         reset();
         HIP_CHECK(hipEventRecord(ev0, stream));
-        if (use_graph) HIP_CHECK(hipGraphLaunch(exec, stream));
+        if (use_graph && !persistent) HIP_CHECK(hipGraphLaunch(exec, stream));
         else enqueue_epochs(epochs);
         HIP_CHECK(hipEventRecord(ev1, stream));
         ran = true;
@@ -303,6 +318,14 @@ umaphip_status umaphip_batch_set_shape(umaphip_batch *batch, int32_t shape) {
         UMAPHIP_REQUIRE(shape >= 0 && shape < kTileShapes, UMAPHIP_INVALID_CONFIG,
                         "tile shape must be in 0..%d", kTileShapes - 1);
         batch->build(shape);
+    });
+}
+
+umaphip_status umaphip_batch_set_persistent(umaphip_batch *batch, int32_t persistent) {
+    return guarded([&] {
+        UMAPHIP_REQUIRE(batch, UMAPHIP_INVALID_INPUT, "batch is null");
+        HIP_CHECK(hipStreamSynchronize(batch->stream));
+        batch->persistent = persistent != 0;
     });
 }
 

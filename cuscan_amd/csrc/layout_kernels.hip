@@ -4,6 +4,7 @@
 // The build disables floating-point contraction so that the f64 clocks
 // (next_negative += count * period) and every f32 force term round exactly
 // as the reference's uncontracted arithmetic does.
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -355,8 +356,11 @@ __device__ inline float unit_coef(float q, bool att, float a, float b, float k_a
     return g16 * tanh_pos(c * q * rq * inv_gamma) * rq;
 }
 
-// x mod m for x < 2^64, m < 2^32, through a double-precision quotient
-// estimate corrected by at most two steps.
+// x mod m through a double-precision quotient estimate. Precondition:
+// 0 < m < 2^31 and x < m * 2^31 (every caller passes x < m^2; m is the
+// negative range, at most N <= INT32_MAX). Then x / m < 2^31, the estimate
+// is within one of the true quotient, x - q m fits an int64 and at most two
+// correction steps run.
 __device__ inline uint64_t mod_u64(uint64_t x, uint64_t m, double inv_m) {
     const uint64_t q = static_cast<uint64_t>(static_cast<double>(x) * inv_m);
     int64_t r = static_cast<int64_t>(x - q * m);
@@ -670,6 +674,65 @@ __global__ __launch_bounds__(THREADS) void segment_sweep_kernel(const SweepSegme
     const int64_t tile = b - segs[lo].tile0;
     if (tile * ROWS >= A.h) return;
     sweep_tile<ROWS, THREADS>(A, tile);
+}
+
+// Batched layouts in one persistent launch: every epoch of every segment.
+// Epoch t's segments are segs[t * n_segs, (t + 1) * n_segs). Its tiles are
+// split into 8 contiguous ranges, one per XCD; workgroups take chunks of
+// kPersistChunk tiles from their XCD's range through that range's counter
+// (ctr[t * 8 + range], zero at launch), then from the other ranges, so slow
+// tiles (hub rows) do not hold the epoch back. Each tile runs as the
+// per-epoch launch would (sweep_tile with the same arguments); a grid
+// barrier separates the epochs. Launched cooperatively, so every workgroup
+// is resident and the barrier cannot wait on a workgroup that never runs.
+// Every counter only grows, so every workgroup leaves the chunk loops.
+constexpr int kPersistChunk = 4;
+
+template <int ROWS, int THREADS>
+__global__ __launch_bounds__(THREADS, THREADS == 64 ? 5 : 4) void persistent_segment_kernel(
+    const SweepSegment *segs,
+                                                                     int32_t n_segs,
+                                                                     int64_t total_tiles,
+                                                                     int32_t epochs,
+                                                                     int32_t *ctr) {
+    cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+    __shared__ int32_t s_take;
+    const int xcd = static_cast<int>(blockIdx.x % 8);
+    for (int32_t t = 0; t < epochs; ++t) {
+        const SweepSegment *es = segs + static_cast<int64_t>(t) * n_segs;
+        int seg = 0;
+        int64_t seg_lo = 0, seg_hi = -1;   // tiles of the cached segment
+        for (int r = 0; r < 8; ++r) {
+            const int range = (xcd + r) & 7;
+            const int64_t lo_t = total_tiles * range / 8, hi_t = total_tiles * (range + 1) / 8;
+            for (;;) {
+                if (threadIdx.x == 0) s_take = atomicAdd(&ctr[t * 8 + range], kPersistChunk);
+                __syncthreads();
+                const int64_t c0 = lo_t + s_take;
+                __syncthreads();
+                if (c0 >= hi_t) break;
+                const int64_t c1 = min<int64_t>(c0 + kPersistChunk, hi_t);
+                for (int64_t b = c0; b < c1; ++b) {
+                    if (b < seg_lo || b >= seg_hi) {
+                        int lo = 0, hi = n_segs;
+                        while (hi - lo > 1) {
+                            const int mid = (lo + hi) >> 1;
+                            if (es[mid].tile0 <= b) lo = mid;
+                            else hi = mid;
+                        }
+                        seg = lo;
+                        seg_lo = es[lo].tile0;
+                        seg_hi = lo + 1 < n_segs ? es[lo + 1].tile0 : total_tiles;
+                    }
+                    const SweepArgs A = es[seg].args;
+                    const int64_t tile = b - seg_lo;
+                    if (tile * ROWS < A.h) sweep_tile<ROWS, THREADS>(A, tile);
+                    __syncthreads();   // the next tile reuses the LDS tables
+                }
+            }
+        }
+        grid.sync();
+    }
 }
 
 // Transform layouts: workgroup b runs every epoch of tile b - tile0[s] of
@@ -2097,6 +2160,45 @@ void launch_segment_sweep(const SweepSegment *segs, int32_t n_segs, int64_t tota
             break;
     }
     HIP_CHECK(hipGetLastError());
+}
+
+__global__ void zero_i32_range_kernel(int32_t *p, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+template <int ROWS, int THREADS>
+hipError_t launch_persistent_shape(const SweepSegment *segs, int32_t n_segs, int64_t total_tiles,
+                                   int32_t epochs, int32_t *ctr, hipStream_t s) {
+    auto kernel = persistent_segment_kernel<ROWS, THREADS>;
+    int dev = 0, cus = 0, per_cu = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, THREADS, 0));
+    UMAPHIP_REQUIRE(per_cu > 0, UMAPHIP_BACKEND, "persistent sweep: no resident workgroup fits");
+    const int64_t blocks = std::min<int64_t>(total_tiles, static_cast<int64_t>(per_cu) * cus);
+    const int64_t nctr = static_cast<int64_t>(epochs) * 8;
+    hipLaunchKernelGGL(zero_i32_range_kernel, dim3(grid_for(nctr)), dim3(kThreads), 0, s, ctr, nctr);
+    HIP_CHECK(hipGetLastError());
+    void *args[] = {const_cast<SweepSegment **>(&segs), &n_segs, &total_tiles, &epochs, &ctr};
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(kernel),
+                                      dim3(static_cast<unsigned>(blocks)), dim3(THREADS), args, 0,
+                                      s);
+}
+
+hipError_t launch_persistent_segments(const SweepSegment *segs, int32_t n_segs,
+                                      int64_t total_tiles, int32_t epochs, int32_t *counters,
+                                      hipStream_t s, int shape) {
+    if (n_segs == 0 || total_tiles == 0 || epochs == 0) return hipSuccess;
+    switch (shape) {
+        case 1:
+            return launch_persistent_shape<32, 128>(segs, n_segs, total_tiles, epochs, counters, s);
+        case 2:
+            return launch_persistent_shape<16, 64>(segs, n_segs, total_tiles, epochs, counters, s);
+        default:
+            return launch_persistent_shape<kTileRows, kThreads>(segs, n_segs, total_tiles, epochs,
+                                                                counters, s);
+    }
 }
 
 int64_t transform_tile_rows() { return kTransformRows; }

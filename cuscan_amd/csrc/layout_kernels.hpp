@@ -71,6 +71,14 @@ constexpr int kTileShapes = 3;
 int64_t tile_rows(int shape = 0);
 void launch_segment_sweep(const SweepSegment *segs, int32_t n_segs, int64_t total_tiles,
                           hipStream_t s, int shape = 0);
+// Every epoch of a batch in one cooperative launch: segs holds `epochs`
+// consecutive tables of n_segs segments (epoch t's at t * n_segs); a grid
+// barrier separates the epochs; counters: 8 * epochs ints of workspace
+// (cleared by the launcher). Results equal one launch_segment_sweep per
+// epoch, bit for bit. Returns the cooperative launch's status.
+hipError_t launch_persistent_segments(const SweepSegment *segs, int32_t n_segs,
+                                      int64_t total_tiles, int32_t epochs, int32_t *counters,
+                                      hipStream_t s, int shape = 0);
 
 // Whole transform layouts in one launch. In a transform every row moves
 // alone against the fixed tail, and the tail's negative tables for each

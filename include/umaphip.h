@@ -642,6 +642,12 @@ umaphip_status umaphip_batch_create(int32_t graphs, const int64_t *rows, const i
 umaphip_status umaphip_batch_tune(umaphip_batch *batch, int32_t probe_epochs, int32_t *shape_out,
                                   double *ms_out);
 umaphip_status umaphip_batch_set_shape(umaphip_batch *batch, int32_t shape);
+/* Launch form of later runs and tuning (no reference counterpart; the
+ * reference runs one layout at a time): persistent = 1 runs every epoch of
+ * every graph in one cooperative launch (issued directly, whatever
+ * use_graph says), the workgroups meeting at a grid barrier between epochs;
+ * 0 launches one segment sweep per epoch. Both give the same bits. */
+umaphip_status umaphip_batch_set_persistent(umaphip_batch *batch, int32_t persistent);
 /* Resets to the uploaded initial state and enqueues every epoch: as one
  * hipGraph launch (use_graph = 1, captured on first use) or as plain
  * launches. Does not synchronise. */

@@ -1,4 +1,5 @@
-"""Batched layouts (many graphs per launch, hipGraph-captured epoch loop).
+"""Batched layouts (many graphs per launch: one persistent cooperative launch
+for every epoch, or a hipGraph-captured loop of per-epoch launches).
 
 Every graph must match its own single-graph plan: clocks bit for bit;
 coordinates bit for bit for small graphs (the plan runs the same tile kernel
@@ -58,6 +59,19 @@ def test_batch_matches_single_graph_plans_and_replays(exclude):
     for a, b, c in zip(r1, r2, r3):
         assert (a == b).all() and (a == c).all()
     assert (p1 == p2).all() and (p1 == p3).all() and (n1 == n3).all()
+    # one persistent cooperative launch for every epoch equals one launch per
+    # epoch; the autotuner also runs in that form
+    assert not batch.persistent
+    batch.set_persistent(True)
+    pshape, pms = batch.tune(probe_epochs=5)
+    assert pshape in (0, 1, 2) and all(t > 0 for t in pms)
+    batch.set_shape(0)
+    batch.run(use_graph=True)
+    r4, _, p4, n4 = batch.download(clocks=True)
+    batch.set_persistent(False)
+    for a, b in zip(r1, r4):
+        assert (a == b).all()
+    assert (p1 == p4).all() and (n1 == n4).all()
     e0 = 0
     for g, (graph, head) in enumerate(zip(graphs, heads)):
         p = L.LayoutParams(epochs=epochs, a=A_CURVE, b=B_CURVE, exclude_graph_neighbors=exclude,
