@@ -10,9 +10,10 @@
 // (32 per wave, held for the whole scan as MFMA A-fragments in VGPRs) and
 // streams one slice of the references through a double-buffered LDS stage
 // in chunks of 16 rows: the next chunk is loaded straight from global
-// memory into LDS (global_load_lds) while the current one feeds the MFMAs.
+// memory into LDS (global_load_lds, one instruction per MFMA K step of the
+// chunk) while the current one feeds the MFMAs.
 // Approximate distances ||r||^2 - 2 q.r are compared against each query's
-// running 64th-best threshold (kept in registers); survivors are appended to
+// running 64th-best threshold (kept in LDS); survivors are appended to
 // a per-(query, slice) buffer that is compacted to its sorted best 64 when
 // it could overflow. A wave owns the candidate state of its own 32 queries,
 // so the threshold epilogue needs no workgroup barrier. Slices are merged
@@ -72,6 +73,7 @@ constexpr int64_t kWideMergeMaxQ = 8192;   // batches up to this many rows merge
 constexpr int kMomentRows = 2048;          // rows per workgroup of the column moments
 constexpr int kExactQB = 4;                // queries per workgroup of the exact rescan
 constexpr int64_t kExactRound = 1024;      // uncertified queries per rescan round
+constexpr int kScanRing = 4;               // B fragments in flight in the large-grid scan
 
 __device__ inline uint16_t f32_to_bf16_rne(float f) {
     uint32_t u = __float_as_uint(f);
@@ -282,14 +284,14 @@ __device__ __forceinline__ void put_chunk(const u32x4 (&src)[LW], float rn, uint
     if (wave == 0 && lane < 16) rn_buf[lane] = rn;
 }
 
-// SMALL: grids of at most one workgroup per CU (small batches). The
-// global_load_lds staging waits for each chunk before the next is computed
-// (the compiler cannot tell which LDS buffer an in-flight DMA writes, so it
-// waits for all of them before an LDS read); SMALL instead loads chunks
-// into registers two chunks ahead and writes them to LDS itself, so two
-// chunks stay in flight while one is computed.
-template <int KSTEPS, bool SMALL>
-__global__ __launch_bounds__(256, SMALL ? 1 : 2) void knn_scan_kernel(
+// Grids of at most one workgroup per CU (small batches): the scan with
+// chunks loaded into registers two chunks ahead and written to LDS by the
+// waves themselves, so that two chunks stay in flight while one is computed
+// (one workgroup per CU has no second workgroup to cover a chunk's load).
+// The LDS image keeps padded rows (SLOTS_ROW words); the fragments, MFMA
+// order and rn - 2 q.r are those of knn_scan_kernel.
+template <int KSTEPS>
+__global__ __launch_bounds__(256, 1) void knn_scan_small_kernel(
     const uint16_t *__restrict__ qb, int64_t nq, const uint16_t *__restrict__ rb,
     const float *__restrict__ rnorm, int64_t nr, int64_t per_slice, int slices,
     float *__restrict__ cand_d, int32_t *__restrict__ cand_i, int32_t *__restrict__ cand_n,
@@ -301,7 +303,7 @@ __global__ __launch_bounds__(256, SMALL ? 1 : 2) void knn_scan_kernel(
     constexpr int BUF = INSTR * 64 * 8;         // bf16 per buffer (whole 1 KB loads)
     constexpr int kRefresh = 32;                // chunks between reads of the global thresholds
     __shared__ __attribute__((aligned(16))) uint16_t sB[2 * BUF];
-    __shared__ float s_rn[SMALL ? 2 : 1][16];   // SMALL: reference norms of each buffered chunk
+    __shared__ float s_rn[2][16];   // reference norms of each buffered chunk
     __shared__ float s_tau[kTQ];
     __shared__ int s_cnt[kTQ];
 
@@ -312,30 +314,13 @@ __global__ __launch_bounds__(256, SMALL ? 1 : 2) void knn_scan_kernel(
     const int64_t r_begin = static_cast<int64_t>(slice) * per_slice;
     const int64_t r_end = min<int64_t>(nr, r_begin + per_slice);
 
-    // One chunk into buffer `buf`: LDS word `slot` of the padded image holds
-    // row slot / SLOTS_ROW, word slot % SLOTS_ROW (the last word of a row and
-    // the slots past the chunk are padding; they load a harmless in-bounds
-    // word).
-    auto stage = [&](int buf, int64_t chunk) {
-        for (int j = wave; j < INSTR; j += 4) {
-            const int slot = j * 64 + lane;
-            const int row = slot / SLOTS_ROW, col = slot % SLOTS_ROW;
-            const int64_t ref = chunk + row;
-            const uint16_t *src = rb;
-            if (row < kTR && col < DP / 8 && ref < r_end) src = rb + ref * DP + col * 8;
-            __builtin_amdgcn_global_load_lds(
-                (const void __attribute__((address_space(1))) *)src,
-                (void __attribute__((address_space(3))) *)(sB + buf * BUF + j * 64 * 8), 16, 0, 0);
-        }
-    };
-    // SMALL: chunk loads into registers (slot 0 or 1), and from there into an
-    // LDS buffer; past the slice they load harmless in-bounds words.
-    constexpr int LW = SMALL ? (INSTR + 3) / 4 : 1;   // staging loads per wave (some use LW - 1)
+    // Chunk loads into registers (slot 0 or 1), and from there into an LDS
+    // buffer: LDS word `slot` of the padded image holds row slot / SLOTS_ROW,
+    // word slot % SLOTS_ROW (the last word of a row and the slots past the
+    // chunk are padding); past the slice they load harmless in-bounds words.
+    constexpr int LW = (INSTR + 3) / 4;   // staging loads per wave (some use LW - 1)
     u32x4 pre0[LW], pre1[LW];
     float rn0 = 0.0f, rn1 = 0.0f;
-    if constexpr (!SMALL) {
-        if (r_begin < r_end) stage(0, r_begin);
-    }
 
     // A fragments: lane holds row (lane % 16) and K elements 8*(lane/16)..+7
     // of each 32-wide K step, for two 16-query sub-blocks.
@@ -371,36 +356,27 @@ __global__ __launch_bounds__(256, SMALL ? 1 : 2) void knn_scan_kernel(
             const int64_t q = q_block + wave * 32 + qa * 16 + 4 * (lane >> 4) + i;
             tau_r[qa][i] = q < nq ? key_float(tau_key[q]) : INFINITY;
         }
-    float rn_next = 0.0f;
-    if (!SMALL && r_begin + (lane & 15) < r_end) rn_next = rnorm[r_begin + (lane & 15)];
-    if constexpr (SMALL) {
-        // the query fragments and thresholds have landed before the chunk
-        // loads start, so no wait in the loop counts them
-        wait_vmcnt<0>();
-        if (r_begin < r_end) {
-            fetch_chunk<LW, INSTR, SLOTS_ROW, DP>(pre0, rn0, rb, rnorm, r_begin, r_end, wave, lane);
-            fetch_chunk<LW, INSTR, SLOTS_ROW, DP>(pre1, rn1, rb, rnorm, r_begin + kTR, r_end, wave, lane);
-            put_chunk<LW, INSTR>(pre0, rn0, sB, &s_rn[0][0], wave, lane);
-            fetch_chunk<LW, INSTR, SLOTS_ROW, DP>(pre0, rn0, rb, rnorm, r_begin + 2 * kTR, r_end, wave, lane);
-        }
+    // the query fragments and thresholds have landed before the chunk loads
+    // start, so no wait in the loop counts them
+    wait_vmcnt<0>();
+    if (r_begin < r_end) {
+        fetch_chunk<LW, INSTR, SLOTS_ROW, DP>(pre0, rn0, rb, rnorm, r_begin, r_end, wave, lane);
+        fetch_chunk<LW, INSTR, SLOTS_ROW, DP>(pre1, rn1, rb, rnorm, r_begin + kTR, r_end, wave, lane);
+        put_chunk<LW, INSTR>(pre0, rn0, sB, &s_rn[0][0], wave, lane);
+        fetch_chunk<LW, INSTR, SLOTS_ROW, DP>(pre0, rn0, rb, rnorm, r_begin + 2 * kTR, r_end, wave, lane);
     }
 
     // One chunk: wait for its buffer, compute it against the queries and keep
-    // the survivors. SMALL passes the buffer as a constant so that each of
-    // the two unrolled copies stores and refills a fixed register set.
+    // the survivors. The buffer is a constant so that each of the two
+    // unrolled copies stores and refills a fixed register set.
     int since = 0;
     auto scan_chunk = [&](int64_t chunk, auto bufc) {
         const int buf = bufc;
-        if constexpr (SMALL) {
-            // chunk's LDS writes are done and the other buffer is free; the
-            // fences order LDS only, so the chunks in flight are not waited on
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-            __builtin_amdgcn_s_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-        } else {
-            __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): this lane's loads of the current buffer
-            __syncthreads();                      // every lane's loads landed; the other buffer is free
-        }
+        // chunk's LDS writes are done and the other buffer is free; the fences
+        // order LDS only, so the chunks in flight are not waited on
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+        __builtin_amdgcn_s_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
         if (++since == kRefresh) {
             // thresholds other slices have published for this wave's queries
             since = 0;
@@ -415,31 +391,20 @@ __global__ __launch_bounds__(256, SMALL ? 1 : 2) void knn_scan_kernel(
                 for (int i = 0; i < 4; ++i)
                     tau_r[qa][i] = s_tau[wave * 32 + qa * 16 + 4 * (lane >> 4) + i];
         }
-        float rn;
-        if constexpr (SMALL) {
-            rn = s_rn[buf][lane & 15];
-            // registers: chunk + kTR in slot buf ^ 1, chunk + 2 kTR in slot buf;
-            // chunk + kTR goes to LDS and chunk + 3 kTR into its slot. Done
-            // past the slice's end too (harmless words into a buffer nobody
-            // reads): a conditional refill would leave the compiler unsure
-            // which register set holds the newest loads, and it would then
-            // wait for all of them.
-            if constexpr (decltype(bufc)::value == 0) {
-                put_chunk<LW, INSTR>(pre1, rn1, sB + BUF, &s_rn[1][0], wave, lane);
-                fetch_chunk<LW, INSTR, SLOTS_ROW, DP>(pre1, rn1, rb, rnorm, chunk + 3 * kTR, r_end,
-                                                      wave, lane);
-            } else {
-                put_chunk<LW, INSTR>(pre0, rn0, sB, &s_rn[0][0], wave, lane);
-                fetch_chunk<LW, INSTR, SLOTS_ROW, DP>(pre0, rn0, rb, rnorm, chunk + 3 * kTR, r_end,
-                                                      wave, lane);
-            }
+        const float rn = s_rn[buf][lane & 15];
+        // registers: chunk + kTR in slot buf ^ 1, chunk + 2 kTR in slot buf;
+        // chunk + kTR goes to LDS and chunk + 3 kTR into its slot. Done past
+        // the slice's end too (harmless words into a buffer nobody reads): a
+        // conditional refill would leave the compiler unsure which register
+        // set holds the newest loads, and it would then wait for all of them.
+        if constexpr (decltype(bufc)::value == 0) {
+            put_chunk<LW, INSTR>(pre1, rn1, sB + BUF, &s_rn[1][0], wave, lane);
+            fetch_chunk<LW, INSTR, SLOTS_ROW, DP>(pre1, rn1, rb, rnorm, chunk + 3 * kTR, r_end,
+                                                  wave, lane);
         } else {
-            rn = rn_next;
-            if (chunk + kTR < r_end) {
-                stage(buf ^ 1, chunk + kTR);
-                const int64_t nref = chunk + kTR + (lane & 15);
-                rn_next = nref < r_end ? rnorm[nref] : 0.0f;
-            }
+            put_chunk<LW, INSTR>(pre0, rn0, sB, &s_rn[0][0], wave, lane);
+            fetch_chunk<LW, INSTR, SLOTS_ROW, DP>(pre0, rn0, rb, rnorm, chunk + 3 * kTR, r_end,
+                                                  wave, lane);
         }
         const uint16_t *B = sB + buf * BUF;
         floatx4 acc[2] = {floatx4{0.0f, 0.0f, 0.0f, 0.0f}, floatx4{0.0f, 0.0f, 0.0f, 0.0f}};
@@ -508,21 +473,13 @@ __global__ __launch_bounds__(256, SMALL ? 1 : 2) void knn_scan_kernel(
             }
         }
     };
-    if constexpr (SMALL) {
-        // pairs of chunks, then an odd last one (in buffer 0)
-        int64_t chunk = r_begin;
-        for (; chunk + kTR < r_end; chunk += 2 * kTR) {
-            scan_chunk(chunk, std::integral_constant<int, 0>{});
-            scan_chunk(chunk + kTR, std::integral_constant<int, 1>{});
-        }
-        if (chunk < r_end) scan_chunk(chunk, std::integral_constant<int, 0>{});
-    } else {
-        int b = 0;
-        for (int64_t chunk = r_begin; chunk < r_end; chunk += kTR) {
-            scan_chunk(chunk, b);
-            b ^= 1;
-        }
+    // pairs of chunks, then an odd last one (in buffer 0)
+    int64_t chunk = r_begin;
+    for (; chunk + kTR < r_end; chunk += 2 * kTR) {
+        scan_chunk(chunk, std::integral_constant<int, 0>{});
+        scan_chunk(chunk + kTR, std::integral_constant<int, 1>{});
     }
+    if (chunk < r_end) scan_chunk(chunk, std::integral_constant<int, 0>{});
     __threadfence_block();
     for (int l = 0; l < 32; ++l) {
         const int ql = wave * 32 + l;
@@ -532,6 +489,205 @@ __global__ __launch_bounds__(256, SMALL ? 1 : 2) void knn_scan_kernel(
         const int64_t base = (q * slices + slice) * kCap;
         if (n > 0) compact_buffer(cand_d + base, cand_i + base, n, lane);
         if (lane == 0) cand_n[q * slices + slice] = n < kTop ? n : kTop;
+    }
+}
+
+// The prefilter scan of one (query block, slice) for grids of two workgroups
+// per CU. Each chunk of 16 references is staged with global_load_lds into
+// the other half of an LDS double buffer while the current one feeds the
+// MFMAs.
+//
+// LDS image of a chunk, K-step major: KSTEPS blocks of 1 KB, block ks holds
+// the 16 rows x 32 features of MFMA K step ks, row r's four 16-byte words at
+// byte 64 r + 16 (w ^ ((r >> 2) & 3)) (the XOR puts any 16 rows of one word
+// on distinct banks). One DMA instruction fills one block: lane l loads row
+// l >> 2, word (l & 3) ^ ((l >> 4) & 3), so a lane's source offset is the
+// same for every block and chunk up to a multiple of 64 bytes (no address
+// arithmetic per 16-byte word), and a B fragment is one ds_read_b128 at a
+// lane-constant offset plus 1 KB per K step.
+//
+// Registers: the A fragments take 8 KSTEPS VGPRs (200 at 784 features) of
+// the 256 that two waves per SIMD allow. The rest must hold a ring of RING
+// B fragments (the read of K step ks + RING is in flight while step ks feeds
+// the MFMAs), the accumulators and the epilogue: thresholds are read from
+// LDS per chunk and the epilogue's indices are 32-bit, so nothing spills. A
+// spill would be costly: with an LDS DMA in flight the compiler turns every
+// wait on a vector memory load (a scratch reload included) into vmcnt(0).
+// Above 25 K steps the A fragments alone fill 256 VGPRs, so those widths
+// run one wave per SIMD (512 VGPRs).
+template <int KSTEPS, int RING>
+__global__ __launch_bounds__(256, KSTEPS > 25 ? 1 : 2) void knn_scan_kernel(
+    const uint16_t *__restrict__ qb, int64_t nq, const uint16_t *__restrict__ rb,
+    const float *__restrict__ rnorm, int64_t nr, int64_t per_slice, int slices,
+    float *__restrict__ cand_d, int32_t *__restrict__ cand_i, int32_t *__restrict__ cand_n,
+    uint32_t *__restrict__ tau_key, int exclude_self, int64_t query_offset) {
+    constexpr int DP = KSTEPS * 32;
+    constexpr int BUF = KSTEPS * 512;   // bf16 per buffer
+    constexpr int kRefresh = 32;
+    __shared__ __attribute__((aligned(16))) uint16_t sB[2 * BUF];
+    __shared__ __attribute__((aligned(16))) float s_tau[kTQ];
+    __shared__ int s_cnt[kTQ];
+
+    const int tid = threadIdx.x;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    const int q_block = static_cast<int>(blockIdx.x) * kTQ;
+    const int slice = blockIdx.y;
+    const int r_begin = static_cast<int>(slice * per_slice);
+    const int r_end = static_cast<int>(min<int64_t>(nr, r_begin + per_slice));
+    const int qnq = static_cast<int>(nq);
+    const int inr = static_cast<int>(nr);
+
+    // DMA lane mapping (row and source word of this lane's 16 bytes)
+    const int d_row = lane >> 2;
+    const int d_word = (lane & 3) ^ ((d_row >> 2) & 3);
+    const uint16_t *const d_src = rb + static_cast<int64_t>(d_row) * DP + d_word * 8;
+    auto stage = [&](int buf, int chunk) {
+        // rows past the references (last chunk only) load row 0 instead
+        const uint16_t *src = d_src + static_cast<int64_t>(chunk) * DP + wave * 32;
+        if (chunk + kTR > inr && chunk + d_row >= inr) src = rb + d_word * 8 + wave * 32;
+#pragma unroll
+        for (int t = 0; t < (KSTEPS + 3) / 4; ++t) {
+            if (wave + 4 * t < KSTEPS)
+                __builtin_amdgcn_global_load_lds(
+                    (const void __attribute__((address_space(1))) *)(src + t * 128),
+                    (void __attribute__((address_space(3))) *)(sB + buf * BUF + (wave + 4 * t) * 512), 16, 0, 0);
+        }
+    };
+    if (r_begin < r_end) stage(0, r_begin);
+
+    bf16x8 A[2][KSTEPS];
+#pragma unroll
+    for (int qa = 0; qa < 2; ++qa) {
+        const int64_t row = q_block + wave * 32 + qa * 16 + (lane & 15);
+#pragma unroll
+        for (int ks = 0; ks < KSTEPS; ++ks) {
+            if (row < nq) {
+                const uint4 raw = *reinterpret_cast<const uint4 *>(qb + row * DP + ks * 32 + 8 * (lane >> 4));
+                A[qa][ks] = __builtin_bit_cast(bf16x8, raw);
+            } else {
+                A[qa][ks] = __builtin_bit_cast(bf16x8, make_uint4(0, 0, 0, 0));
+            }
+        }
+    }
+    if (lane < 32) {
+        // queries past nq get -inf: no distance passes dt <= -inf
+        const int q = q_block + wave * 32 + lane;
+        s_tau[wave * 32 + lane] = q < qnq ? key_float(tau_key[q]) : -INFINITY;
+        s_cnt[wave * 32 + lane] = 0;
+    }
+    float rn_next = 0.0f;
+    if (r_begin + (lane & 15) < r_end) rn_next = rnorm[r_begin + (lane & 15)];
+    // lane's C rows: queries ql_lane + qa * 16 + i
+    const int ql_lane = wave * 32 + 4 * (lane >> 4);
+    const int self_base = static_cast<int>(query_offset) + q_block + ql_lane;
+    // B fragment: row lane & 15, word lane >> 4 of each K block
+    const int b_row = lane & 15;
+    const uint16_t *const Bl0 = sB + b_row * 32 + (((lane >> 4) ^ ((b_row >> 2) & 3)) * 8);
+
+    int since = 0, buf = 0;
+    for (int chunk = r_begin; chunk < r_end; chunk += kTR, buf ^= 1) {
+        __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): this lane's loads of the current buffer
+        __syncthreads();                      // every lane's loads landed; the other buffer is free
+        if (++since == kRefresh) {
+            since = 0;
+            const int q = q_block + wave * 32 + (lane & 31);
+            if (lane < 32 && q < qnq) {
+                const float g = key_float(tau_key[q]);
+                if (g < s_tau[wave * 32 + lane]) s_tau[wave * 32 + lane] = g;
+            }
+        }
+        const float rn = rn_next;
+        if (chunk + kTR < r_end) {
+            stage(buf ^ 1, chunk + kTR);
+            const int nref = chunk + kTR + (lane & 15);
+            rn_next = nref < r_end ? rnorm[nref] : 0.0f;
+        }
+        const uint16_t *Bl = Bl0 + buf * BUF;
+        floatx4 acc[2] = {floatx4{0.0f, 0.0f, 0.0f, 0.0f}, floatx4{0.0f, 0.0f, 0.0f, 0.0f}};
+        uint4 ring[RING];
+#pragma unroll
+        for (int r = 0; r < RING; ++r)
+            if (r < KSTEPS) ring[r] = *reinterpret_cast<const uint4 *>(Bl + r * 512);
+#pragma unroll
+        for (int ks = 0; ks < KSTEPS; ++ks) {
+            const bf16x8 bfrag = __builtin_bit_cast(bf16x8, ring[ks % RING]);
+            if (ks + RING < KSTEPS) ring[ks % RING] = *reinterpret_cast<const uint4 *>(Bl + (ks + RING) * 512);
+#pragma unroll
+            for (int qa = 0; qa < 2; ++qa)
+                acc[qa] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[qa][ks], bfrag, acc[qa], 0, 0, 0);
+        }
+        // issue order: the RING first reads, then per K step its two MFMAs
+        // and the read RING steps ahead (the scheduler would otherwise batch
+        // the reads and wait for each batch)
+#pragma unroll
+        for (int r = 0; r < RING; ++r) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+#pragma unroll
+        for (int ks = 0; ks < KSTEPS; ++ks) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+            if (ks + RING < KSTEPS) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+        }
+        const int ref = chunk + (lane & 15);
+        const bool ref_ok = ref < r_end;
+        const int self_d = exclude_self ? ref - self_base : -1;
+        bool any = false;
+#pragma unroll
+        for (int qa = 0; qa < 2; ++qa) {
+            const floatx4 tau = *reinterpret_cast<const floatx4 *>(&s_tau[ql_lane + qa * 16]);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int ql = ql_lane + qa * 16 + i;
+                const float dt = rn - 2.0f * acc[qa][i];
+                const bool keep = ref_ok && dt <= tau[i] && self_d != qa * 16 + i;
+                const uint64_t m = __builtin_amdgcn_ballot_w64(keep);
+                if (m) {
+                    const int base_n = s_cnt[ql];
+                    const uint32_t gm = static_cast<uint32_t>(m >> (16 * (lane >> 4))) & 0xFFFFu;
+                    if (keep) {
+                        const int pos = base_n + __popc(gm & ((1u << (lane & 15)) - 1u));
+                        int qv = q_block + ql;
+                        asm volatile("" : "+v"(qv));   // keeps the address computation in the branch
+                        const int64_t base = (static_cast<int64_t>(qv) * slices + slice) * kCap;
+                        cand_d[base + pos] = dt;
+                        cand_i[base + pos] = ref;
+                    }
+                    if ((lane & 15) == 0) s_cnt[ql] = base_n + __popc(gm);
+                    any = true;
+                }
+            }
+        }
+        if (any) {
+            const int n_l = lane < 32 ? s_cnt[wave * 32 + lane] : 0;
+            uint64_t full = __builtin_amdgcn_ballot_w64(lane < 32 && n_l > kCap - kTR);
+            if (full) {
+                __threadfence_block();
+                while (full) {
+                    const int l = __builtin_ctzll(full);
+                    full &= full - 1;
+                    const int ql = wave * 32 + l;
+                    const int q = q_block + ql;
+                    const int n = __shfl(n_l, l, 64);
+                    const int64_t base = (static_cast<int64_t>(q) * slices + slice) * kCap;
+                    const float tau = compact_buffer(cand_d + base, cand_i + base, n, lane);
+                    if (lane == 0) {
+                        const uint32_t old = atomicMin(&tau_key[q], float_key(tau));
+                        const float g = key_float(old);
+                        s_tau[ql] = g < tau ? g : tau;
+                        s_cnt[ql] = kTop;
+                    }
+                }
+                __threadfence_block();
+            }
+        }
+    }
+    __threadfence_block();
+    for (int l = 0; l < 32; ++l) {
+        const int ql = wave * 32 + l;
+        const int q = q_block + ql;
+        if (q >= qnq) continue;
+        const int n = s_cnt[ql];
+        const int64_t base = (static_cast<int64_t>(q) * slices + slice) * kCap;
+        if (n > 0) compact_buffer(cand_d + base, cand_i + base, n, lane);
+        if (lane == 0) cand_n[static_cast<int64_t>(q) * slices + slice] = n < kTop ? n : kTop;
     }
 }
 
@@ -1073,11 +1229,11 @@ void launch_scan(dim3 grid, hipStream_t s, const uint16_t *qb, int64_t nq, const
                  const float *rnorm, int64_t nr, int64_t per_slice, int slices, float *cd,
                  int32_t *ci, int32_t *cn, uint32_t *tk, int exclude_self, int64_t qoff) {
     if (static_cast<int64_t>(grid.x) * grid.y <= 256)
-        hipLaunchKernelGGL((knn_scan_kernel<KS, true>), grid, dim3(256), 0, s, qb, nq, rb, rnorm, nr,
+        hipLaunchKernelGGL((knn_scan_small_kernel<KS>), grid, dim3(256), 0, s, qb, nq, rb, rnorm, nr,
                            per_slice, slices, cd, ci, cn, tk, exclude_self, qoff);
     else
-        hipLaunchKernelGGL((knn_scan_kernel<KS, false>), grid, dim3(256), 0, s, qb, nq, rb, rnorm, nr,
-                           per_slice, slices, cd, ci, cn, tk, exclude_self, qoff);
+        hipLaunchKernelGGL((knn_scan_kernel<KS, kScanRing>), grid, dim3(256), 0, s, qb, nq, rb, rnorm,
+                           nr, per_slice, slices, cd, ci, cn, tk, exclude_self, qoff);
 }
 
 template <int KS>

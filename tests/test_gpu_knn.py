@@ -47,6 +47,28 @@ def test_integer_data_bit_exact(n, d, q, k):
     assert (gd.view(np.uint32) == ed.view(np.uint32)).all()
 
 
+# Grids above 256 workgroups (query blocks x slices) run the DMA-staged scan
+# (two workgroups per CU, one wave per SIMD above 800 features); 40,000
+# references give 9 slices, so 4,000 queries (32 blocks) make 288
+# workgroups. 39,990 references end the last slice in a ragged chunk, and
+# 300,000 with 512 queries (4 blocks x 73 slices) add the sample bound.
+@pytest.mark.parametrize("n,d,q", [(40_000, 1000, 4000), (40_000, 500, 4000),
+                                   (39_990, 200, 4000), (40_000, 20, 4000),
+                                   (300_000, 200, 512)])
+def test_large_grid_scan_widths_bit_exact(n, d, q):
+    from cuscan_amd.knn import KnnIndex
+
+    ref = image_like(n, d, 10, n + d)
+    qry = image_like(q, d, 10, 3 * n + q)
+    print(f"[gpu] knn large grid n={n} d={d} q={q}", flush=True)
+    idx = KnnIndex(ref, max_queries=q)
+    gi, gd = idx.query(qry, 15)
+    idx.close()
+    ei, ed = O.exact_knn(qry, ref, 15, threads=8)
+    assert (gi == ei).all()
+    assert (gd.view(np.uint32) == ed.view(np.uint32)).all()
+
+
 def test_fit_graph_exclude_self_and_ties():
     from cuscan_amd.knn import KnnIndex
 
