@@ -327,7 +327,7 @@ struct umaphip_plan {
                                        keys.ptr, keys_sorted.ptr, proj_order.ptr, ranks.ptr,
                                        table.ptr, stream, ranked ? offsets.ptr : nullptr,
                                        ranked ? rank_seg.ptr : nullptr,
-                                       ranked ? rank_deg.ptr : nullptr);
+                                       ranked ? rank_deg.ptr : nullptr, error_flag.ptr);
             return;
         }
         launch_project_keys(reference, n, d, dir, keys.ptr, nullptr, status.ptr, tag, stream);
@@ -740,8 +740,12 @@ umaphip_status umaphip_plan_create(int64_t h, int64_t n, int64_t d, const float 
             P->keys.alloc(n);
             P->keys_sorted.alloc(n);
             P->proj_order.alloc(n);
-            P->sort_ws.alloc(sort_pairs_workspace(n));
-            clear_sort_workspace(P->sort_ws.ptr, P->sort_ws.count, n, P->stream);
+            if (P->shared && d == 2) {
+                P->sort_ws.alloc(project_sort_workspace(n));
+                clear_project_sort_workspace(P->sort_ws.ptr, P->sort_ws.count, n, P->stream);
+            } else {
+                P->sort_ws.alloc(sort_pairs_workspace(n));
+            }
         }
         if (!P->hard) {
             P->host_orders[0].alloc(n);
@@ -1046,6 +1050,7 @@ umaphip_status umaphip_plan_download(umaphip_plan *plan, float *head, double *ne
                         "non-finite %s projection at row %llu (refresh %llu)",
                         plan->shared ? "reference" : "reference or query",
                         status & 0xFFFFFFFFull, status >> 32);
+        UMAPHIP_REQUIRE((err & 2) == 0, UMAPHIP_BACKEND, "projection sort: a pass stalled");
         UMAPHIP_REQUIRE(err == 0, UMAPHIP_NUMERICAL, "invalid negative force ratio");
     });
 }

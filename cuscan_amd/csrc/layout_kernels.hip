@@ -359,14 +359,53 @@ __device__ inline float unit_coef(float q, bool att, float a, float b, float k_a
 // x mod m through a double-precision quotient estimate. Precondition:
 // 0 < m < 2^31 and x < m * 2^31 (every caller passes x < m^2; m is the
 // negative range, at most N <= INT32_MAX). Then x / m < 2^31, the estimate
-// is within one of the true quotient, x - q m fits an int64 and at most two
-// correction steps run.
+// x * RN(1/m) is within 2^-20 of x / m, so its truncation q is the true
+// quotient or one off, x - q m lies in [-m, 2m) and one correction step each
+// way finishes. The steps are written without loops: a loop here lets the
+// compiler replace its trip count by a 64-bit division.
 __device__ inline uint64_t mod_u64(uint64_t x, uint64_t m, double inv_m) {
-    const uint64_t q = static_cast<uint64_t>(static_cast<double>(x) * inv_m);
+    const uint64_t q = static_cast<uint32_t>(static_cast<double>(x) * inv_m);
     int64_t r = static_cast<int64_t>(x - q * m);
-    while (r < 0) r += static_cast<int64_t>(m);
-    while (r >= static_cast<int64_t>(m)) r -= static_cast<int64_t>(m);
+    r += r < 0 ? static_cast<int64_t>(m) : 0;
+    r -= r >= static_cast<int64_t>(m) ? static_cast<int64_t>(m) : 0;
     return static_cast<uint64_t>(r);
+}
+
+// x mod m for an integer-valued double 0 <= x < 2^53 with x / m < 2^40 and
+// m >= 1 an integer: trunc(x * RN(1/m)) is floor(x / m) or one off, the
+// remainder x - q m is an integer in [-m, 2m) that the fma yields exactly,
+// and one correction each way finishes.
+__device__ inline uint32_t mod_f64(double x, double m, double inv_m) {
+    const double q = trunc(x * inv_m);
+    double r = fma(-q, m, x);
+    r = r < 0.0 ? r + m : r;
+    r = r >= m ? r - m : r;
+    return static_cast<uint32_t>(r);
+}
+
+// Largest f for which e_mod * f (e_mod < m) stays below 2^53, less a margin
+// of two for the rounding of inv_m: products up to it are exact in f64.
+__device__ inline uint64_t exact_product_limit(double inv_m) {
+    const double lim = 9007199254740992.0 * inv_m;
+    return lim > 4.0 ? static_cast<uint64_t>(lim) - 2u : 0u;
+}
+
+// Candidate offset of the k-th (k >= 1) negative an edge draws at epoch t,
+// e (t + k) mod M (modern_layout.rs:433-514), from em = e mod M and
+// t1 = (t + 1) mod M. With f = t1 + k - 1 the value is em * f mod M; when
+// that product is exact in f64 (f <= limit) one f64 reduction gives it.
+// Otherwise e (t + 1) mod M goes through 64-bit integers and the remaining
+// k - 1 steps of em are added in f64 (exact for k < 2^22).
+__device__ inline uint32_t negative_offset(uint32_t em, uint32_t t1, int k, uint64_t modulus,
+                                           double m_d, double inv_m, uint64_t limit) {
+    const uint64_t f = static_cast<uint64_t>(t1) + static_cast<uint64_t>(k - 1);
+    if (f <= limit)
+        return mod_f64(static_cast<double>(em) * static_cast<double>(static_cast<uint32_t>(f)),
+                       m_d, inv_m);
+    const uint64_t s0 = mod_u64(static_cast<uint64_t>(em) * t1, modulus, inv_m);
+    const double x = static_cast<double>(static_cast<uint32_t>(s0)) +
+                     static_cast<double>(k - 1) * static_cast<double>(em);
+    return mod_f64(x, m_d, inv_m);
 }
 
 template <int WAVES>
@@ -416,7 +455,9 @@ __device__ inline void sweep_tile(const SweepArgs &A, int64_t tile) {
     const uint64_t modulus = small ? n64 : A.range;
     const bool windowed = !small && static_cast<uint64_t>(A.range) < n64;
     const bool retry = small && A.exclude;
-    const double inv_mod = 1.0 / static_cast<double>(modulus);
+    const double mod_d = static_cast<double>(static_cast<uint32_t>(modulus));
+    const double inv_mod = 1.0 / mod_d;
+    const uint64_t prod_limit = exact_product_limit(inv_mod);
     const uint64_t t1 = (static_cast<uint64_t>(A.epoch) + 1u) % modulus;
     int len = 0;
     if (tid < rows) {
@@ -569,10 +610,9 @@ __device__ inline void sweep_tile(const SweepArgs &A, int64_t tile) {
                     } else if (retry) {
                         live[u] = false;  // combined retry term already stored
                     } else {
-                        uint64_t f = t1 + static_cast<uint64_t>(k - 1);
-                        if (f >= modulus) f = f % modulus;
                         const uint64_t sel =
-                            mod_u64(static_cast<uint64_t>(s_em[owner]) * f, modulus, inv_mod);
+                            negative_offset(s_em[owner], static_cast<uint32_t>(t1), k, modulus,
+                                            mod_d, inv_mod, prod_limit);
                         const uint64_t rank = static_cast<uint64_t>(s_first[rr]) + sel;
                         if (A.exclude) {
                             const int64_t jj = static_cast<uint32_t>(A.order[rank]);
@@ -930,6 +970,9 @@ __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
     const uint64_t n64 = static_cast<uint64_t>(A.n);
     const uint64_t modulus = A.range;
     const bool windowed = modulus < n64;
+    const double mod_d = static_cast<double>(static_cast<uint32_t>(modulus));
+    const uint64_t prod_limit = exact_product_limit(U.inv_mod);
+    const uint32_t t1 = static_cast<uint32_t>(U.t1);
     // 1. unit rows and firing shadow
     // first and last row (ranked: rank) of the unit
     const int32_t k0 = U.unit_k[2 * u], k1 = U.unit_k[2 * u + 1];
@@ -1054,12 +1097,10 @@ __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
     // clocks (bit-exact f64, reference order of operations) and term counts
     int running = 0;
     int t_cnt[EPL], t_base[EPL];
-    uint32_t sel0r[EPL];   // e (epoch + 1) mod M of the lane's own edge slots
 #pragma unroll
     for (int k = 0; k < EPL; ++k) {
         const int slot = k * 64 + lane;
         int t = 0, r = 0;
-        sel0r[k] = 0u;
         if (slot < ue) {
             if constexpr (RANKED) {
                 r = s_row[slot];
@@ -1100,14 +1141,12 @@ __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
                     if (negs > U.tr_cap) atomicOr(U.tr_overflow, 1);
                 }
                 t = 1 + negs;
+                // e < 2^31: exact in f64
                 const uint32_t em =
-                    COMPACT ? static_cast<uint32_t>(mod_u64(static_cast<uint64_t>(e), modulus, U.inv_mod))
+                    COMPACT ? mod_f64(static_cast<double>(static_cast<uint32_t>(e)), mod_d, U.inv_mod)
                             : static_cast<uint32_t>(r_w[k] >> 32);
                 s_j[slot] = static_cast<int32_t>(static_cast<uint32_t>(r_w[k]));
                 s_em[slot] = em;
-                sel0r[k] = negs ? static_cast<uint32_t>(mod_u64(static_cast<uint64_t>(em) * U.t1,
-                                                                modulus, U.inv_mod))
-                                : 0u;
             }
         }
         s_row[slot] = static_cast<uint8_t>(r);
@@ -1152,15 +1191,8 @@ __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
                     y[g] = ld2(A.tgt, static_cast<uint32_t>(s_j[owner]));
                     continue;
                 }
-                // e*(epoch+kq) mod M from e*(epoch+1) mod M by adding e mod M
-                const uint64_t em_o = s_em[owner];
-                uint64_t sel = mod_u64(em_o * U.t1, modulus, U.inv_mod) +
-                               static_cast<uint64_t>(kq - 1) * em_o;
-                if (kq <= 4) {
-                    while (sel >= modulus) sel -= modulus;
-                } else {
-                    sel = mod_u64(sel, modulus, U.inv_mod);
-                }
+                const uint64_t sel =
+                    negative_offset(s_em[owner], t1, kq, modulus, mod_d, U.inv_mod, prod_limit);
                 const int rr = s_row[owner];
                 const uint64_t rank = row_first(rr) + sel;
                 if (A.exclude) {
@@ -1220,8 +1252,10 @@ __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
                     const float c = attract_fast(q, A.a, A.b, 1.0f);
                     g = make_float2(c * dx, c * dy);
                 }
-                uint64_t sel = sel0r[k];
                 const uint64_t em = s_em[slot];
+                uint64_t sel = t_cnt[k] > 1 ? negative_offset(static_cast<uint32_t>(em), t1, 1,
+                                                              modulus, mod_d, U.inv_mod, prod_limit)
+                                            : 0u;
                 for (int p = 0; p + 1 < t_cnt[k]; ++p) {
                     if (p > 0) {
                         sel += em;
@@ -1801,6 +1835,282 @@ __global__ __launch_bounds__(kSortThreads) void project_keys_hist_kernel(
         if (t4[d][tid]) atomicAdd(&totals[d * 256 + tid], t4[d][tid]);
 }
 
+// ------------------------------------------- single-launch radix passes
+// The projection sort of rank-ordered fits: the same stable LSD radix sort
+// (four 8-bit passes over (key, row) pairs), with one launch per pass and no
+// histogram or scan launches. A pass kernel takes its tile id from a counter
+// in the order its workgroups start, ranks the tile's keys among equal digits
+// with ballots (as radix_scatter_kernel) and publishes the tile's digit
+// counts with a ready bit. Its output offsets add the counts of the earlier
+// tiles of its group of kOsGroup tiles and the totals of the earlier groups,
+// which the last tile of each group publishes once it has its group's
+// counts. Every tile and group total waited on belongs to a workgroup that
+// started earlier and waits only on tiles before it, so each wait ends (a
+// spin limit reports a stall instead of waiting forever).
+// The key-byte totals of all four passes are counted by the projection
+// kernel, which also clears the published counts and the tile counters; the
+// last workgroup to finish the last pass clears the totals.
+constexpr int kOsThreads = 512;
+constexpr int kOsItems = 8;
+constexpr int kOsTile = kOsThreads * kOsItems;
+constexpr uint32_t kOsReady = 0x80000000u;
+constexpr int kOsSpinLimit = 1 << 22;
+constexpr int kOsGroup = 16;   // tiles per group (the group totals make the second level)
+
+__device__ inline uint32_t os_load(const uint32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Exclusive scans over the 256 digits of two vectors by threads 0..255 (every
+// thread of the workgroup must call it); s_tmp holds 8 words.
+__device__ inline void os_scan_digits(uint32_t a, uint32_t b, uint32_t &ea, uint32_t &eb,
+                                      uint32_t *s_tmp) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    uint32_t ia = a, ib = b;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t ua = __shfl_up(ia, off, 64), ub = __shfl_up(ib, off, 64);
+        if (lane >= off) {
+            ia += ua;
+            ib += ub;
+        }
+    }
+    if (tid < 256 && lane == 63) {
+        s_tmp[wave] = ia;
+        s_tmp[4 + wave] = ib;
+    }
+    __syncthreads();
+    uint32_t ba = 0, bb = 0;
+    for (int w = 0; w < 4; ++w)
+        if (w < wave) {
+            ba += s_tmp[w];
+            bb += s_tmp[4 + w];
+        }
+    ea = ba + ia - a;
+    eb = bb + ib - b;
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kOsThreads) void radix_onesweep_kernel(
+    const uint32_t *keys_in, const int32_t *vals_in, uint32_t *keys_out, int32_t *vals_out,
+    int64_t n, int shift, int64_t tiles, const uint32_t *totals, uint32_t *counts,
+    uint32_t *group_totals, uint32_t *tile_ctr, uint32_t *done_ctr, uint32_t *clear_totals,
+    int32_t *err,
+    int32_t *ranks_out, const float *table_src, float *table_out, const int32_t *csr_off,
+    int32_t *seg_out, int32_t *deg_out) {
+    constexpr int WAVES = kOsThreads / 64;
+    constexpr int PER_WAVE = kOsTile / WAVES;
+    __shared__ uint32_t s_off[256];            // output position of the tile's run of each digit
+    __shared__ uint32_t s_start[256];          // start of each digit in the tile sorted by digit
+    __shared__ uint32_t s_wcnt[WAVES][256];    // per-wave digit counts, then their wave prefix
+    __shared__ uint32_t s_part[2][WAVES][256]; // per-wave sums: earlier tiles of the group, earlier groups
+    __shared__ uint32_t s_tmp[8];
+    __shared__ uint32_t s_key[kOsTile];
+    __shared__ int32_t s_val[kOsTile];
+    __shared__ uint32_t s_tile, s_last;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) s_tile = atomicAdd(tile_ctr, 1u);
+    for (int i = tid; i < WAVES * 256; i += kOsThreads) (&s_wcnt[0][0])[i] = 0u;
+    __syncthreads();
+    const int64_t tile = s_tile;
+    const int64_t base = tile * kOsTile;
+    const int cnt = static_cast<int>(min<int64_t>(kOsTile, n - base));
+    uint32_t k[kOsItems];
+    int32_t v[kOsItems];
+#pragma unroll
+    for (int i = 0; i < kOsItems; ++i) {
+        const int local = wave * PER_WAVE + i * 64 + lane;
+        k[i] = 0u;
+        v[i] = 0;
+        if (local < cnt) {
+            k[i] = keys_in[base + local];
+            v[i] = vals_in ? vals_in[base + local] : static_cast<int32_t>(base + local);
+        }
+    }
+    // rank of each key among the equal digits of its wave before it
+    const uint64_t below_lane = (1ull << lane) - 1ull;
+    uint32_t lrank[kOsItems];
+#pragma unroll
+    for (int i = 0; i < kOsItems; ++i) {
+        const bool valid = wave * PER_WAVE + i * 64 + lane < cnt;
+        const uint32_t digit = valid ? ((k[i] >> shift) & 255u) : 256u;
+        uint64_t same = ~0ull;
+#pragma unroll
+        for (int bit = 0; bit < 9; ++bit) {
+            const bool set = (digit >> bit) & 1u;
+            const uint64_t ballot = __ballot(set);
+            same &= set ? ballot : ~ballot;
+        }
+        const uint32_t r = static_cast<uint32_t>(__popcll(same & below_lane));
+        const uint32_t before = valid ? s_wcnt[wave][digit] : 0u;
+        lrank[i] = before + r;
+        if (valid && r == 0u) s_wcnt[wave][digit] = before + static_cast<uint32_t>(__popcll(same));
+    }
+    __syncthreads();
+    // digit tid: wave prefix of its counts and the tile's count, published
+    uint32_t tot = 0u, gtot = 0u;
+    if (tid < 256) {
+        for (int w = 0; w < WAVES; ++w) {
+            const uint32_t c = s_wcnt[w][tid];
+            s_wcnt[w][tid] = tot;
+            tot += c;
+        }
+        __hip_atomic_store(counts + tile * 256 + tid, tot | kOsReady, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        gtot = totals[tid];
+    }
+    uint32_t start, gbase;
+    os_scan_digits(tot, gtot, start, gbase, s_tmp);
+    if (tid < 256) s_start[tid] = start;
+    // Rows to add: the earlier tiles of the group (counts) and the earlier
+    // group totals. Wave w adds rows w, w + WAVES, ... of a list; lane l
+    // holds digits 4l..4l+3. The group's last tile publishes the group total
+    // as soon as it has its group's counts, before it waits on earlier
+    // groups, so group totals never wait on each other.
+    const int64_t grp = tile / kOsGroup, pos_in_group = tile % kOsGroup;
+    const bool publishes = pos_in_group == kOsGroup - 1;
+    int spins = 0;
+    // adds rows [0, n_own) of the group's counts and [0, n_grp) of the group
+    // totals into s_part[0] / s_part[1] (per wave)
+    auto add_rows = [&](int64_t n_own, int64_t n_grp) {
+        uint32_t acc[2][4] = {{0u, 0u, 0u, 0u}, {0u, 0u, 0u, 0u}};
+        const int64_t n_rows = n_own + n_grp;
+        for (int64_t r0 = wave; r0 < n_rows; r0 += 4 * WAVES) {
+            uint32_t c[4][4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int64_t r = r0 + u * WAVES;
+                const uint32_t *row = r < n_own ? counts + (grp * kOsGroup + r) * 256
+                                                : group_totals + (r - n_own) * 256;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) c[u][j] = r < n_rows ? os_load(row + 4 * lane + j) : kOsReady;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int64_t r = r0 + u * WAVES;
+                const int side = r < n_own ? 0 : 1;
+                const uint32_t *row = r < n_own ? counts + (grp * kOsGroup + r) * 256
+                                                : group_totals + (r - n_own) * 256;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    while (!(c[u][j] & kOsReady)) {
+                        __builtin_amdgcn_s_sleep(1);
+                        if (++spins > kOsSpinLimit) {
+                            atomicOr(err, 2);
+                            c[u][j] = kOsReady;
+                            break;
+                        }
+                        c[u][j] = os_load(row + 4 * lane + j);
+                    }
+                    acc[side][j] += c[u][j] & ~kOsReady;
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            s_part[0][wave][4 * lane + j] += acc[0][j];
+            s_part[1][wave][4 * lane + j] += acc[1][j];
+        }
+    };
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        s_part[0][wave][4 * lane + j] = 0u;
+        s_part[1][wave][4 * lane + j] = 0u;
+    }
+    if (publishes) {
+        add_rows(pos_in_group, 0);
+        __syncthreads();
+        if (tid < 256) {
+            uint32_t in_group = tot;
+            for (int w = 0; w < WAVES; ++w) in_group += s_part[0][w][tid];
+            __hip_atomic_store(group_totals + grp * 256 + tid, in_group | kOsReady,
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        add_rows(0, grp);
+    } else {
+        add_rows(pos_in_group, grp);
+    }
+    __syncthreads();
+    if (tid < 256) {
+        uint32_t before = gbase;
+        for (int w = 0; w < WAVES; ++w) before += s_part[0][w][tid] + s_part[1][w][tid];
+        s_off[tid] = before;
+    }
+#pragma unroll
+    for (int i = 0; i < kOsItems; ++i) {
+        if (wave * PER_WAVE + i * 64 + lane < cnt) {
+            const uint32_t digit = (k[i] >> shift) & 255u;
+            const uint32_t lp = s_start[digit] + s_wcnt[wave][digit] + lrank[i];
+            s_key[lp] = k[i];
+            s_val[lp] = v[i];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kOsItems; ++i) {
+        const int idx = i * kOsThreads + tid;
+        if (idx < cnt) {
+            const uint32_t key = s_key[idx];
+            const int32_t val = s_val[idx];
+            const uint32_t digit = (key >> shift) & 255u;
+            const uint32_t pos = s_off[digit] + (static_cast<uint32_t>(idx) - s_start[digit]);
+            keys_out[pos] = key;
+            vals_out[pos] = val;
+            if (ranks_out) ranks_out[val] = static_cast<int32_t>(pos);
+            if (table_out)
+                reinterpret_cast<float2 *>(table_out)[pos] =
+                    reinterpret_cast<const float2 *>(table_src)[val];
+            if (seg_out) {
+                const int32_t b = csr_off[val];
+                seg_out[pos] = b;
+                deg_out[pos] = csr_off[val + 1] - b;
+            }
+        }
+    }
+    if (clear_totals) {
+        // every workgroup has read its totals before it counts itself done
+        __syncthreads();
+        if (tid == 0) s_last = atomicAdd(done_ctr, 1u) == static_cast<uint32_t>(tiles - 1);
+        __syncthreads();
+        if (s_last)
+            for (int t = tid; t < 1024; t += kOsThreads) clear_totals[t] = 0u;
+    }
+}
+
+// Keys of the d = 2 projection x . dir (the arithmetic of
+// project_keys_kernel) and the key-byte totals of all four passes
+// (totals must be zero on entry); also clears the published tile counts
+// (`clear_words` words) and the five pass counters of the single-launch sort.
+__global__ __launch_bounds__(kSortThreads) void project_keys_totals_kernel(
+    const float *x, int64_t rows, const float *dir, uint32_t *keys, unsigned long long *status,
+    unsigned long long tag, uint32_t *totals, uint32_t *clear, int64_t clear_words,
+    uint32_t *ctr) {
+    __shared__ uint32_t t4[4][256];
+    const int tid = threadIdx.x;
+    for (int d = 0; d < 4; ++d) t4[d][tid] = 0u;
+    const int64_t per = (clear_words + gridDim.x - 1) / gridDim.x;
+    const int64_t c0 = static_cast<int64_t>(blockIdx.x) * per;
+    for (int64_t i = c0 + tid; i < min(c0 + per, clear_words); i += kSortThreads) clear[i] = 0u;
+    if (blockIdx.x == 0 && tid < 5) ctr[tid] = 0u;
+    __syncthreads();
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kSortTile;
+    for (int i = 0; i < kSortItems; ++i) {
+        const int64_t p = base + i * kSortThreads + tid;
+        if (p < rows) {
+            const float2 xy = reinterpret_cast<const float2 *>(x)[p];
+            float sum = 0.0f;
+            sum += xy.x * dir[0];
+            sum += xy.y * dir[1];
+            const uint32_t k = radix_key(sum);
+            keys[p] = k;
+            if (!isfinite(sum)) atomicMin(status, tag | static_cast<unsigned long long>(p));
+            for (int d = 0; d < 4; ++d) atomicAdd(&t4[d][(k >> (8 * d)) & 255u], 1u);
+        }
+    }
+    __syncthreads();
+    for (int d = 0; d < 4; ++d)
+        if (t4[d][tid]) atomicAdd(&totals[d * 256 + tid], t4[d][tid]);
+}
+
 __global__ void keys_from_values_kernel(const float *values, int64_t n, uint32_t *keys,
                                         unsigned long long *status) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -1952,7 +2262,7 @@ __global__ __launch_bounds__(256) void ratio_update_kernel(const double *res, in
     __syncthreads();
     const int count = count_s;
     if (bad_s) {
-        if (tid == 0) *error_flag = 1;
+        if (tid == 0) atomicOr(error_flag, 1);
         return;
     }
     // Bitonic sort of logs[0, pow2) padded with +inf.
@@ -1995,7 +2305,7 @@ __global__ __launch_bounds__(256) void ratio_update_kernel(const double *res, in
         }
         ratio = exp(sum / static_cast<double>(count));
         if (!isfinite(ratio) || ratio <= 0.0) {
-            *error_flag = 1;
+            atomicOr(error_flag, 1);
             return;
         }
     }
@@ -2020,7 +2330,7 @@ __global__ __launch_bounds__(256) void ratio_update_kernel(const double *res, in
     } else {
         const double anchor = current / ratio;
         if (!isfinite(anchor) || anchor <= 0.0) {
-            *error_flag = 1;
+            atomicOr(error_flag, 1);
             return;
         }
         state[2] = anchor;
@@ -2385,18 +2695,87 @@ void clear_sort_workspace(void *ws, size_t ws_bytes, int64_t n, hipStream_t s) {
     HIP_CHECK(hipGetLastError());
 }
 
+int64_t onesweep_tiles(int64_t n) { return (n + kOsTile - 1) / kOsTile; }
+int64_t onesweep_groups(int64_t n) { return (onesweep_tiles(n) + kOsGroup - 1) / kOsGroup; }
+
+// per pass: tile counts [tiles][256], then group totals [groups][256]
+int64_t onesweep_count_words(int64_t n) {
+    return 4 * 256 * (onesweep_tiles(n) + onesweep_groups(n));
+}
+
+size_t project_sort_workspace(int64_t n) {
+    auto up = [](size_t b) { return (b + 255) & ~static_cast<size_t>(255); };
+    return up(static_cast<size_t>(n) * 4) * 2 +
+           up(static_cast<size_t>(onesweep_count_words(n)) * 4) + up(1024 * 4) + up(8 * 4);
+}
+
+namespace {
+struct OnesweepWorkspace {
+    uint32_t *tmp_k;
+    int32_t *tmp_v;
+    uint32_t *counts, *group_totals, *totals, *ctr;
+    int64_t tiles, groups;
+};
+OnesweepWorkspace onesweep_workspace(void *ws, size_t ws_bytes, int64_t n) {
+    // published counts carry a ready bit in bit 31
+    UMAPHIP_REQUIRE(n < (int64_t(1) << 31), UMAPHIP_OVERFLOW, "sort size must be below 2^31");
+    UMAPHIP_REQUIRE(ws_bytes >= project_sort_workspace(n), UMAPHIP_CAPACITY,
+                    "projection sort workspace too small");
+    auto up = [](size_t b) { return (b + 255) & ~static_cast<size_t>(255); };
+    uint8_t *w = static_cast<uint8_t *>(ws);
+    OnesweepWorkspace W;
+    W.tiles = onesweep_tiles(n);
+    W.groups = onesweep_groups(n);
+    const size_t kb = up(static_cast<size_t>(n) * 4);
+    const size_t cb = up(static_cast<size_t>(onesweep_count_words(n)) * 4);
+    W.tmp_k = reinterpret_cast<uint32_t *>(w);
+    W.tmp_v = reinterpret_cast<int32_t *>(w + kb);
+    W.counts = reinterpret_cast<uint32_t *>(w + 2 * kb);
+    W.group_totals = W.counts + 4 * 256 * W.tiles;
+    W.totals = reinterpret_cast<uint32_t *>(w + 2 * kb + cb);
+    W.ctr = reinterpret_cast<uint32_t *>(w + 2 * kb + cb + up(1024 * 4));
+    return W;
+}
+}  // namespace
+
+void clear_project_sort_workspace(void *ws, size_t ws_bytes, int64_t n, hipStream_t s) {
+    if (n == 0) return;
+    const OnesweepWorkspace W = onesweep_workspace(ws, ws_bytes, n);
+    hipLaunchKernelGGL(radix_zero_kernel, dim3(4), dim3(kSortThreads), 0, s, W.totals, int64_t(1024));
+    HIP_CHECK(hipGetLastError());
+}
+
 void launch_project_sort_ranked(void *ws, size_t ws_bytes, const float *x, int64_t n,
                                 const float *dir, unsigned long long *status,
                                 unsigned long long tag, uint32_t *keys, uint32_t *keys_out,
                                 int32_t *vals_out, int32_t *ranks, float *table_out, hipStream_t s,
-                                const int32_t *csr_off, int32_t *seg_out, int32_t *deg_out) {
+                                const int32_t *csr_off, int32_t *seg_out, int32_t *deg_out,
+                                int32_t *err) {
     if (n == 0) return;
-    const SortWorkspace W = sort_workspace(ws, ws_bytes, n);
-    hipLaunchKernelGGL(project_keys_hist_kernel, dim3(static_cast<unsigned>(W.tiles)),
-                       dim3(kSortThreads), 0, s, x, n, dir, keys, status, tag, W.tiles, W.hist,
-                       W.totals);
-    sort_passes(W, true, keys, keys_out, nullptr, vals_out, n, ranks, x, table_out, s, csr_off,
-                seg_out, deg_out);
+    UMAPHIP_REQUIRE(err != nullptr, UMAPHIP_INVALID_INPUT, "projection sort needs an error word");
+    const OnesweepWorkspace W = onesweep_workspace(ws, ws_bytes, n);
+    const int64_t count_words = onesweep_count_words(n);
+    hipLaunchKernelGGL(project_keys_totals_kernel, dim3(static_cast<unsigned>(sort_tiles(n))),
+                       dim3(kSortThreads), 0, s, x, n, dir, keys, status, tag, W.totals, W.counts,
+                       count_words, W.ctr);
+    const uint32_t *src_k = keys;
+    const int32_t *src_v = nullptr;
+    for (int pass = 0; pass < 4; ++pass) {
+        uint32_t *dst_k = (pass & 1) ? keys_out : W.tmp_k;
+        int32_t *dst_v = (pass & 1) ? vals_out : W.tmp_v;
+        const bool last = pass == 3;
+        hipLaunchKernelGGL(radix_onesweep_kernel, dim3(static_cast<unsigned>(W.tiles)),
+                           dim3(kOsThreads), 0, s, src_k, src_v, dst_k, dst_v, n, 8 * pass,
+                           W.tiles, W.totals + 256 * pass, W.counts + 256 * W.tiles * pass,
+                           W.group_totals + 256 * W.groups * pass, W.ctr + pass, W.ctr + 4,
+                           last ? W.totals : nullptr, err,
+                           last ? ranks : nullptr, last ? x : nullptr, last ? table_out : nullptr,
+                           last ? csr_off : nullptr, last ? seg_out : nullptr,
+                           last ? deg_out : nullptr);
+        src_k = dst_k;
+        src_v = dst_v;
+    }
+    HIP_CHECK(hipGetLastError());
 }
 
 void launch_sort_pairs(void *ws, size_t ws_bytes, const uint32_t *keys_in, uint32_t *keys_out,

@@ -207,20 +207,26 @@ void launch_sort_pairs_ranked(void *ws, size_t ws_bytes, const uint32_t *keys_in
                               uint32_t *keys_out, const int32_t *vals_in, int32_t *vals_out,
                               int64_t n, int32_t *ranks, const float *table_src, float *table_out,
                               hipStream_t s);
-// Clears the digit totals of a sort workspace; launch_project_sort_ranked
-// needs them clear on entry (every sort leaves them clear).
+// Clears the digit totals of a sort workspace; launch_sort_pairs_ranked
+// leaves them clear.
 void clear_sort_workspace(void *ws, size_t ws_bytes, int64_t n, hipStream_t s);
 // Fit refresh for d = 2: keys of x . dir (non-finite rows tracked in status as
-// launch_project_keys does), then the ranked sort of (key, row) with
-// ranks[row] = position and table_out[position] = x[row].
+// launch_project_keys does), then the stable sort of (key, row), one launch
+// per 8-bit pass, with ranks[row] = position and table_out[position] = x[row].
 // With csr_off (the CSR offsets of the n rows), the last pass also writes
 // seg_out[position] / deg_out[position], the CSR start and degree of the row.
+// A pass whose wait for earlier tiles exceeds its spin limit sets bit 2 of
+// *err. The workspace (project_sort_workspace bytes, n < 2^31) needs its
+// digit totals clear on entry (clear_project_sort_workspace; every sort
+// leaves them clear).
+size_t project_sort_workspace(int64_t n);
+void clear_project_sort_workspace(void *ws, size_t ws_bytes, int64_t n, hipStream_t s);
 void launch_project_sort_ranked(void *ws, size_t ws_bytes, const float *x, int64_t n,
                                 const float *dir, unsigned long long *status,
                                 unsigned long long tag, uint32_t *keys, uint32_t *keys_out,
                                 int32_t *vals_out, int32_t *ranks, float *table_out, hipStream_t s,
-                                const int32_t *csr_off = nullptr, int32_t *seg_out = nullptr,
-                                int32_t *deg_out = nullptr);
+                                const int32_t *csr_off, int32_t *seg_out, int32_t *deg_out,
+                                int32_t *err);
 void launch_values_from_keys(const uint32_t *keys, int64_t n, float *values, hipStream_t s);
 void launch_inverse_ranks(const int32_t *order, int64_t n, int32_t *ranks, hipStream_t s);
 void launch_lower_bound(const float *sorted, int64_t n, const float *query, int64_t h,

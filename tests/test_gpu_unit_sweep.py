@@ -123,6 +123,13 @@ def check_trace(g, params, epoch, tr):
         inv = np.empty(n, np.int64)
         inv[tr["order"]] = np.arange(n)
         assert (tr["ranks"] == inv).all(), "window ranks are not the inverse of the order"
+        # The order in force at epoch t is the stable argsort of the epoch's
+        # positions projected on direction t (refresh 0 precedes epoch 0).
+        rng = O.StdRng(params.seed)
+        for _ in range(epoch):
+            rng.direction(2)
+        want, _ = O.stable_argsort(O.project(tr["head"], rng.direction(2)))
+        assert (tr["order"] == want).all(), "projection order differs from the stable argsort"
 
     def epoch_args():
         return O.Epoch(head=tr["head"], tail=tr["head"], offsets=g["offsets"],
