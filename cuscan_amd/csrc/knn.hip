@@ -290,6 +290,23 @@ __device__ __forceinline__ void put_chunk(const u32x4 (&src)[LW], float rn, uint
 // (one workgroup per CU has no second workgroup to cover a chunk's load).
 // The LDS image keeps padded rows (SLOTS_ROW words); the fragments, MFMA
 // order and rn - 2 q.r are those of knn_scan_kernel.
+// Scan cells (query block, slice) in slice-major order: cell = slice *
+// query_blocks + query block. Workgroups are dealt to the 8 XCDs in turn;
+// the scan launches 8 * ceil(cells / 8) of them and workgroup b takes cell
+// (b mod 8) * ceil(cells / 8) + b / 8, so each XCD runs one contiguous run
+// of cells and the query blocks of a slice stream that slice's references
+// through the same L2 together. Returns false for the padding workgroups.
+__device__ inline bool scan_cell(int64_t nq, int slices, int64_t &q_block, int &slice) {
+    const int64_t qblocks = (nq + kTQ - 1) / kTQ;
+    const int64_t cells = qblocks * slices;
+    const int64_t chunk = (cells + 7) / 8;
+    const int64_t cell = (blockIdx.x % 8) * chunk + blockIdx.x / 8;
+    if (cell >= cells) return false;
+    slice = static_cast<int>(cell / qblocks);
+    q_block = (cell % qblocks) * kTQ;
+    return true;
+}
+
 template <int KSTEPS>
 __global__ __launch_bounds__(256, 1) void knn_scan_small_kernel(
     const uint16_t *__restrict__ qb, int64_t nq, const uint16_t *__restrict__ rb,
@@ -309,8 +326,9 @@ __global__ __launch_bounds__(256, 1) void knn_scan_small_kernel(
 
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
-    const int64_t q_block = static_cast<int64_t>(blockIdx.x) * kTQ;
-    const int slice = blockIdx.y;
+    int64_t q_block = 0;
+    int slice = 0;
+    if (!scan_cell(nq, slices, q_block, slice)) return;
     const int64_t r_begin = static_cast<int64_t>(slice) * per_slice;
     const int64_t r_end = min<int64_t>(nr, r_begin + per_slice);
 
@@ -530,8 +548,10 @@ __global__ __launch_bounds__(256, KSTEPS > 25 ? 1 : 2) void knn_scan_kernel(
 
     const int tid = threadIdx.x;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
-    const int q_block = static_cast<int>(blockIdx.x) * kTQ;
-    const int slice = blockIdx.y;
+    int64_t q_block64 = 0;
+    int slice = 0;
+    if (!scan_cell(nq, slices, q_block64, slice)) return;
+    const int q_block = static_cast<int>(q_block64);
     const int r_begin = static_cast<int>(slice * per_slice);
     const int r_end = static_cast<int>(min<int64_t>(nr, r_begin + per_slice));
     const int qnq = static_cast<int>(nq);
@@ -1225,10 +1245,13 @@ double sum_bound(int64_t n) {
 }
 
 template <int KS>
-void launch_scan(dim3 grid, hipStream_t s, const uint16_t *qb, int64_t nq, const uint16_t *rb,
+void launch_scan(dim3 cells, hipStream_t s, const uint16_t *qb, int64_t nq, const uint16_t *rb,
                  const float *rnorm, int64_t nr, int64_t per_slice, int slices, float *cd,
                  int32_t *ci, int32_t *cn, uint32_t *tk, int exclude_self, int64_t qoff) {
-    if (static_cast<int64_t>(grid.x) * grid.y <= 256)
+    // cells.x query blocks by cells.y slices, as 8 * ceil(cells / 8) workgroups (scan_cell)
+    const int64_t n_cells = static_cast<int64_t>(cells.x) * cells.y;
+    const dim3 grid(static_cast<unsigned>(8 * ((n_cells + 7) / 8)));
+    if (n_cells <= 256)
         hipLaunchKernelGGL((knn_scan_small_kernel<KS>), grid, dim3(256), 0, s, qb, nq, rb, rnorm, nr,
                            per_slice, slices, cd, ci, cn, tk, exclude_self, qoff);
     else

@@ -56,7 +56,7 @@ class TransformServer:
         if not hasattr(model, "embedding_"):
             raise ValueError("This UMAP instance is not fitted yet")
         epochs = int(model.n_epochs) // 3 if model.n_epochs is not None else 100
-        return cls(model._train, model.embedding_, a=model._a, b=model._b,
+        return cls(model._train, model._fit_embedding, a=model._a, b=model._b,
                    n_neighbors=model.n_neighbors, epochs=epochs, learning_rate=model.learning_rate,
                    repulsion=model.repulsion_strength, local_connectivity=model.local_connectivity,
                    set_op_mix_ratio=model.set_op_mix_ratio,

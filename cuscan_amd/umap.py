@@ -36,6 +36,23 @@ def _is_int(v) -> bool:
     return isinstance(v, (int, np.integer)) and not isinstance(v, (bool, np.bool_))
 
 
+def _distinct_rows(x):
+    """distinct_training_rows (crates/umap/src/standard_umap/estimator.rs:
+    38-77): the distinct rows in order of first occurrence, keyed on their
+    f32 bit patterns with +0 and -0 equal, each kept as its first occurrence,
+    and for every input row the index of its distinct row."""
+    bits = np.ascontiguousarray(x, np.float32).view(np.uint32).copy()
+    bits[x == 0.0] = 0
+    _, first, inverse = np.unique(bits, axis=0, return_index=True, return_inverse=True)
+    order = np.argsort(first, kind="stable")
+    rank = np.empty(len(first), np.int64)
+    rank[order] = np.arange(len(first))
+    rows = np.ascontiguousarray(x[first[order]])
+    if rows.shape[0] < 2:
+        raise ValueError("unique fitting requires at least two distinct rows")
+    return rows, rank[inverse.reshape(-1)]
+
+
 class UMAP:
     """Standard nonparametric UMAP estimator on a HIP device."""
 
@@ -187,8 +204,6 @@ class UMAP:
             raise NotImplementedError(
                 "init='spectral' (the block Krylov spectral solver) is not part of this build; "
                 "pass init='recursive', 'auto' or 'random'")
-        if self.unique:
-            raise NotImplementedError("unique=True is not part of this build")
 
     def _recursive_config(self) -> dict:
         """recursive_init_kwds checks of python/cuscan/standard_umap.py:146-182."""
@@ -257,14 +272,21 @@ class UMAP:
 
         x = self._input_array(X)
         self._validate()
+        features = x.shape[1]
+        # unique: fit the distinct rows; every input row takes its distinct
+        # row's coordinates (FittedUMAP::embedding, estimator.rs:1320-1330)
+        inverse = None
+        if self.unique:
+            x, inverse = _distinct_rows(x)
         self._device()
-        n, features = x.shape
+        n = x.shape[0]
         d = int(self.n_components)
         seed = DEFAULT_RANDOM_SEED if self.random_state is None else int(self.random_state)
         self.n_features_in_ = features
         self._train = x
+        self._inverse = inverse
         if n == 1:
-            self.embedding_ = np.zeros((1, d), np.float32)
+            self._set_embedding(np.zeros((1, d), np.float32))
             self._index, self._a, self._b = None, 0.0, 0.0
             return self
         self_inclusive = min(int(self.n_neighbors), n - 1)
@@ -302,9 +324,17 @@ class UMAP:
             params = self._layout_params(n_epochs, a, b, self.learning_rate, seed, good=good)
             emb, _, _, self.layout_stats_ = L.optimize_compact(init, None, compact, params,
                                                                shared=True)
-        self.embedding_ = emb
+        self._set_embedding(emb)
         self._index, self._a, self._b = index, a, b
         return self
+
+    def _set_embedding(self, emb):
+        """The fitted rows' coordinates (the transform's reference
+        embedding) and embedding_, expanded to every input row under
+        unique=True."""
+        self._fit_embedding = emb
+        self._embedding_dev = None
+        self.embedding_ = emb if self._inverse is None else emb[self._inverse]
 
     def _layout_params(self, epochs, a, b, alpha, seed, good=False):
         from .layout import LayoutParams
@@ -340,11 +370,11 @@ class UMAP:
         from . import layout as L
 
         q = x.shape[0]
-        n = self.embedding_.shape[0]
+        n = self._fit_embedding.shape[0]
         if q == 0:
-            return np.zeros((0, self.embedding_.shape[1]), np.float32)
+            return np.zeros((0, self._fit_embedding.shape[1]), np.float32)
         if self._index is None:
-            return np.zeros((q, self.embedding_.shape[1]), np.float32)
+            return np.zeros((q, self._fit_embedding.shape[1]), np.float32)
         k = min(int(self.n_neighbors), n)
         idx, dist = self._index.query(x, k)
         epochs = int(self.n_epochs) // 3 if self.n_epochs is not None else (100 if q <= 10_000 else 30)
@@ -359,21 +389,21 @@ class UMAP:
             if epochs == 0:
                 return compact["init"]
             params = self._layout_params(epochs, self._a, self._b, self.learning_rate / 4.0, tseed)
-            emb, _, _, _ = L.optimize_compact(compact["init"], self.embedding_, compact, params,
+            emb, _, _, _ = L.optimize_compact(compact["init"], self._fit_embedding, compact, params,
                                               shared=False)
             return emb
         g = G.fuzzy_graph(idx, dist, local_connectivity=lc, bandwidth=1.0,
                           mix=self.set_op_mix_ratio, bipartite=True, n_reference=n)
         if len(g["weights"]) == 0 or not (g["weights"] > 0).any():
             raise ValueError("transform fuzzy graph is empty after positive-edge pruning")
-        init = G.transform_init(idx, g["directed"], self.embedding_)
+        init = G.transform_init(idx, g["directed"], self._fit_embedding)
         if epochs == 0:
             return init
         from .classic import optimize_classic
 
         # compatibility transform: prepare_schedule(.., n_epochs, n_epochs, ..)
         # (estimator.rs:2158-2166), learning rate / 4, transform_seed
-        return optimize_classic(init, self.embedding_, g["heads"], g["tails"], g["weights"],
+        return optimize_classic(init, self._fit_embedding, g["heads"], g["tails"], g["weights"],
                                 n_epochs=epochs, default_epochs=epochs, a=self._a, b=self._b,
                                 learning_rate=self.learning_rate / 4.0,
                                 gamma=self.repulsion_strength,
@@ -385,8 +415,8 @@ class UMAP:
         import torch
 
         cached = getattr(self, "_embedding_dev", None)
-        if cached is None or cached.shape != self.embedding_.shape:
-            cached = torch.as_tensor(np.ascontiguousarray(self.embedding_, np.float32),
+        if cached is None:
+            cached = torch.as_tensor(np.ascontiguousarray(self._fit_embedding, np.float32),
                                      device=torch.device("cuda:0"))
             self._embedding_dev = cached
         return cached

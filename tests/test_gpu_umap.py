@@ -75,3 +75,33 @@ def test_seeded_fit_is_reproducible():
     a = UMAP(init="random", n_epochs=60, random_state=5).fit_transform(x)
     b = UMAP(init="random", n_epochs=60, random_state=5).fit_transform(x)
     assert (a == b).all()
+
+
+@pytest.mark.parametrize("optimizer", ["adam", "momentum"])
+def test_unique_fits_distinct_rows(optimizer):
+    """modern_unique_windows_use_distinct_rows (estimator.rs:3647-3672):
+    duplicates share their distinct row's coordinates, which are those of a
+    fit of the distinct rows alone; transform runs against the distinct
+    rows."""
+    from cuscan_amd import UMAP
+
+    base = blobs(60, 6, 4, 11)
+    rng = np.random.default_rng(5)
+    dup = base[rng.integers(0, len(base), 120)]
+    x = np.concatenate([base, dup])[rng.permutation(len(base) + len(dup))]
+    kw = dict(optimizer=optimizer, init="random", n_epochs=60, random_state=9,
+              negative_selection_range=50)
+    model = UMAP(unique=True, **kw)
+    emb = model.fit_transform(x)
+    assert emb.shape == (len(x), 2)
+    from cuscan_amd.umap import _distinct_rows
+
+    rows, inverse = _distinct_rows(x)
+    alone = UMAP(**kw).fit_transform(rows)
+    assert (emb == alone[inverse]).all()
+    xq = base[::9] + np.float32(0.05)
+    assert (model.transform(xq) == UMAP(**kw).fit(rows).transform(xq)).all()
+    tiny = np.array([[0.0], [1.0], [2.0], [0.0], [1.0]], np.float32)
+    e = UMAP(unique=True, optimizer=optimizer, n_neighbors=2, n_epochs=4, init="random",
+             negative_selection_range=1).fit_transform(tiny)
+    assert (e[0] == e[3]).all() and (e[1] == e[4]).all()

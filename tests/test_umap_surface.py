@@ -76,3 +76,34 @@ def test_auto_init_resolves_to_recursive():
     u._validate()
     assert u._recursive == {"base_init_threshold": 8, "n_embedding_epochs": 50,
                             "hub_selection": "diverse", "anchor_sample_size": 16384}
+
+
+def test_distinct_rows_first_occurrence_and_signed_zero():
+    """distinct_training_rows (crates/umap/src/standard_umap/estimator.rs:
+    38-77): first-occurrence order, +0 and -0 one key (the first occurrence
+    kept), every row mapped to its distinct row."""
+    from cuscan_amd.umap import _distinct_rows
+
+    x = np.array([[1.0, -0.0], [2.0, 0.0], [1.0, 0.0], [3.0, 5.0], [2.0, -0.0], [3.0, 5.0]],
+                 np.float32)
+    rows, inverse = _distinct_rows(x)
+    assert rows.tolist() == [[1.0, 0.0], [2.0, 0.0], [3.0, 5.0]]
+    assert np.signbit(rows[0, 1]) and not np.signbit(rows[1, 1])
+    assert inverse.tolist() == [0, 1, 0, 2, 1, 2]
+    assert (rows[inverse] == x).all()
+    # a plain brute-force restatement on random rows with many repeats
+    rng = np.random.default_rng(3)
+    y = rng.integers(0, 3, (500, 3)).astype(np.float32)
+    y[rng.random(y.shape) < 0.2] *= -1.0
+    rows, inverse = _distinct_rows(y)
+    seen = {}
+    for i, r in enumerate(y):
+        key = tuple(0 if v == 0 else float(v) for v in r)
+        seen.setdefault(key, len(seen))
+        assert inverse[i] == seen[key]
+    assert len(rows) == len(seen)
+
+
+def test_unique_needs_two_distinct_rows():
+    with pytest.raises(ValueError, match="two distinct rows"):
+        UMAP(unique=True, init="random").fit(np.ones((5, 3), np.float32))
