@@ -67,7 +67,7 @@ constexpr int kTop = 64;      // candidates kept per query
 constexpr int kMaxSlices = 256;   // slices per query batch (KnnIndex::slices_for)
 constexpr int kSample = 8192;      // references in the sample bound
 constexpr int kSampleRefs = 128;   // sample references per workgroup of the sample distances
-constexpr int kSampleMinSlices = 32;   // batches with at least this many slices use the bound
+constexpr int kSampleMinSlices = 8;   // batches with at least this many slices use the bound
 constexpr int64_t kSampleMinRefs = 8 * kSample;
 constexpr int64_t kWideMergeMaxQ = 8192;   // batches up to this many rows merge with 4 waves
 constexpr int kMomentRows = 2048;          // rows per workgroup of the column moments

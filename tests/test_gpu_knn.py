@@ -104,19 +104,23 @@ def test_continuous_data_recall():
     assert (gi == ei).all() and (gd.view(np.uint32) == ed.view(np.uint32)).all()
 
 
-@pytest.mark.parametrize("q", [1, 64, 700])
+@pytest.mark.parametrize("q", [1, 64, 700, 4096])
 def test_sample_bound_bit_exact(q):
-    """150,000 references: batches of up to 1,792 rows take at least 32
+    """150,000 references: batches of up to 8,192 rows take at least 8
     slices, so their thresholds start from the 8,192-row sample bound.
     Results stay bit-identical to the exact kernel, and the merged prefilter
-    lists equal those of a 2,200-row batch (29 slices, no bound): the bound
+    lists equal those of a 9,472-row batch (7 slices, no bound): the bound
     drops no candidate of the top 64."""
     from cuscan_amd.knn import KnnIndex
 
     ref = image_like(150_000, 64, 10, 11)
-    qry = image_like(2200, 64, 10, 12)
+    qry = image_like(9472, 64, 10, 12)
     print(f"[gpu] knn sample bound q={q}", flush=True)
-    idx = KnnIndex(ref, max_queries=2200)
+    idx = KnnIndex(ref, max_queries=9472)
+    # slices per batch (KnnIndex::slices_for): ceil(512 / query blocks),
+    # at most references / 4096; the bound needs at least 8 slices
+    slices = lambda rows: min(-(-512 // -(-rows // 128)), 150_000 // 4096)
+    assert slices(q) >= 8 and slices(9472) == 7
     gi, gd = idx.query(qry[:q], 15)
     ei, ed = O.exact_knn(qry[:q], ref, 15, threads=8)
     assert (gi == ei).all() and (gd.view(np.uint32) == ed.view(np.uint32)).all()
