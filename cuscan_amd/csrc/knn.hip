@@ -66,7 +66,7 @@ constexpr int kCap = 128;     // candidate buffer per (query, slice)
 constexpr int kTop = 64;      // candidates kept per query
 constexpr int kMaxSlices = 256;   // slices per query batch (KnnIndex::slices_for)
 constexpr int kSample = 8192;      // references in the sample bound
-constexpr int kSampleRefs = 128;   // sample references per workgroup of the sample distances
+constexpr int kSampleRefs = 128;   // fewest sample references per workgroup of the sample distances
 constexpr int kSampleMinSlices = 8;   // batches with at least this many slices use the bound
 constexpr int64_t kSampleMinRefs = 8 * kSample;
 constexpr int64_t kWideMergeMaxQ = 8192;   // batches up to this many rows merge with 4 waves
@@ -720,7 +720,7 @@ __global__ __launch_bounds__(256, KSTEPS > 25 ? 1 : 2) void knn_scan_kernel(
 template <int KSTEPS>
 __global__ __launch_bounds__(256) void knn_sample_dist_kernel(
     const uint16_t *__restrict__ qb, int64_t nq, const uint16_t *__restrict__ rb,
-    const float *__restrict__ rnorm, int64_t nr, int m, float *__restrict__ dist,
+    const float *__restrict__ rnorm, int64_t nr, int m, int refs_per_wg, float *__restrict__ dist,
     int exclude_self, int64_t query_offset) {
     constexpr int DP = KSTEPS * 32;
     const int tid = threadIdx.x;
@@ -740,8 +740,8 @@ __global__ __launch_bounds__(256) void knn_sample_dist_kernel(
             }
         }
     }
-    for (int c = 0; c < kSampleRefs / kTR; ++c) {
-        const int i = blockIdx.y * kSampleRefs + c * kTR + (lane & 15);   // sample entry
+    for (int c = 0; c < refs_per_wg / kTR; ++c) {
+        const int i = blockIdx.y * refs_per_wg + c * kTR + (lane & 15);   // sample entry
         const int64_t ref = static_cast<int64_t>(i) * nr / m;
         floatx4 acc[2] = {floatx4{0.0f, 0.0f, 0.0f, 0.0f}, floatx4{0.0f, 0.0f, 0.0f, 0.0f}};
 #pragma unroll
@@ -1263,8 +1263,9 @@ template <int KS>
 void launch_sample_dist(dim3 grid, hipStream_t s, const uint16_t *qb, int64_t nq, const uint16_t *rb,
                         const float *rnorm, int64_t nr, int m, float *dist, int exclude_self,
                         int64_t qoff) {
+    // grid.y sample slices of m / grid.y references each
     hipLaunchKernelGGL(knn_sample_dist_kernel<KS>, grid, dim3(256), 0, s, qb, nq, rb, rnorm, nr, m,
-                       dist, exclude_self, qoff);
+                       static_cast<int>(m / grid.y), dist, exclude_self, qoff);
 }
 
 int ksteps_for(int64_t d) {
@@ -1381,8 +1382,13 @@ void KnnIndex::query(const float *q_dev, int64_t nq, int k, bool exclude_self, i
     const dim3 grid(static_cast<unsigned>((nq + kTQ - 1) / kTQ), static_cast<unsigned>(slices));
     const int ex = exclude_self ? 1 : 0;
     if (uses_sample(nq)) {
-        // every query's global threshold starts at its sample bound
-        const dim3 sgrid(static_cast<unsigned>((nq + kTQ - 1) / kTQ), kSample / kSampleRefs);
+        // every query's global threshold starts at its sample bound; about
+        // 512 workgroups, each loading its queries' fragments once for
+        // kSample / (sample slices) references
+        const int64_t qblocks = (nq + kTQ - 1) / kTQ;
+        int sample_slices = kSample / kSampleRefs;
+        while (sample_slices > 1 && qblocks * sample_slices > 512) sample_slices /= 2;
+        const dim3 sgrid(static_cast<unsigned>(qblocks), static_cast<unsigned>(sample_slices));
         switch (ks_) {
             case 1: launch_sample_dist<1>(sgrid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, kSample, sample_d_.ptr, ex, query_offset); break;
             case 2: launch_sample_dist<2>(sgrid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, kSample, sample_d_.ptr, ex, query_offset); break;
