@@ -1668,8 +1668,7 @@ __global__ __launch_bounds__(kSortThreads) void radix_scan_kernel(uint32_t *hist
 __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     const uint32_t *keys_in, const int32_t *vals_in, uint32_t *keys_out, int32_t *vals_out,
     int64_t n, int shift, int64_t tiles, const uint32_t *offsets, int32_t *ranks_out,
-    const float *table_src, float *table_out, uint32_t *clear_totals, const int32_t *csr_off,
-    int32_t *seg_out, int32_t *deg_out) {
+    const float *table_src, float *table_out, uint32_t *clear_totals) {
     constexpr int WAVES = kSortThreads / 64;
     constexpr int PER_WAVE = kSortTile / WAVES;
     __shared__ uint32_t s_off[256];             // output position of the tile's run of each digit
@@ -1752,12 +1751,6 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
             if (table_out)
                 reinterpret_cast<float2 *>(table_out)[pos] =
                     reinterpret_cast<const float2 *>(table_src)[val];
-            if (seg_out) {
-                // CSR start and degree of the row at this rank
-                const int32_t b = csr_off[val];
-                seg_out[pos] = b;
-                deg_out[pos] = csr_off[val + 1] - b;
-            }
         }
     }
 }
@@ -1799,40 +1792,6 @@ __global__ void project_keys_kernel(const float *x, int64_t rows, int64_t d, con
     if (proj) proj[i] = sum;
     if (keys) keys[i] = radix_key(sum);
     if (!isfinite(sum)) atomicMin(status, tag | static_cast<unsigned long long>(i));
-}
-
-// Keys of the d = 2 projection x . dir for one sort tile (2048 rows): the
-// arithmetic of project_keys_kernel, plus the tile's histogram of the lowest
-// key byte and the key-byte totals of all four sort passes (totals must be
-// zero on entry).
-__global__ __launch_bounds__(kSortThreads) void project_keys_hist_kernel(
-    const float *x, int64_t rows, const float *dir, uint32_t *keys, unsigned long long *status,
-    unsigned long long tag, int64_t tiles, uint32_t *hist, uint32_t *totals) {
-    __shared__ uint32_t h[256];
-    __shared__ uint32_t t4[4][256];
-    const int tid = threadIdx.x;
-    h[tid] = 0u;
-    for (int d = 0; d < 4; ++d) t4[d][tid] = 0u;
-    __syncthreads();
-    const int64_t base = static_cast<int64_t>(blockIdx.x) * kSortTile;
-    for (int i = 0; i < kSortItems; ++i) {
-        const int64_t p = base + i * kSortThreads + tid;
-        if (p < rows) {
-            const float2 xy = reinterpret_cast<const float2 *>(x)[p];
-            float sum = 0.0f;
-            sum += xy.x * dir[0];
-            sum += xy.y * dir[1];
-            const uint32_t k = radix_key(sum);
-            keys[p] = k;
-            if (!isfinite(sum)) atomicMin(status, tag | static_cast<unsigned long long>(p));
-            atomicAdd(&h[k & 255u], 1u);
-            for (int d = 0; d < 4; ++d) atomicAdd(&t4[d][(k >> (8 * d)) & 255u], 1u);
-        }
-    }
-    __syncthreads();
-    hist[static_cast<int64_t>(tid) * tiles + blockIdx.x] = h[tid];
-    for (int d = 0; d < 4; ++d)
-        if (t4[d][tid]) atomicAdd(&totals[d * 256 + tid], t4[d][tid]);
 }
 
 // ------------------------------------------- single-launch radix passes
@@ -2644,13 +2603,11 @@ SortWorkspace sort_workspace(void *ws, size_t ws_bytes, int64_t n) {
     return W;
 }
 
-// Passes 1..3 (pass 0 too unless its histogram is already in W.hist):
-// in -> tmp -> out -> tmp -> out; the last pass clears the totals.
-void sort_passes(const SortWorkspace &W, bool hist0_done, const uint32_t *keys_in,
-                 uint32_t *keys_out, const int32_t *vals_in, int32_t *vals_out, int64_t n,
-                 int32_t *ranks, const float *table_src, float *table_out, hipStream_t s,
-                 const int32_t *csr_off = nullptr, int32_t *seg_out = nullptr,
-                 int32_t *deg_out = nullptr) {
+// The four passes: in -> tmp -> out -> tmp -> out; the last pass clears the
+// totals.
+void sort_passes(const SortWorkspace &W, const uint32_t *keys_in, uint32_t *keys_out,
+                 const int32_t *vals_in, int32_t *vals_out, int64_t n, int32_t *ranks,
+                 const float *table_src, float *table_out, hipStream_t s) {
     const unsigned tiles = static_cast<unsigned>(W.tiles);
     const uint32_t *src_k = keys_in;
     const int32_t *src_v = vals_in;
@@ -2658,17 +2615,15 @@ void sort_passes(const SortWorkspace &W, bool hist0_done, const uint32_t *keys_i
         uint32_t *dst_k = (pass & 1) ? keys_out : W.tmp_k;
         int32_t *dst_v = (pass & 1) ? vals_out : W.tmp_v;
         const int shift = 8 * pass;
-        if (pass > 0 || !hist0_done)
-            hipLaunchKernelGGL(radix_hist_kernel, dim3(tiles), dim3(kSortThreads), 0, s, src_k, n,
-                               shift, W.tiles, W.hist, pass == 0 ? W.totals : nullptr);
+        hipLaunchKernelGGL(radix_hist_kernel, dim3(tiles), dim3(kSortThreads), 0, s, src_k, n,
+                           shift, W.tiles, W.hist, pass == 0 ? W.totals : nullptr);
         hipLaunchKernelGGL(radix_scan_kernel, dim3(256), dim3(kSortThreads), 0, s, W.hist, W.tiles,
                            W.totals + pass * 256);
         const bool last = pass == 3;
         hipLaunchKernelGGL(radix_scatter_kernel, dim3(tiles), dim3(kSortThreads), 0, s, src_k, src_v,
                            dst_k, dst_v, n, shift, W.tiles, W.hist, last ? ranks : nullptr,
                            last ? table_src : nullptr, last ? table_out : nullptr,
-                           last ? W.totals : nullptr, last ? csr_off : nullptr,
-                           last ? seg_out : nullptr, last ? deg_out : nullptr);
+                           last ? W.totals : nullptr);
         src_k = dst_k;
         src_v = dst_v;
     }
@@ -2685,14 +2640,7 @@ void launch_sort_pairs_ranked(void *ws, size_t ws_bytes, const uint32_t *keys_in
     if (n == 0) return;
     const SortWorkspace W = sort_workspace(ws, ws_bytes, n);
     hipLaunchKernelGGL(radix_zero_kernel, dim3(4), dim3(kSortThreads), 0, s, W.totals, int64_t(1024));
-    sort_passes(W, false, keys_in, keys_out, vals_in, vals_out, n, ranks, table_src, table_out, s);
-}
-
-void clear_sort_workspace(void *ws, size_t ws_bytes, int64_t n, hipStream_t s) {
-    if (n == 0) return;
-    const SortWorkspace W = sort_workspace(ws, ws_bytes, n);
-    hipLaunchKernelGGL(radix_zero_kernel, dim3(4), dim3(kSortThreads), 0, s, W.totals, int64_t(1024));
-    HIP_CHECK(hipGetLastError());
+    sort_passes(W, keys_in, keys_out, vals_in, vals_out, n, ranks, table_src, table_out, s);
 }
 
 int64_t onesweep_tiles(int64_t n) { return (n + kOsTile - 1) / kOsTile; }

@@ -207,9 +207,6 @@ void launch_sort_pairs_ranked(void *ws, size_t ws_bytes, const uint32_t *keys_in
                               uint32_t *keys_out, const int32_t *vals_in, int32_t *vals_out,
                               int64_t n, int32_t *ranks, const float *table_src, float *table_out,
                               hipStream_t s);
-// Clears the digit totals of a sort workspace; launch_sort_pairs_ranked
-// leaves them clear.
-void clear_sort_workspace(void *ws, size_t ws_bytes, int64_t n, hipStream_t s);
 // Fit refresh for d = 2: keys of x . dir (non-finite rows tracked in status as
 // launch_project_keys does), then the stable sort of (key, row), one launch
 // per 8-bit pass, with ranks[row] = position and table_out[position] = x[row].
