@@ -902,6 +902,19 @@ __device__ inline void seg_scan_step(float &sx, float &sy, int rid) {
     }
 }
 
+// Inclusive prefix sum over the wave's 64 lanes through DPP (row_shr:1..8
+// within 16-lane rows, then row_bcast:15 and row_bcast:31 across them): a
+// few cycles per step instead of the LDS round trips of lane permutes.
+__device__ inline int wave_inclusive_sum(int v) {
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);   // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);   // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);   // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);   // row_shr:8
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);   // row_bcast:15 into rows 1, 3
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);   // row_bcast:31 into rows 2, 3
+    return v;
+}
+
 // Adds the segmented (by row) inclusive sums of one 64-term group into the
 // row accumulators; `row` is -1 for lanes without a term. Rows are
 // non-decreasing over the lanes that hold terms, so a partial sum ending at
@@ -1150,15 +1163,11 @@ __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
             }
         }
         s_row[slot] = static_cast<uint8_t>(r);
-        int incl = t;
-        for (int o = 1; o < 64; o <<= 1) {
-            const int up = __shfl_up(incl, o, 64);
-            if (lane >= o) incl += up;
-        }
+        const int incl = wave_inclusive_sum(t);
         t_cnt[k] = t;
         t_base[k] = running + incl - t;
         s_base[slot] = static_cast<uint16_t>(t_base[k]);
-        running += __shfl(incl, 63, 64);
+        running += __builtin_amdgcn_readlane(incl, 63);
     }
     const int total = running;
     if (lane == 0) s_base[UE] = static_cast<uint16_t>(total);
