@@ -64,12 +64,14 @@ constexpr int kTQ = 128;      // queries per workgroup
 constexpr int kTR = 16;       // reference rows per LDS chunk (one MFMA column block)
 constexpr int kCap = 128;     // candidate buffer per (query, slice)
 constexpr int kTop = 64;      // candidates kept per query
-constexpr int kMaxSlices = 256;   // slices per query batch (KnnIndex::slices_for)
+constexpr int kMaxSlices = 512;   // slices per query batch (KnnIndex::slices_for)
+constexpr int64_t kMinSliceRefs = 2048;   // fewest references per slice
 constexpr int kSample = 8192;      // references in the sample bound
-constexpr int kSampleRefs = 128;   // fewest sample references per workgroup of the sample distances
+constexpr int kSampleRefs = 16;    // fewest sample references per workgroup of the sample distances
 constexpr int kSampleMinSlices = 8;   // batches with at least this many slices use the bound
 constexpr int64_t kSampleMinRefs = 8 * kSample;
 constexpr int64_t kWideMergeMaxQ = 8192;   // batches up to this many rows merge with 4 waves
+constexpr int64_t kDeepMergeMaxQ = 512;    // and up to this many with 16
 constexpr int kMomentRows = 2048;          // rows per workgroup of the column moments
 constexpr int kExactQB = 4;                // queries per workgroup of the exact rescan
 constexpr int64_t kExactRound = 1024;      // uncertified queries per rescan round
@@ -180,14 +182,17 @@ __global__ void rounding_ratio_kernel(const float *norms, const float *err, int6
 // lane e % 64, register e / 64), keeps the best kTop = 64 sorted in place
 // and returns the new threshold (the kTop-th distance, or +inf when fewer).
 // Ids are unique, so the order is total.
-__device__ inline float compact_buffer(float *cd, int32_t *ci, int n, int lane) {
-    static_assert(kCap == 128 && kTop == 64, "two entries per lane");
-    float d[2];
-    int32_t id[2];
+__device__ inline void load_buffer(const float *cd, const int32_t *ci, int n, int lane,
+                                   float (&d)[2], int32_t (&id)[2]) {
     d[0] = lane < n ? cd[lane] : INFINITY;
     id[0] = lane < n ? ci[lane] : INT32_MAX;
     d[1] = lane + 64 < n ? cd[lane + 64] : INFINITY;
     id[1] = lane + 64 < n ? ci[lane + 64] : INT32_MAX;
+}
+
+__device__ inline float sort_buffer(float (&d)[2], int32_t (&id)[2], int n, float *cd, int32_t *ci,
+                                    int lane) {
+    static_assert(kCap == 128 && kTop == 64, "two entries per lane");
 #pragma unroll
     for (int k = 2; k <= 128; k <<= 1) {
 #pragma unroll
@@ -227,6 +232,49 @@ __device__ inline float compact_buffer(float *cd, int32_t *ci, int n, int lane) 
     ci[lane] = id[0];
     const float kth = __shfl(d[0], kTop - 1, 64);
     return n >= kTop ? kth : INFINITY;
+}
+
+__device__ inline float compact_buffer(float *cd, int32_t *ci, int n, int lane) {
+    float d[2];
+    int32_t id[2];
+    load_buffer(cd, ci, n, lane, d, id);
+    return sort_buffer(d, id, n, cd, ci, lane);
+}
+
+// End of a scan cell: the final sort of the buffers of the block's kTQ
+// queries and their counts (at most kTop). Query ql goes to wave ql % 4, so
+// a batch of fewer than kTQ rows spreads over all four waves, and each wave
+// loads its next query's buffer while it sorts the current one. Call after
+// every wave's appends (global and s_cnt) are visible to the block.
+__device__ inline void finish_cell(const int *s_cnt, int64_t q_block, int64_t nq, int slices,
+                                   int slice, float *cand_d, int32_t *cand_i, int32_t *cand_n,
+                                   int wave, int lane) {
+    int ql = wave;
+    int64_t q = q_block + ql;
+    int n = q < nq ? s_cnt[ql] : 0;
+    float nd[2], d[2];
+    int32_t nid[2], id[2];
+    if (q < nq) load_buffer(cand_d + (q * slices + slice) * kCap, cand_i + (q * slices + slice) * kCap,
+                            n, lane, nd, nid);
+    while (q < nq) {
+        const int cur_n = n;
+        const int64_t cur = q * slices + slice;
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            d[r] = nd[r];
+            id[r] = nid[r];
+        }
+        ql += 4;
+        q += 4;
+        if (ql >= kTQ) q = nq;   // past the block
+        if (q < nq) {
+            n = s_cnt[ql];
+            load_buffer(cand_d + (q * slices + slice) * kCap, cand_i + (q * slices + slice) * kCap,
+                        n, lane, nd, nid);
+        }
+        if (cur_n > 0) sort_buffer(d, id, cur_n, cand_d + cur * kCap, cand_i + cur * kCap, lane);
+        if (lane == 0) cand_n[cur] = cur_n < kTop ? cur_n : kTop;
+    }
 }
 
 // Order-preserving u32 key of a float (the global threshold is kept as a
@@ -285,9 +333,10 @@ __device__ __forceinline__ void put_chunk(const u32x4 (&src)[LW], float rn, uint
 }
 
 // Grids of at most one workgroup per CU (small batches): the scan with
-// chunks loaded into registers two chunks ahead and written to LDS by the
-// waves themselves, so that two chunks stay in flight while one is computed
-// (one workgroup per CU has no second workgroup to cover a chunk's load).
+// chunks loaded into registers up to four chunks ahead and written to LDS by
+// the waves themselves, so that four chunks stay in flight while one is
+// computed (one workgroup per CU has no second workgroup to cover a chunk's
+// load).
 // The LDS image keeps padded rows (SLOTS_ROW words); the fragments, MFMA
 // order and rn - 2 q.r are those of knn_scan_kernel.
 // Scan cells (query block, slice) in slice-major order: cell = slice *
@@ -332,13 +381,17 @@ __global__ __launch_bounds__(256, 1) void knn_scan_small_kernel(
     const int64_t r_begin = static_cast<int64_t>(slice) * per_slice;
     const int64_t r_end = min<int64_t>(nr, r_begin + per_slice);
 
-    // Chunk loads into registers (slot 0 or 1), and from there into an LDS
+    // Chunk loads into registers (one of SETS sets), and from there into an LDS
     // buffer: LDS word `slot` of the padded image holds row slot / SLOTS_ROW,
     // word slot % SLOTS_ROW (the last word of a row and the slots past the
     // chunk are padding); past the slice they load harmless in-bounds words.
     constexpr int LW = (INSTR + 3) / 4;   // staging loads per wave (some use LW - 1)
-    u32x4 pre0[LW], pre1[LW];
-    float rn0 = 0.0f, rn1 = 0.0f;
+    // register sets of staged chunks: chunk i (counted from the slice's
+    // start) is in set i mod SETS and LDS buffer i mod 2; four sets keep four
+    // chunks in flight, two where the query fragments leave no room for more
+    constexpr int SETS = KSTEPS >= 32 ? 2 : 4;
+    u32x4 pre[SETS][LW];
+    float rnv[SETS];
 
     // A fragments: lane holds row (lane % 16) and K elements 8*(lane/16)..+7
     // of each 32-wide K step, for two 16-query sub-blocks.
@@ -378,18 +431,22 @@ __global__ __launch_bounds__(256, 1) void knn_scan_small_kernel(
     // start, so no wait in the loop counts them
     wait_vmcnt<0>();
     if (r_begin < r_end) {
-        fetch_chunk<LW, INSTR, SLOTS_ROW, DP>(pre0, rn0, rb, rnorm, r_begin, r_end, wave, lane);
-        fetch_chunk<LW, INSTR, SLOTS_ROW, DP>(pre1, rn1, rb, rnorm, r_begin + kTR, r_end, wave, lane);
-        put_chunk<LW, INSTR>(pre0, rn0, sB, &s_rn[0][0], wave, lane);
-        fetch_chunk<LW, INSTR, SLOTS_ROW, DP>(pre0, rn0, rb, rnorm, r_begin + 2 * kTR, r_end, wave, lane);
+#pragma unroll
+        for (int s = 0; s < SETS; ++s)
+            fetch_chunk<LW, INSTR, SLOTS_ROW, DP>(pre[s], rnv[s], rb, rnorm, r_begin + s * kTR,
+                                                  r_end, wave, lane);
+        put_chunk<LW, INSTR>(pre[0], rnv[0], sB, &s_rn[0][0], wave, lane);
+        fetch_chunk<LW, INSTR, SLOTS_ROW, DP>(pre[0], rnv[0], rb, rnorm, r_begin + SETS * kTR,
+                                              r_end, wave, lane);
     }
 
     // One chunk: wait for its buffer, compute it against the queries and keep
-    // the survivors. The buffer is a constant so that each of the two
+    // the survivors. The chunk's set is a constant so that each of the SETS
     // unrolled copies stores and refills a fixed register set.
     int since = 0;
-    auto scan_chunk = [&](int64_t chunk, auto bufc) {
-        const int buf = bufc;
+    auto scan_chunk = [&](int64_t chunk, auto setc) {
+        constexpr int set = decltype(setc)::value;
+        const int buf = set & 1;
         // chunk's LDS writes are done and the other buffer is free; the fences
         // order LDS only, so the chunks in flight are not waited on
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
@@ -410,20 +467,16 @@ __global__ __launch_bounds__(256, 1) void knn_scan_small_kernel(
                     tau_r[qa][i] = s_tau[wave * 32 + qa * 16 + 4 * (lane >> 4) + i];
         }
         const float rn = s_rn[buf][lane & 15];
-        // registers: chunk + kTR in slot buf ^ 1, chunk + 2 kTR in slot buf;
-        // chunk + kTR goes to LDS and chunk + 3 kTR into its slot. Done past
-        // the slice's end too (harmless words into a buffer nobody reads): a
+        // registers: chunks + kTR .. + SETS kTR; chunk + kTR goes to the other
+        // LDS buffer and chunk + (SETS + 1) kTR into its set. Done past the
+        // slice's end too (harmless words into a buffer nobody reads): a
         // conditional refill would leave the compiler unsure which register
         // set holds the newest loads, and it would then wait for all of them.
-        if constexpr (decltype(bufc)::value == 0) {
-            put_chunk<LW, INSTR>(pre1, rn1, sB + BUF, &s_rn[1][0], wave, lane);
-            fetch_chunk<LW, INSTR, SLOTS_ROW, DP>(pre1, rn1, rb, rnorm, chunk + 3 * kTR, r_end,
-                                                  wave, lane);
-        } else {
-            put_chunk<LW, INSTR>(pre0, rn0, sB, &s_rn[0][0], wave, lane);
-            fetch_chunk<LW, INSTR, SLOTS_ROW, DP>(pre0, rn0, rb, rnorm, chunk + 3 * kTR, r_end,
-                                                  wave, lane);
-        }
+        constexpr int next = (set + 1) % SETS;
+        put_chunk<LW, INSTR>(pre[next], rnv[next], sB + (buf ^ 1) * BUF, &s_rn[buf ^ 1][0], wave,
+                             lane);
+        fetch_chunk<LW, INSTR, SLOTS_ROW, DP>(pre[next], rnv[next], rb, rnorm,
+                                              chunk + (SETS + 1) * kTR, r_end, wave, lane);
         const uint16_t *B = sB + buf * BUF;
         floatx4 acc[2] = {floatx4{0.0f, 0.0f, 0.0f, 0.0f}, floatx4{0.0f, 0.0f, 0.0f, 0.0f}};
 #pragma unroll
@@ -491,23 +544,24 @@ __global__ __launch_bounds__(256, 1) void knn_scan_small_kernel(
             }
         }
     };
-    // pairs of chunks, then an odd last one (in buffer 0)
+    // groups of SETS chunks, then up to SETS - 1 more
     int64_t chunk = r_begin;
-    for (; chunk + kTR < r_end; chunk += 2 * kTR) {
+    for (; chunk + (SETS - 1) * kTR < r_end; chunk += SETS * kTR) {
         scan_chunk(chunk, std::integral_constant<int, 0>{});
         scan_chunk(chunk + kTR, std::integral_constant<int, 1>{});
+        if constexpr (SETS == 4) {
+            scan_chunk(chunk + 2 * kTR, std::integral_constant<int, 2>{});
+            scan_chunk(chunk + 3 * kTR, std::integral_constant<int, 3>{});
+        }
     }
     if (chunk < r_end) scan_chunk(chunk, std::integral_constant<int, 0>{});
-    __threadfence_block();
-    for (int l = 0; l < 32; ++l) {
-        const int ql = wave * 32 + l;
-        const int64_t q = q_block + ql;
-        if (q >= nq) continue;
-        const int n = s_cnt[ql];
-        const int64_t base = (q * slices + slice) * kCap;
-        if (n > 0) compact_buffer(cand_d + base, cand_i + base, n, lane);
-        if (lane == 0) cand_n[q * slices + slice] = n < kTop ? n : kTop;
+    if constexpr (SETS == 4) {
+        if (chunk + kTR < r_end) scan_chunk(chunk + kTR, std::integral_constant<int, 1>{});
+        if (chunk + 2 * kTR < r_end) scan_chunk(chunk + 2 * kTR, std::integral_constant<int, 2>{});
     }
+    __threadfence_block();
+    __syncthreads();
+    finish_cell(s_cnt, q_block, nq, slices, slice, cand_d, cand_i, cand_n, wave, lane);
 }
 
 // The prefilter scan of one (query block, slice) for grids of two workgroups
@@ -700,15 +754,8 @@ __global__ __launch_bounds__(256, KSTEPS > 25 ? 1 : 2) void knn_scan_kernel(
         }
     }
     __threadfence_block();
-    for (int l = 0; l < 32; ++l) {
-        const int ql = wave * 32 + l;
-        const int q = q_block + ql;
-        if (q >= qnq) continue;
-        const int n = s_cnt[ql];
-        const int64_t base = (static_cast<int64_t>(q) * slices + slice) * kCap;
-        if (n > 0) compact_buffer(cand_d + base, cand_i + base, n, lane);
-        if (lane == 0) cand_n[static_cast<int64_t>(q) * slices + slice] = n < kTop ? n : kTop;
-    }
+    __syncthreads();
+    finish_cell(s_cnt, q_block, nq, slices, slice, cand_d, cand_i, cand_n, wave, lane);
 }
 
 // Approximate distances of a block of 128 queries to kSampleRefs sample
@@ -766,29 +813,40 @@ __global__ __launch_bounds__(256) void knn_sample_dist_kernel(
     }
 }
 
-// Workgroup q: the kTop-th smallest of the query's m sample distances by a
-// most-significant-byte-first radix select over their order-preserving
-// keys, widened by 2^-20 of its magnitude, becomes the query's global
-// threshold (+inf when fewer than kTop sample distances are finite).
+// Workgroup q: the kTop-th smallest of the query's m (at most kSample)
+// sample distances by a most-significant-byte-first radix select over their
+// order-preserving keys, widened by 2^-20 of its magnitude, becomes the
+// query's global threshold (+inf when fewer than kTop sample distances are
+// finite). The keys are read once into registers (kSample / 256 per
+// thread); each wave counts into its own histogram, so the four waves do
+// not contend for the few bins the exponent byte fills.
 __global__ __launch_bounds__(256) void knn_sample_select_kernel(const float *__restrict__ dist, int m,
                                                                 uint32_t *__restrict__ tau_key) {
-    __shared__ uint32_t s_hist[256];
+    constexpr int KPT = kSample / 256;
+    __shared__ uint32_t s_hist[4][256];
     __shared__ uint32_t s_wsum[4];
     __shared__ uint32_t s_sel[2];   // chosen byte, rank left within it
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t q = blockIdx.x;
     const float *row = dist + q * m;
+    uint32_t key[KPT];
+#pragma unroll
+    for (int j = 0; j < KPT; ++j) {
+        const int i = j * 256 + tid;
+        key[j] = i < m ? float_key(row[i]) : 0u;
+    }
     uint32_t prefix = 0u, mask = 0u, want = kTop - 1;   // 0-based rank of the kTop-th smallest
     for (int shift = 24; shift >= 0; shift -= 8) {
-        s_hist[tid] = 0u;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) s_hist[w][tid] = 0u;
         __syncthreads();
-        for (int i = tid; i < m; i += 256) {
-            const uint32_t k = float_key(row[i]);
-            if ((k & mask) == prefix) atomicAdd(&s_hist[(k >> shift) & 255u], 1u);
-        }
+#pragma unroll
+        for (int j = 0; j < KPT; ++j)
+            if (j * 256 + tid < m && (key[j] & mask) == prefix)
+                atomicAdd(&s_hist[wave][(key[j] >> shift) & 255u], 1u);
         __syncthreads();
         // the byte whose count range [below, below + count) holds rank want
-        const uint32_t c = s_hist[tid];
+        const uint32_t c = s_hist[0][tid] + s_hist[1][tid] + s_hist[2][tid] + s_hist[3][tid];
         uint32_t incl = c;
         for (int o = 1; o < 64; o <<= 1) {
             const uint32_t up = __shfl_up(incl, o, 64);
@@ -893,31 +951,35 @@ __global__ __launch_bounds__(64) void knn_merge_kernel(const float *cand_d, cons
     if (lane == 0) top_n[q] = na;
 }
 
-// knn_merge_kernel for small batches: four waves each merge every fourth
-// listed slice, then wave 0 merges the four lists. The best kTop under the
-// total order (distance, id) does not depend on the order of the merges, so
-// the result equals knn_merge_kernel's.
-__global__ __launch_bounds__(256) void knn_merge_wide_kernel(const float *cand_d,
-                                                             const int32_t *cand_i,
-                                                             const int32_t *cand_n,
-                                                             const uint32_t *tau_key, int64_t nq,
-                                                             int slices, float *top_d,
-                                                             int32_t *top_i, int32_t *top_n) {
+// knn_merge_kernel for small batches: WAVES waves each merge every
+// WAVES-th listed slice, then the wave lists merge pairwise in log2(WAVES)
+// rounds. The best kTop under the total order (distance, id) does not
+// depend on the order of the merges, so the result equals
+// knn_merge_kernel's.
+template <int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void knn_merge_wide_kernel(const float *cand_d,
+                                                                    const int32_t *cand_i,
+                                                                    const int32_t *cand_n,
+                                                                    const uint32_t *tau_key,
+                                                                    int64_t nq, int slices,
+                                                                    float *top_d, int32_t *top_i,
+                                                                    int32_t *top_n) {
+    static_assert(WAVES >= 2 && (WAVES & (WAVES - 1)) == 0, "a power of two of waves");
     __shared__ int32_t s_list[kMaxSlices];
     __shared__ int s_count;
-    __shared__ float s_d[4][kTop];
-    __shared__ int32_t s_i[4][kTop];
-    __shared__ int s_na[4];
+    __shared__ float s_d[WAVES][kTop];
+    __shared__ int32_t s_i[WAVES][kTop];
+    __shared__ int s_na[WAVES];
     const int64_t q = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (q >= nq) return;
     const float tau = key_float(tau_key[q]);
     if (tid == 0) s_count = 0;
     __syncthreads();
-    if (tid < slices) {
+    for (int sl = tid; sl < slices; sl += 64 * WAVES) {
         // list order is free: the merged result does not depend on it
-        const int n = cand_n[q * slices + tid];
-        if (n > 0 && cand_d[(q * slices + tid) * kCap] <= tau) s_list[atomicAdd(&s_count, 1)] = tid;
+        const int n = cand_n[q * slices + sl];
+        if (n > 0 && cand_d[(q * slices + sl) * kCap] <= tau) s_list[atomicAdd(&s_count, 1)] = sl;
     }
     __syncthreads();
     const int listed = s_count;
@@ -954,24 +1016,29 @@ __global__ __launch_bounds__(256) void knn_merge_wide_kernel(const float *cand_d
         next_i = r < next_n ? cand_i[base + r] : INT32_MAX;
     };
     if (wave < listed) fetch(wave);
-    for (int li = wave; li < listed; li += 4) {
+    for (int li = wave; li < listed; li += WAVES) {
         const float bd = next_d;
         const int32_t bi = next_i;
         const int nb = next_n;
-        if (li + 4 < listed) fetch(li + 4);
+        if (li + WAVES < listed) fetch(li + WAVES);
         merge_in(bd, bi);
         na = min(kTop, na + nb);
     }
-    s_d[wave][lane] = ad;
-    s_i[wave][lane] = ai;
-    if (lane == 0) s_na[wave] = na;
-    __syncthreads();
-    if (wave != 0) return;
+    // pairwise rounds: wave w < h takes in the list of wave w + h
 #pragma unroll
-    for (int w = 1; w < 4; ++w) {
-        merge_in(s_d[w][kTop - 1 - lane], s_i[w][kTop - 1 - lane]);
-        na = min(kTop, na + s_na[w]);
+    for (int h = WAVES / 2; h >= 1; h /= 2) {
+        if (wave >= h && wave < 2 * h) {
+            s_d[wave][lane] = ad;
+            s_i[wave][lane] = ai;
+            if (lane == 0) s_na[wave] = na;
+        }
+        __syncthreads();
+        if (wave < h) {
+            merge_in(s_d[wave + h][kTop - 1 - lane], s_i[wave + h][kTop - 1 - lane]);
+            na = min(kTop, na + s_na[wave + h]);
+        }
     }
+    if (wave != 0) return;
     top_d[q * kTop + lane] = ad;
     top_i[q * kTop + lane] = ai;
     if (lane == 0) top_n[q] = na;
@@ -1328,10 +1395,10 @@ bool KnnIndex::uses_sample(int64_t q) const {
 int KnnIndex::slices_for(int64_t q) const {
     const int64_t blocks = (q + kTQ - 1) / kTQ;
     // Enough workgroups to fill the chip (256 CUs, two resident each), but
-    // slices of at least 4096 references.
+    // slices of at least kMinSliceRefs references.
     int64_t s = (512 + blocks - 1) / blocks;
-    s = std::min<int64_t>(s, std::max<int64_t>(1, n_ / 4096));
-    s = std::max<int64_t>(1, std::min<int64_t>(s, 256));
+    s = std::min<int64_t>(s, std::max<int64_t>(1, n_ / kMinSliceRefs));
+    s = std::max<int64_t>(1, std::min<int64_t>(s, kMaxSlices));
     return static_cast<int>(s);
 }
 
@@ -1416,8 +1483,12 @@ void KnnIndex::query(const float *q_dev, int64_t nq, int k, bool exclude_self, i
         default: launch_scan<32>(grid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, per_slice, slices, cand_d_.ptr, cand_i_.ptr, cand_n_.ptr, tau_key_.ptr, ex, query_offset); break;
     }
     HIP_CHECK(hipGetLastError());
-    if (nq <= kWideMergeMaxQ)
-        hipLaunchKernelGGL(knn_merge_wide_kernel, dim3(static_cast<unsigned>(nq)), dim3(256), 0, s,
+    if (nq <= kDeepMergeMaxQ)
+        hipLaunchKernelGGL(knn_merge_wide_kernel<16>, dim3(static_cast<unsigned>(nq)), dim3(1024), 0,
+                           s, cand_d_.ptr, cand_i_.ptr, cand_n_.ptr, tau_key_.ptr, nq, slices,
+                           top_d_.ptr, top_i_.ptr, top_n_.ptr);
+    else if (nq <= kWideMergeMaxQ)
+        hipLaunchKernelGGL(knn_merge_wide_kernel<4>, dim3(static_cast<unsigned>(nq)), dim3(256), 0, s,
                            cand_d_.ptr, cand_i_.ptr, cand_n_.ptr, tau_key_.ptr, nq, slices,
                            top_d_.ptr, top_i_.ptr, top_n_.ptr);
     else

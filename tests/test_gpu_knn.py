@@ -49,9 +49,9 @@ def test_integer_data_bit_exact(n, d, q, k):
 
 # Grids above 256 workgroups (query blocks x slices) run the DMA-staged scan
 # (two workgroups per CU, one wave per SIMD above 800 features); 40,000
-# references give 9 slices, so 4,000 queries (32 blocks) make 288
-# workgroups. 39,990 references end the last slice in a ragged chunk, and
-# 300,000 with 512 queries (4 blocks x 73 slices) add the sample bound.
+# references and 4,000 queries (32 blocks) take 16 slices, 512 workgroups.
+# 39,990 references end the last slice in a ragged chunk, and 300,000 with
+# 512 queries (4 blocks x 128 slices) add the sample bound.
 @pytest.mark.parametrize("n,d,q", [(40_000, 1000, 4000), (40_000, 500, 4000),
                                    (39_990, 200, 4000), (40_000, 20, 4000),
                                    (300_000, 200, 512)])
@@ -104,6 +104,24 @@ def test_continuous_data_recall():
     assert (gi == ei).all() and (gd.view(np.uint32) == ed.view(np.uint32)).all()
 
 
+@pytest.mark.parametrize("q", [1, 100])
+def test_most_slices_bit_exact(q):
+    """1,048,600 references: batches of up to 128 rows take 512 slices (the
+    most a batch takes) of 2,064 references, the last one ragged, so the
+    scan runs 512 DMA-staged workgroups and the merge lists more than 256
+    slices per query."""
+    from cuscan_amd.knn import KnnIndex
+
+    ref = image_like(1_048_600, 20, 10, 21)
+    qry = image_like(q, 20, 10, 22)
+    print(f"[gpu] knn most slices q={q}", flush=True)
+    idx = KnnIndex(ref, max_queries=q)
+    gi, gd = idx.query(qry, 15)
+    idx.close()
+    ei, ed = O.exact_knn(qry, ref, 15, threads=8)
+    assert (gi == ei).all() and (gd.view(np.uint32) == ed.view(np.uint32)).all()
+
+
 @pytest.mark.parametrize("q", [1, 64, 700, 4096])
 def test_sample_bound_bit_exact(q):
     """150,000 references: batches of up to 8,192 rows take at least 8
@@ -118,8 +136,8 @@ def test_sample_bound_bit_exact(q):
     print(f"[gpu] knn sample bound q={q}", flush=True)
     idx = KnnIndex(ref, max_queries=9472)
     # slices per batch (KnnIndex::slices_for): ceil(512 / query blocks),
-    # at most references / 4096; the bound needs at least 8 slices
-    slices = lambda rows: min(-(-512 // -(-rows // 128)), 150_000 // 4096)
+    # at most references / 2048; the bound needs at least 8 slices
+    slices = lambda rows: min(-(-512 // -(-rows // 128)), 150_000 // 2048)
     assert slices(q) >= 8 and slices(9472) == 7
     gi, gd = idx.query(qry[:q], 15)
     ei, ed = O.exact_knn(qry[:q], ref, 15, threads=8)
