@@ -72,6 +72,7 @@ constexpr int kSampleMinSlices = 8;   // batches with at least this many slices 
 constexpr int64_t kSampleMinRefs = 8 * kSample;
 constexpr int64_t kWideMergeMaxQ = 8192;   // batches up to this many rows merge with 4 waves
 constexpr int64_t kDeepMergeMaxQ = 512;    // and up to this many with 16
+constexpr int kMergeAhead = 4;             // slice lists in flight per merging wave
 constexpr int kMomentRows = 2048;          // rows per workgroup of the column moments
 constexpr int kExactQB = 4;                // queries per workgroup of the exact rescan
 constexpr int64_t kExactRound = 1024;      // uncertified queries per rescan round
@@ -587,12 +588,19 @@ __global__ __launch_bounds__(256, 1) void knn_scan_small_kernel(
 // wait on a vector memory load (a scratch reload included) into vmcnt(0).
 // Above 25 K steps the A fragments alone fill 256 VGPRs, so those widths
 // run one wave per SIMD (512 VGPRs).
-template <int KSTEPS, int RING>
+//
+// SAMPLE: the sample distances instead. The rows are the sample entries
+// 0 .. sample_m - 1 (entry i is reference i * nr / sample_m), sliced like
+// the references, and every (query, entry) value is written to
+// sample_dist[q * sample_m + i] (+inf for an excluded self pair), so each
+// equals the value the scan computes for the same (query, reference).
+template <int KSTEPS, int RING, bool SAMPLE>
 __global__ __launch_bounds__(256, KSTEPS > 25 ? 1 : 2) void knn_scan_kernel(
     const uint16_t *__restrict__ qb, int64_t nq, const uint16_t *__restrict__ rb,
     const float *__restrict__ rnorm, int64_t nr, int64_t per_slice, int slices,
     float *__restrict__ cand_d, int32_t *__restrict__ cand_i, int32_t *__restrict__ cand_n,
-    uint32_t *__restrict__ tau_key, int exclude_self, int64_t query_offset) {
+    uint32_t *__restrict__ tau_key, int exclude_self, int64_t query_offset, int64_t sample_m,
+    float *__restrict__ sample_dist) {
     constexpr int DP = KSTEPS * 32;
     constexpr int BUF = KSTEPS * 512;   // bf16 per buffer
     constexpr int kRefresh = 32;
@@ -606,19 +614,29 @@ __global__ __launch_bounds__(256, KSTEPS > 25 ? 1 : 2) void knn_scan_kernel(
     int slice = 0;
     if (!scan_cell(nq, slices, q_block64, slice)) return;
     const int q_block = static_cast<int>(q_block64);
+    const int inr = static_cast<int>(SAMPLE ? sample_m : nr);   // rows: references or entries
     const int r_begin = static_cast<int>(slice * per_slice);
-    const int r_end = static_cast<int>(min<int64_t>(nr, r_begin + per_slice));
+    const int r_end = static_cast<int>(min<int64_t>(inr, r_begin + per_slice));
     const int qnq = static_cast<int>(nq);
-    const int inr = static_cast<int>(nr);
+    // the reference of row r
+    auto ref_of = [&](int r) -> int {
+        return SAMPLE ? static_cast<int>(static_cast<int64_t>(r) * nr / sample_m) : r;
+    };
 
     // DMA lane mapping (row and source word of this lane's 16 bytes)
     const int d_row = lane >> 2;
     const int d_word = (lane & 3) ^ ((d_row >> 2) & 3);
     const uint16_t *const d_src = rb + static_cast<int64_t>(d_row) * DP + d_word * 8;
     auto stage = [&](int buf, int chunk) {
-        // rows past the references (last chunk only) load row 0 instead
-        const uint16_t *src = d_src + static_cast<int64_t>(chunk) * DP + wave * 32;
-        if (chunk + kTR > inr && chunk + d_row >= inr) src = rb + d_word * 8 + wave * 32;
+        // rows past the last (last chunk only) load row 0 instead
+        const uint16_t *src;
+        if constexpr (SAMPLE) {
+            const int r = chunk + d_row < inr ? chunk + d_row : 0;
+            src = rb + static_cast<int64_t>(ref_of(r)) * DP + d_word * 8 + wave * 32;
+        } else {
+            src = d_src + static_cast<int64_t>(chunk) * DP + wave * 32;
+            if (chunk + kTR > inr && chunk + d_row >= inr) src = rb + d_word * 8 + wave * 32;
+        }
 #pragma unroll
         for (int t = 0; t < (KSTEPS + 3) / 4; ++t) {
             if (wave + 4 * t < KSTEPS)
@@ -643,14 +661,14 @@ __global__ __launch_bounds__(256, KSTEPS > 25 ? 1 : 2) void knn_scan_kernel(
             }
         }
     }
-    if (lane < 32) {
+    if (!SAMPLE && lane < 32) {
         // queries past nq get -inf: no distance passes dt <= -inf
         const int q = q_block + wave * 32 + lane;
         s_tau[wave * 32 + lane] = q < qnq ? key_float(tau_key[q]) : -INFINITY;
         s_cnt[wave * 32 + lane] = 0;
     }
     float rn_next = 0.0f;
-    if (r_begin + (lane & 15) < r_end) rn_next = rnorm[r_begin + (lane & 15)];
+    if (r_begin + (lane & 15) < r_end) rn_next = rnorm[ref_of(r_begin + (lane & 15))];
     // lane's C rows: queries ql_lane + qa * 16 + i
     const int ql_lane = wave * 32 + 4 * (lane >> 4);
     const int self_base = static_cast<int>(query_offset) + q_block + ql_lane;
@@ -662,7 +680,7 @@ __global__ __launch_bounds__(256, KSTEPS > 25 ? 1 : 2) void knn_scan_kernel(
     for (int chunk = r_begin; chunk < r_end; chunk += kTR, buf ^= 1) {
         __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): this lane's loads of the current buffer
         __syncthreads();                      // every lane's loads landed; the other buffer is free
-        if (++since == kRefresh) {
+        if (!SAMPLE && ++since == kRefresh) {
             since = 0;
             const int q = q_block + wave * 32 + (lane & 31);
             if (lane < 32 && q < qnq) {
@@ -674,7 +692,7 @@ __global__ __launch_bounds__(256, KSTEPS > 25 ? 1 : 2) void knn_scan_kernel(
         if (chunk + kTR < r_end) {
             stage(buf ^ 1, chunk + kTR);
             const int nref = chunk + kTR + (lane & 15);
-            rn_next = nref < r_end ? rnorm[nref] : 0.0f;
+            rn_next = nref < r_end ? rnorm[ref_of(nref)] : 0.0f;
         }
         const uint16_t *Bl = Bl0 + buf * BUF;
         floatx4 acc[2] = {floatx4{0.0f, 0.0f, 0.0f, 0.0f}, floatx4{0.0f, 0.0f, 0.0f, 0.0f}};
@@ -699,6 +717,24 @@ __global__ __launch_bounds__(256, KSTEPS > 25 ? 1 : 2) void knn_scan_kernel(
         for (int ks = 0; ks < KSTEPS; ++ks) {
             __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
             if (ks + RING < KSTEPS) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+        }
+        if constexpr (SAMPLE) {
+            const int e = chunk + (lane & 15);
+            if (e < r_end) {
+                const int64_t ref = ref_of(e);
+#pragma unroll
+                for (int qa = 0; qa < 2; ++qa)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const int q = q_block + ql_lane + qa * 16 + i;
+                        if (q < qnq) {
+                            const float dt = rn - 2.0f * acc[qa][i];
+                            const bool excluded = exclude_self && ref == query_offset + q;
+                            sample_dist[static_cast<int64_t>(q) * sample_m + e] = excluded ? INFINITY : dt;
+                        }
+                    }
+            }
+            continue;
         }
         const int ref = chunk + (lane & 15);
         const bool ref_ok = ref < r_end;
@@ -753,63 +789,10 @@ __global__ __launch_bounds__(256, KSTEPS > 25 ? 1 : 2) void knn_scan_kernel(
             }
         }
     }
-    __threadfence_block();
-    __syncthreads();
-    finish_cell(s_cnt, q_block, nq, slices, slice, cand_d, cand_i, cand_n, wave, lane);
-}
-
-// Approximate distances of a block of 128 queries to kSampleRefs sample
-// references (sample entry i is reference i * nr / m), in chunks of 16:
-// the A fragments, B fragments, MFMA sequence over the K steps and
-// rn - 2 q.r of knn_scan_kernel, so each value equals the one the scan
-// computes for the same (query, reference). Excluded pairs (self) and
-// queries past nq get +inf.
-template <int KSTEPS>
-__global__ __launch_bounds__(256) void knn_sample_dist_kernel(
-    const uint16_t *__restrict__ qb, int64_t nq, const uint16_t *__restrict__ rb,
-    const float *__restrict__ rnorm, int64_t nr, int m, int refs_per_wg, float *__restrict__ dist,
-    int exclude_self, int64_t query_offset) {
-    constexpr int DP = KSTEPS * 32;
-    const int tid = threadIdx.x;
-    const int wave = tid >> 6, lane = tid & 63;
-    const int64_t q_block = static_cast<int64_t>(blockIdx.x) * kTQ;
-    bf16x8 A[2][KSTEPS];
-#pragma unroll
-    for (int qa = 0; qa < 2; ++qa) {
-        const int64_t row = q_block + wave * 32 + qa * 16 + (lane & 15);
-#pragma unroll
-        for (int ks = 0; ks < KSTEPS; ++ks) {
-            if (row < nq) {
-                const uint4 raw = *reinterpret_cast<const uint4 *>(qb + row * DP + ks * 32 + 8 * (lane >> 4));
-                A[qa][ks] = __builtin_bit_cast(bf16x8, raw);
-            } else {
-                A[qa][ks] = __builtin_bit_cast(bf16x8, make_uint4(0, 0, 0, 0));
-            }
-        }
-    }
-    for (int c = 0; c < refs_per_wg / kTR; ++c) {
-        const int i = blockIdx.y * refs_per_wg + c * kTR + (lane & 15);   // sample entry
-        const int64_t ref = static_cast<int64_t>(i) * nr / m;
-        floatx4 acc[2] = {floatx4{0.0f, 0.0f, 0.0f, 0.0f}, floatx4{0.0f, 0.0f, 0.0f, 0.0f}};
-#pragma unroll
-        for (int ks = 0; ks < KSTEPS; ++ks) {
-            const uint4 raw = *reinterpret_cast<const uint4 *>(rb + ref * DP + ks * 32 + 8 * (lane >> 4));
-            const bf16x8 bfrag = __builtin_bit_cast(bf16x8, raw);
-#pragma unroll
-            for (int qa = 0; qa < 2; ++qa)
-                acc[qa] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[qa][ks], bfrag, acc[qa], 0, 0, 0);
-        }
-        const float rn = rnorm[ref];
-#pragma unroll
-        for (int qa = 0; qa < 2; ++qa)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int64_t q = q_block + wave * 32 + qa * 16 + 4 * (lane >> 4) + r;
-                if (q >= nq) continue;
-                const float dt = rn - 2.0f * acc[qa][r];
-                const bool excluded = exclude_self && ref == query_offset + q;
-                dist[q * m + i] = excluded ? INFINITY : dt;
-            }
+    if constexpr (!SAMPLE) {
+        __threadfence_block();
+        __syncthreads();
+        finish_cell(s_cnt, q_block, nq, slices, slice, cand_d, cand_i, cand_n, wave, lane);
     }
 }
 
@@ -818,8 +801,9 @@ __global__ __launch_bounds__(256) void knn_sample_dist_kernel(
 // order-preserving keys, widened by 2^-20 of its magnitude, becomes the
 // query's global threshold (+inf when fewer than kTop sample distances are
 // finite). The keys are read once into registers (kSample / 256 per
-// thread); each wave counts into its own histogram, so the four waves do
-// not contend for the few bins the exponent byte fills.
+// thread); each wave counts into its own histogram. The first (exponent)
+// byte takes only a few values, so there a wave adds each distinct byte
+// once (lanes grouped by ballot) instead of 64 atomics on one bin.
 __global__ __launch_bounds__(256) void knn_sample_select_kernel(const float *__restrict__ dist, int m,
                                                                 uint32_t *__restrict__ tau_key) {
     constexpr int KPT = kSample / 256;
@@ -840,10 +824,25 @@ __global__ __launch_bounds__(256) void knn_sample_select_kernel(const float *__r
 #pragma unroll
         for (int w = 0; w < 4; ++w) s_hist[w][tid] = 0u;
         __syncthreads();
+        if (shift == 24) {
 #pragma unroll
-        for (int j = 0; j < KPT; ++j)
-            if (j * 256 + tid < m && (key[j] & mask) == prefix)
-                atomicAdd(&s_hist[wave][(key[j] >> shift) & 255u], 1u);
+            for (int j = 0; j < KPT; ++j) {
+                const uint32_t b = key[j] >> 24;
+                uint64_t todo = __builtin_amdgcn_ballot_w64(j * 256 + tid < m);
+                while (todo) {
+                    const uint32_t lb = __builtin_amdgcn_readlane(b, static_cast<int>(__builtin_ctzll(todo)));
+                    const uint64_t same = todo & __builtin_amdgcn_ballot_w64(b == lb);
+                    if (lane == static_cast<int>(__builtin_ctzll(todo)))
+                        s_hist[wave][lb] += static_cast<uint32_t>(__popcll(same));
+                    todo &= ~same;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < KPT; ++j)
+                if (j * 256 + tid < m && (key[j] & mask) == prefix)
+                    atomicAdd(&s_hist[wave][(key[j] >> shift) & 255u], 1u);
+        }
         __syncthreads();
         // the byte whose count range [below, below + count) holds rank want
         const uint32_t c = s_hist[0][tid] + s_hist[1][tid] + s_hist[2][tid] + s_hist[3][tid];
@@ -875,96 +874,27 @@ __global__ __launch_bounds__(256) void knn_sample_select_kernel(const float *__r
 }
 
 // Merge the sorted per-slice lists of each query into its best kTop
-// (sorted by distance, then id). One wave per query; lane l holds entry l
-// of the running list A. Each slice list B is loaded reversed, so A followed
-// by B is bitonic: a half-cleaner keeps the 64 smallest and a bitonic merge
-// sorts them, all in registers. Every member of the query's best 64 is at
-// most its published threshold, so slices whose best candidate is above it
-// are skipped; the next listed slice is loaded while the current one merges.
-__global__ __launch_bounds__(64) void knn_merge_kernel(const float *cand_d, const int32_t *cand_i,
-                                                       const int32_t *cand_n,
-                                                       const uint32_t *tau_key, int64_t nq,
-                                                       int slices, float *top_d, int32_t *top_i,
-                                                       int32_t *top_n) {
-    __shared__ int32_t s_list[kMaxSlices];
-    __shared__ int s_count;
-    const int64_t q = blockIdx.x;
-    const int lane = threadIdx.x;
-    if (q >= nq) return;
-    const float tau = key_float(tau_key[q]);
-    if (lane == 0) s_count = 0;
-    __syncthreads();
-    for (int s0 = 0; s0 < slices; s0 += 64) {
-        const int s = s0 + lane;
-        bool need = false;
-        if (s < slices) {
-            const int n = cand_n[q * slices + s];
-            need = n > 0 && cand_d[(q * slices + s) * kCap] <= tau;
-        }
-        const uint64_t m = __ballot(need);
-        if (need) s_list[s_count + __popcll(m & ((1ull << lane) - 1ull))] = s;
-        __syncthreads();
-        if (lane == 0) s_count += __popcll(m);
-        __syncthreads();
-    }
-    const int listed = s_count;
-    float ad = INFINITY;
-    int32_t ai = INT32_MAX;
-    int na = 0;
-    float next_d = INFINITY;
-    int32_t next_i = INT32_MAX;
-    int next_n = 0;
-    auto fetch = [&](int li) {
-        const int s = s_list[li];
-        next_n = cand_n[q * slices + s];
-        const int64_t base = (q * slices + s) * kCap;
-        const int r = kTop - 1 - lane;   // reversed
-        next_d = r < next_n ? cand_d[base + r] : INFINITY;
-        next_i = r < next_n ? cand_i[base + r] : INT32_MAX;
-    };
-    if (listed > 0) fetch(0);
-    for (int li = 0; li < listed; ++li) {
-        const float bd = next_d;
-        const int32_t bi = next_i;
-        const int nb = next_n;
-        if (li + 1 < listed) fetch(li + 1);
-        // half-cleaner: entry l against entry 64 + l of the bitonic A ++ reverse(B)
-        if (bd < ad || (bd == ad && bi < ai)) {
-            ad = bd;
-            ai = bi;
-        }
-#pragma unroll
-        for (int j = 32; j > 0; j >>= 1) {
-            const float dp = __shfl_xor(ad, j, 64);
-            const int32_t ip = __shfl_xor(ai, j, 64);
-            const bool mine_gt = ad > dp || (ad == dp && ai > ip);
-            const bool lower = (lane & j) == 0;
-            if (lower == mine_gt) {
-                ad = dp;
-                ai = ip;
-            }
-        }
-        na = min(kTop, na + nb);
-    }
-    top_d[q * kTop + lane] = ad;
-    top_i[q * kTop + lane] = ai;
-    if (lane == 0) top_n[q] = na;
-}
-
-// knn_merge_kernel for small batches: WAVES waves each merge every
-// WAVES-th listed slice, then the wave lists merge pairwise in log2(WAVES)
-// rounds. The best kTop under the total order (distance, id) does not
-// depend on the order of the merges, so the result equals
-// knn_merge_kernel's.
+// (sorted by distance, then id). One workgroup of WAVES waves per query:
+// the query's listed slices are dealt to the waves in turn; in a wave, lane
+// l holds entry l of the running list A, and each slice list B is loaded
+// reversed, so A followed by B is bitonic: a half-cleaner keeps the 64
+// smallest and a bitonic merge sorts them, all in registers. The wave lists
+// then merge pairwise in log2(WAVES) rounds. The best kTop under the total
+// order (distance, id) does not depend on the order of the merges. Every
+// member of the query's best 64 is at most its published threshold, so
+// slices whose best candidate is above it are not listed. A wave keeps
+// kMergeAhead slices in flight; a slice's 64 entries are read whatever its
+// count (words of its own buffer) and masked when merged, so a fetch is one
+// round trip.
 template <int WAVES>
-__global__ __launch_bounds__(64 * WAVES) void knn_merge_wide_kernel(const float *cand_d,
-                                                                    const int32_t *cand_i,
-                                                                    const int32_t *cand_n,
-                                                                    const uint32_t *tau_key,
-                                                                    int64_t nq, int slices,
-                                                                    float *top_d, int32_t *top_i,
-                                                                    int32_t *top_n) {
-    static_assert(WAVES >= 2 && (WAVES & (WAVES - 1)) == 0, "a power of two of waves");
+__global__ __launch_bounds__(64 * WAVES) void knn_merge_kernel(const float *cand_d,
+                                                               const int32_t *cand_i,
+                                                               const int32_t *cand_n,
+                                                               const uint32_t *tau_key, int64_t nq,
+                                                               int slices, float *top_d,
+                                                               int32_t *top_i, int32_t *top_n) {
+    static_assert((WAVES & (WAVES - 1)) == 0, "a power of two of waves");
+    constexpr int PF = kMergeAhead;
     __shared__ int32_t s_list[kMaxSlices];
     __shared__ int s_count;
     __shared__ float s_d[WAVES][kTop];
@@ -1004,25 +934,32 @@ __global__ __launch_bounds__(64 * WAVES) void knn_merge_wide_kernel(const float 
             }
         }
     };
-    float next_d = INFINITY;
-    int32_t next_i = INT32_MAX;
-    int next_n = 0;
-    auto fetch = [&](int li) {
-        const int sl = s_list[li];
-        next_n = cand_n[q * slices + sl];
-        const int64_t base = (q * slices + sl) * kCap;
-        const int r = kTop - 1 - lane;   // reversed
-        next_d = r < next_n ? cand_d[base + r] : INFINITY;
-        next_i = r < next_n ? cand_i[base + r] : INT32_MAX;
+    const int r = kTop - 1 - lane;   // this lane's entry of a reversed list
+    float pd[PF];
+    int32_t pi[PF];
+    int pn[PF];
+    auto fetch = [&](int li, int p) {
+        const int64_t cell = q * slices + s_list[li];
+        pn[p] = cand_n[cell];
+        pd[p] = cand_d[cell * kCap + r];
+        pi[p] = cand_i[cell * kCap + r];
     };
-    if (wave < listed) fetch(wave);
-    for (int li = wave; li < listed; li += WAVES) {
-        const float bd = next_d;
-        const int32_t bi = next_i;
-        const int nb = next_n;
-        if (li + WAVES < listed) fetch(li + WAVES);
-        merge_in(bd, bi);
-        na = min(kTop, na + nb);
+#pragma unroll
+    for (int p = 0; p < PF; ++p)
+        if (wave + p * WAVES < listed) fetch(wave + p * WAVES, p);
+    for (int li = wave; li < listed; li += PF * WAVES) {
+#pragma unroll
+        for (int p = 0; p < PF; ++p) {
+            const int lp = li + p * WAVES;
+            if (lp < listed) {
+                const int nb = pn[p];
+                const float bd = r < nb ? pd[p] : INFINITY;
+                const int32_t bi = r < nb ? pi[p] : INT32_MAX;
+                if (lp + PF * WAVES < listed) fetch(lp + PF * WAVES, p);
+                merge_in(bd, bi);
+                na = min(kTop, na + nb);
+            }
+        }
     }
     // pairwise rounds: wave w < h takes in the list of wave w + h
 #pragma unroll
@@ -1322,17 +1259,24 @@ void launch_scan(dim3 cells, hipStream_t s, const uint16_t *qb, int64_t nq, cons
         hipLaunchKernelGGL((knn_scan_small_kernel<KS>), grid, dim3(256), 0, s, qb, nq, rb, rnorm, nr,
                            per_slice, slices, cd, ci, cn, tk, exclude_self, qoff);
     else
-        hipLaunchKernelGGL((knn_scan_kernel<KS, kScanRing>), grid, dim3(256), 0, s, qb, nq, rb, rnorm,
-                           nr, per_slice, slices, cd, ci, cn, tk, exclude_self, qoff);
+        hipLaunchKernelGGL((knn_scan_kernel<KS, kScanRing, false>), grid, dim3(256), 0, s, qb, nq, rb,
+                           rnorm, nr, per_slice, slices, cd, ci, cn, tk, exclude_self, qoff,
+                           int64_t{0}, static_cast<float *>(nullptr));
 }
 
 template <int KS>
 void launch_sample_dist(dim3 grid, hipStream_t s, const uint16_t *qb, int64_t nq, const uint16_t *rb,
                         const float *rnorm, int64_t nr, int m, float *dist, int exclude_self,
                         int64_t qoff) {
-    // grid.y sample slices of m / grid.y references each
-    hipLaunchKernelGGL(knn_sample_dist_kernel<KS>, grid, dim3(256), 0, s, qb, nq, rb, rnorm, nr, m,
-                       static_cast<int>(m / grid.y), dist, exclude_self, qoff);
+    // grid.x query blocks by grid.y sample slices of m / grid.y entries
+    // each, through the scan's own pipeline (cells as in launch_scan)
+    const int64_t n_cells = static_cast<int64_t>(grid.x) * grid.y;
+    hipLaunchKernelGGL((knn_scan_kernel<KS, kScanRing, true>),
+                       dim3(static_cast<unsigned>(8 * ((n_cells + 7) / 8))), dim3(256), 0, s, qb, nq,
+                       rb, rnorm, nr, static_cast<int64_t>(m / grid.y), static_cast<int>(grid.y),
+                       static_cast<float *>(nullptr), static_cast<int32_t *>(nullptr),
+                       static_cast<int32_t *>(nullptr), static_cast<uint32_t *>(nullptr), exclude_self,
+                       qoff, static_cast<int64_t>(m), dist);
 }
 
 int ksteps_for(int64_t d) {
@@ -1484,15 +1428,15 @@ void KnnIndex::query(const float *q_dev, int64_t nq, int k, bool exclude_self, i
     }
     HIP_CHECK(hipGetLastError());
     if (nq <= kDeepMergeMaxQ)
-        hipLaunchKernelGGL(knn_merge_wide_kernel<16>, dim3(static_cast<unsigned>(nq)), dim3(1024), 0,
-                           s, cand_d_.ptr, cand_i_.ptr, cand_n_.ptr, tau_key_.ptr, nq, slices,
+        hipLaunchKernelGGL(knn_merge_kernel<16>, dim3(static_cast<unsigned>(nq)), dim3(1024), 0, s,
+                           cand_d_.ptr, cand_i_.ptr, cand_n_.ptr, tau_key_.ptr, nq, slices,
                            top_d_.ptr, top_i_.ptr, top_n_.ptr);
     else if (nq <= kWideMergeMaxQ)
-        hipLaunchKernelGGL(knn_merge_wide_kernel<4>, dim3(static_cast<unsigned>(nq)), dim3(256), 0, s,
+        hipLaunchKernelGGL(knn_merge_kernel<4>, dim3(static_cast<unsigned>(nq)), dim3(256), 0, s,
                            cand_d_.ptr, cand_i_.ptr, cand_n_.ptr, tau_key_.ptr, nq, slices,
                            top_d_.ptr, top_i_.ptr, top_n_.ptr);
     else
-        hipLaunchKernelGGL(knn_merge_kernel, dim3(static_cast<unsigned>(nq)), dim3(64), 0, s,
+        hipLaunchKernelGGL(knn_merge_kernel<1>, dim3(static_cast<unsigned>(nq)), dim3(64), 0, s,
                            cand_d_.ptr, cand_i_.ptr, cand_n_.ptr, tau_key_.ptr, nq, slices,
                            top_d_.ptr, top_i_.ptr, top_n_.ptr);
     HIP_CHECK(hipGetLastError());
