@@ -52,6 +52,7 @@
 
 #include "common.hpp"
 #include "knn.hpp"
+#include "wave.hpp"
 
 namespace umaphip {
 
@@ -67,6 +68,9 @@ constexpr int kTop = 64;      // candidates kept per query
 constexpr int kMaxSlices = 512;   // slices per query batch (KnnIndex::slices_for)
 constexpr int64_t kMinSliceRefs = 2048;   // fewest references per slice
 constexpr int kSample = 8192;      // references in the sample bound
+constexpr int kSampleLarge = 32768;   // and for batches of at most kLargeSampleMaxQ rows
+constexpr int64_t kLargeSampleMaxQ = 4096;
+constexpr int kSelectKeys = 32;       // sample keys per thread of the select
 constexpr int kSampleRefs = 16;    // fewest sample references per workgroup of the sample distances
 constexpr int kSampleMinSlices = 8;   // batches with at least this many slices use the bound
 constexpr int64_t kSampleMinRefs = 8 * kSample;
@@ -235,6 +239,29 @@ __device__ inline float sort_buffer(float (&d)[2], int32_t (&id)[2], int n, floa
     return n >= kTop ? kth : INFINITY;
 }
 
+// sort_buffer for n <= 64 entries, one per lane (lanes n.. hold +inf): a
+// bitonic network over the smallest power of two >= n lanes only, since the
+// final buffers of small batches hold a few entries each. Writes the same
+// 64 entries as sort_buffer.
+__device__ inline void sort_short(float d, int32_t id, int n, float *cd, int32_t *ci, int lane) {
+    int size = 2;
+    while (size < n) size <<= 1;
+    for (int k = 2; k <= size; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const float dp = __shfl_xor(d, j, 64);
+            const int32_t ip = __shfl_xor(id, j, 64);
+            const bool lower = (lane & j) == 0;     // lane is the lower entry of its pair
+            const bool asc = (lane & k) == 0;       // the pair sorts ascending
+            const bool mine_gt = d > dp || (d == dp && id > ip);
+            if (lower == asc ? mine_gt : !mine_gt) {
+                d = dp;
+                id = ip;
+            }
+        }
+    cd[lane] = d;
+    ci[lane] = id;
+}
+
 __device__ inline float compact_buffer(float *cd, int32_t *ci, int n, int lane) {
     float d[2];
     int32_t id[2];
@@ -273,7 +300,10 @@ __device__ inline void finish_cell(const int *s_cnt, int64_t q_block, int64_t nq
             load_buffer(cand_d + (q * slices + slice) * kCap, cand_i + (q * slices + slice) * kCap,
                         n, lane, nd, nid);
         }
-        if (cur_n > 0) sort_buffer(d, id, cur_n, cand_d + cur * kCap, cand_i + cur * kCap, lane);
+        if (cur_n > kTop)
+            sort_buffer(d, id, cur_n, cand_d + cur * kCap, cand_i + cur * kCap, lane);
+        else if (cur_n > 0)
+            sort_short(d[0], id[0], cur_n, cand_d + cur * kCap, cand_i + cur * kCap, lane);
         if (lane == 0) cand_n[cur] = cur_n < kTop ? cur_n : kTop;
     }
 }
@@ -796,18 +826,22 @@ __global__ __launch_bounds__(256, KSTEPS > 25 ? 1 : 2) void knn_scan_kernel(
     }
 }
 
-// Workgroup q: the kTop-th smallest of the query's m (at most kSample)
-// sample distances by a most-significant-byte-first radix select over their
-// order-preserving keys, widened by 2^-20 of its magnitude, becomes the
-// query's global threshold (+inf when fewer than kTop sample distances are
-// finite). The keys are read once into registers (kSample / 256 per
-// thread); each wave counts into its own histogram. The first (exponent)
-// byte takes only a few values, so there a wave adds each distinct byte
-// once (lanes grouped by ballot) instead of 64 atomics on one bin.
-__global__ __launch_bounds__(256) void knn_sample_select_kernel(const float *__restrict__ dist, int m,
-                                                                uint32_t *__restrict__ tau_key) {
-    constexpr int KPT = kSample / 256;
-    __shared__ uint32_t s_hist[4][256];
+// Workgroup q: the kTop-th smallest of the query's m (at most 64 WAVES
+// KPT) sample distances by a most-significant-byte-first radix select over
+// their order-preserving keys, widened by 2^-20 of its magnitude, becomes
+// the query's global threshold (+inf when fewer than kTop sample distances
+// are finite). The keys are read once into registers (KPT per thread);
+// each wave counts into its own histogram. The first (exponent) byte takes
+// only a few values, so there a wave adds each distinct byte once (lanes
+// grouped by ballot) instead of 64 atomics on one bin.
+template <int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void knn_sample_select_kernel(const float *__restrict__ dist,
+                                                                       int m,
+                                                                       uint32_t *__restrict__ tau_key) {
+    constexpr int T = 64 * WAVES;
+    constexpr int KPT = kSelectKeys;
+    static_assert(WAVES >= 4, "threads 0..255 own the 256 bins");
+    __shared__ uint32_t s_hist[WAVES][256];
     __shared__ uint32_t s_wsum[4];
     __shared__ uint32_t s_sel[2];   // chosen byte, rank left within it
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -816,48 +850,50 @@ __global__ __launch_bounds__(256) void knn_sample_select_kernel(const float *__r
     uint32_t key[KPT];
 #pragma unroll
     for (int j = 0; j < KPT; ++j) {
-        const int i = j * 256 + tid;
+        const int i = j * T + tid;
         key[j] = i < m ? float_key(row[i]) : 0u;
     }
     uint32_t prefix = 0u, mask = 0u, want = kTop - 1;   // 0-based rank of the kTop-th smallest
     for (int shift = 24; shift >= 0; shift -= 8) {
-#pragma unroll
-        for (int w = 0; w < 4; ++w) s_hist[w][tid] = 0u;
+        for (int e = tid; e < WAVES * 256; e += T) (&s_hist[0][0])[e] = 0u;
         __syncthreads();
         if (shift == 24) {
 #pragma unroll
             for (int j = 0; j < KPT; ++j) {
                 const uint32_t b = key[j] >> 24;
-                uint64_t todo = __builtin_amdgcn_ballot_w64(j * 256 + tid < m);
+                uint64_t todo = __builtin_amdgcn_ballot_w64(j * T + tid < m);
                 while (todo) {
                     const uint32_t lb = __builtin_amdgcn_readlane(b, static_cast<int>(__builtin_ctzll(todo)));
                     const uint64_t same = todo & __builtin_amdgcn_ballot_w64(b == lb);
                     if (lane == static_cast<int>(__builtin_ctzll(todo)))
-                        s_hist[wave][lb] += static_cast<uint32_t>(__popcll(same));
+                        atomicAdd(&s_hist[wave][lb], static_cast<uint32_t>(__popcll(same)));
                     todo &= ~same;
                 }
             }
         } else {
 #pragma unroll
             for (int j = 0; j < KPT; ++j)
-                if (j * 256 + tid < m && (key[j] & mask) == prefix)
+                if (j * T + tid < m && (key[j] & mask) == prefix)
                     atomicAdd(&s_hist[wave][(key[j] >> shift) & 255u], 1u);
         }
         __syncthreads();
         // the byte whose count range [below, below + count) holds rank want
-        const uint32_t c = s_hist[0][tid] + s_hist[1][tid] + s_hist[2][tid] + s_hist[3][tid];
-        uint32_t incl = c;
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t up = __shfl_up(incl, o, 64);
-            if (lane >= o) incl += up;
+        // (threads 0..255, one bin each)
+        uint32_t c = 0u, incl = 0u;
+        if (tid < 256) {
+#pragma unroll
+            for (int w = 0; w < WAVES; ++w) c += s_hist[w][tid];
+            incl = static_cast<uint32_t>(wave_inclusive_sum(static_cast<int>(c)));
+            if (lane == 63) s_wsum[wave] = incl;
         }
-        if (lane == 63) s_wsum[wave] = incl;
         __syncthreads();
-        for (int w = 0; w < wave; ++w) incl += s_wsum[w];
-        const uint32_t below = incl - c;
-        if (c > 0u && below <= want && want < incl) {
-            s_sel[0] = static_cast<uint32_t>(tid);
-            s_sel[1] = want - below;
+        if (tid < 256) {
+            for (int w = 0; w < wave; ++w) incl += s_wsum[w];
+            const uint32_t below = incl - c;
+            if (c > 0u && below <= want && want < incl) {
+                s_sel[0] = static_cast<uint32_t>(tid);
+                s_sel[1] = want - below;
+            }
         }
         __syncthreads();
         prefix |= s_sel[0] << shift;
@@ -1336,6 +1372,15 @@ bool KnnIndex::uses_sample(int64_t q) const {
     return n_ >= kSampleMinRefs && slices_for(q) >= kSampleMinSlices;
 }
 
+int KnnIndex::sample_size(int64_t q) const {
+    // the larger sample (a bound about 4x tighter, so fewer survivors to
+    // append, sort and merge) where its distances stay cheap next to the scan
+    static_assert(kSample == 256 * kSelectKeys && kSampleLarge == 1024 * kSelectKeys,
+                  "the select holds the sample in registers");
+    return q <= kLargeSampleMaxQ && n_ >= 8 * static_cast<int64_t>(kSampleLarge) ? kSampleLarge
+                                                                               : kSample;
+}
+
 int KnnIndex::slices_for(int64_t q) const {
     const int64_t blocks = (q + kTQ - 1) / kTQ;
     // Enough workgroups to fill the chip (256 CUs, two resident each), but
@@ -1364,7 +1409,9 @@ void KnnIndex::reserve(int64_t q) {
     int64_t sq = 0;
     for (int64_t qq = q; qq >= 1 && sq == 0; --qq)
         if (uses_sample(qq)) sq = qq;
-    if (sq > 0) sample_d_.alloc(static_cast<size_t>(sq) * kSample);
+    if (sq > 0)
+        sample_d_.alloc(static_cast<size_t>(std::max<int64_t>(
+            sq * kSample, std::min<int64_t>(sq, kLargeSampleMaxQ) * sample_size(1))));
     qbf_.alloc(static_cast<size_t>(q * dp_));
     qnorm_.alloc(static_cast<size_t>(q));
     qerr_.alloc(static_cast<size_t>(q));
@@ -1395,23 +1442,28 @@ void KnnIndex::query(const float *q_dev, int64_t nq, int k, bool exclude_self, i
     if (uses_sample(nq)) {
         // every query's global threshold starts at its sample bound; about
         // 512 workgroups, each loading its queries' fragments once for
-        // kSample / (sample slices) references
+        // m / (sample slices) references
+        const int m = sample_size(nq);
         const int64_t qblocks = (nq + kTQ - 1) / kTQ;
-        int sample_slices = kSample / kSampleRefs;
+        int sample_slices = m / kSampleRefs;
         while (sample_slices > 1 && qblocks * sample_slices > 512) sample_slices /= 2;
         const dim3 sgrid(static_cast<unsigned>(qblocks), static_cast<unsigned>(sample_slices));
         switch (ks_) {
-            case 1: launch_sample_dist<1>(sgrid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, kSample, sample_d_.ptr, ex, query_offset); break;
-            case 2: launch_sample_dist<2>(sgrid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, kSample, sample_d_.ptr, ex, query_offset); break;
-            case 4: launch_sample_dist<4>(sgrid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, kSample, sample_d_.ptr, ex, query_offset); break;
-            case 8: launch_sample_dist<8>(sgrid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, kSample, sample_d_.ptr, ex, query_offset); break;
-            case 16: launch_sample_dist<16>(sgrid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, kSample, sample_d_.ptr, ex, query_offset); break;
-            case 25: launch_sample_dist<25>(sgrid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, kSample, sample_d_.ptr, ex, query_offset); break;
-            default: launch_sample_dist<32>(sgrid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, kSample, sample_d_.ptr, ex, query_offset); break;
+            case 1: launch_sample_dist<1>(sgrid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, m, sample_d_.ptr, ex, query_offset); break;
+            case 2: launch_sample_dist<2>(sgrid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, m, sample_d_.ptr, ex, query_offset); break;
+            case 4: launch_sample_dist<4>(sgrid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, m, sample_d_.ptr, ex, query_offset); break;
+            case 8: launch_sample_dist<8>(sgrid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, m, sample_d_.ptr, ex, query_offset); break;
+            case 16: launch_sample_dist<16>(sgrid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, m, sample_d_.ptr, ex, query_offset); break;
+            case 25: launch_sample_dist<25>(sgrid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, m, sample_d_.ptr, ex, query_offset); break;
+            default: launch_sample_dist<32>(sgrid, s, qbf_.ptr, nq, ref_bf_.ptr, rnorm_.ptr, n_, m, sample_d_.ptr, ex, query_offset); break;
         }
         HIP_CHECK(hipGetLastError());
-        hipLaunchKernelGGL(knn_sample_select_kernel, dim3(static_cast<unsigned>(nq)), dim3(256), 0, s,
-                           sample_d_.ptr, kSample, tau_key_.ptr);
+        if (m == kSampleLarge)
+            hipLaunchKernelGGL(knn_sample_select_kernel<16>, dim3(static_cast<unsigned>(nq)), dim3(1024),
+                               0, s, sample_d_.ptr, m, tau_key_.ptr);
+        else
+            hipLaunchKernelGGL(knn_sample_select_kernel<4>, dim3(static_cast<unsigned>(nq)), dim3(256), 0,
+                               s, sample_d_.ptr, m, tau_key_.ptr);
     } else {
         hipLaunchKernelGGL(fill_u32_kernel, dim3(static_cast<unsigned>((nq + 255) / 256)), dim3(256), 0,
                            s, tau_key_.ptr, nq, 0xFF800000u);   // key of +inf

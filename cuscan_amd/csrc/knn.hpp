@@ -30,6 +30,7 @@ public:
     int slices_for(int64_t q) const;
     // Whether a batch of q rows seeds its thresholds from the sample bound.
     bool uses_sample(int64_t q) const;
+    int sample_size(int64_t q) const;
     // Queries of the last query() that the certificate left open and the
     // exact FP32 rescan answered (synchronises s).
     int64_t rescans(hipStream_t s) const;

@@ -13,6 +13,7 @@
 
 #include "common.hpp"
 #include "layout_kernels.hpp"
+#include "wave.hpp"
 #include "host_parallel.hpp"
 
 namespace umaphip {
@@ -900,19 +901,6 @@ __device__ inline void seg_scan_step(float &sx, float &sy, int rid) {
         sx = sx + ux;
         sy = sy + uy;
     }
-}
-
-// Inclusive prefix sum over the wave's 64 lanes through DPP (row_shr:1..8
-// within 16-lane rows, then row_bcast:15 and row_bcast:31 across them): a
-// few cycles per step instead of the LDS round trips of lane permutes.
-__device__ inline int wave_inclusive_sum(int v) {
-    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);   // row_shr:1
-    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);   // row_shr:2
-    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);   // row_shr:4
-    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);   // row_shr:8
-    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);   // row_bcast:15 into rows 1, 3
-    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);   // row_bcast:31 into rows 2, 3
-    return v;
 }
 
 // Adds the segmented (by row) inclusive sums of one 64-term group into the

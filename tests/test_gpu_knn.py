@@ -150,6 +150,30 @@ def test_sample_bound_bit_exact(q):
     idx.close()
 
 
+@pytest.mark.parametrize("q", [1, 100, 1024])
+def test_large_sample_bound_bit_exact(q):
+    """262,200 references (at least 8 x 32,768): batches of up to 1,024 rows
+    seed their thresholds from a 32,768-row sample. The merged prefilter
+    lists equal those of a 9,472-row batch (7 slices, no bound), so the
+    tighter bound drops no candidate of the top 64, and the results stay
+    bit-identical to the exact kernel."""
+    from cuscan_amd.knn import KnnIndex
+
+    ref = image_like(262_200, 32, 10, 31)
+    qry = image_like(9472, 32, 10, 32)
+    print(f"[gpu] knn large sample bound q={q}", flush=True)
+    idx = KnnIndex(ref, max_queries=9472)
+    gi, gd = idx.query(qry[:q], 15)
+    ei, ed = O.exact_knn(qry[:q], ref, 15, threads=8)
+    assert (gi == ei).all() and (gd.view(np.uint32) == ed.view(np.uint32)).all()
+    qt = torch.from_numpy(qry).cuda()
+    small_ids, small_n = idx.candidates_device(qt[:q])
+    big_ids, big_n = idx.candidates_device(qt)
+    assert torch.equal(small_n, big_n[:q]) and torch.equal(small_ids, big_ids[:q])
+    assert (small_n == 64).all()
+    idx.close()
+
+
 def test_sample_bound_exclude_self_with_offset():
     """Fit-graph style queries (rows of the reference set itself, excluding
     self) through the sample bound, for a batch that starts at row 1,000."""
