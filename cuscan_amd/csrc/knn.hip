@@ -77,6 +77,8 @@ constexpr int64_t kSampleMinRefs = 8 * kSample;
 constexpr int64_t kWideMergeMaxQ = 8192;   // batches up to this many rows merge with 4 waves
 constexpr int64_t kDeepMergeMaxQ = 512;    // and up to this many with 16
 constexpr int kMergeAhead = 4;             // slice lists in flight per merging wave
+constexpr int kRerankSlots = 16;           // features per lane per row of the wide rerank
+constexpr int64_t kRerankWideMaxQ = 512;   // batches up to this many rows rerank wide
 constexpr int kMomentRows = 2048;          // rows per workgroup of the column moments
 constexpr int kExactQB = 4;                // queries per workgroup of the exact rescan
 constexpr int64_t kExactRound = 1024;      // uncertified queries per rescan round
@@ -1017,62 +1019,18 @@ __global__ __launch_bounds__(64 * WAVES) void knn_merge_kernel(const float *cand
     if (lane == 0) top_n[q] = na;
 }
 
-// Exact rerank: one workgroup of four waves per query, one lane of wave 0
-// per candidate; the distance is the reference's sequential f32 sum over the
-// features (no contraction in this build). The four waves stage a
-// 128-feature block of all 64 candidate rows through LDS at once (32 loads
-// in flight per lane, every load part of a contiguous row segment), then
-// wave 0 adds the block in feature order.
-__global__ __launch_bounds__(256) void knn_rerank_kernel(const float *query, int64_t nq,
-                                                              const float *ref, int64_t nr,
-                                                              int64_t d, const int32_t *top_i,
-                                                              const int32_t *top_n, int k,
-                                                              int32_t *out_i, float *out_d,
-                                                              float *kth_d2) {
-    constexpr int FB = 128;
-    __shared__ float s_q[FB];
-    __shared__ float s_r[kTop][FB + 1];
-    const int64_t q = blockIdx.x;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    if (q >= nq) return;
-    const int n = top_n[q];
-    const int32_t id = lane < n ? top_i[q * kTop + lane] : INT32_MAX;
-    float acc = 0.0f;
-    for (int64_t f0 = 0; f0 < d; f0 += FB) {
-        const int fb = static_cast<int>(min<int64_t>(FB, d - f0));
-        __syncthreads();
-        if (tid < FB) s_q[tid] = tid < fb ? query[q * d + f0 + tid] : 0.0f;
-        // wave w: candidates 16w .. 16w + 15, features lane and lane + 64
-        float v[16][2];
-#pragma unroll
-        for (int u = 0; u < 16; ++u) {
-            const int c = 16 * wave + u;
-            const int64_t row = __shfl(id, c, 64);
-            v[u][0] = (c < n && lane < fb) ? ref[row * d + f0 + lane] : 0.0f;
-            v[u][1] = (c < n && lane + 64 < fb) ? ref[row * d + f0 + 64 + lane] : 0.0f;
-        }
-#pragma unroll
-        for (int u = 0; u < 16; ++u) {
-            const int c = 16 * wave + u;
-            if (c < n) {
-                s_r[c][lane] = v[u][0];
-                s_r[c][lane + 64] = v[u][1];
-            }
-        }
-        __syncthreads();
-        if (wave == 0 && lane < n) {
-            for (int f = 0; f < fb; ++f) {
-                const float delta = s_q[f] - s_r[lane][f];
-                acc += delta * delta;
-            }
-        }
-    }
-    if (wave != 0) return;
+// The reranked k of one query from wave 0's lanes (lane c: candidate c, its
+// exact squared distance `acc`): ranks by (distance, id), the k best in
+// order with their square-rooted distances, the k-th squared distance for
+// the certificate, and sentinels (id nr, distance sqrt(FLT_MAX)) past n.
+__device__ inline void rerank_output(float acc, int32_t id, int n, int lane, int64_t q, int k,
+                                     int64_t nr, int32_t *out_i, float *out_d, float *kth_d2) {
     const float dist = lane < n ? acc : INFINITY;
     int rank = 0;
+#pragma unroll 8
     for (int f = 0; f < 64; ++f) {
-        const float df = __shfl(dist, f, 64);
-        const int32_t idf = __shfl(id, f, 64);
+        const float df = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(dist), f));
+        const int32_t idf = __builtin_amdgcn_readlane(id, f);
         rank += (df < dist || (df == dist && idf < id)) ? 1 : 0;
     }
     if (lane < n && rank < k) {
@@ -1085,6 +1043,127 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(const float *query, int
         out_i[q * k + lane] = static_cast<int32_t>(nr);
         out_d[q * k + lane] = sqrtf(FLT_MAX);
     }
+}
+
+// Exact rerank: one workgroup of four waves per query, one lane of wave 0
+// per candidate; the distance is the reference's sequential f32 sum over the
+// features (no contraction in this build). The four waves load a
+// 128-feature block of all 64 candidate rows at once (32 loads in flight
+// per lane, every load part of a contiguous row segment) and put the
+// squared differences (q - r)^2, each rounded to f32 as the sum's own
+// product is, into LDS; wave 0 then only adds the block in feature order.
+__global__ __launch_bounds__(256) void knn_rerank_kernel(const float *query, int64_t nq,
+                                                              const float *ref, int64_t nr,
+                                                              int64_t d, const int32_t *top_i,
+                                                              const int32_t *top_n, int k,
+                                                              int32_t *out_i, float *out_d,
+                                                              float *kth_d2) {
+    constexpr int FB = 128;
+    __shared__ float s_r[kTop][FB + 1];
+    const int64_t q = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (q >= nq) return;
+    const int n = top_n[q];
+    const int32_t id = lane < n ? top_i[q * kTop + lane] : INT32_MAX;
+    float acc = 0.0f;
+    for (int64_t f0 = 0; f0 < d; f0 += FB) {
+        const int fb = static_cast<int>(min<int64_t>(FB, d - f0));
+        // wave w: candidates 16w .. 16w + 15, features lane and lane + 64
+        const float q0 = lane < fb ? query[q * d + f0 + lane] : 0.0f;
+        const float q1 = lane + 64 < fb ? query[q * d + f0 + 64 + lane] : 0.0f;
+        float v[16][2];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int c = 16 * wave + u;
+            const int64_t row = __shfl(id, c, 64);
+            v[u][0] = (c < n && lane < fb) ? ref[row * d + f0 + lane] : 0.0f;
+            v[u][1] = (c < n && lane + 64 < fb) ? ref[row * d + f0 + 64 + lane] : 0.0f;
+        }
+        __syncthreads();   // wave 0 is done with the previous block
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int c = 16 * wave + u;
+            if (c < n) {
+                const float e0 = q0 - v[u][0], e1 = q1 - v[u][1];
+                s_r[c][lane] = e0 * e0;
+                s_r[c][lane + 64] = e1 * e1;
+            }
+        }
+        __syncthreads();
+        if (wave == 0 && lane < n) {
+#pragma unroll 8
+            for (int f = 0; f < fb; ++f) acc += s_r[lane][f];
+        }
+    }
+    if (wave != 0) return;
+    rerank_output(acc, id, n, lane, q, k, nr, out_i, out_d, kth_d2);
+}
+
+// knn_rerank_kernel for small batches with d <= 64 kRerankSlots: sixteen
+// waves load all 64 candidate rows at once (wave w: candidates 4w .. 4w + 3,
+// lane l: features l + 64 t), so the rows arrive in one round trip instead
+// of one per 128-feature block. They pass through LDS in 512-feature
+// phases as squared differences, and wave 0 adds each candidate's features
+// in order with one accumulator, exactly the operations of
+// knn_rerank_kernel.
+__global__ __launch_bounds__(1024) void knn_rerank_wide_kernel(const float *query, int64_t nq,
+                                                               const float *ref, int64_t nr,
+                                                               int64_t d, const int32_t *top_i,
+                                                               const int32_t *top_n, int k,
+                                                               int32_t *out_i, float *out_d,
+                                                               float *kth_d2) {
+    constexpr int PH = 512;                 // features per LDS phase
+    constexpr int TPP = PH / 64;            // register slots per phase
+    constexpr int PHASES = kRerankSlots / TPP;
+    __shared__ float s_r[kTop][PH + 1];
+    const int64_t q = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (q >= nq) return;
+    const int n = top_n[q];
+    const int32_t id = lane < n ? top_i[q * kTop + lane] : INT32_MAX;
+    float qv[kRerankSlots];
+#pragma unroll
+    for (int t = 0; t < kRerankSlots; ++t) {
+        const int64_t f = lane + 64 * t;
+        qv[t] = f < d ? query[q * d + f] : 0.0f;
+    }
+    float v[4][kRerankSlots];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int c = 4 * wave + u;
+        const int64_t row = __shfl(id, c, 64);
+#pragma unroll
+        for (int t = 0; t < kRerankSlots; ++t) {
+            const int64_t f = lane + 64 * t;
+            v[u][t] = (c < n && f < d) ? ref[row * d + f] : 0.0f;
+        }
+    }
+    float acc = 0.0f;
+#pragma unroll
+    for (int ph = 0; ph < PHASES; ++ph) {
+        const int64_t f0 = static_cast<int64_t>(ph) * PH;
+        if (f0 >= d) break;
+        const int fb = static_cast<int>(min<int64_t>(PH, d - f0));
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int c = 4 * wave + u;
+            if (c < n) {
+#pragma unroll
+                for (int t = 0; t < TPP; ++t) {
+                    const float e = qv[ph * TPP + t] - v[u][ph * TPP + t];
+                    s_r[c][t * 64 + lane] = e * e;
+                }
+            }
+        }
+        __syncthreads();
+        if (wave == 0 && lane < n) {
+#pragma unroll 8
+            for (int f = 0; f < fb; ++f) acc += s_r[lane][f];
+        }
+    }
+    if (wave != 0) return;
+    rerank_output(acc, id, n, lane, q, k, nr, out_i, out_d, kth_d2);
 }
 
 // Certificate of one query's reranked top k (one lane per query).
@@ -1500,13 +1579,18 @@ void KnnIndex::query(const float *q_dev, int64_t nq, int k, bool exclude_self, i
                                  hipMemcpyDeviceToDevice, s));
         return;
     }
-    // one four-wave workgroup per query: its staging rounds are the
-    // latency at small batches, and it also has more loads in flight per CU
-    // at large ones (87 ns per query with one wave at 16,384 rows)
+    // one four-wave workgroup per query: it also has more loads in flight
+    // per CU at large batches (87 ns per query with one wave at 16,384
+    // rows); small batches load each query's rows in one round trip
     const float *q_exact = q_f32_for_rerank ? q_f32_for_rerank : q_dev;
-    hipLaunchKernelGGL(knn_rerank_kernel, dim3(static_cast<unsigned>(nq)), dim3(256), 0, s,
-                       q_exact, nq, ref_.ptr, n_, d_, top_i_.ptr, top_n_.ptr, k, idx_out, dist_out,
-                       kth_.ptr);
+    if (nq <= kRerankWideMaxQ && d_ <= 64 * kRerankSlots)
+        hipLaunchKernelGGL(knn_rerank_wide_kernel, dim3(static_cast<unsigned>(nq)), dim3(1024), 0, s,
+                           q_exact, nq, ref_.ptr, n_, d_, top_i_.ptr, top_n_.ptr, k, idx_out,
+                           dist_out, kth_.ptr);
+    else
+        hipLaunchKernelGGL(knn_rerank_kernel, dim3(static_cast<unsigned>(nq)), dim3(256), 0, s,
+                           q_exact, nq, ref_.ptr, n_, d_, top_i_.ptr, top_n_.ptr, k, idx_out,
+                           dist_out, kth_.ptr);
     HIP_CHECK(hipGetLastError());
     // certificate, then the exact rescan of the queries it leaves open
     hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(1), 0, s, fail_count_.ptr);
