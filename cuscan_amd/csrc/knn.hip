@@ -930,7 +930,8 @@ __global__ __launch_bounds__(64 * WAVES) void knn_merge_kernel(const float *cand
                                                                const int32_t *cand_n,
                                                                const uint32_t *tau_key, int64_t nq,
                                                                int slices, float *top_d,
-                                                               int32_t *top_i, int32_t *top_n) {
+                                                               int32_t *top_i, int32_t *top_n,
+                                                               int32_t *open_count) {
     static_assert((WAVES & (WAVES - 1)) == 0, "a power of two of waves");
     constexpr int PF = kMergeAhead;
     __shared__ int32_t s_list[kMaxSlices];
@@ -940,6 +941,7 @@ __global__ __launch_bounds__(64 * WAVES) void knn_merge_kernel(const float *cand
     __shared__ int s_na[WAVES];
     const int64_t q = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (q == 0 && tid == 0) *open_count = 0;   // the rerank counts the uncertified queries
     if (q >= nq) return;
     const float tau = key_float(tau_key[q]);
     if (tid == 0) s_count = 0;
@@ -1019,12 +1021,60 @@ __global__ __launch_bounds__(64 * WAVES) void knn_merge_kernel(const float *cand
     if (lane == 0) top_n[q] = na;
 }
 
+// Certificate of one query's reranked top k.
+// g bounds the relative error of an f32 sum of up to dp products or
+// squares, doubled (2 dp u / (1 - dp u), u = 2^-24). With x~ / y~ the BF16
+// rows (x - s) / (y - s), every reference left out of the 64 has an
+// approximate distance dt >= T, the 64th. |dt - (|y~|^2 - 2 x~.y~)| <=
+// c (|y~|^2 + 2 |x~| |y~|) with c = g + 2u, and |y~| <= |x~| + t for
+// t = |x~ - y~|, so t >= t_lo, the positive root of
+// (1 + c) t^2 + 4 c X t = X2l + T - 3 c X^2 (X, X2l: upper and lower bounds
+// of |x~| and |x~|^2). The rounding of the rows is at most err(x) and
+// eps |y~| <= eps (X + t), so |x - y| >= t_lo (1 - eps) - err(x) - eps X = L,
+// and the reference's f32 distance of y is at least L^2 (1 - g). When that
+// exceeds the k-th reranked distance, no left-out reference can enter the
+// top k (ties included). The rerank appends uncertified queries to
+// fail_list.
+__device__ inline bool certified(int n, float T_f, float kth_f, float qn_f, float qerr_f,
+                                 uint32_t eps_bits, double g, int force) {
+    if (force) return false;
+    if (n < kTop) return true;   // every eligible reference was reranked
+    const double u = 0x1.0p-24, c = g + 2.0 * u;
+    const double T = T_f;
+    const double qn = qn_f;
+    const double x2l = qn * (1.0 - g), xu = sqrt(qn * (1.0 + g));
+    const double K = x2l + T - 3.0 * c * xu * xu;
+    double tlo = 0.0;
+    if (K > 0.0)
+        tlo = (sqrt(16.0 * c * c * xu * xu + 4.0 * (1.0 + c) * K) - 4.0 * c * xu) / (2.0 * (1.0 + c));
+    const double eps = __uint_as_float(eps_bits);
+    const double L = tlo * (1.0 - eps) - static_cast<double>(qerr_f) - eps * xu;
+    const double kth = kth_f;
+    return eps < 1.0 && L > 0.0 && L * L * (1.0 - g) * (1.0 - 0x1.0p-40) > kth;
+}
+
+// Everything the rerank needs besides the rows: the certificate's inputs
+// and the list of queries it leaves open.
+struct CertifyArgs {
+    const float *top_d;
+    const float *qnorm;
+    const float *qerr;
+    const uint32_t *eps_key;
+    double g;
+    int force;
+    int32_t *fail_list;
+    int32_t *fail_count;
+};
+
 // The reranked k of one query from wave 0's lanes (lane c: candidate c, its
 // exact squared distance `acc`): ranks by (distance, id), the k best in
 // order with their square-rooted distances, the k-th squared distance for
-// the certificate, and sentinels (id nr, distance sqrt(FLT_MAX)) past n.
+// the exact rescan, and sentinels (id nr, distance sqrt(FLT_MAX)) past n.
+// Then the query's certificate; an uncertified query is appended to
+// fail_list.
 __device__ inline void rerank_output(float acc, int32_t id, int n, int lane, int64_t q, int k,
-                                     int64_t nr, int32_t *out_i, float *out_d, float *kth_d2) {
+                                     int64_t nr, int32_t *out_i, float *out_d, float *kth_d2,
+                                     const CertifyArgs &ca) {
     const float dist = lane < n ? acc : INFINITY;
     int rank = 0;
 #pragma unroll 8
@@ -1043,6 +1093,13 @@ __device__ inline void rerank_output(float acc, int32_t id, int n, int lane, int
         out_i[q * k + lane] = static_cast<int32_t>(nr);
         out_d[q * k + lane] = sqrtf(FLT_MAX);
     }
+    const uint64_t at_k = __builtin_amdgcn_ballot_w64(lane < n && rank == k - 1);
+    const float kth = at_k ? __int_as_float(__builtin_amdgcn_readlane(
+                                 __float_as_int(dist), static_cast<int>(__builtin_ctzll(at_k))))
+                           : INFINITY;
+    if (lane == 0 && !certified(n, ca.top_d[q * kTop + kTop - 1], kth, ca.qnorm[q], ca.qerr[q],
+                                *ca.eps_key, ca.g, ca.force))
+        ca.fail_list[atomicAdd(ca.fail_count, 1)] = static_cast<int32_t>(q);
 }
 
 // Exact rerank: one workgroup of four waves per query, one lane of wave 0
@@ -1057,7 +1114,7 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(const float *query, int
                                                               int64_t d, const int32_t *top_i,
                                                               const int32_t *top_n, int k,
                                                               int32_t *out_i, float *out_d,
-                                                              float *kth_d2) {
+                                                              float *kth_d2, CertifyArgs ca) {
     constexpr int FB = 128;
     __shared__ float s_r[kTop][FB + 1];
     const int64_t q = blockIdx.x;
@@ -1096,7 +1153,7 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(const float *query, int
         }
     }
     if (wave != 0) return;
-    rerank_output(acc, id, n, lane, q, k, nr, out_i, out_d, kth_d2);
+    rerank_output(acc, id, n, lane, q, k, nr, out_i, out_d, kth_d2, ca);
 }
 
 // knn_rerank_kernel for small batches with d <= 64 kRerankSlots: sixteen
@@ -1111,7 +1168,7 @@ __global__ __launch_bounds__(1024) void knn_rerank_wide_kernel(const float *quer
                                                                int64_t d, const int32_t *top_i,
                                                                const int32_t *top_n, int k,
                                                                int32_t *out_i, float *out_d,
-                                                               float *kth_d2) {
+                                                               float *kth_d2, CertifyArgs ca) {
     constexpr int PH = 512;                 // features per LDS phase
     constexpr int TPP = PH / 64;            // register slots per phase
     constexpr int PHASES = kRerankSlots / TPP;
@@ -1163,50 +1220,9 @@ __global__ __launch_bounds__(1024) void knn_rerank_wide_kernel(const float *quer
         }
     }
     if (wave != 0) return;
-    rerank_output(acc, id, n, lane, q, k, nr, out_i, out_d, kth_d2);
+    rerank_output(acc, id, n, lane, q, k, nr, out_i, out_d, kth_d2, ca);
 }
 
-// Certificate of one query's reranked top k (one lane per query).
-// g bounds the relative error of an f32 sum of up to dp products or
-// squares, doubled (2 dp u / (1 - dp u), u = 2^-24). With x~ / y~ the BF16
-// rows (x - s) / (y - s), every reference left out of the 64 has an
-// approximate distance dt >= T, the 64th. |dt - (|y~|^2 - 2 x~.y~)| <=
-// c (|y~|^2 + 2 |x~| |y~|) with c = g + 2u, and |y~| <= |x~| + t for
-// t = |x~ - y~|, so t >= t_lo, the positive root of
-// (1 + c) t^2 + 4 c X t = X2l + T - 3 c X^2 (X, X2l: upper and lower bounds
-// of |x~| and |x~|^2). The rounding of the rows is at most err(x) and
-// eps |y~| <= eps (X + t), so |x - y| >= t_lo (1 - eps) - err(x) - eps X = L,
-// and the reference's f32 distance of y is at least L^2 (1 - g). When that
-// exceeds the k-th reranked distance, no left-out reference can enter the
-// top k (ties included). Uncertified queries are appended to fail_list.
-__global__ void knn_certify_kernel(const float *top_d, const int32_t *top_n, const float *qnorm,
-                                   const float *qerr, const uint32_t *eps_key,
-                                   const float *kth_d2, int64_t nq, double g, int force,
-                                   int32_t *fail_list, int32_t *fail_count) {
-    const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (q >= nq) return;
-    bool ok = false;
-    if (!force) {
-        if (top_n[q] < kTop) {
-            ok = true;   // every eligible reference was reranked
-        } else {
-            const double u = 0x1.0p-24, c = g + 2.0 * u;
-            const double T = top_d[q * kTop + kTop - 1];
-            const double qn = qnorm[q];
-            const double x2l = qn * (1.0 - g), xu = sqrt(qn * (1.0 + g));
-            const double K = x2l + T - 3.0 * c * xu * xu;
-            double tlo = 0.0;
-            if (K > 0.0)
-                tlo = (sqrt(16.0 * c * c * xu * xu + 4.0 * (1.0 + c) * K) - 4.0 * c * xu) /
-                      (2.0 * (1.0 + c));
-            const double eps = __uint_as_float(*eps_key);
-            const double L = tlo * (1.0 - eps) - static_cast<double>(qerr[q]) - eps * xu;
-            const double kth = kth_d2[q];
-            ok = eps < 1.0 && L > 0.0 && L * L * (1.0 - g) * (1.0 - 0x1.0p-40) > kth;
-        }
-    }
-    if (!ok) fail_list[atomicAdd(fail_count, 1)] = static_cast<int32_t>(q);
-}
 
 // Exact rescan of the uncertified queries fail_list[base + slot] for slots
 // of this round (kExactQB per workgroup; blockIdx.y = a slice of the
@@ -1354,8 +1370,6 @@ __global__ __launch_bounds__(64) void knn_exact_merge_kernel(
         out_d[q * k + lane] = lane < na ? sqrtf(ad) : sqrtf(FLT_MAX);
     }
 }
-
-__global__ void zero_i32_kernel(int32_t *p) { *p = 0; }
 
 // 2 n u / (1 - n u) for f32 sums of n terms, doubled.
 double sum_bound(int64_t n) {
@@ -1561,15 +1575,15 @@ void KnnIndex::query(const float *q_dev, int64_t nq, int k, bool exclude_self, i
     if (nq <= kDeepMergeMaxQ)
         hipLaunchKernelGGL(knn_merge_kernel<16>, dim3(static_cast<unsigned>(nq)), dim3(1024), 0, s,
                            cand_d_.ptr, cand_i_.ptr, cand_n_.ptr, tau_key_.ptr, nq, slices,
-                           top_d_.ptr, top_i_.ptr, top_n_.ptr);
+                           top_d_.ptr, top_i_.ptr, top_n_.ptr, fail_count_.ptr);
     else if (nq <= kWideMergeMaxQ)
         hipLaunchKernelGGL(knn_merge_kernel<4>, dim3(static_cast<unsigned>(nq)), dim3(256), 0, s,
                            cand_d_.ptr, cand_i_.ptr, cand_n_.ptr, tau_key_.ptr, nq, slices,
-                           top_d_.ptr, top_i_.ptr, top_n_.ptr);
+                           top_d_.ptr, top_i_.ptr, top_n_.ptr, fail_count_.ptr);
     else
         hipLaunchKernelGGL(knn_merge_kernel<1>, dim3(static_cast<unsigned>(nq)), dim3(64), 0, s,
                            cand_d_.ptr, cand_i_.ptr, cand_n_.ptr, tau_key_.ptr, nq, slices,
-                           top_d_.ptr, top_i_.ptr, top_n_.ptr);
+                           top_d_.ptr, top_i_.ptr, top_n_.ptr, fail_count_.ptr);
     HIP_CHECK(hipGetLastError());
     if (cand_ids) {
         // prefilter lists only (best first), no rerank
@@ -1582,22 +1596,20 @@ void KnnIndex::query(const float *q_dev, int64_t nq, int k, bool exclude_self, i
     // one four-wave workgroup per query: it also has more loads in flight
     // per CU at large batches (87 ns per query with one wave at 16,384
     // rows); small batches load each query's rows in one round trip
+    // with each query's certificate (the merge cleared the open count)
     const float *q_exact = q_f32_for_rerank ? q_f32_for_rerank : q_dev;
+    const CertifyArgs ca{top_d_.ptr,      qnorm_.ptr,         qerr_.ptr,      eps_key_.ptr,
+                         sum_bound(dp_), force_rescan_ ? 1 : 0, fail_list_.ptr, fail_count_.ptr};
     if (nq <= kRerankWideMaxQ && d_ <= 64 * kRerankSlots)
         hipLaunchKernelGGL(knn_rerank_wide_kernel, dim3(static_cast<unsigned>(nq)), dim3(1024), 0, s,
                            q_exact, nq, ref_.ptr, n_, d_, top_i_.ptr, top_n_.ptr, k, idx_out,
-                           dist_out, kth_.ptr);
+                           dist_out, kth_.ptr, ca);
     else
         hipLaunchKernelGGL(knn_rerank_kernel, dim3(static_cast<unsigned>(nq)), dim3(256), 0, s,
                            q_exact, nq, ref_.ptr, n_, d_, top_i_.ptr, top_n_.ptr, k, idx_out,
-                           dist_out, kth_.ptr);
+                           dist_out, kth_.ptr, ca);
     HIP_CHECK(hipGetLastError());
-    // certificate, then the exact rescan of the queries it leaves open
-    hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(1), 0, s, fail_count_.ptr);
-    hipLaunchKernelGGL(knn_certify_kernel, dim3(static_cast<unsigned>((nq + 255) / 256)), dim3(256),
-                       0, s, top_d_.ptr, top_n_.ptr, qnorm_.ptr, qerr_.ptr, eps_key_.ptr, kth_.ptr,
-                       nq, sum_bound(dp_), force_rescan_ ? 1 : 0, fail_list_.ptr, fail_count_.ptr);
-    HIP_CHECK(hipGetLastError());
+    // the exact rescan of the queries the certificate leaves open
     const int64_t round = std::min<int64_t>(nq, kExactRound);
     int64_t open = nq;
     if (nq > kWideMergeMaxQ) {
