@@ -642,6 +642,7 @@ __global__ __launch_bounds__(256, KSTEPS > 25 ? 1 : 2) void knn_scan_kernel(
 
     const int tid = threadIdx.x;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    __builtin_assume(wave >= 0 && wave < 4);   // lost by readfirstlane; simplifies the DMA predicates
     int64_t q_block64 = 0;
     int slice = 0;
     if (!scan_cell(nq, slices, q_block64, slice)) return;
@@ -753,16 +754,17 @@ __global__ __launch_bounds__(256, KSTEPS > 25 ? 1 : 2) void knn_scan_kernel(
         if constexpr (SAMPLE) {
             const int e = chunk + (lane & 15);
             if (e < r_end) {
-                const int64_t ref = ref_of(e);
+                // the entry's own query, counted from the lane's first query
+                const int64_t self0 = ref_of(e) - query_offset - (q_block + ql_lane);
+                float *out = sample_dist + static_cast<int64_t>(q_block + ql_lane) * sample_m + e;
 #pragma unroll
                 for (int qa = 0; qa < 2; ++qa)
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
-                        const int q = q_block + ql_lane + qa * 16 + i;
-                        if (q < qnq) {
+                        if (q_block + ql_lane + qa * 16 + i < qnq) {
                             const float dt = rn - 2.0f * acc[qa][i];
-                            const bool excluded = exclude_self && ref == query_offset + q;
-                            sample_dist[static_cast<int64_t>(q) * sample_m + e] = excluded ? INFINITY : dt;
+                            const bool excluded = exclude_self && self0 == qa * 16 + i;
+                            out[static_cast<int64_t>(qa * 16 + i) * sample_m] = excluded ? INFINITY : dt;
                         }
                     }
             }
