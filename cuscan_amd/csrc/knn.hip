@@ -23,12 +23,13 @@
 // infinite threshold keeps its first 128 references and sorts its buffer
 // about every 64 survivors until its threshold tightens, and a wave does
 // that one query at a time, so small batches spent most of the scan there.
-// Before such a scan, the approximate distances of every query to a strided
-// sample of 8192 references are computed with the scan's own MFMA
-// arithmetic, and each query's 64th best of them seeds its global
-// threshold: 64 real references lie at or below it, so it bounds the
-// query's final 64th best from above and no candidate of the exact top 64
-// is dropped. With it a slice of 4096 references keeps about 32.
+// Before such a scan, the approximate distances of every query to a sample
+// of the references (8,192, or 32,768 for batches up to 4,096 rows, in
+// blocks of 16 consecutive rows spread evenly over the references) are
+// computed by the scan kernel itself, and each query's 64th best of them
+// seeds its global threshold: 64 real references lie at or below it, so it
+// bounds the query's final 64th best from above and no candidate of the
+// exact top 64 is dropped.
 //
 // Centring and the certificate: the BF16 copies hold x - s for a per-feature
 // shift s near the references' mean (distances do not change), so the
@@ -622,10 +623,12 @@ __global__ __launch_bounds__(256, 1) void knn_scan_small_kernel(
 // run one wave per SIMD (512 VGPRs).
 //
 // SAMPLE: the sample distances instead. The rows are the sample entries
-// 0 .. sample_m - 1 (entry i is reference i * nr / sample_m), sliced like
-// the references, and every (query, entry) value is written to
-// sample_dist[q * sample_m + i] (+inf for an excluded self pair), so each
-// equals the value the scan computes for the same (query, reference).
+// 0 .. sample_m - 1, in blocks of 16: entry i is reference
+// (i / 16) nr / (sample_m / 16) + i mod 16, so a chunk is 16 consecutive
+// rows as in the scan. They are sliced like the references, and every
+// (query, entry) value is written to sample_dist[q * sample_m + i] (+inf
+// for an excluded self pair), so each equals the value the scan computes
+// for the same (query, reference).
 template <int KSTEPS, int RING, bool SAMPLE>
 __global__ __launch_bounds__(256, KSTEPS > 25 ? 1 : 2) void knn_scan_kernel(
     const uint16_t *__restrict__ qb, int64_t nq, const uint16_t *__restrict__ rb,
@@ -653,7 +656,8 @@ __global__ __launch_bounds__(256, KSTEPS > 25 ? 1 : 2) void knn_scan_kernel(
     const int qnq = static_cast<int>(nq);
     // the reference of row r
     auto ref_of = [&](int r) -> int {
-        return SAMPLE ? static_cast<int>(static_cast<int64_t>(r) * nr / sample_m) : r;
+        return SAMPLE ? static_cast<int>(static_cast<int64_t>(r / kTR) * nr / (sample_m / kTR)) + r % kTR
+                      : r;
     };
 
     // DMA lane mapping (row and source word of this lane's 16 bytes)
@@ -709,6 +713,24 @@ __global__ __launch_bounds__(256, KSTEPS > 25 ? 1 : 2) void knn_scan_kernel(
     const int b_row = lane & 15;
     const uint16_t *const Bl0 = sB + b_row * 32 + (((lane >> 4) ^ ((b_row >> 2) & 3)) * 8);
 
+    // SAMPLE: a chunk's values are written in the next chunk, after its DMA
+    // is issued, so that the wait at the top of a chunk (vmcnt(0)) finds the
+    // stores as far along as the DMA instead of just issued
+    float pend[2][4] = {{0.0f, 0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f, 0.0f}};
+    int pend_chunk = -1;
+    auto write_pending = [&]() {
+        const int e = pend_chunk + (lane & 15);
+        if (pend_chunk >= 0 && e < r_end) {
+            float *out = sample_dist + static_cast<int64_t>(q_block + ql_lane) * sample_m + e;
+#pragma unroll
+            for (int qa = 0; qa < 2; ++qa)
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    if (q_block + ql_lane + qa * 16 + i < qnq)
+                        out[static_cast<int64_t>(qa * 16 + i) * sample_m] = pend[qa][i];
+        }
+    };
+
     int since = 0, buf = 0;
     for (int chunk = r_begin; chunk < r_end; chunk += kTR, buf ^= 1) {
         __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): this lane's loads of the current buffer
@@ -727,6 +749,7 @@ __global__ __launch_bounds__(256, KSTEPS > 25 ? 1 : 2) void knn_scan_kernel(
             const int nref = chunk + kTR + (lane & 15);
             rn_next = nref < r_end ? rnorm[ref_of(nref)] : 0.0f;
         }
+        if constexpr (SAMPLE) write_pending();
         const uint16_t *Bl = Bl0 + buf * BUF;
         floatx4 acc[2] = {floatx4{0.0f, 0.0f, 0.0f, 0.0f}, floatx4{0.0f, 0.0f, 0.0f, 0.0f}};
         uint4 ring[RING];
@@ -752,22 +775,17 @@ __global__ __launch_bounds__(256, KSTEPS > 25 ? 1 : 2) void knn_scan_kernel(
             if (ks + RING < KSTEPS) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
         }
         if constexpr (SAMPLE) {
-            const int e = chunk + (lane & 15);
-            if (e < r_end) {
-                // the entry's own query, counted from the lane's first query
-                const int64_t self0 = ref_of(e) - query_offset - (q_block + ql_lane);
-                float *out = sample_dist + static_cast<int64_t>(q_block + ql_lane) * sample_m + e;
+            // the entry's own query, counted from the lane's first query
+            const int64_t self0 = ref_of(min(chunk + (lane & 15), r_end - 1)) - query_offset -
+                                  (q_block + ql_lane);
 #pragma unroll
-                for (int qa = 0; qa < 2; ++qa)
+            for (int qa = 0; qa < 2; ++qa)
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        if (q_block + ql_lane + qa * 16 + i < qnq) {
-                            const float dt = rn - 2.0f * acc[qa][i];
-                            const bool excluded = exclude_self && self0 == qa * 16 + i;
-                            out[static_cast<int64_t>(qa * 16 + i) * sample_m] = excluded ? INFINITY : dt;
-                        }
-                    }
-            }
+                for (int i = 0; i < 4; ++i) {
+                    const float dt = rn - 2.0f * acc[qa][i];
+                    pend[qa][i] = exclude_self && self0 == qa * 16 + i ? INFINITY : dt;
+                }
+            pend_chunk = chunk;
             continue;
         }
         const int ref = chunk + (lane & 15);
@@ -823,7 +841,9 @@ __global__ __launch_bounds__(256, KSTEPS > 25 ? 1 : 2) void knn_scan_kernel(
             }
         }
     }
-    if constexpr (!SAMPLE) {
+    if constexpr (SAMPLE) {
+        write_pending();
+    } else {
         __threadfence_block();
         __syncthreads();
         finish_cell(s_cnt, q_block, nq, slices, slice, cand_d, cand_i, cand_n, wave, lane);
