@@ -629,7 +629,10 @@ __global__ __launch_bounds__(256, 1) void knn_scan_small_kernel(
 // (query, entry) value is written to sample_dist[q * sample_m + i] (+inf
 // for an excluded self pair), so each equals the value the scan computes
 // for the same (query, reference).
-template <int KSTEPS, int RING, bool SAMPLE>
+//
+// EXCL: the self pairs may need excluding (exclude_self); without it the
+// keep test is one compare per (query, reference).
+template <int KSTEPS, int RING, bool SAMPLE, bool EXCL>
 __global__ __launch_bounds__(256, KSTEPS > 25 ? 1 : 2) void knn_scan_kernel(
     const uint16_t *__restrict__ qb, int64_t nq, const uint16_t *__restrict__ rb,
     const float *__restrict__ rnorm, int64_t nr, int64_t per_slice, int slices,
@@ -704,7 +707,9 @@ __global__ __launch_bounds__(256, KSTEPS > 25 ? 1 : 2) void knn_scan_kernel(
         s_tau[wave * 32 + lane] = q < qnq ? key_float(tau_key[q]) : -INFINITY;
         s_cnt[wave * 32 + lane] = 0;
     }
-    float rn_next = 0.0f;
+    // a row past the slice gets a NaN norm: its distances fail every keep test
+    const float kNoRow = __builtin_nanf("");
+    float rn_next = kNoRow;
     if (r_begin + (lane & 15) < r_end) rn_next = rnorm[ref_of(r_begin + (lane & 15))];
     // lane's C rows: queries ql_lane + qa * 16 + i
     const int ql_lane = wave * 32 + 4 * (lane >> 4);
@@ -747,7 +752,7 @@ __global__ __launch_bounds__(256, KSTEPS > 25 ? 1 : 2) void knn_scan_kernel(
         if (chunk + kTR < r_end) {
             stage(buf ^ 1, chunk + kTR);
             const int nref = chunk + kTR + (lane & 15);
-            rn_next = nref < r_end ? rnorm[ref_of(nref)] : 0.0f;
+            rn_next = nref < r_end ? rnorm[ref_of(nref)] : kNoRow;
         }
         if constexpr (SAMPLE) write_pending();
         const uint16_t *Bl = Bl0 + buf * BUF;
@@ -789,8 +794,7 @@ __global__ __launch_bounds__(256, KSTEPS > 25 ? 1 : 2) void knn_scan_kernel(
             continue;
         }
         const int ref = chunk + (lane & 15);
-        const bool ref_ok = ref < r_end;
-        const int self_d = exclude_self ? ref - self_base : -1;
+        const int self_d = EXCL && exclude_self ? ref - self_base : -1;
         bool any = false;
 #pragma unroll
         for (int qa = 0; qa < 2; ++qa) {
@@ -799,7 +803,7 @@ __global__ __launch_bounds__(256, KSTEPS > 25 ? 1 : 2) void knn_scan_kernel(
             for (int i = 0; i < 4; ++i) {
                 const int ql = ql_lane + qa * 16 + i;
                 const float dt = rn - 2.0f * acc[qa][i];
-                const bool keep = ref_ok && dt <= tau[i] && self_d != qa * 16 + i;
+                const bool keep = dt <= tau[i] && (!EXCL || self_d != qa * 16 + i);
                 const uint64_t m = __builtin_amdgcn_ballot_w64(keep);
                 if (m) {
                     const int base_n = s_cnt[ql];
@@ -1409,9 +1413,13 @@ void launch_scan(dim3 cells, hipStream_t s, const uint16_t *qb, int64_t nq, cons
     if (n_cells <= 256)
         hipLaunchKernelGGL((knn_scan_small_kernel<KS>), grid, dim3(256), 0, s, qb, nq, rb, rnorm, nr,
                            per_slice, slices, cd, ci, cn, tk, exclude_self, qoff);
+    else if (exclude_self)
+        hipLaunchKernelGGL((knn_scan_kernel<KS, kScanRing, false, true>), grid, dim3(256), 0, s, qb, nq,
+                           rb, rnorm, nr, per_slice, slices, cd, ci, cn, tk, exclude_self, qoff,
+                           int64_t{0}, static_cast<float *>(nullptr));
     else
-        hipLaunchKernelGGL((knn_scan_kernel<KS, kScanRing, false>), grid, dim3(256), 0, s, qb, nq, rb,
-                           rnorm, nr, per_slice, slices, cd, ci, cn, tk, exclude_self, qoff,
+        hipLaunchKernelGGL((knn_scan_kernel<KS, kScanRing, false, false>), grid, dim3(256), 0, s, qb,
+                           nq, rb, rnorm, nr, per_slice, slices, cd, ci, cn, tk, exclude_self, qoff,
                            int64_t{0}, static_cast<float *>(nullptr));
 }
 
@@ -1422,7 +1430,7 @@ void launch_sample_dist(dim3 grid, hipStream_t s, const uint16_t *qb, int64_t nq
     // grid.x query blocks by grid.y sample slices of m / grid.y entries
     // each, through the scan's own pipeline (cells as in launch_scan)
     const int64_t n_cells = static_cast<int64_t>(grid.x) * grid.y;
-    hipLaunchKernelGGL((knn_scan_kernel<KS, kScanRing, true>),
+    hipLaunchKernelGGL((knn_scan_kernel<KS, kScanRing, true, true>),
                        dim3(static_cast<unsigned>(8 * ((n_cells + 7) / 8))), dim3(256), 0, s, qb, nq,
                        rb, rnorm, nr, static_cast<int64_t>(m / grid.y), static_cast<int>(grid.y),
                        static_cast<float *>(nullptr), static_cast<int32_t *>(nullptr),
