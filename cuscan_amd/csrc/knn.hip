@@ -279,7 +279,7 @@ __device__ inline float compact_buffer(float *cd, int32_t *ci, int n, int lane) 
 // every wave's appends (global and s_cnt) are visible to the block.
 __device__ inline void finish_cell(const int *s_cnt, int64_t q_block, int64_t nq, int slices,
                                    int slice, float *cand_d, int32_t *cand_i, int32_t *cand_n,
-                                   int wave, int lane) {
+                                   int wave, int lane, int bq = kTQ) {
     int ql = wave;
     int64_t q = q_block + ql;
     int n = q < nq ? s_cnt[ql] : 0;
@@ -297,7 +297,7 @@ __device__ inline void finish_cell(const int *s_cnt, int64_t q_block, int64_t nq
         }
         ql += 4;
         q += 4;
-        if (ql >= kTQ) q = nq;   // past the block
+        if (ql >= bq) q = nq;   // past the block
         if (q < nq) {
             n = s_cnt[ql];
             load_buffer(cand_d + (q * slices + slice) * kCap, cand_i + (q * slices + slice) * kCap,
@@ -379,14 +379,21 @@ __device__ __forceinline__ void put_chunk(const u32x4 (&src)[LW], float rn, uint
 // (b mod 8) * ceil(cells / 8) + b / 8, so each XCD runs one contiguous run
 // of cells and the query blocks of a slice stream that slice's references
 // through the same L2 together. Returns false for the padding workgroups.
-__device__ inline bool scan_cell(int64_t nq, int slices, int64_t &q_block, int &slice) {
-    const int64_t qblocks = (nq + kTQ - 1) / kTQ;
+// Query rows of a block of kTQ = 128 across the four waves: MFMA row r of
+// sub-block qa on wave w is query qa * 64 + 16 w + r, so a batch of fewer
+// than 128 rows still gives every wave its share of the survivors. Lane
+// l < 32 of wave w owns the thresholds and counts of query own_query(w, l).
+__device__ inline int own_query(int wave, int l) { return (l >> 4) * 64 + wave * 16 + (l & 15); }
+
+__device__ inline bool scan_cell(int64_t nq, int slices, int64_t &q_block, int &slice,
+                                 int bq = kTQ) {
+    const int64_t qblocks = (nq + bq - 1) / bq;
     const int64_t cells = qblocks * slices;
     const int64_t chunk = (cells + 7) / 8;
     const int64_t cell = (blockIdx.x % 8) * chunk + blockIdx.x / 8;
     if (cell >= cells) return false;
     slice = static_cast<int>(cell / qblocks);
-    q_block = (cell % qblocks) * kTQ;
+    q_block = (cell % qblocks) * bq;
     return true;
 }
 
@@ -432,7 +439,7 @@ __global__ __launch_bounds__(256, 1) void knn_scan_small_kernel(
     bf16x8 A[2][KSTEPS];
 #pragma unroll
     for (int qa = 0; qa < 2; ++qa) {
-        const int64_t row = q_block + wave * 32 + qa * 16 + (lane & 15);
+        const int64_t row = q_block + qa * 64 + wave * 16 + (lane & 15);
 #pragma unroll
         for (int ks = 0; ks < KSTEPS; ++ks) {
             if (row < nq) {
@@ -443,11 +450,12 @@ __global__ __launch_bounds__(256, 1) void knn_scan_small_kernel(
             }
         }
     }
+    const int ql_own = own_query(wave, lane & 31);
     if (lane < 32) {
         // the query's global threshold so far (+inf, or its sample bound)
-        const int64_t q = q_block + wave * 32 + lane;
-        s_tau[wave * 32 + lane] = q < nq ? key_float(tau_key[q]) : INFINITY;
-        s_cnt[wave * 32 + lane] = 0;
+        const int64_t q = q_block + ql_own;
+        s_tau[ql_own] = q < nq ? key_float(tau_key[q]) : INFINITY;
+        s_cnt[ql_own] = 0;
     }
     // C layout: lane holds rows 4*(lane/16)+i (queries), column lane%16
     // (reference). A candidate is kept when dt <= tau, tau being the lower
@@ -458,7 +466,7 @@ __global__ __launch_bounds__(256, 1) void knn_scan_small_kernel(
     for (int qa = 0; qa < 2; ++qa)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const int64_t q = q_block + wave * 32 + qa * 16 + 4 * (lane >> 4) + i;
+            const int64_t q = q_block + qa * 64 + wave * 16 + 4 * (lane >> 4) + i;
             tau_r[qa][i] = q < nq ? key_float(tau_key[q]) : INFINITY;
         }
     // the query fragments and thresholds have landed before the chunk loads
@@ -489,16 +497,16 @@ __global__ __launch_bounds__(256, 1) void knn_scan_small_kernel(
         if (++since == kRefresh) {
             // thresholds other slices have published for this wave's queries
             since = 0;
-            const int64_t q = q_block + wave * 32 + (lane & 31);
+            const int64_t q = q_block + ql_own;
             if (lane < 32 && q < nq) {
                 const float g = key_float(tau_key[q]);
-                if (g < s_tau[wave * 32 + lane]) s_tau[wave * 32 + lane] = g;
+                if (g < s_tau[ql_own]) s_tau[ql_own] = g;
             }
 #pragma unroll
             for (int qa = 0; qa < 2; ++qa)
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
-                    tau_r[qa][i] = s_tau[wave * 32 + qa * 16 + 4 * (lane >> 4) + i];
+                    tau_r[qa][i] = s_tau[qa * 64 + wave * 16 + 4 * (lane >> 4) + i];
         }
         const float rn = s_rn[buf][lane & 15];
         // registers: chunks + kTR .. + SETS kTR; chunk + kTR goes to the other
@@ -527,7 +535,7 @@ __global__ __launch_bounds__(256, 1) void knn_scan_small_kernel(
         for (int qa = 0; qa < 2; ++qa)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const int ql = wave * 32 + qa * 16 + 4 * (lane >> 4) + i;
+                const int ql = qa * 64 + wave * 16 + 4 * (lane >> 4) + i;
                 const int64_t q = q_block + ql;
                 const float dt = rn - 2.0f * acc[qa][i];
                 const bool keep = ref < r_end && q < nq && dt <= tau_r[qa][i] &&
@@ -550,14 +558,14 @@ __global__ __launch_bounds__(256, 1) void knn_scan_small_kernel(
             }
         if (any) {
             // this wave's buffers that could overflow with the next chunk
-            const int n_l = lane < 32 ? s_cnt[wave * 32 + lane] : 0;
+            const int n_l = lane < 32 ? s_cnt[ql_own] : 0;
             uint64_t full = __ballot(lane < 32 && n_l > kCap - kTR);
             if (full) {
                 __threadfence_block();   // the wave's appends are visible to all its lanes
                 while (full) {
                     const int l = __builtin_ctzll(full);
                     full &= full - 1;
-                    const int ql = wave * 32 + l;
+                    const int ql = own_query(wave, l);
                     const int64_t q = q_block + ql;
                     const int n = __shfl(n_l, l, 64);
                     const int64_t base = (q * slices + slice) * kCap;
@@ -574,7 +582,7 @@ __global__ __launch_bounds__(256, 1) void knn_scan_small_kernel(
                 for (int qa = 0; qa < 2; ++qa)
 #pragma unroll
                     for (int i = 0; i < 4; ++i)
-                        tau_r[qa][i] = s_tau[wave * 32 + qa * 16 + 4 * (lane >> 4) + i];
+                        tau_r[qa][i] = s_tau[qa * 64 + wave * 16 + 4 * (lane >> 4) + i];
             }
         }
     };
@@ -632,7 +640,11 @@ __global__ __launch_bounds__(256, 1) void knn_scan_small_kernel(
 //
 // EXCL: the self pairs may need excluding (exclude_self); without it the
 // keep test is one compare per (query, reference).
-template <int KSTEPS, int RING, bool SAMPLE, bool EXCL>
+//
+// QA: 16-query sub-blocks per wave, so a block holds 64 QA queries. Batches
+// of at most 64 rows take QA = 1: half the A fragments and half the MFMAs
+// per chunk, and three workgroups fit a CU.
+template <int KSTEPS, int RING, bool SAMPLE, bool EXCL, int QA>
 __global__ __launch_bounds__(256, KSTEPS > 25 ? 1 : 2) void knn_scan_kernel(
     const uint16_t *__restrict__ qb, int64_t nq, const uint16_t *__restrict__ rb,
     const float *__restrict__ rnorm, int64_t nr, int64_t per_slice, int slices,
@@ -642,6 +654,8 @@ __global__ __launch_bounds__(256, KSTEPS > 25 ? 1 : 2) void knn_scan_kernel(
     constexpr int DP = KSTEPS * 32;
     constexpr int BUF = KSTEPS * 512;   // bf16 per buffer
     constexpr int kRefresh = 32;
+    constexpr int BQ = 64 * QA;         // queries per block
+    static_assert(QA == 1 || QA == 2, "one or two sub-blocks per wave");
     __shared__ __attribute__((aligned(16))) uint16_t sB[2 * BUF];
     __shared__ __attribute__((aligned(16))) float s_tau[kTQ];
     __shared__ int s_cnt[kTQ];
@@ -651,7 +665,7 @@ __global__ __launch_bounds__(256, KSTEPS > 25 ? 1 : 2) void knn_scan_kernel(
     __builtin_assume(wave >= 0 && wave < 4);   // lost by readfirstlane; simplifies the DMA predicates
     int64_t q_block64 = 0;
     int slice = 0;
-    if (!scan_cell(nq, slices, q_block64, slice)) return;
+    if (!scan_cell(nq, slices, q_block64, slice, BQ)) return;
     const int q_block = static_cast<int>(q_block64);
     const int inr = static_cast<int>(SAMPLE ? sample_m : nr);   // rows: references or entries
     const int r_begin = static_cast<int>(slice * per_slice);
@@ -687,10 +701,10 @@ __global__ __launch_bounds__(256, KSTEPS > 25 ? 1 : 2) void knn_scan_kernel(
     };
     if (r_begin < r_end) stage(0, r_begin);
 
-    bf16x8 A[2][KSTEPS];
+    bf16x8 A[QA][KSTEPS];
 #pragma unroll
-    for (int qa = 0; qa < 2; ++qa) {
-        const int64_t row = q_block + wave * 32 + qa * 16 + (lane & 15);
+    for (int qa = 0; qa < QA; ++qa) {
+        const int64_t row = q_block + qa * 64 + wave * 16 + (lane & 15);
 #pragma unroll
         for (int ks = 0; ks < KSTEPS; ++ks) {
             if (row < nq) {
@@ -701,18 +715,20 @@ __global__ __launch_bounds__(256, KSTEPS > 25 ? 1 : 2) void knn_scan_kernel(
             }
         }
     }
-    if (!SAMPLE && lane < 32) {
+    const int ql_own = own_query(wave, lane & 31);
+    constexpr int kOwners = 16 * QA;    // lanes that own a query's threshold and count
+    if (!SAMPLE && lane < kOwners) {
         // queries past nq get -inf: no distance passes dt <= -inf
-        const int q = q_block + wave * 32 + lane;
-        s_tau[wave * 32 + lane] = q < qnq ? key_float(tau_key[q]) : -INFINITY;
-        s_cnt[wave * 32 + lane] = 0;
+        const int q = q_block + ql_own;
+        s_tau[ql_own] = q < qnq ? key_float(tau_key[q]) : -INFINITY;
+        s_cnt[ql_own] = 0;
     }
     // a row past the slice gets a NaN norm: its distances fail every keep test
     const float kNoRow = __builtin_nanf("");
     float rn_next = kNoRow;
     if (r_begin + (lane & 15) < r_end) rn_next = rnorm[ref_of(r_begin + (lane & 15))];
-    // lane's C rows: queries ql_lane + qa * 16 + i
-    const int ql_lane = wave * 32 + 4 * (lane >> 4);
+    // lane's C rows: queries ql_lane + qa * 64 + i
+    const int ql_lane = wave * 16 + 4 * (lane >> 4);
     const int self_base = static_cast<int>(query_offset) + q_block + ql_lane;
     // B fragment: row lane & 15, word lane >> 4 of each K block
     const int b_row = lane & 15;
@@ -721,18 +737,18 @@ __global__ __launch_bounds__(256, KSTEPS > 25 ? 1 : 2) void knn_scan_kernel(
     // SAMPLE: a chunk's values are written in the next chunk, after its DMA
     // is issued, so that the wait at the top of a chunk (vmcnt(0)) finds the
     // stores as far along as the DMA instead of just issued
-    float pend[2][4] = {{0.0f, 0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f, 0.0f}};
+    float pend[QA][4] = {};
     int pend_chunk = -1;
     auto write_pending = [&]() {
         const int e = pend_chunk + (lane & 15);
         if (pend_chunk >= 0 && e < r_end) {
             float *out = sample_dist + static_cast<int64_t>(q_block + ql_lane) * sample_m + e;
 #pragma unroll
-            for (int qa = 0; qa < 2; ++qa)
+            for (int qa = 0; qa < QA; ++qa)
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
-                    if (q_block + ql_lane + qa * 16 + i < qnq)
-                        out[static_cast<int64_t>(qa * 16 + i) * sample_m] = pend[qa][i];
+                    if (q_block + ql_lane + qa * 64 + i < qnq)
+                        out[static_cast<int64_t>(qa * 64 + i) * sample_m] = pend[qa][i];
         }
     };
 
@@ -742,10 +758,10 @@ __global__ __launch_bounds__(256, KSTEPS > 25 ? 1 : 2) void knn_scan_kernel(
         __syncthreads();                      // every lane's loads landed; the other buffer is free
         if (!SAMPLE && ++since == kRefresh) {
             since = 0;
-            const int q = q_block + wave * 32 + (lane & 31);
-            if (lane < 32 && q < qnq) {
+            const int q = q_block + ql_own;
+            if (lane < kOwners && q < qnq) {
                 const float g = key_float(tau_key[q]);
-                if (g < s_tau[wave * 32 + lane]) s_tau[wave * 32 + lane] = g;
+                if (g < s_tau[ql_own]) s_tau[ql_own] = g;
             }
         }
         const float rn = rn_next;
@@ -756,7 +772,9 @@ __global__ __launch_bounds__(256, KSTEPS > 25 ? 1 : 2) void knn_scan_kernel(
         }
         if constexpr (SAMPLE) write_pending();
         const uint16_t *Bl = Bl0 + buf * BUF;
-        floatx4 acc[2] = {floatx4{0.0f, 0.0f, 0.0f, 0.0f}, floatx4{0.0f, 0.0f, 0.0f, 0.0f}};
+        floatx4 acc[QA];
+#pragma unroll
+        for (int qa = 0; qa < QA; ++qa) acc[qa] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
         uint4 ring[RING];
 #pragma unroll
         for (int r = 0; r < RING; ++r)
@@ -766,7 +784,7 @@ __global__ __launch_bounds__(256, KSTEPS > 25 ? 1 : 2) void knn_scan_kernel(
             const bf16x8 bfrag = __builtin_bit_cast(bf16x8, ring[ks % RING]);
             if (ks + RING < KSTEPS) ring[ks % RING] = *reinterpret_cast<const uint4 *>(Bl + (ks + RING) * 512);
 #pragma unroll
-            for (int qa = 0; qa < 2; ++qa)
+            for (int qa = 0; qa < QA; ++qa)
                 acc[qa] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[qa][ks], bfrag, acc[qa], 0, 0, 0);
         }
         // issue order: the RING first reads, then per K step its two MFMAs
@@ -776,7 +794,7 @@ __global__ __launch_bounds__(256, KSTEPS > 25 ? 1 : 2) void knn_scan_kernel(
         for (int r = 0; r < RING; ++r) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
 #pragma unroll
         for (int ks = 0; ks < KSTEPS; ++ks) {
-            __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+            __builtin_amdgcn_sched_group_barrier(0x008, QA, 0);
             if (ks + RING < KSTEPS) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
         }
         if constexpr (SAMPLE) {
@@ -784,11 +802,11 @@ __global__ __launch_bounds__(256, KSTEPS > 25 ? 1 : 2) void knn_scan_kernel(
             const int64_t self0 = ref_of(min(chunk + (lane & 15), r_end - 1)) - query_offset -
                                   (q_block + ql_lane);
 #pragma unroll
-            for (int qa = 0; qa < 2; ++qa)
+            for (int qa = 0; qa < QA; ++qa)
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     const float dt = rn - 2.0f * acc[qa][i];
-                    pend[qa][i] = exclude_self && self0 == qa * 16 + i ? INFINITY : dt;
+                    pend[qa][i] = exclude_self && self0 == qa * 64 + i ? INFINITY : dt;
                 }
             pend_chunk = chunk;
             continue;
@@ -797,13 +815,13 @@ __global__ __launch_bounds__(256, KSTEPS > 25 ? 1 : 2) void knn_scan_kernel(
         const int self_d = EXCL && exclude_self ? ref - self_base : -1;
         bool any = false;
 #pragma unroll
-        for (int qa = 0; qa < 2; ++qa) {
-            const floatx4 tau = *reinterpret_cast<const floatx4 *>(&s_tau[ql_lane + qa * 16]);
+        for (int qa = 0; qa < QA; ++qa) {
+            const floatx4 tau = *reinterpret_cast<const floatx4 *>(&s_tau[ql_lane + qa * 64]);
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const int ql = ql_lane + qa * 16 + i;
+                const int ql = ql_lane + qa * 64 + i;
                 const float dt = rn - 2.0f * acc[qa][i];
-                const bool keep = dt <= tau[i] && (!EXCL || self_d != qa * 16 + i);
+                const bool keep = dt <= tau[i] && (!EXCL || self_d != qa * 64 + i);
                 const uint64_t m = __builtin_amdgcn_ballot_w64(keep);
                 if (m) {
                     const int base_n = s_cnt[ql];
@@ -822,14 +840,14 @@ __global__ __launch_bounds__(256, KSTEPS > 25 ? 1 : 2) void knn_scan_kernel(
             }
         }
         if (any) {
-            const int n_l = lane < 32 ? s_cnt[wave * 32 + lane] : 0;
-            uint64_t full = __builtin_amdgcn_ballot_w64(lane < 32 && n_l > kCap - kTR);
+            const int n_l = lane < kOwners ? s_cnt[ql_own] : 0;
+            uint64_t full = __builtin_amdgcn_ballot_w64(lane < kOwners && n_l > kCap - kTR);
             if (full) {
                 __threadfence_block();
                 while (full) {
                     const int l = __builtin_ctzll(full);
                     full &= full - 1;
-                    const int ql = wave * 32 + l;
+                    const int ql = own_query(wave, l);
                     const int q = q_block + ql;
                     const int n = __shfl(n_l, l, 64);
                     const int64_t base = (static_cast<int64_t>(q) * slices + slice) * kCap;
@@ -850,7 +868,7 @@ __global__ __launch_bounds__(256, KSTEPS > 25 ? 1 : 2) void knn_scan_kernel(
     } else {
         __threadfence_block();
         __syncthreads();
-        finish_cell(s_cnt, q_block, nq, slices, slice, cand_d, cand_i, cand_n, wave, lane);
+        finish_cell(s_cnt, q_block, nq, slices, slice, cand_d, cand_i, cand_n, wave, lane, BQ);
     }
 }
 
@@ -1413,12 +1431,20 @@ void launch_scan(dim3 cells, hipStream_t s, const uint16_t *qb, int64_t nq, cons
     if (n_cells <= 256)
         hipLaunchKernelGGL((knn_scan_small_kernel<KS>), grid, dim3(256), 0, s, qb, nq, rb, rnorm, nr,
                            per_slice, slices, cd, ci, cn, tk, exclude_self, qoff);
+    else if (exclude_self && nq <= 64)
+        hipLaunchKernelGGL((knn_scan_kernel<KS, kScanRing, false, true, 1>), grid, dim3(256), 0, s, qb,
+                           nq, rb, rnorm, nr, per_slice, slices, cd, ci, cn, tk, exclude_self, qoff,
+                           int64_t{0}, static_cast<float *>(nullptr));
     else if (exclude_self)
-        hipLaunchKernelGGL((knn_scan_kernel<KS, kScanRing, false, true>), grid, dim3(256), 0, s, qb, nq,
-                           rb, rnorm, nr, per_slice, slices, cd, ci, cn, tk, exclude_self, qoff,
+        hipLaunchKernelGGL((knn_scan_kernel<KS, kScanRing, false, true, 2>), grid, dim3(256), 0, s, qb,
+                           nq, rb, rnorm, nr, per_slice, slices, cd, ci, cn, tk, exclude_self, qoff,
+                           int64_t{0}, static_cast<float *>(nullptr));
+    else if (nq <= 64)
+        hipLaunchKernelGGL((knn_scan_kernel<KS, kScanRing, false, false, 1>), grid, dim3(256), 0, s, qb,
+                           nq, rb, rnorm, nr, per_slice, slices, cd, ci, cn, tk, exclude_self, qoff,
                            int64_t{0}, static_cast<float *>(nullptr));
     else
-        hipLaunchKernelGGL((knn_scan_kernel<KS, kScanRing, false, false>), grid, dim3(256), 0, s, qb,
+        hipLaunchKernelGGL((knn_scan_kernel<KS, kScanRing, false, false, 2>), grid, dim3(256), 0, s, qb,
                            nq, rb, rnorm, nr, per_slice, slices, cd, ci, cn, tk, exclude_self, qoff,
                            int64_t{0}, static_cast<float *>(nullptr));
 }
@@ -1430,12 +1456,19 @@ void launch_sample_dist(dim3 grid, hipStream_t s, const uint16_t *qb, int64_t nq
     // grid.x query blocks by grid.y sample slices of m / grid.y entries
     // each, through the scan's own pipeline (cells as in launch_scan)
     const int64_t n_cells = static_cast<int64_t>(grid.x) * grid.y;
-    hipLaunchKernelGGL((knn_scan_kernel<KS, kScanRing, true, true>),
-                       dim3(static_cast<unsigned>(8 * ((n_cells + 7) / 8))), dim3(256), 0, s, qb, nq,
-                       rb, rnorm, nr, static_cast<int64_t>(m / grid.y), static_cast<int>(grid.y),
-                       static_cast<float *>(nullptr), static_cast<int32_t *>(nullptr),
-                       static_cast<int32_t *>(nullptr), static_cast<uint32_t *>(nullptr), exclude_self,
-                       qoff, static_cast<int64_t>(m), dist);
+    const dim3 wgs(static_cast<unsigned>(8 * ((n_cells + 7) / 8)));
+    if (nq <= 64)
+        hipLaunchKernelGGL((knn_scan_kernel<KS, kScanRing, true, true, 1>), wgs, dim3(256), 0, s, qb, nq,
+                           rb, rnorm, nr, static_cast<int64_t>(m / grid.y), static_cast<int>(grid.y),
+                           static_cast<float *>(nullptr), static_cast<int32_t *>(nullptr),
+                           static_cast<int32_t *>(nullptr), static_cast<uint32_t *>(nullptr),
+                           exclude_self, qoff, static_cast<int64_t>(m), dist);
+    else
+        hipLaunchKernelGGL((knn_scan_kernel<KS, kScanRing, true, true, 2>), wgs, dim3(256), 0, s, qb, nq,
+                           rb, rnorm, nr, static_cast<int64_t>(m / grid.y), static_cast<int>(grid.y),
+                           static_cast<float *>(nullptr), static_cast<int32_t *>(nullptr),
+                           static_cast<int32_t *>(nullptr), static_cast<uint32_t *>(nullptr),
+                           exclude_self, qoff, static_cast<int64_t>(m), dist);
 }
 
 int ksteps_for(int64_t d) {
@@ -1505,9 +1538,9 @@ int KnnIndex::sample_size(int64_t q) const {
 }
 
 int KnnIndex::slices_for(int64_t q) const {
-    const int64_t blocks = (q + kTQ - 1) / kTQ;
     // Enough workgroups to fill the chip (256 CUs, two resident each), but
     // slices of at least kMinSliceRefs references.
+    const int64_t blocks = (q + kTQ - 1) / kTQ;
     int64_t s = (512 + blocks - 1) / blocks;
     s = std::min<int64_t>(s, std::max<int64_t>(1, n_ / kMinSliceRefs));
     s = std::max<int64_t>(1, std::min<int64_t>(s, kMaxSlices));

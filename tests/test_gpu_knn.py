@@ -104,22 +104,44 @@ def test_continuous_data_recall():
     assert (gi == ei).all() and (gd.view(np.uint32) == ed.view(np.uint32)).all()
 
 
-@pytest.mark.parametrize("q", [1, 100])
+@pytest.mark.parametrize("q", [1, 37, 64, 100])
 def test_most_slices_bit_exact(q):
     """1,048,600 references: batches of up to 128 rows take 512 slices (the
     most a batch takes) of 2,064 references, the last one ragged, so the
-    scan runs 512 DMA-staged workgroups and the merge lists more than 256
-    slices per query."""
+    scan runs 512 DMA-staged workgroups (64-query blocks up to 64 rows) and
+    the merge lists more than 256 slices per query. The thresholds start
+    from the 32,768-row sample bound; the merged prefilter lists equal those
+    of a 9,472-row batch (7 slices, no bound)."""
     from cuscan_amd.knn import KnnIndex
 
     ref = image_like(1_048_600, 20, 10, 21)
-    qry = image_like(q, 20, 10, 22)
+    qry = image_like(9472, 20, 10, 22)
     print(f"[gpu] knn most slices q={q}", flush=True)
-    idx = KnnIndex(ref, max_queries=q)
-    gi, gd = idx.query(qry, 15)
-    idx.close()
-    ei, ed = O.exact_knn(qry, ref, 15, threads=8)
+    idx = KnnIndex(ref, max_queries=9472)
+    gi, gd = idx.query(qry[:q], 15)
+    ei, ed = O.exact_knn(qry[:q], ref, 15, threads=8)
     assert (gi == ei).all() and (gd.view(np.uint32) == ed.view(np.uint32)).all()
+    qt = torch.from_numpy(qry).cuda()
+    small_ids, small_n = idx.candidates_device(qt[:q])
+    big_ids, big_n = idx.candidates_device(qt)
+    assert torch.equal(small_n, big_n[:q]) and torch.equal(small_ids, big_ids[:q])
+    idx.close()
+
+
+def test_most_slices_exclude_self():
+    """Fit-graph style queries (40 rows of the 1,048,600 references, starting
+    at row 1,000, excluding self) through the 64-query-block scan with the
+    self test."""
+    from cuscan_amd.knn import KnnIndex
+
+    x = image_like(1_048_600, 20, 10, 23)
+    print("[gpu] knn most slices exclude self", flush=True)
+    idx = KnnIndex(x, max_queries=40)
+    gi, gd = idx.query(x[1000:1040], 14, exclude_self=True, query_offset=1000)
+    idx.close()
+    ei, ed = O.exact_knn(x[1000:1040], x, 14, exclude_self=True, query_offset=1000, threads=8)
+    assert (gi == ei).all() and (gd.view(np.uint32) == ed.view(np.uint32)).all()
+    assert not (gi == np.arange(1000, 1040)[:, None]).any()
 
 
 @pytest.mark.parametrize("q", [1, 64, 700, 4096])
