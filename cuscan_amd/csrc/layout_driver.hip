@@ -253,6 +253,7 @@ struct umaphip_plan {
     DeviceArray<uint16_t> fire;
     DeviceArray<int32_t> unit_k, unit_frag, nz_rows, nz_off, row_split, split_off;
     DeviceArray<float> grad, frag;
+    DeviceArray<uint8_t> split;   // rank-ordered sweep: row split over units (set by the sweep)
     // rank-ordered unit sweep (projection windows, every row with an edge)
     bool ranked = false;
     int64_t ranked_epochs = 0;
@@ -487,7 +488,7 @@ void umaphip_plan::run() {
                 U.per_num = static_cast<double>(p.epochs);
                 U.unit_k = unit_k.ptr; U.unit_frag = unit_frag.ptr;
                 U.nz_rows = nz_rows.count ? nz_rows.ptr : nullptr; U.nz_off = nz_off.ptr;
-                U.grad = grad.ptr; U.frag = frag.ptr;
+                U.grad = grad.ptr; U.frag = frag.ptr; U.split = split.ptr;
                 U.units = units; U.n_edges = E;
                 U.rate = static_cast<double>(p.negative_sample_rate);
                 // with this epoch's projection order, sweep the rows by rank
@@ -528,9 +529,9 @@ void umaphip_plan::run() {
                 float *grad_out = tracing ? tr_grad.ptr : nullptr;
                 launch_unit_sweep(U, stream);
                 if (by_rank)
-                    launch_unit_adam_ranked(h, pos[c].ptr, pos[1 - c].ptr, grad.ptr, ranks.ptr,
-                                            pref.ptr, frag.ptr, m.ptr, v.ptr, alpha[t], c1[t],
-                                            c2[t], stream, grad_out);
+                    launch_unit_adam_ranked(h, pos[c].ptr, pos[1 - c].ptr, grad.ptr, split.ptr,
+                                            frag.ptr, m.ptr, v.ptr, alpha[t], c1[t], c2[t], stream,
+                                            grad_out);
                 else
                     launch_unit_adam(h, pos[c].ptr, pos[1 - c].ptr, grad.ptr, row_split.ptr,
                                      split_off.ptr, frag.ptr, m.ptr, v.ptr, alpha[t], c1[t],
@@ -790,6 +791,10 @@ umaphip_status umaphip_plan_create(int64_t h, int64_t n, int64_t d, const float 
             // the epochs with a projection order sweep rows by rank (fragment
             // slots 2u and 2u + 1 of every unit).
             P->ranked = P->hard && T.nz_rows.empty() && E < INT32_MAX;
+            if (P->ranked) {
+                P->split.alloc(static_cast<size_t>(h));
+                HIP_CHECK(hipMemset(P->split.ptr, 0, P->split.bytes()));
+            }
             // Rank-ordered plans keep 24-byte records when every period is
             // rebuilt exactly from an f32 sample count.
             P->compactable = P->ranked &&

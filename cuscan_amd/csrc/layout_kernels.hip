@@ -1297,10 +1297,17 @@ __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
         int slot = -1;
         if constexpr (RANKED) {
             // a row begun in an earlier unit stores slot 2u; one that
-            // continues into the next unit (and began here) stores 2u + 1
-            if (lr == 0 && s_pst[0] < e0) slot = static_cast<int>(2 * u);
-            else if (lr == nr - 1 && s_off[lr] + (s_end[lr] - s_seg[lr]) > ue)
+            // continues into the next unit (and began here) stores 2u + 1,
+            // and its flag and unit range for the Adam step
+            if (lr == 0 && s_pst[0] < e0) {
+                slot = static_cast<int>(2 * u);
+            } else if (lr == nr - 1 && s_off[lr] + (s_end[lr] - s_seg[lr]) > ue) {
                 slot = static_cast<int>(2 * u + 1);
+                const int64_t last = (s_pst[lr] + (s_end[lr] - s_seg[lr]) - 1) / UE;
+                reinterpret_cast<int2 *>(U.grad)[s_rowid[lr]] =
+                    make_int2(static_cast<int>(u), static_cast<int>(last));
+                U.split[s_rowid[lr]] = 1;   // cleared again by the Adam step
+            }
         } else {
             if (lr == 0) slot = U.unit_frag[2 * u];
             if (lr == nr - 1 && slot < 0) slot = U.unit_frag[2 * u + 1];
@@ -1379,23 +1386,22 @@ __global__ void unit_adam_kernel(int64_t h, const float *pos, float *pos_out, co
     reinterpret_cast<float2 *>(pos_out)[i] = out;
 }
 
-// Lazy Adam for the rank-ordered sweep: row i (rank r) covers sequence slots
-// [pref[r], pref[r+1]); a row inside one unit has its gradient in grad,
-// otherwise its fragments are slot 2u_a + 1 of its first unit and slot 2u of
-// every later unit, added in unit order.
+// Lazy Adam for the rank-ordered sweep: a row inside one unit has its
+// gradient in grad (split[i] = 0); otherwise (split[i] = 1, set by the
+// sweep's first unit of the row and cleared here for the next epoch) grad
+// holds the bits of its first and last unit u_a, u_b, and its fragments are
+// slot 2u_a + 1 of the first unit and slot 2u of every later unit, added in
+// unit order.
 __global__ void unit_adam_ranked_kernel(int64_t h, const float *pos, float *pos_out,
-                                        const float *grad, const int32_t *ranks,
-                                        const int32_t *pref, const float *frag, float *m, float *v,
-                                        float alpha, float c1, float c2, float *grad_out) {
+                                        const float *grad, uint8_t *split, const float *frag,
+                                        float *m, float *v, float alpha, float c1, float c2,
+                                        float *grad_out) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= h) return;
-    const int32_t r = ranks[i];
-    const int64_t b = pref[r], en = pref[r + 1];
-    const int64_t ua = b / kUnitEdges, ub = (en - 1) / kUnitEdges;
-    float2 g;
-    if (ua == ub) {
-        g = ld2(grad, i);
-    } else {
+    float2 g = ld2(grad, i);
+    if (split[i]) {
+        split[i] = 0;
+        const int64_t ua = __float_as_int(g.x), ub = __float_as_int(g.y);
         g = make_float2(0.0f, 0.0f);
         const float2 f0 = ld2(frag, 2 * ua + 1);
         g.x = g.x + f0.x;
@@ -2392,12 +2398,11 @@ void launch_rank_prefix(const int32_t *rank_deg, int64_t n, int32_t total_edges,
 }
 
 void launch_unit_adam_ranked(int64_t h, const float *pos, float *pos_out, const float *grad,
-                             const int32_t *ranks, const int32_t *pref, const float *frag,
-                             float *m, float *v, float alpha, float c1, float c2, hipStream_t s,
-                             float *grad_out) {
+                             uint8_t *split, const float *frag, float *m, float *v,
+                             float alpha, float c1, float c2, hipStream_t s, float *grad_out) {
     if (h == 0) return;
     hipLaunchKernelGGL(unit_adam_ranked_kernel, dim3(grid_for(h)), dim3(kThreads), 0, s, h, pos,
-                       pos_out, grad, ranks, pref, frag, m, v, alpha, c1, c2, grad_out);
+                       pos_out, grad, split, frag, m, v, alpha, c1, c2, grad_out);
     HIP_CHECK(hipGetLastError());
 }
 

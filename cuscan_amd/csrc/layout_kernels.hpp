@@ -121,6 +121,10 @@ struct UnitSweepArgs {
     const int32_t *nz_rows;    // nonzero-degree row ids (null when every row has an edge)
     const int32_t *nz_off;     // CSR start of each nonzero-degree row, then E
     float *grad, *frag;
+    // rank-ordered sweep: [n] set to 1 when the row's gradient is split over
+    // units (grad then holds the bits of its first and last unit); zero
+    // otherwise (the Adam step clears it after reading it)
+    uint8_t *split;
     const int32_t *pref;       // rank-ordered sweep: [n + 1] sequence start of each rank, or null
     const int32_t *rank_seg;   // rank-ordered sweep: [n] CSR start of order[r]
     int64_t units, n_edges;
@@ -165,8 +169,8 @@ void launch_rank_prefix(const int32_t *rank_deg, int64_t n, int32_t total_edges,
 // grad_out (may be null) receives each row's assembled gradient, the value
 // the Adam step consumes.
 void launch_unit_adam_ranked(int64_t h, const float *pos, float *pos_out, const float *grad,
-                             const int32_t *ranks, const int32_t *pref, const float *frag,
-                             float *m, float *v, float alpha, float c1, float c2, hipStream_t s,
+                             uint8_t *split, const float *frag, float *m, float *v,
+                             float alpha, float c1, float c2, hipStream_t s,
                              float *grad_out = nullptr);
 void launch_unit_adam(int64_t h, const float *pos, float *pos_out, const float *grad,
                       const int32_t *row_split, const int32_t *split_off, const float *frag,
