@@ -708,7 +708,9 @@ def main():
                 "gather_bytes_per_epoch": int(8 * (F + S) / args.epochs),
                 "refresh_avg_ms": round(pstats["refresh_ms"] / max(pstats["refresh_launches"], 1), 4),
                 "rank_ordered_epochs": pstats["rank_ordered_epochs"],
-                "compact_records": pstats["compact_records"]}
+                "compact_records": pstats["compact_records"],
+                # epochs per clock-schedule pass (its launches are inside the averaged time)
+                "clock_schedule": pstats["clock_schedule"]}
     out = {
         "metric": "layout edge-epochs/s at 1M rows", "value": value, "unit": "edge-epochs/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

@@ -98,7 +98,7 @@ class LayoutStats(C.Structure):
         ("sweep_ms", C.c_double), ("sweep_launches", C.c_int64),
         ("refresh_ms", C.c_double), ("refresh_launches", C.c_int64),
         ("edge_balanced", C.c_int64), ("rank_ordered_epochs", C.c_int64),
-        ("compact_records", C.c_int64),
+        ("compact_records", C.c_int64), ("clock_schedule", C.c_int64),
     ]
 
 
@@ -153,6 +153,7 @@ SIGNATURES = {
     "umaphip_plan_stream": (_vp, [_vp]),
     "umaphip_plan_set_profiling": (C.c_int, [_vp, _i32]),
     "umaphip_plan_set_record_format": (C.c_int, [_vp, _i32]),
+    "umaphip_plan_set_clock_schedule": (C.c_int, [_vp, _i32]),
     "umaphip_plan_set_trace": (C.c_int, [_vp, _i32, _i32]),
     "umaphip_plan_trace": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "umaphip_plan_destroy": (None, [_vp]),

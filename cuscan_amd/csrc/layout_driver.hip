@@ -126,6 +126,10 @@ void bias_corrections(int epochs, std::vector<float> &c1, std::vector<float> &c2
 
 namespace {
 
+// Device bytes a plan gives its clock schedule (one byte per edge and epoch):
+// 26.3M edges hold 163 epochs per pass.
+constexpr int64_t kScheduleBytes = int64_t(4) << 30;
+
 void check_layout_params(const umaphip_layout_params *p, int64_t h, int64_t n, int64_t d,
                          int64_t E) {
     UMAPHIP_REQUIRE(p != nullptr, UMAPHIP_INVALID_INPUT, "layout params are null");
@@ -259,6 +263,13 @@ struct umaphip_plan {
     int64_t ranked_epochs = 0;
     int32_t row_cap_id = 0, row_cap_ranked = 0;   // unit row-table capacity per order
     DeviceArray<int32_t> pref, pref_blocks, rank_unit_k, rank_seg, rank_deg;
+    // Clock schedule of the rank-ordered sweep: each edge's term count per
+    // epoch, computed sched_span epochs at a time (0: off, the sweep
+    // advances the clocks from the records every epoch).
+    bool schedulable = false;
+    int32_t sched_span = 0;
+    int64_t sched_stride = 0;
+    DeviceArray<uint8_t> sched;
 
     // optional trace of one unit-sweep epoch: the state it starts from
     // (positions, negative order, ranks, scale), each row's assembled
@@ -286,6 +297,21 @@ struct umaphip_plan {
     }
 
     int64_t adaptation_stride() const { return sample_count * 11; }
+
+    // cap < 0: automatic span (every epoch in one pass when the slabs fit
+    // kScheduleBytes); 0: off; > 0: at most cap epochs per pass.
+    void configure_schedule(int32_t cap) {
+        if (!schedulable || cap == 0) {
+            sched_span = 0;
+            sched.alloc(0);
+            return;
+        }
+        sched_stride = unit_schedule_stride(E);
+        int64_t span = std::max<int64_t>(std::min<int64_t>(p.epochs, kScheduleBytes / sched_stride), 1);
+        if (cap > 0) span = std::min<int64_t>(span, cap);
+        sched_span = static_cast<int32_t>(span);
+        sched.alloc(static_cast<size_t>(sched_stride * span));
+    }
 
     void mark(std::vector<hipEvent_t> &pool, int &used) {
         if (!profiling) return;
@@ -479,13 +505,28 @@ void umaphip_plan::run() {
             A.multiplier = 1.0f; A.alpha = alpha[t]; A.c1 = c1[t]; A.c2 = c2[t];
             A.shared = shared; A.negatives = p.negative_sample_rate != 0;
             A.exclude = p.exclude_graph_neighbors; A.fused = 1;
-            mark(sweep_ev, sweep_used);
+            mark(sweep_ev, sweep_used);   // schedule passes count as sweep time
+            const uint8_t *sched_now = nullptr;
+            if (unit_path && sched_span > 0) {
+                // every epoch of a schedulable plan sweeps by rank
+                UMAPHIP_REQUIRE(ranked && projection_ready, UMAPHIP_BACKEND,
+                                "clock schedule: epoch %d without a projection order", t);
+                const int slab = t % sched_span;
+                if (slab == 0)
+                    launch_unit_schedule(rec.ptr, E, compact, static_cast<double>(p.epochs),
+                                         static_cast<double>(p.negative_sample_rate),
+                                         p.negative_sample_rate != 0, t,
+                                         std::min(sched_span, epochs - t), sched.ptr, sched_stride,
+                                         error_flag.ptr, stream);
+                sched_now = sched.ptr + slab * sched_stride;
+            }
             if (unit_path) {
                 UnitSweepArgs U{};
                 U.s = A;
                 U.rec = rec.ptr; U.fire = fire.ptr;
                 U.compact = compact;
                 U.per_num = static_cast<double>(p.epochs);
+                U.sched = sched_now;
                 U.unit_k = unit_k.ptr; U.unit_frag = unit_frag.ptr;
                 U.nz_rows = nz_rows.count ? nz_rows.ptr : nullptr; U.nz_off = nz_off.ptr;
                 U.grad = grad.ptr; U.frag = frag.ptr; U.split = split.ptr;
@@ -801,6 +842,12 @@ umaphip_status umaphip_plan_create(int64_t h, int64_t n, int64_t d, const float 
                              periods_compactable(periods, E, static_cast<double>(params->epochs));
             P->compact = P->compactable;
             P->rec.alloc(static_cast<size_t>((P->compact ? 3 : 4) * E));
+            // The clocks run ahead of the sweep as a per-epoch schedule of
+            // term counts (one byte per edge and epoch); a firing edge draws
+            // at most about 1.34 rate + 3 negatives, so rate <= 64 keeps
+            // every count in a byte.
+            P->schedulable = P->ranked && params->negative_sample_rate <= 64;
+            P->configure_schedule(-1);
             // Row-table capacity: the rows of a unit in CSR order are known;
             // in any rank order a unit of UE slots holds at most
             // 2 + (UE - 2) / (smallest degree) rows (two partial rows).
@@ -942,6 +989,17 @@ umaphip_status umaphip_plan_set_record_format(umaphip_plan *plan, int32_t compac
     });
 }
 
+umaphip_status umaphip_plan_set_clock_schedule(umaphip_plan *plan, int32_t span) {
+    return guarded([&] {
+        UMAPHIP_REQUIRE(plan != nullptr, UMAPHIP_INVALID_INPUT, "plan is null");
+        UMAPHIP_REQUIRE(span <= 0 || plan->schedulable, UMAPHIP_INVALID_CONFIG,
+                        "a clock schedule needs a rank-ordered plan with negative_sample_rate "
+                        "<= 64");
+        HIP_CHECK(hipStreamSynchronize(plan->stream));
+        plan->configure_schedule(span);
+    });
+}
+
 umaphip_status umaphip_plan_set_trace(umaphip_plan *plan, int32_t epoch, int32_t negative_cap) {
     return guarded([&] {
         UMAPHIP_REQUIRE(plan != nullptr, UMAPHIP_INVALID_INPUT, "plan is null");
@@ -1050,12 +1108,15 @@ umaphip_status umaphip_plan_download(umaphip_plan *plan, float *head, double *ne
             stats->edge_balanced = plan->unit_path ? 1 : 0;
             stats->rank_ordered_epochs = plan->ran ? plan->ranked_epochs : 0;
             stats->compact_records = plan->compact ? 1 : 0;
+            stats->clock_schedule = plan->sched_span;
         }
         UMAPHIP_REQUIRE(status == ~0ull, UMAPHIP_NUMERICAL,
                         "non-finite %s projection at row %llu (refresh %llu)",
                         plan->shared ? "reference" : "reference or query",
                         status & 0xFFFFFFFFull, status >> 32);
         UMAPHIP_REQUIRE((err & 2) == 0, UMAPHIP_BACKEND, "projection sort: a pass stalled");
+        UMAPHIP_REQUIRE((err & 4) == 0, UMAPHIP_BACKEND,
+                        "clock schedule: an edge drew more than 254 negatives in one epoch");
         UMAPHIP_REQUIRE(err == 0, UMAPHIP_NUMERICAL, "invalid negative force ratio");
     });
 }

@@ -934,13 +934,20 @@ __device__ inline void segmented_accumulate(float2 term, int row, int lane, floa
 // RC: capacity of the unit row table; the plan picks 64 when no unit can
 // hold more rows, which keeps the workgroup at about 7 KB of LDS (6 waves
 // per SIMD) instead of about 14 KB (3 waves).
-// COMPACT (rank-ordered only): 24-byte records, the period rebuilt as
-// per_num / samples and e mod M from the edge index.
+// FMT (rank-ordered only for 1 and 2): how a firing edge is found and its
+// terms counted.
+//   0: 32-byte records (above), clocks advanced here;
+//   1: 24-byte records, the period rebuilt as per_num / samples and e mod M
+//      from the edge index, clocks advanced here;
+//   2: the epoch's slab of the clock schedule (unit_schedule_kernel) gives
+//      the term count of each edge and the CSR columns give its tail; the
+//      records are not touched (the schedule kernel has advanced them).
 // TRACE: also records every drawn negative (candidate id and whether it
 // repels) per CSR edge, with the same selection code (tests).
-template <bool RANKED, int RC, bool COMPACT, bool TRACE>
+template <bool RANKED, int RC, int FMT, bool TRACE>
 __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
-    static_assert(RANKED || !COMPACT, "compact records are read by rank only");
+    constexpr bool COMPACT = FMT == 1, SCHED = FMT == 2;
+    static_assert(RANKED || FMT == 0, "compact records and schedules are read by rank only");
     const SweepArgs &A = U.s;
     constexpr int EPL = kUnitEPL, UE = kUnitEdges, TCAP = kUnitTermCap, G = kUnitGroups;
     __shared__ uint16_t s_off[RC + 1];  // unit-local first edge of each unit row
@@ -1077,6 +1084,15 @@ __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
 #pragma unroll
         for (int k = 0; k < EPL; ++k) {
             r_npos[k] = 0.0; r_nn[k] = 0.0; r_per[k] = 0.0; r_w[k] = 0;
+            if constexpr (SCHED) {
+                // fire[k] holds the edge's term count (0: it does not fire)
+                fire[k] = 0;
+                if (k * 64 + lane < ue) {
+                    fire[k] = U.sched[eidx[k]];
+                    r_w[k] = static_cast<uint32_t>(A.indices[eidx[k]]);
+                }
+                continue;
+            }
             if (k * 64 + lane < ue) {
                 if constexpr (COMPACT) {
                     const double *rec = U.rec + 3 * eidx[k];
@@ -1114,7 +1130,21 @@ __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
                 }
                 r = lo;
             }
-            if (fire[k] <= epoch) {
+            if constexpr (SCHED) {
+                if (fire[k] != 0) {
+                    const int64_t e = eidx[k];
+                    if constexpr (TRACE) {
+                        const int negs = static_cast<int>(fire[k]) - 1;
+                        U.tr_count[e] = negs;
+                        if (negs > U.tr_cap) atomicOr(U.tr_overflow, 1);
+                    }
+                    t = static_cast<int>(fire[k]);
+                    // e < 2^31: exact in f64
+                    s_j[slot] = static_cast<int32_t>(static_cast<uint32_t>(r_w[k]));
+                    s_em[slot] = mod_f64(static_cast<double>(static_cast<uint32_t>(e)), mod_d,
+                                         U.inv_mod);
+                }
+            } else if (fire[k] <= epoch) {
                 const int64_t e = eidx[k];
                 double per = r_per[k];
                 if constexpr (COMPACT)
@@ -1345,6 +1375,75 @@ __global__ void unit_unpack_kernel(const double *rec, int64_t E, bool compact, d
     const int64_t at = compact ? 3 * e : 4 * e;
     npos[e] = rec[at];
     nneg[e] = rec[at + 1];
+}
+
+// Clock schedule of the rank-ordered sweep for epochs [t0, t0 + span):
+// sched[(t - t0) * stride + e] = 1 + the negatives edge e draws when it
+// fires in epoch t, 0 when it does not fire. Neither clock depends on the
+// positions, so one pass runs both through the span with the f64
+// operations of the sweep's own clock update, in the same order
+// (modern_layout.rs:662-691): the edge fires when next_positive <= t, then
+// next_positive += period; count = trunc((t - next_negative) / negative
+// period), next_negative += count * negative period. The records are left
+// at the state after the span's last epoch. A lane holds four consecutive
+// edges and writes their bytes as one word per epoch (stride is a multiple
+// of 4 and at least E rounded up to one). A term count above 255 is stored
+// as 255 and ORs 4 into *flag.
+__global__ void unit_schedule_kernel(double *rec, int64_t E, bool compact, double per_num,
+                                     double rate, bool negatives, int32_t t0, int32_t span,
+                                     uint8_t *sched, int64_t stride, int32_t *flag) {
+    const int64_t e0 = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) * 4;
+    if (e0 >= E) return;
+    double npos[4], nn[4], per[4], np[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int64_t e = e0 + k;
+        npos[k] = INFINITY;   // past the end: never fires
+        nn[k] = 0.0;
+        per[k] = 1.0;
+        if (e < E) {
+            if (compact) {
+                npos[k] = rec[3 * e];
+                nn[k] = rec[3 * e + 1];
+                const uint64_t w = static_cast<uint64_t>(__double_as_longlong(rec[3 * e + 2]));
+                per[k] = per_num / static_cast<double>(__uint_as_float(static_cast<uint32_t>(w >> 32)));
+            } else {
+                npos[k] = rec[4 * e];
+                nn[k] = rec[4 * e + 1];
+                per[k] = rec[4 * e + 2];
+            }
+        }
+        np[k] = negatives ? (per[k] / rate) * 1.5 : 1.0;
+    }
+    bool over = false;
+    for (int32_t i = 0; i < span; ++i) {
+        const double now = static_cast<double>(t0 + i);
+        uint32_t word = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (npos[k] <= now) {
+                npos[k] = npos[k] + per[k];
+                int32_t count = 0;
+                if (negatives) {
+                    count = static_cast<int32_t>((now - nn[k]) / np[k]);
+                    nn[k] = nn[k] + static_cast<double>(count) * np[k];
+                }
+                const int32_t terms = 1 + (count > 0 ? count : 0);
+                over = over || terms > 255;
+                word |= static_cast<uint32_t>(terms < 255 ? terms : 255) << (8 * k);
+            }
+        }
+        *reinterpret_cast<uint32_t *>(sched + i * stride + e0) = word;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int64_t e = e0 + k;
+        if (e >= E) break;
+        const int64_t at = compact ? 3 * e : 4 * e;
+        rec[at] = npos[k];
+        rec[at + 1] = nn[k];
+    }
+    if (over) atomicOr(flag, 4);
 }
 
 // Lazy Adam over rows; rows split across units add their fragments in unit
@@ -2347,18 +2446,20 @@ void launch_edge_sweep(const SweepArgs &a, hipStream_t s) {
 int64_t unit_edges() { return kUnitEdges; }
 int64_t unit_narrow_rows() { return kUnitNarrowRows; }
 
+template <bool RANKED, int FMT, bool TRACE>
+void launch_unit_sweep_rows(const UnitSweepArgs &a, dim3 grid, bool narrow, hipStream_t s) {
+    if (narrow)
+        hipLaunchKernelGGL((unit_sweep_kernel<RANKED, kUnitNarrowRows, FMT, TRACE>), grid, dim3(64), 0, s, a);
+    else
+        hipLaunchKernelGGL((unit_sweep_kernel<RANKED, kUnitEdges, FMT, TRACE>), grid, dim3(64), 0, s, a);
+}
+
 template <bool TRACE>
 void launch_unit_sweep_shape(const UnitSweepArgs &a, dim3 grid, bool narrow, hipStream_t s) {
-    if (a.pref && a.compact) {
-        if (narrow) hipLaunchKernelGGL((unit_sweep_kernel<true, kUnitNarrowRows, true, TRACE>), grid, dim3(64), 0, s, a);
-        else hipLaunchKernelGGL((unit_sweep_kernel<true, kUnitEdges, true, TRACE>), grid, dim3(64), 0, s, a);
-    } else if (a.pref) {
-        if (narrow) hipLaunchKernelGGL((unit_sweep_kernel<true, kUnitNarrowRows, false, TRACE>), grid, dim3(64), 0, s, a);
-        else hipLaunchKernelGGL((unit_sweep_kernel<true, kUnitEdges, false, TRACE>), grid, dim3(64), 0, s, a);
-    } else {
-        if (narrow) hipLaunchKernelGGL((unit_sweep_kernel<false, kUnitNarrowRows, false, TRACE>), grid, dim3(64), 0, s, a);
-        else hipLaunchKernelGGL((unit_sweep_kernel<false, kUnitEdges, false, TRACE>), grid, dim3(64), 0, s, a);
-    }
+    if (a.pref && a.sched) launch_unit_sweep_rows<true, 2, TRACE>(a, grid, narrow, s);
+    else if (a.pref && a.compact) launch_unit_sweep_rows<true, 1, TRACE>(a, grid, narrow, s);
+    else if (a.pref) launch_unit_sweep_rows<true, 0, TRACE>(a, grid, narrow, s);
+    else launch_unit_sweep_rows<false, 0, TRACE>(a, grid, narrow, s);
 }
 
 void launch_unit_sweep(UnitSweepArgs a, hipStream_t s) {
@@ -2369,8 +2470,8 @@ void launch_unit_sweep(UnitSweepArgs a, hipStream_t s) {
     a.xcd_chunk = (a.units + 7) / 8;
     const dim3 grid(static_cast<unsigned>(8 * a.xcd_chunk));
     const bool narrow = a.row_cap == kUnitNarrowRows;
-    UMAPHIP_REQUIRE(a.pref || !a.compact, UMAPHIP_BACKEND,
-                    "unit sweep: compact records need the rank-ordered sweep");
+    UMAPHIP_REQUIRE(a.pref || (!a.compact && !a.sched), UMAPHIP_BACKEND,
+                    "unit sweep: compact records and clock schedules need the rank-ordered sweep");
     if (a.tr_count) {
         UMAPHIP_REQUIRE(a.tr_cap > 0 && a.tr_cand && a.tr_kept && a.tr_overflow && a.tr_paths,
                         UMAPHIP_INVALID_INPUT, "unit sweep trace buffers are null");
@@ -2506,6 +2607,20 @@ void launch_unit_pack(const double *npos, const double *nneg, const double *peri
     if (E == 0) return;
     hipLaunchKernelGGL(unit_pack_kernel, dim3(grid_for(E)), dim3(kThreads), 0, s, npos, nneg,
                        periods, indices, E, modulus, compact, per_num, rec, fire);
+    HIP_CHECK(hipGetLastError());
+}
+
+int64_t unit_schedule_stride(int64_t E) { return (E + 255) / 256 * 256; }
+
+void launch_unit_schedule(double *rec, int64_t E, bool compact, double per_num, double rate,
+                          bool negatives, int32_t t0, int32_t span, uint8_t *sched,
+                          int64_t stride, int32_t *flag, hipStream_t s) {
+    if (E == 0 || span <= 0) return;
+    UMAPHIP_REQUIRE(stride % 4 == 0 && stride >= (E + 3) / 4 * 4, UMAPHIP_BACKEND,
+                    "clock schedule: slab stride %lld does not hold %lld edges",
+                    static_cast<long long>(stride), static_cast<long long>(E));
+    hipLaunchKernelGGL(unit_schedule_kernel, dim3(grid_for((E + 3) / 4)), dim3(kThreads), 0, s,
+                       rec, E, compact, per_num, rate, negatives, t0, span, sched, stride, flag);
     HIP_CHECK(hipGetLastError());
 }
 

@@ -140,6 +140,12 @@ class LayoutPlan:
         """24-byte (compact) or 32-byte edge records for the edge-balanced sweep."""
         _lib.check(_lib.lib().umaphip_plan_set_record_format(self._handle, int(bool(compact))))
 
+    def set_clock_schedule(self, span: int = -1) -> None:
+        """Clock schedule of a rank-ordered plan: span < 0 automatic (the
+        default), 0 off (the sweep advances the clocks from the records),
+        span > 0 at most that many epochs per schedule pass."""
+        _lib.check(_lib.lib().umaphip_plan_set_clock_schedule(self._handle, int(span)))
+
     def set_trace(self, epoch: int, negative_cap: int = 16) -> None:
         """Record one epoch of the edge-balanced sweep on later runs (epoch < 0: off)."""
         self._trace_cap = int(negative_cap)

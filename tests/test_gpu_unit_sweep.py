@@ -21,8 +21,9 @@ gradient_kernel (crates/umap/src/kernels/standard_umap/modern_layout.rs:
 
 Paths covered: rows in CSR order (global negatives, exclusion; sparse rows
 and hubs split over units) and in projection order (windows; 24- and
-32-byte records), the term-parallel path and the per-edge path (rate 24,
-more terms than a unit holds).
+32-byte records; term counts from the clock schedule, whole-run and in
+7-epoch passes, or clocks advanced by the sweep itself), the term-parallel
+path and the per-edge path (rate 24, more terms than a unit holds).
 """
 import numpy as np
 import pytest
@@ -93,7 +94,7 @@ def mixture_graph_rate24(mixture_rows):
     return _graph_from_data(mixture_rows, rate=24.0)
 
 
-def traced_epoch(g, params, epoch, *, cap=16, compact=None):
+def traced_epoch(g, params, epoch, *, cap=16, compact=None, schedule=None):
     from cuscan_amd import graph as G
     from cuscan_amd import layout as L
 
@@ -103,6 +104,8 @@ def traced_epoch(g, params, epoch, *, cap=16, compact=None):
     try:
         if compact is not None:
             plan.set_record_format(compact)
+        if schedule is not None:
+            plan.set_clock_schedule(schedule)
         plan.set_trace(epoch, cap)
         plan.run()
         _, npos, nneg, stats = plan.download(clocks=True)
@@ -110,6 +113,7 @@ def traced_epoch(g, params, epoch, *, cap=16, compact=None):
     finally:
         plan.close()
     assert stats["edge_balanced"] == 1
+    tr["clock_schedule"] = stats["clock_schedule"]
     return tr, npos, nneg
 
 
@@ -203,11 +207,25 @@ def test_cfg2_csr_order_exclusion(cfg2_graph, epoch):
 @pytest.mark.parametrize("epoch", [1, 50, 199])
 def test_mixture_windows_rank_order(mixture_graph, epoch):
     """Projection windows (Q = 20,000 of 100,000 rows, adapted scale):
-    rows swept in projection order with 24-byte records."""
+    rows swept in projection order with 24-byte records, term counts from
+    the whole-run clock schedule (the default)."""
     params = _params(negative_selection_range=20_000)
     print(f"[gpu] unit sweep mixture epoch {epoch}", flush=True)
     tr, npos, nneg = traced_epoch(mixture_graph, params, epoch)
-    assert tr["rank_ordered"] and tr["compact_records"]
+    assert tr["rank_ordered"] and tr["compact_records"] and tr["clock_schedule"] == 200
+    check_trace(mixture_graph, params, epoch, tr)
+    want = advance_graph(mixture_graph, 200)
+    assert (npos == want["next_positive"]).all() and (nneg == want["next_negative"]).all()
+
+
+@pytest.mark.parametrize("epoch", [1, 199])
+def test_mixture_windows_clocks_in_sweep(mixture_graph, epoch):
+    """Schedule off: the sweep reads the 24-byte records and advances the
+    clocks itself."""
+    params = _params(negative_selection_range=20_000)
+    print(f"[gpu] unit sweep mixture, clocks in the sweep, epoch {epoch}", flush=True)
+    tr, npos, nneg = traced_epoch(mixture_graph, params, epoch, schedule=0)
+    assert tr["rank_ordered"] and tr["compact_records"] and tr["clock_schedule"] == 0
     check_trace(mixture_graph, params, epoch, tr)
     want = advance_graph(mixture_graph, 200)
     assert (npos == want["next_positive"]).all() and (nneg == want["next_negative"]).all()
@@ -217,10 +235,61 @@ def test_mixture_windows_rank_order(mixture_graph, epoch):
 def test_mixture_windows_32_byte_records(mixture_graph, epoch):
     params = _params(negative_selection_range=20_000, exclude_graph_neighbors=True)
     print(f"[gpu] unit sweep mixture 32-byte epoch {epoch}", flush=True)
-    tr, npos, _ = traced_epoch(mixture_graph, params, epoch, compact=False)
-    assert tr["rank_ordered"] and not tr["compact_records"]
+    # 7-epoch schedule passes: epochs 50 and 199 are slabs 1 and 3 of a pass
+    tr, npos, nneg = traced_epoch(mixture_graph, params, epoch, compact=False, schedule=7)
+    assert tr["rank_ordered"] and not tr["compact_records"] and tr["clock_schedule"] == 7
     check_trace(mixture_graph, params, epoch, tr)
-    assert (npos == advance_graph(mixture_graph, 200)["next_positive"]).all()
+    want = advance_graph(mixture_graph, 200)
+    assert (npos == want["next_positive"]).all() and (nneg == want["next_negative"]).all()
+
+
+def test_schedule_forms_identical(mixture_graph):
+    """Whole runs with the schedule in one pass, in 7- and 1-epoch passes and
+    off, with 24- and 32-byte records: the same embedding and clocks, bit for
+    bit."""
+    from cuscan_amd import graph as G
+    from cuscan_amd import layout as L
+
+    params = _params(negative_selection_range=20_000, exclude_graph_neighbors=True)
+    head = G.random_init(len(mixture_graph["offsets"]) - 1, 2, 42)
+    outs = []
+    for compact, span in [(True, -1), (True, 0), (True, 7), (False, 1), (False, 0)]:
+        print(f"[gpu] schedule forms: 24-byte {compact}, span {span}", flush=True)
+        plan = L.LayoutPlan(head, None, mixture_graph, params, shared=True)
+        try:
+            plan.set_record_format(compact)
+            plan.set_clock_schedule(span)
+            plan.run()
+            y, npos, nneg, stats = plan.download(clocks=True)
+        finally:
+            plan.close()
+        assert stats["clock_schedule"] == {-1: 200, 0: 0}.get(span, span)
+        assert stats["rank_ordered_epochs"] == 200
+        outs.append((y, npos, nneg))
+    for y, npos, nneg in outs[1:]:
+        assert np.array_equal(y.view(np.uint32), outs[0][0].view(np.uint32))
+        assert (npos == outs[0][1]).all() and (nneg == outs[0][2]).all()
+
+
+def test_schedule_needs_small_rate():
+    """Rates above 64 keep the clocks in the sweep; asking for a schedule
+    is an error."""
+    from cuscan_amd import graph as G
+    from cuscan_amd import layout as L
+    from tests.gpu_helpers import hub_graph
+
+    g = hub_graph(5000, 8, 3, 200, rate=65.0, isolated=())
+    params = _params(negative_selection_range=1_000, negative_sample_rate=65)
+    plan = L.LayoutPlan(G.random_init(5000, 2, 1), None, g, params, shared=True)
+    try:
+        with pytest.raises(Exception, match="clock schedule"):
+            plan.set_clock_schedule(5)
+        plan.set_clock_schedule(-1)
+        plan.run()
+        _, _, _, stats = plan.download(clocks=True)
+        assert stats["clock_schedule"] == 0 and stats["rank_ordered_epochs"] == 200
+    finally:
+        plan.close()
 
 
 @pytest.mark.parametrize("q", [20_000, 200_000])
