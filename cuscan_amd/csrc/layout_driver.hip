@@ -263,6 +263,9 @@ struct umaphip_plan {
     int64_t ranked_epochs = 0;
     int32_t row_cap_id = 0, row_cap_ranked = 0;   // unit row-table capacity per order
     DeviceArray<int32_t> pref, pref_blocks, rank_unit_k, rank_seg, rank_deg;
+    // {x, y, CSR start, degree} per row, kept by the Adam step for the
+    // refresh's last sort pass (rank-ordered plans)
+    DeviceArray<float> rows4;
     // Clock schedule of the rank-ordered sweep: each edge's term count per
     // epoch, computed sched_span epochs at a time (0: off, the sweep
     // advances the clocks from the records every epoch).
@@ -350,11 +353,15 @@ struct umaphip_plan {
             // projection, its first histogram and the four sort passes; the
             // last pass writes the ranks and the table
             // (rank-ordered sweeps: also each rank's CSR start and degree)
+            // (rank-ordered plans: from the packed rows the Adam step keeps,
+            // and no ranks, which only the epoch trace reads)
             launch_project_sort_ranked(sort_ws.ptr, sort_ws.count, head_now, n, dir, status.ptr, tag,
-                                       keys.ptr, keys_sorted.ptr, proj_order.ptr, ranks.ptr,
+                                       keys.ptr, keys_sorted.ptr, proj_order.ptr,
+                                       ranked ? nullptr : ranks.ptr,
                                        table.ptr, stream, ranked ? offsets.ptr : nullptr,
                                        ranked ? rank_seg.ptr : nullptr,
-                                       ranked ? rank_deg.ptr : nullptr, error_flag.ptr);
+                                       ranked ? rank_deg.ptr : nullptr, error_flag.ptr,
+                                       ranked ? rows4.ptr : nullptr);
             return;
         }
         launch_project_keys(reference, n, d, dir, keys.ptr, nullptr, status.ptr, tag, stream);
@@ -388,6 +395,7 @@ void umaphip_plan::run() {
     if (unit_path)
         launch_unit_pack(npos0.ptr, nneg0.ptr, periods.ptr, indices.ptr, E, range, compact,
                          static_cast<double>(p.epochs), rec.ptr, fire.ptr, stream);
+    if (ranked) launch_pack_rows(pos[0].ptr, offsets.ptr, h, rows4.ptr, stream);
     HIP_CHECK(hipMemsetAsync(updates.ptr, 0, updates.bytes(), stream));
     if (adam) {
         HIP_CHECK(hipMemsetAsync(m.ptr, 0, m.bytes(), stream));
@@ -551,6 +559,8 @@ void umaphip_plan::run() {
                 const bool tracing = t == trace_epoch;
                 if (tracing) {
                     const hipMemcpyKind dd = hipMemcpyDeviceToDevice;
+                    // rank-ordered refreshes leave the ranks unwritten
+                    if (ranked) launch_inverse_ranks(order_now, n, ranks.ptr, stream);
                     HIP_CHECK(hipMemcpyAsync(tr_head.ptr, pos[c].ptr, tr_head.bytes(), dd, stream));
                     HIP_CHECK(hipMemcpyAsync(tr_order.ptr, order_now, tr_order.bytes(), dd, stream));
                     HIP_CHECK(hipMemcpyAsync(tr_ranks.ptr, hard ? ranks.ptr : iota.ptr,
@@ -572,7 +582,7 @@ void umaphip_plan::run() {
                 if (by_rank)
                     launch_unit_adam_ranked(h, pos[c].ptr, pos[1 - c].ptr, grad.ptr, split.ptr,
                                             frag.ptr, m.ptr, v.ptr, alpha[t], c1[t], c2[t], stream,
-                                            grad_out);
+                                            grad_out, offsets.ptr, rows4.ptr);
                 else
                     launch_unit_adam(h, pos[c].ptr, pos[1 - c].ptr, grad.ptr, row_split.ptr,
                                      split_off.ptr, frag.ptr, m.ptr, v.ptr, alpha[t], c1[t],
@@ -868,6 +878,7 @@ umaphip_status umaphip_plan_create(int64_t h, int64_t n, int64_t d, const float 
                 P->rank_unit_k.alloc(static_cast<size_t>(2 * T.units));
                 P->rank_seg.alloc(static_cast<size_t>(n));
                 P->rank_deg.alloc(static_cast<size_t>(n));
+                P->rows4.alloc(static_cast<size_t>(4 * h));
             }
             P->frag.alloc(static_cast<size_t>(2 * frag_slots));
         }

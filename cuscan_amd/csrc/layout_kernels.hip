@@ -960,7 +960,7 @@ __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
     __shared__ uint16_t s_base[UE + 1]; // first term of each edge slot (read when total <= TCAP)
     __shared__ int32_t s_j[UE];
     __shared__ uint8_t s_row[UE];
-    __shared__ uint32_t s_em[UE];
+    __shared__ uint32_t s_em[SCHED ? 1 : UE];   // e mod M (SCHED: recomputed per term)
     __shared__ int32_t s_pst[RANKED ? RC + 1 : 1];  // sequence start of each unit row
     __shared__ int32_t s_end[RANKED ? RC + 1 : 1];  // CSR end of each unit row
     static_assert(UE <= 256 && RC <= UE, "edge slots and unit rows are indexed by uint8");
@@ -1139,10 +1139,7 @@ __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
                         if (negs > U.tr_cap) atomicOr(U.tr_overflow, 1);
                     }
                     t = static_cast<int>(fire[k]);
-                    // e < 2^31: exact in f64
                     s_j[slot] = static_cast<int32_t>(static_cast<uint32_t>(r_w[k]));
-                    s_em[slot] = mod_f64(static_cast<double>(static_cast<uint32_t>(e)), mod_d,
-                                         U.inv_mod);
                 }
             } else if (fire[k] <= epoch) {
                 const int64_t e = eidx[k];
@@ -1218,9 +1215,17 @@ __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
                     y[g] = ld2(A.tgt, static_cast<uint32_t>(s_j[owner]));
                     continue;
                 }
-                const uint64_t sel =
-                    negative_offset(s_em[owner], t1, kq, modulus, mod_d, U.inv_mod, prod_limit);
                 const int rr = s_row[owner];
+                uint32_t em;
+                if constexpr (SCHED) {
+                    // e < 2^31: exact in f64
+                    const int64_t e = s_seg[rr] + (e0 + owner - s_pst[rr]);
+                    em = mod_f64(static_cast<double>(static_cast<uint32_t>(e)), mod_d, U.inv_mod);
+                } else {
+                    em = s_em[owner];
+                }
+                const uint64_t sel =
+                    negative_offset(em, t1, kq, modulus, mod_d, U.inv_mod, prod_limit);
                 const uint64_t rank = row_first(rr) + sel;
                 if (A.exclude) {
                     const int64_t jj = static_cast<uint32_t>(A.order[rank]);
@@ -1279,7 +1284,10 @@ __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
                     const float c = attract_fast(q, A.a, A.b, 1.0f);
                     g = make_float2(c * dx, c * dy);
                 }
-                const uint64_t em = s_em[slot];
+                const uint64_t em =
+                    SCHED ? mod_f64(static_cast<double>(static_cast<uint32_t>(eidx[k])), mod_d,
+                                    U.inv_mod)
+                          : s_em[slot];
                 uint64_t sel = t_cnt[k] > 1 ? negative_offset(static_cast<uint32_t>(em), t1, 1,
                                                               modulus, mod_d, U.inv_mod, prod_limit)
                                             : 0u;
@@ -1494,7 +1502,8 @@ __global__ void unit_adam_kernel(int64_t h, const float *pos, float *pos_out, co
 __global__ void unit_adam_ranked_kernel(int64_t h, const float *pos, float *pos_out,
                                         const float *grad, uint8_t *split, const float *frag,
                                         float *m, float *v, float alpha, float c1, float c2,
-                                        float *grad_out) {
+                                        float *grad_out, const int32_t *csr_off,
+                                        float4 *rows_out) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= h) return;
     float2 g = ld2(grad, i);
@@ -1528,6 +1537,21 @@ __global__ void unit_adam_ranked_kernel(int64_t h, const float *pos, float *pos_
     reinterpret_cast<float2 *>(m)[i] = mm;
     reinterpret_cast<float2 *>(v)[i] = vv;
     reinterpret_cast<float2 *>(pos_out)[i] = out;
+    if (rows_out) {
+        const int32_t b = csr_off[i];
+        rows_out[i] = make_float4(out.x, out.y, __int_as_float(b), __int_as_float(csr_off[i + 1] - b));
+    }
+}
+
+// rows[i] = {x_i, y_i, CSR start, degree} (bits of the two integers), the
+// record the projection sort's last pass gathers per row.
+__global__ void pack_rows_kernel(const float *pos, const int32_t *csr_off, int64_t h,
+                                 float4 *rows) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= h) return;
+    const float2 x = ld2(pos, i);
+    const int32_t b = csr_off[i];
+    rows[i] = make_float4(x.x, x.y, __int_as_float(b), __int_as_float(csr_off[i + 1] - b));
 }
 
 // Exclusive prefix sum of the row degrees in projection order:
@@ -1957,7 +1981,7 @@ __global__ __launch_bounds__(kOsThreads) void radix_onesweep_kernel(
     uint32_t *group_totals, uint32_t *tile_ctr, uint32_t *done_ctr, uint32_t *clear_totals,
     int32_t *err,
     int32_t *ranks_out, const float *table_src, float *table_out, const int32_t *csr_off,
-    int32_t *seg_out, int32_t *deg_out) {
+    int32_t *seg_out, int32_t *deg_out, const float4 *rows) {
     constexpr int WAVES = kOsThreads / 64;
     constexpr int PER_WAVE = kOsTile / WAVES;
     __shared__ uint32_t s_off[256];            // output position of the tile's run of each digit
@@ -2117,10 +2141,17 @@ __global__ __launch_bounds__(kOsThreads) void radix_onesweep_kernel(
             keys_out[pos] = key;
             vals_out[pos] = val;
             if (ranks_out) ranks_out[val] = static_cast<int32_t>(pos);
-            if (table_out)
+            if (rows) {
+                // one 16-byte gather per key
+                const float4 r = rows[val];
+                reinterpret_cast<float2 *>(table_out)[pos] = make_float2(r.x, r.y);
+                seg_out[pos] = __float_as_int(r.z);
+                deg_out[pos] = __float_as_int(r.w);
+            } else if (table_out) {
                 reinterpret_cast<float2 *>(table_out)[pos] =
                     reinterpret_cast<const float2 *>(table_src)[val];
-            if (seg_out) {
+            }
+            if (seg_out && !rows) {
                 const int32_t b = csr_off[val];
                 seg_out[pos] = b;
                 deg_out[pos] = csr_off[val + 1] - b;
@@ -2500,10 +2531,22 @@ void launch_rank_prefix(const int32_t *rank_deg, int64_t n, int32_t total_edges,
 
 void launch_unit_adam_ranked(int64_t h, const float *pos, float *pos_out, const float *grad,
                              uint8_t *split, const float *frag, float *m, float *v,
-                             float alpha, float c1, float c2, hipStream_t s, float *grad_out) {
+                             float alpha, float c1, float c2, hipStream_t s, float *grad_out,
+                             const int32_t *csr_off, float *rows_out) {
     if (h == 0) return;
+    UMAPHIP_REQUIRE(!rows_out || csr_off, UMAPHIP_INVALID_INPUT,
+                    "packed rows need the CSR offsets");
     hipLaunchKernelGGL(unit_adam_ranked_kernel, dim3(grid_for(h)), dim3(kThreads), 0, s, h, pos,
-                       pos_out, grad, split, frag, m, v, alpha, c1, c2, grad_out);
+                       pos_out, grad, split, frag, m, v, alpha, c1, c2, grad_out, csr_off,
+                       reinterpret_cast<float4 *>(rows_out));
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_pack_rows(const float *pos, const int32_t *csr_off, int64_t h, float *rows,
+                      hipStream_t s) {
+    if (h == 0) return;
+    hipLaunchKernelGGL(pack_rows_kernel, dim3(grid_for(h)), dim3(kThreads), 0, s, pos, csr_off, h,
+                       reinterpret_cast<float4 *>(rows));
     HIP_CHECK(hipGetLastError());
 }
 
@@ -2815,9 +2858,11 @@ void launch_project_sort_ranked(void *ws, size_t ws_bytes, const float *x, int64
                                 unsigned long long tag, uint32_t *keys, uint32_t *keys_out,
                                 int32_t *vals_out, int32_t *ranks, float *table_out, hipStream_t s,
                                 const int32_t *csr_off, int32_t *seg_out, int32_t *deg_out,
-                                int32_t *err) {
+                                int32_t *err, const float *rows) {
     if (n == 0) return;
     UMAPHIP_REQUIRE(err != nullptr, UMAPHIP_INVALID_INPUT, "projection sort needs an error word");
+    UMAPHIP_REQUIRE(!rows || (table_out && seg_out && deg_out), UMAPHIP_INVALID_INPUT,
+                    "packed rows feed the table and the rank-ordered CSR starts and degrees");
     const OnesweepWorkspace W = onesweep_workspace(ws, ws_bytes, n);
     const int64_t count_words = onesweep_count_words(n);
     hipLaunchKernelGGL(project_keys_totals_kernel, dim3(static_cast<unsigned>(sort_tiles(n))),
@@ -2836,7 +2881,8 @@ void launch_project_sort_ranked(void *ws, size_t ws_bytes, const float *x, int64
                            last ? W.totals : nullptr, err,
                            last ? ranks : nullptr, last ? x : nullptr, last ? table_out : nullptr,
                            last ? csr_off : nullptr, last ? seg_out : nullptr,
-                           last ? deg_out : nullptr);
+                           last ? deg_out : nullptr,
+                           last ? reinterpret_cast<const float4 *>(rows) : nullptr);
         src_k = dst_k;
         src_v = dst_v;
     }

@@ -179,11 +179,18 @@ int64_t rank_prefix_blocks(int64_t n);
 void launch_rank_prefix(const int32_t *rank_deg, int64_t n, int32_t total_edges, int32_t *pref,
                         int32_t *block_sums, int32_t *unit_k, hipStream_t s);
 // grad_out (may be null) receives each row's assembled gradient, the value
-// the Adam step consumes.
+// the Adam step consumes. With rows_out (4 floats per row), it also writes
+// each row's packed record {x, y, CSR start, degree} (launch_pack_rows).
 void launch_unit_adam_ranked(int64_t h, const float *pos, float *pos_out, const float *grad,
                              uint8_t *split, const float *frag, float *m, float *v,
                              float alpha, float c1, float c2, hipStream_t s,
-                             float *grad_out = nullptr);
+                             float *grad_out = nullptr, const int32_t *csr_off = nullptr,
+                             float *rows_out = nullptr);
+// rows[i] = {x_i, y_i, CSR start, degree} (the two integers as float bits)
+// for the projection sort's last pass: one 16-byte gather per key instead of
+// a position and two offsets.
+void launch_pack_rows(const float *pos, const int32_t *csr_off, int64_t h, float *rows,
+                      hipStream_t s);
 void launch_unit_adam(int64_t h, const float *pos, float *pos_out, const float *grad,
                       const int32_t *row_split, const int32_t *split_off, const float *frag,
                       float *m, float *v, float alpha, float c1, float c2, hipStream_t s,
@@ -228,6 +235,8 @@ void launch_sort_pairs_ranked(void *ws, size_t ws_bytes, const uint32_t *keys_in
 // per 8-bit pass, with ranks[row] = position and table_out[position] = x[row].
 // With csr_off (the CSR offsets of the n rows), the last pass also writes
 // seg_out[position] / deg_out[position], the CSR start and degree of the row.
+// With rows (launch_pack_rows of x and csr_off), the last pass reads the
+// position, CSR start and degree from that one record; ranks may be null.
 // A pass whose wait for earlier tiles exceeds its spin limit sets bit 2 of
 // *err. The workspace (project_sort_workspace bytes, n < 2^31) needs its
 // digit totals clear on entry (clear_project_sort_workspace; every sort
@@ -239,7 +248,7 @@ void launch_project_sort_ranked(void *ws, size_t ws_bytes, const float *x, int64
                                 unsigned long long tag, uint32_t *keys, uint32_t *keys_out,
                                 int32_t *vals_out, int32_t *ranks, float *table_out, hipStream_t s,
                                 const int32_t *csr_off, int32_t *seg_out, int32_t *deg_out,
-                                int32_t *err);
+                                int32_t *err, const float *rows = nullptr);
 void launch_values_from_keys(const uint32_t *keys, int64_t n, float *values, hipStream_t s);
 void launch_inverse_ranks(const int32_t *order, int64_t n, int32_t *ranks, hipStream_t s);
 void launch_lower_bound(const float *sorted, int64_t n, const float *query, int64_t h,
