@@ -212,6 +212,8 @@ SIGNATURES = {
     "umaphip_batch_set_shape": (C.c_int, [_vp, _i32]),
     "umaphip_batch_run": (C.c_int, [_vp, _i32]),
     "umaphip_batch_set_persistent": (C.c_int, [_vp, _i32]),
+    "umaphip_batch_set_clock_schedule": (C.c_int, [_vp, _i32]),
+    "umaphip_batch_clock_schedule": (_i32, [_vp]),
     "umaphip_batch_download": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "umaphip_batch_destroy": (None, [_vp]),
     "umaphip_classic_optimize": (C.c_int, [_i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _i64, _i32,

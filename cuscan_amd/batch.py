@@ -57,6 +57,16 @@ class LayoutBatch:
         _lib.check(_lib.lib().umaphip_batch_set_persistent(self._h, int(bool(persistent))))
         self.persistent = bool(persistent)
 
+    def set_clock_schedule(self, enabled: bool) -> None:
+        """Clock schedule on (the default when the batch allows it) or off
+        (same results either way)."""
+        _lib.check(_lib.lib().umaphip_batch_set_clock_schedule(self._h, int(bool(enabled))))
+
+    @property
+    def clock_schedule(self) -> int:
+        """Epochs the clock schedule covers (0: off)."""
+        return int(_lib.lib().umaphip_batch_clock_schedule(self._h))
+
     def tune(self, probe_epochs: int = 20):
         """Times each tile shape on the first epochs; keeps the fastest.
         Returns (shape index, [ms per shape])."""

@@ -17,6 +17,12 @@
 // set-up would have no grid barrier to meet at. The tile shape (rows per
 // tile x lanes) is chosen by timing the candidate shapes on the first epochs
 // in the chosen form (umaphip_batch_tune).
+//
+// The clocks do not depend on the positions: when every epoch's term counts
+// fit kBatchScheduleBytes (one byte per edge and epoch), a schedule pass
+// before the first epoch runs all clocks ahead (launch_edge_schedule) and
+// the tile sweeps read each edge's count instead of its four f64 clock
+// words. Results are the same bits either way.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -32,6 +38,25 @@
 #include "stdrng.hpp"
 
 using namespace umaphip;
+
+namespace {
+constexpr int64_t kBatchScheduleBytes = int64_t(8) << 30;
+
+// Whether every term count the clocks can produce fits a byte: in steady
+// state a firing edge draws at most (period + 1) / negative_period + 1
+// negatives, and the first firing at most (next_positive + 1 -
+// next_negative) / negative_period + 1.
+bool counts_fit_byte(const double *per, const double *nper, const double *npos,
+                     const double *nneg, int64_t E) {
+    for (int64_t e = 0; e < E; ++e) {
+        if (!(per[e] > 0.0) || !(nper[e] > 0.0)) return false;
+        const double steady = (per[e] + 1.0) / nper[e] + 1.0;
+        const double first = (npos[e] + 1.0 - nneg[e]) / nper[e] + 1.0;
+        if (!(steady < 250.0) || !(first < 250.0)) return false;
+    }
+    return true;
+}
+}  // namespace
 
 struct umaphip_batch {
     int32_t G = 0;
@@ -49,6 +74,19 @@ struct umaphip_batch {
     DeviceArray<double> periods, negperiods, npos, nneg, npos0, nneg0;
     DeviceArray<SweepSegment> segs;
     DeviceArray<int32_t> counters;      // persistent launch: tile counters per epoch and range
+    // clock schedule: every epoch's term counts, computed by one pass before
+    // the first epoch (sched_span = epochs), or off (0)
+    bool schedulable = false;
+    int32_t sched_span = 0;
+    int64_t sched_stride = 0;
+    DeviceArray<uint8_t> sched;
+    DeviceArray<int32_t> flag;          // bit 4: a term count overflowed its byte
+    // per-epoch negative tables (table[i] = position at rank i - r0 of the
+    // graph's order) for the per-epoch launch form: one gather per negative
+    DeviceArray<float> table;
+    DeviceArray<int32_t> graph_of_row;
+    DeviceArray<int64_t> row_start_dev, order_start_dev;
+    bool built_persistent = false;
     int shape = 0;
     int64_t tiles = 0;
     bool built = false;
@@ -69,7 +107,7 @@ struct umaphip_batch {
 
     // Segment arguments of every epoch for a tile shape.
     void build(int s) {
-        if (built && shape == s) return;
+        if (built && shape == s && built_persistent == persistent) return;
         if (exec) {
             HIP_CHECK(hipGraphExecDestroy(exec));
             exec = nullptr;
@@ -79,6 +117,7 @@ struct umaphip_batch {
             graph = nullptr;
         }
         shape = s;
+        built_persistent = persistent;
         const int64_t tr = tile_rows(s);
         std::vector<SweepSegment> host(static_cast<size_t>(epochs) * G);
         tiles = 0;
@@ -96,7 +135,8 @@ struct umaphip_batch {
                 A.pos = pos[t & 1].ptr + r0 * d;
                 A.pos_out = pos[(t + 1) & 1].ptr + r0 * d;
                 A.tgt = A.pos;
-                A.table = nullptr;
+                // the persistent launch reads positions through the order
+                A.table = persistent ? nullptr : table.ptr + r0 * d;
                 A.order = orders.ptr + order_start[g] + static_cast<int64_t>(t) * rows[g];
                 A.ranks = nullptr;
                 A.offsets = offsets.ptr + off_start[g];
@@ -113,6 +153,7 @@ struct umaphip_batch {
                 A.multiplier = 1.0f; A.alpha = alpha[t]; A.c1 = c1[t]; A.c2 = c2[t];
                 A.shared = 1; A.negatives = p.negative_sample_rate != 0;
                 A.exclude = p.exclude_graph_neighbors; A.fused = 1;
+                A.sched = sched_span ? sched.ptr + (t % sched_span) * sched_stride + e0 : nullptr;
                 host[static_cast<size_t>(t) * G + g].args = A;
                 host[static_cast<size_t>(t) * G + g].tile0 = tile0[g];
             }
@@ -125,7 +166,35 @@ struct umaphip_batch {
         built = true;
     }
 
+    // Schedule on (when the batch allows it) or off; segments are rebuilt.
+    void set_schedule(bool on) {
+        const int32_t span = on && schedulable ? epochs : 0;
+        if (span == sched_span && (span == 0 || sched.ptr)) return;
+        HIP_CHECK(hipStreamSynchronize(stream));
+        sched_span = span;
+        sched_stride = span ? unit_schedule_stride(total_edges) : 0;
+        sched.alloc(static_cast<size_t>(sched_stride * span));
+        if (exec) {
+            HIP_CHECK(hipGraphExecDestroy(exec));
+            exec = nullptr;
+        }
+        if (graph) {
+            HIP_CHECK(hipGraphDestroy(graph));
+            graph = nullptr;
+        }
+        built = false;
+    }
+
+    // the schedule pass that epoch t starts, if any
+    void schedule_from(int t, int count) {
+        if (sched_span && t % sched_span == 0)
+            launch_edge_schedule(npos.ptr, nneg.ptr, periods.ptr, negperiods.ptr, total_edges,
+                                 p.negative_sample_rate != 0, t, std::min(sched_span, count - t),
+                                 sched.ptr, sched_stride, flag.ptr, stream);
+    }
+
     void reset() {
+        HIP_CHECK(hipMemsetAsync(flag.ptr, 0, sizeof(int32_t), stream));
         HIP_CHECK(hipMemcpyAsync(pos[0].ptr, head0.ptr, head0.bytes(), hipMemcpyDeviceToDevice, stream));
         HIP_CHECK(hipMemsetAsync(m.ptr, 0, m.bytes(), stream));
         HIP_CHECK(hipMemsetAsync(v.ptr, 0, v.bytes(), stream));
@@ -137,12 +206,18 @@ struct umaphip_batch {
 
     void enqueue_epochs(int count) {
         if (persistent) {
+            // sched_span is 0 or every epoch: one pass covers the launch
+            schedule_from(0, count);
             HIP_CHECK(launch_persistent_segments(segs.ptr, G, tiles, count, counters.ptr, stream,
                                                  shape));
             return;
         }
-        for (int t = 0; t < count; ++t)
+        for (int t = 0; t < count; ++t) {
+            schedule_from(t, count);
+            launch_batch_table(pos[t & 1].ptr, orders.ptr, row_start_dev.ptr, order_start_dev.ptr,
+                               graph_of_row.ptr, total_rows, t, table.ptr, stream);
             launch_segment_sweep(segs.ptr + static_cast<size_t>(t) * G, G, tiles, stream, shape);
+        }
     }
 
     // the per-epoch launches of every epoch as one graph
@@ -156,7 +231,7 @@ struct umaphip_batch {
 
     // persistent: one cooperative launch whatever use_graph says
     void run(bool use_graph) {
-        if (!built) build(shape);
+        build(shape);   // no-op unless the shape or the launch form changed
         if (use_graph && !persistent) capture();
         reset();
         HIP_CHECK(hipEventRecord(ev0, stream));
@@ -275,6 +350,28 @@ umaphip_status umaphip_batch_create(int32_t G, const int64_t *rows, const int64_
             HIP_CHECK(hipMemcpy(B->npos0.ptr, next_positive, E * 8, hipMemcpyHostToDevice));
             HIP_CHECK(hipMemcpy(B->nneg0.ptr, next_negative, E * 8, hipMemcpyHostToDevice));
         }
+        B->table.alloc(hd);
+        {
+            std::vector<int32_t> gor(static_cast<size_t>(B->total_rows));
+            for (int32_t g = 0; g < G; ++g)
+                std::fill(gor.begin() + B->row_start[g], gor.begin() + B->row_start[g + 1], g);
+            B->graph_of_row.alloc(gor.size());
+            HIP_CHECK(hipMemcpy(B->graph_of_row.ptr, gor.data(), gor.size() * sizeof(int32_t),
+                                hipMemcpyHostToDevice));
+            B->row_start_dev.alloc(G + 1);
+            B->order_start_dev.alloc(G + 1);
+            HIP_CHECK(hipMemcpy(B->row_start_dev.ptr, B->row_start.data(), (G + 1) * sizeof(int64_t),
+                                hipMemcpyHostToDevice));
+            HIP_CHECK(hipMemcpy(B->order_start_dev.ptr, B->order_start.data(),
+                                (G + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+        }
+        B->flag.alloc(1);
+        HIP_CHECK(hipMemset(B->flag.ptr, 0, sizeof(int32_t)));
+        B->schedulable = B->total_edges > 0 && p.epochs > 0 &&
+                         unit_schedule_stride(B->total_edges) * p.epochs <= kBatchScheduleBytes &&
+                         counts_fit_byte(periods, negative_periods, next_positive, next_negative,
+                                         B->total_edges);
+        B->set_schedule(true);
         B->orders.alloc(orders.size());
         HIP_CHECK(hipMemcpy(B->orders.ptr, orders.data(), orders.size() * sizeof(int32_t),
                             hipMemcpyHostToDevice));
@@ -325,8 +422,21 @@ umaphip_status umaphip_batch_set_persistent(umaphip_batch *batch, int32_t persis
     return guarded([&] {
         UMAPHIP_REQUIRE(batch, UMAPHIP_INVALID_INPUT, "batch is null");
         HIP_CHECK(hipStreamSynchronize(batch->stream));
-        batch->persistent = persistent != 0;
+        batch->persistent = persistent != 0;   // segments are rebuilt for the form
     });
+}
+
+umaphip_status umaphip_batch_set_clock_schedule(umaphip_batch *batch, int32_t enabled) {
+    return guarded([&] {
+        UMAPHIP_REQUIRE(batch, UMAPHIP_INVALID_INPUT, "batch is null");
+        const int shape = batch->shape;
+        batch->set_schedule(enabled != 0);
+        batch->build(shape);
+    });
+}
+
+int32_t umaphip_batch_clock_schedule(const umaphip_batch *batch) {
+    return batch ? batch->sched_span : 0;
 }
 
 umaphip_status umaphip_batch_run(umaphip_batch *batch, int32_t use_graph) {
@@ -345,6 +455,12 @@ umaphip_status umaphip_batch_download(umaphip_batch *batch, float *head, double 
             float ms = 0.0f;
             if (batch->ran) HIP_CHECK(hipEventElapsedTime(&ms, batch->ev0, batch->ev1));
             *device_ms = ms;
+        }
+        if (batch->ran) {
+            int32_t flag = 0;
+            HIP_CHECK(hipMemcpy(&flag, batch->flag.ptr, sizeof(flag), hipMemcpyDeviceToHost));
+            UMAPHIP_REQUIRE((flag & 4) == 0, UMAPHIP_BACKEND,
+                            "clock schedule: an edge drew more than 254 negatives in one epoch");
         }
         const float *src = batch->ran ? batch->pos[batch->epochs & 1].ptr : batch->head0.ptr;
         if (head)

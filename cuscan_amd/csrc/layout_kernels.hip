@@ -512,8 +512,13 @@ __device__ inline void sweep_tile(const SweepArgs &A, int64_t tile) {
             }
             pf_r = lo;
             pf_e = s_seg[lo] + (v - s_off[lo]);
-            pf_npos = A.next_positive[pf_e];
             pf_idx = A.indices[pf_e];
+            if (A.sched) {
+                // term count from the clock schedule: 1 + negatives, 0 when idle
+                pf_npos = static_cast<double>(A.sched[pf_e]);
+                return;
+            }
+            pf_npos = A.next_positive[pf_e];
             pf_per = A.periods[pf_e];
             if (A.negatives) {
                 pf_nneg = A.next_negative[pf_e];
@@ -535,12 +540,16 @@ __device__ inline void sweep_tile(const SweepArgs &A, int64_t tile) {
         float2 retry_sum = make_float2(0.0f, 0.0f);
         if (has_edge) {
             r = r_cur;
-            if (s_active[r] && npos <= now) {
+            if (s_active[r] && (A.sched ? npos != 0.0 : npos <= now)) {
                 s_j[tid] = jidx;
-                A.next_positive[e] = npos + per;
-                if (A.negatives) {
-                    count = static_cast<int32_t>((now - nn) / np);
-                    A.next_negative[e] = nn + static_cast<double>(count) * np;
+                if (A.sched) {
+                    count = static_cast<int32_t>(npos) - 1;
+                } else {
+                    A.next_positive[e] = npos + per;
+                    if (A.negatives) {
+                        count = static_cast<int32_t>((now - nn) / np);
+                        A.next_negative[e] = nn + static_cast<double>(count) * np;
+                    }
                 }
                 const int negs = count > 0 ? count : 0;
                 if (retry && negs > 0) {
@@ -1397,6 +1406,36 @@ __global__ void unit_unpack_kernel(const double *rec, int64_t E, bool compact, d
 // edges and writes their bytes as one word per epoch (stride is a multiple
 // of 4 and at least E rounded up to one). A term count above 255 is stored
 // as 255 and ORs 4 into *flag.
+//
+// schedule_lanes runs the four edges e0..e0+3 of a lane through the span
+// (edges past the end carry next_positive = inf and never fire) and returns
+// whether a count overflowed.
+__device__ inline bool schedule_lanes(double (&npos)[4], double (&nn)[4], const double (&per)[4],
+                                      const double (&np)[4], bool negatives, int32_t t0,
+                                      int32_t span, uint8_t *sched, int64_t stride, int64_t e0) {
+    bool over = false;
+    for (int32_t i = 0; i < span; ++i) {
+        const double now = static_cast<double>(t0 + i);
+        uint32_t word = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (npos[k] <= now) {
+                npos[k] = npos[k] + per[k];
+                int32_t count = 0;
+                if (negatives) {
+                    count = static_cast<int32_t>((now - nn[k]) / np[k]);
+                    nn[k] = nn[k] + static_cast<double>(count) * np[k];
+                }
+                const int32_t terms = 1 + (count > 0 ? count : 0);
+                over = over || terms > 255;
+                word |= static_cast<uint32_t>(terms < 255 ? terms : 255) << (8 * k);
+            }
+        }
+        *reinterpret_cast<uint32_t *>(sched + i * stride + e0) = word;
+    }
+    return over;
+}
+
 __global__ void unit_schedule_kernel(double *rec, int64_t E, bool compact, double per_num,
                                      double rate, bool negatives, int32_t t0, int32_t span,
                                      uint8_t *sched, int64_t stride, int32_t *flag) {
@@ -1423,26 +1462,7 @@ __global__ void unit_schedule_kernel(double *rec, int64_t E, bool compact, doubl
         }
         np[k] = negatives ? (per[k] / rate) * 1.5 : 1.0;
     }
-    bool over = false;
-    for (int32_t i = 0; i < span; ++i) {
-        const double now = static_cast<double>(t0 + i);
-        uint32_t word = 0;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            if (npos[k] <= now) {
-                npos[k] = npos[k] + per[k];
-                int32_t count = 0;
-                if (negatives) {
-                    count = static_cast<int32_t>((now - nn[k]) / np[k]);
-                    nn[k] = nn[k] + static_cast<double>(count) * np[k];
-                }
-                const int32_t terms = 1 + (count > 0 ? count : 0);
-                over = over || terms > 255;
-                word |= static_cast<uint32_t>(terms < 255 ? terms : 255) << (8 * k);
-            }
-        }
-        *reinterpret_cast<uint32_t *>(sched + i * stride + e0) = word;
-    }
+    const bool over = schedule_lanes(npos, nn, per, np, negatives, t0, span, sched, stride, e0);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const int64_t e = e0 + k;
@@ -1452,6 +1472,49 @@ __global__ void unit_schedule_kernel(double *rec, int64_t E, bool compact, doubl
         rec[at + 1] = nn[k];
     }
     if (over) atomicOr(flag, 4);
+}
+
+// The same schedule over clocks kept as four f64 arrays (the tile sweep's
+// form): the negative period is read, not rebuilt.
+__global__ void edge_schedule_kernel(double *npos_a, double *nneg_a, const double *per_a,
+                                     const double *nper_a, int64_t E, bool negatives, int32_t t0,
+                                     int32_t span, uint8_t *sched, int64_t stride, int32_t *flag) {
+    const int64_t e0 = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) * 4;
+    if (e0 >= E) return;
+    double npos[4], nn[4], per[4], np[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int64_t e = e0 + k;
+        const bool in = e < E;
+        npos[k] = in ? npos_a[e] : INFINITY;   // past the end: never fires
+        nn[k] = in ? nneg_a[e] : 0.0;
+        per[k] = in ? per_a[e] : 1.0;
+        np[k] = in && negatives ? nper_a[e] : 1.0;
+    }
+    const bool over = schedule_lanes(npos, nn, per, np, negatives, t0, span, sched, stride, e0);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int64_t e = e0 + k;
+        if (e >= E) break;
+        npos_a[e] = npos[k];
+        nneg_a[e] = nn[k];
+    }
+    if (over) atomicOr(flag, 4);
+}
+
+// Negative tables of a batch for epoch t: for row i of graph g (rows r0 ..
+// r0 + h - 1), table[i] = pos[r0 + order_g,t[i - r0]], the position the
+// graph's epoch-t order puts at that rank.
+__global__ void batch_table_kernel(const float *pos, const int32_t *orders,
+                                   const int64_t *row_start, const int64_t *order_start,
+                                   const int32_t *graph_of_row, int64_t total_rows, int32_t t,
+                                   float *table) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= total_rows) return;
+    const int32_t g = graph_of_row[i];
+    const int64_t r0 = row_start[g], h = row_start[g + 1] - r0;
+    const int64_t j = static_cast<uint32_t>(orders[order_start[g] + t * h + (i - r0)]);
+    reinterpret_cast<float2 *>(table)[i] = ld2(pos, r0 + j);
 }
 
 // Lazy Adam over rows; rows split across units add their fragments in unit
@@ -2664,6 +2727,29 @@ void launch_unit_schedule(double *rec, int64_t E, bool compact, double per_num, 
                     static_cast<long long>(stride), static_cast<long long>(E));
     hipLaunchKernelGGL(unit_schedule_kernel, dim3(grid_for((E + 3) / 4)), dim3(kThreads), 0, s,
                        rec, E, compact, per_num, rate, negatives, t0, span, sched, stride, flag);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_edge_schedule(double *npos, double *nneg, const double *periods,
+                          const double *negative_periods, int64_t E, bool negatives, int32_t t0,
+                          int32_t span, uint8_t *sched, int64_t stride, int32_t *flag,
+                          hipStream_t s) {
+    if (E == 0 || span <= 0) return;
+    UMAPHIP_REQUIRE(stride % 4 == 0 && stride >= (E + 3) / 4 * 4, UMAPHIP_BACKEND,
+                    "clock schedule: slab stride %lld does not hold %lld edges",
+                    static_cast<long long>(stride), static_cast<long long>(E));
+    hipLaunchKernelGGL(edge_schedule_kernel, dim3(grid_for((E + 3) / 4)), dim3(kThreads), 0, s,
+                       npos, nneg, periods, negative_periods, E, negatives, t0, span, sched,
+                       stride, flag);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_batch_table(const float *pos, const int32_t *orders, const int64_t *row_start,
+                        const int64_t *order_start, const int32_t *graph_of_row,
+                        int64_t total_rows, int32_t t, float *table, hipStream_t s) {
+    if (total_rows == 0) return;
+    hipLaunchKernelGGL(batch_table_kernel, dim3(grid_for(total_rows)), dim3(kThreads), 0, s, pos,
+                       orders, row_start, order_start, graph_of_row, total_rows, t, table);
     HIP_CHECK(hipGetLastError());
 }
 

@@ -57,6 +57,9 @@ struct SweepArgs {
     uint32_t epoch; float a, b, gamma; uint32_t range;
     float multiplier, alpha, c1, c2;
     int shared, negatives, exclude, fused;
+    // tile sweep: this epoch's clock-schedule slab ([h rows' edges] term
+    // counts, launch_edge_schedule), or null to advance the clocks here
+    const uint8_t *sched;
 };
 void launch_edge_sweep(const SweepArgs &a, hipStream_t s);
 
@@ -76,6 +79,12 @@ void launch_segment_sweep(const SweepSegment *segs, int32_t n_segs, int64_t tota
 // barrier separates the epochs; counters: 8 * epochs ints of workspace
 // (cleared by the launcher). Results equal one launch_segment_sweep per
 // epoch, bit for bit. Returns the cooperative launch's status.
+// Negative tables of every graph of a batch for epoch t (2-D): table[i] =
+// pos[r0 + order[order_start[g] + t * h + (i - r0)]] for row i of graph g
+// (rows r0 .. r0 + h - 1, graph_of_row[i] = g).
+void launch_batch_table(const float *pos, const int32_t *orders, const int64_t *row_start,
+                        const int64_t *order_start, const int32_t *graph_of_row,
+                        int64_t total_rows, int32_t t, float *table, hipStream_t s);
 hipError_t launch_persistent_segments(const SweepSegment *segs, int32_t n_segs,
                                       int64_t total_tiles, int32_t epochs, int32_t *counters,
                                       hipStream_t s, int shape = 0);
@@ -164,6 +173,12 @@ void launch_unit_unpack(const double *rec, int64_t E, bool compact, double *npos
 // t (1 + negatives, 0 when it does not fire); the records advance to the
 // state after epoch t0 + span - 1. A count above 255 ORs 4 into *flag.
 int64_t unit_schedule_stride(int64_t E);
+// The same schedule over clocks kept as four f64 arrays (npos and nneg are
+// advanced in place), for the tile sweep's SweepArgs::sched.
+void launch_edge_schedule(double *npos, double *nneg, const double *periods,
+                          const double *negative_periods, int64_t E, bool negatives, int32_t t0,
+                          int32_t span, uint8_t *sched, int64_t stride, int32_t *flag,
+                          hipStream_t s);
 void launch_unit_schedule(double *rec, int64_t E, bool compact, double per_num, double rate,
                           bool negatives, int32_t t0, int32_t span, uint8_t *sched,
                           int64_t stride, int32_t *flag, hipStream_t s);

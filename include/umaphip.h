@@ -662,6 +662,18 @@ umaphip_status umaphip_batch_set_shape(umaphip_batch *batch, int32_t shape);
  * use_graph says), the workgroups meeting at a grid barrier between epochs;
  * 0 launches one segment sweep per epoch. Both give the same bits. */
 umaphip_status umaphip_batch_set_persistent(umaphip_batch *batch, int32_t persistent);
+/* Clock schedule of the batch (no reference counterpart; the reference
+ * advances both clocks inside its gradient kernel, kernels/standard_umap/
+ * modern_layout.rs:166-253): enabled = 1 (the default when allowed) runs
+ * every edge's clocks through all epochs in one pass before the first epoch
+ * and stores its term count per epoch (one byte per edge and epoch), which
+ * the tile sweeps read instead of the four f64 clock words; 0 advances the
+ * clocks inside the sweeps. Both give the same bits. A batch allows it when
+ * those bytes fit 8 GiB and every count provably fits a byte; otherwise it
+ * stays off. umaphip_batch_clock_schedule returns the epochs the schedule
+ * covers (0: off). */
+umaphip_status umaphip_batch_set_clock_schedule(umaphip_batch *batch, int32_t enabled);
+int32_t umaphip_batch_clock_schedule(const umaphip_batch *batch);
 /* Resets to the uploaded initial state and enqueues every epoch: as one
  * hipGraph launch (use_graph = 1, captured on first use) or as plain
  * launches. Does not synchronise. */

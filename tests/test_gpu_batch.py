@@ -72,6 +72,22 @@ def test_batch_matches_single_graph_plans_and_replays(exclude):
     for a, b in zip(r1, r4):
         assert (a == b).all()
     assert (p1 == p4).all() and (n1 == n4).all()
+    # the clock schedule (the default) and clocks advanced inside the sweeps
+    # give the same bits, in both launch forms
+    assert batch.clock_schedule == epochs
+    batch.set_clock_schedule(False)
+    assert batch.clock_schedule == 0
+    batch.run(use_graph=True)
+    r5, _, p5, n5 = batch.download(clocks=True)
+    batch.set_persistent(True)
+    batch.run()
+    r6, _, p6, n6 = batch.download(clocks=True)
+    batch.set_persistent(False)
+    batch.set_clock_schedule(True)
+    assert batch.clock_schedule == epochs
+    for a, b, c in zip(r1, r5, r6):
+        assert (a == b).all() and (a == c).all()
+    assert (p1 == p5).all() and (p1 == p6).all() and (n1 == n5).all() and (n1 == n6).all()
     e0 = 0
     for g, (graph, head) in enumerate(zip(graphs, heads)):
         p = L.LayoutParams(epochs=epochs, a=A_CURVE, b=B_CURVE, exclude_graph_neighbors=exclude,
