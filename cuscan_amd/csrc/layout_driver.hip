@@ -266,7 +266,7 @@ struct umaphip_plan {
     // {x, y, CSR start, degree} per row, kept by the Adam step for the
     // refresh's last sort pass (rank-ordered plans)
     DeviceArray<float> rows4;
-    // Clock schedule of the rank-ordered sweep: each edge's term count per
+    // Clock schedule of the edge-balanced sweep: each edge's term count per
     // epoch, computed sched_span epochs at a time (0: off, the sweep
     // advances the clocks from the records every epoch).
     bool schedulable = false;
@@ -516,9 +516,6 @@ void umaphip_plan::run() {
             mark(sweep_ev, sweep_used);   // schedule passes count as sweep time
             const uint8_t *sched_now = nullptr;
             if (unit_path && sched_span > 0) {
-                // every epoch of a schedulable plan sweeps by rank
-                UMAPHIP_REQUIRE(ranked && projection_ready, UMAPHIP_BACKEND,
-                                "clock schedule: epoch %d without a projection order", t);
                 const int slab = t % sched_span;
                 if (slab == 0)
                     launch_unit_schedule(rec.ptr, E, compact, static_cast<double>(p.epochs),
@@ -542,9 +539,10 @@ void umaphip_plan::run() {
                 U.rate = static_cast<double>(p.negative_sample_rate);
                 // with this epoch's projection order, sweep the rows by rank
                 const bool by_rank = ranked && projection_ready;
-                // The rank-ordered sweep decides firing from next_positive in
-                // the record and leaves the firing shadow behind; the id-order
-                // sweep reads the shadow, so it cannot follow a ranked epoch.
+                // Without the clock schedule, the rank-ordered sweep decides
+                // firing from next_positive in the record and leaves the
+                // firing shadow behind; the id-order sweep reads the shadow,
+                // so it cannot follow a ranked epoch.
                 UMAPHIP_REQUIRE(by_rank || ranked_epochs == 0, UMAPHIP_BACKEND,
                                 "unit sweep: id-order epoch after a rank-ordered epoch");
                 U.row_cap = by_rank ? row_cap_ranked : row_cap_id;
@@ -856,7 +854,7 @@ umaphip_status umaphip_plan_create(int64_t h, int64_t n, int64_t d, const float 
             // term counts (one byte per edge and epoch); a firing edge draws
             // at most about 1.34 rate + 3 negatives, so rate <= 64 keeps
             // every count in a byte.
-            P->schedulable = P->ranked && params->negative_sample_rate <= 64;
+            P->schedulable = params->negative_sample_rate <= 64;
             P->configure_schedule(-1);
             // Row-table capacity: the rows of a unit in CSR order are known;
             // in any rank order a unit of UE slots holds at most
@@ -1004,8 +1002,8 @@ umaphip_status umaphip_plan_set_clock_schedule(umaphip_plan *plan, int32_t span)
     return guarded([&] {
         UMAPHIP_REQUIRE(plan != nullptr, UMAPHIP_INVALID_INPUT, "plan is null");
         UMAPHIP_REQUIRE(span <= 0 || plan->schedulable, UMAPHIP_INVALID_CONFIG,
-                        "a clock schedule needs a rank-ordered plan with negative_sample_rate "
-                        "<= 64");
+                        "a clock schedule needs a plan on the edge-balanced sweep with "
+                        "negative_sample_rate <= 64");
         HIP_CHECK(hipStreamSynchronize(plan->stream));
         plan->configure_schedule(span);
     });

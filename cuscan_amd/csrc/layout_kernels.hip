@@ -943,8 +943,8 @@ __device__ inline void segmented_accumulate(float2 term, int row, int lane, floa
 // RC: capacity of the unit row table; the plan picks 64 when no unit can
 // hold more rows, which keeps the workgroup at about 7 KB of LDS (6 waves
 // per SIMD) instead of about 14 KB (3 waves).
-// FMT (rank-ordered only for 1 and 2): how a firing edge is found and its
-// terms counted.
+// FMT (rank-ordered only for 1): how a firing edge is found and its terms
+// counted.
 //   0: 32-byte records (above), clocks advanced here;
 //   1: 24-byte records, the period rebuilt as per_num / samples and e mod M
 //      from the edge index, clocks advanced here;
@@ -956,7 +956,7 @@ __device__ inline void segmented_accumulate(float2 term, int row, int lane, floa
 template <bool RANKED, int RC, int FMT, bool TRACE>
 __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
     constexpr bool COMPACT = FMT == 1, SCHED = FMT == 2;
-    static_assert(RANKED || FMT == 0, "compact records and schedules are read by rank only");
+    static_assert(RANKED || FMT != 1, "compact records are read by rank only");
     const SweepArgs &A = U.s;
     constexpr int EPL = kUnitEPL, UE = kUnitEdges, TCAP = kUnitTermCap, G = kUnitGroups;
     __shared__ uint16_t s_off[RC + 1];  // unit-local first edge of each unit row
@@ -998,7 +998,20 @@ __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
     int64_t eidx[EPL];
     double r_npos[EPL], r_nn[EPL], r_per[EPL];
     uint64_t r_w[EPL];
-    if constexpr (!RANKED) {
+    if constexpr (!RANKED && SCHED) {
+        // the unit's edges are contiguous: term counts and columns at once
+#pragma unroll
+        for (int k = 0; k < EPL; ++k) {
+            const int v = k * 64 + lane;
+            eidx[k] = e0 + v;
+            fire[k] = 0;   // term count (0: the edge does not fire)
+            r_w[k] = 0;
+            if (v < ue) {
+                fire[k] = U.sched[e0 + v];
+                r_w[k] = static_cast<uint32_t>(A.indices[e0 + v]);
+            }
+        }
+    } else if constexpr (!RANKED) {
 #pragma unroll
         for (int k = 0; k < EPL; ++k) {
             const int v = k * 64 + lane;
@@ -1228,7 +1241,7 @@ __global__ __launch_bounds__(64) void unit_sweep_kernel(UnitSweepArgs U) {
                 uint32_t em;
                 if constexpr (SCHED) {
                     // e < 2^31: exact in f64
-                    const int64_t e = s_seg[rr] + (e0 + owner - s_pst[rr]);
+                    const int64_t e = RANKED ? s_seg[rr] + (e0 + owner - s_pst[rr]) : e0 + owner;
                     em = mod_f64(static_cast<double>(static_cast<uint32_t>(e)), mod_d, U.inv_mod);
                 } else {
                     em = s_em[owner];
@@ -1394,7 +1407,7 @@ __global__ void unit_unpack_kernel(const double *rec, int64_t E, bool compact, d
     nneg[e] = rec[at + 1];
 }
 
-// Clock schedule of the rank-ordered sweep for epochs [t0, t0 + span):
+// Clock schedule of the unit sweep for epochs [t0, t0 + span):
 // sched[(t - t0) * stride + e] = 1 + the negatives edge e draws when it
 // fires in epoch t, 0 when it does not fire. Neither clock depends on the
 // positions, so one pass runs both through the span with the f64
@@ -2553,6 +2566,7 @@ void launch_unit_sweep_shape(const UnitSweepArgs &a, dim3 grid, bool narrow, hip
     if (a.pref && a.sched) launch_unit_sweep_rows<true, 2, TRACE>(a, grid, narrow, s);
     else if (a.pref && a.compact) launch_unit_sweep_rows<true, 1, TRACE>(a, grid, narrow, s);
     else if (a.pref) launch_unit_sweep_rows<true, 0, TRACE>(a, grid, narrow, s);
+    else if (a.sched) launch_unit_sweep_rows<false, 2, TRACE>(a, grid, narrow, s);
     else launch_unit_sweep_rows<false, 0, TRACE>(a, grid, narrow, s);
 }
 
@@ -2564,8 +2578,8 @@ void launch_unit_sweep(UnitSweepArgs a, hipStream_t s) {
     a.xcd_chunk = (a.units + 7) / 8;
     const dim3 grid(static_cast<unsigned>(8 * a.xcd_chunk));
     const bool narrow = a.row_cap == kUnitNarrowRows;
-    UMAPHIP_REQUIRE(a.pref || (!a.compact && !a.sched), UMAPHIP_BACKEND,
-                    "unit sweep: compact records and clock schedules need the rank-ordered sweep");
+    UMAPHIP_REQUIRE(a.pref || !a.compact, UMAPHIP_BACKEND,
+                    "unit sweep: compact records need the rank-ordered sweep");
     if (a.tr_count) {
         UMAPHIP_REQUIRE(a.tr_cap > 0 && a.tr_cand && a.tr_kept && a.tr_overflow && a.tr_paths,
                         UMAPHIP_INVALID_INPUT, "unit sweep trace buffers are null");

@@ -144,9 +144,8 @@ struct UnitSweepArgs {
     double inv_mod;            // 1 / M (set by the launcher)
     bool compact;              // 24-byte records (rank-ordered sweep only)
     double per_num;            // compact: period = per_num / samples
-    // rank-ordered sweep: this epoch's slab of the clock schedule ([E] term
-    // counts, 0 when the edge does not fire), or null to advance the clocks
-    // from the records
+    // this epoch's slab of the clock schedule ([E] term counts, 0 when the
+    // edge does not fire), or null to advance the clocks from the records
     const uint8_t *sched;
     // Epoch trace (null tr_count: off). Per CSR edge: tr_count[e] = the
     // negatives drawn this epoch (left untouched when the edge does not
@@ -168,7 +167,7 @@ void launch_unit_pack(const double *npos, const double *nneg, const double *peri
                       uint16_t *fire, hipStream_t s);
 void launch_unit_unpack(const double *rec, int64_t E, bool compact, double *npos, double *nneg,
                         hipStream_t s);
-// Clock schedule of epochs [t0, t0 + span) for the rank-ordered sweep: slab
+// Clock schedule of epochs [t0, t0 + span) for the unit sweep: slab
 // t - t0 (at sched + (t - t0) * stride) holds each edge's term count in epoch
 // t (1 + negatives, 0 when it does not fire); the records advance to the
 // state after epoch t0 + span - 1. A count above 255 ORs 4 into *flag.

@@ -141,7 +141,7 @@ class LayoutPlan:
         _lib.check(_lib.lib().umaphip_plan_set_record_format(self._handle, int(bool(compact))))
 
     def set_clock_schedule(self, span: int = -1) -> None:
-        """Clock schedule of a rank-ordered plan: span < 0 automatic (the
+        """Clock schedule of a plan on the edge-balanced sweep: span < 0 automatic (the
         default), 0 off (the sweep advances the clocks from the records),
         span > 0 at most that many epochs per schedule pass."""
         _lib.check(_lib.lib().umaphip_plan_set_clock_schedule(self._handle, int(span)))

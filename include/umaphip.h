@@ -196,7 +196,7 @@ typedef struct {
                                        in projection order */
     int64_t compact_records;        /* 1 when the rank-ordered sweep kept 24-byte edge records
                                        (every period rebuilt exactly from an f32 sample count) */
-    int64_t clock_schedule;         /* epochs per clock-schedule pass of the rank-ordered sweep,
+    int64_t clock_schedule;         /* epochs per clock-schedule pass of the edge-balanced sweep,
                                        0 when the sweep advances the clocks itself */
 } umaphip_layout_stats;
 
@@ -248,7 +248,7 @@ umaphip_status umaphip_plan_set_profiling(umaphip_plan *plan, int32_t enabled);
  * the f64 period. Results are identical; plans default to 24 bytes when
  * allowed. */
 umaphip_status umaphip_plan_set_record_format(umaphip_plan *plan, int32_t compact);
-/* Clock schedule of a rank-ordered plan (no reference counterpart; the
+/* Clock schedule of a plan on the edge-balanced sweep (no reference counterpart; the
  * reference advances both clocks inside its gradient kernel,
  * kernels/standard_umap/modern_layout.rs:166-253). The clocks do not depend
  * on the positions, so a pass at the start of every span epochs runs them
@@ -257,8 +257,9 @@ umaphip_status umaphip_plan_set_record_format(umaphip_plan *plan, int32_t compac
  * edge record. span < 0 picks the span automatically (all epochs in one
  * pass while the bytes fit 4 GiB; the default), 0 turns the schedule off,
  * span > 0 caps the epochs per pass. Results are identical in every form.
- * Needs a rank-ordered plan with negative_sample_rate <= 64 (any span > 0
- * on another plan is an error; span < 0 leaves it off). */
+ * Needs a plan on the edge-balanced sweep (Adam, d = 2, more than 1024
+ * rows) with negative_sample_rate <= 64 (any span > 0 on another plan is an
+ * error; span < 0 leaves it off). */
 umaphip_status umaphip_plan_set_clock_schedule(umaphip_plan *plan, int32_t span);
 /* Epoch trace, a test hook with no reference counterpart: it stands where
  * the reference's runtime observers watch one epoch's state

@@ -353,7 +353,7 @@ def test_driver_matches_oracle_loop(case):
     assert stats["rank_ordered_epochs"] == (epochs if name in RANKED_PATH else 0)
     # the graph's periods come from the same epoch count: 24-byte records
     assert stats["compact_records"] == (name in RANKED_PATH)
-    assert stats["clock_schedule"] == (epochs if name in RANKED_PATH else 0)
+    assert stats["clock_schedule"] == (epochs if name in UNIT_PATH else 0)
     assert (gp == ep).all() and (gn == en).all()
     assert stats["adaptations"] == est["adaptations"]
     assert stats["refreshes"] == est["refreshes"]

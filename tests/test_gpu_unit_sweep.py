@@ -195,11 +195,16 @@ def _params(**kw):
 
 @pytest.mark.parametrize("epoch", [1, 50, 199])
 def test_cfg2_csr_order_exclusion(cfg2_graph, epoch):
-    """Global negatives with exclusion: rows swept in CSR order."""
+    """Global negatives with exclusion: rows swept in CSR order, term counts
+    from the clock schedule (epochs 1 and 199) or clocks advanced by the
+    sweep from the records and firing shadow (epoch 50)."""
     params = _params(exclude_graph_neighbors=True)
     print(f"[gpu] unit sweep cfg2 epoch {epoch}", flush=True)
-    tr, _, _ = traced_epoch(cfg2_graph, params, epoch)
-    assert not tr["rank_ordered"]
+    span = 0 if epoch == 50 else None
+    tr, npos, nneg = traced_epoch(cfg2_graph, params, epoch, schedule=span)
+    assert not tr["rank_ordered"] and tr["clock_schedule"] == (0 if span == 0 else 200)
+    want = advance_graph(cfg2_graph, 200)
+    assert (npos == want["next_positive"]).all() and (nneg == want["next_negative"]).all()
     fires, _ = check_trace(cfg2_graph, params, epoch, tr)
     assert fires > 0
 
