@@ -700,7 +700,9 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
                 "traffic": measured_traffic() if args.rows == 1_000_000 else None,
-                "kernel": "unit_sweep_kernel + unit_adam_kernel (edge pass + lazy Adam)",
+                "kernel": "unit_schedule_kernel (its passes spread over the epochs) + rank "
+                          "prefix + unit_sweep_kernel + unit_adam_ranked_kernel: the edge pass "
+                          "and lazy Adam of one epoch",
                 "bytes_per_launch": int(bytes_per_launch), "avg_launch_ms": round(sweep_avg_ms, 4),
                 "firing_edges_per_epoch": F / args.epochs, "negatives_per_epoch": S / args.epochs,
                 # B_gather: the 8-byte position gathers (tails and negative table rows),
