@@ -1539,9 +1539,10 @@ int KnnIndex::sample_size(int64_t q) const {
 
 int KnnIndex::slices_for(int64_t q) const {
     // Enough workgroups to fill the chip (256 CUs, two resident each), but
-    // slices of at least kMinSliceRefs references.
+    // slices of at least kMinSliceRefs references. Rounded down: 513 cells
+    // would run one cell in a second round and double the scan.
     const int64_t blocks = (q + kTQ - 1) / kTQ;
-    int64_t s = (512 + blocks - 1) / blocks;
+    int64_t s = 512 / blocks;
     s = std::min<int64_t>(s, std::max<int64_t>(1, n_ / kMinSliceRefs));
     s = std::max<int64_t>(1, std::min<int64_t>(s, kMaxSlices));
     return static_cast<int>(s);

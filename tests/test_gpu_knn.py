@@ -104,14 +104,16 @@ def test_continuous_data_recall():
     assert (gi == ei).all() and (gd.view(np.uint32) == ed.view(np.uint32)).all()
 
 
-@pytest.mark.parametrize("q", [1, 37, 64, 100])
+@pytest.mark.parametrize("q", [1, 37, 64, 100, 384, 700])
 def test_most_slices_bit_exact(q):
     """1,048,600 references: batches of up to 128 rows take 512 slices (the
     most a batch takes) of 2,064 references, the last one ragged, so the
     scan runs 512 DMA-staged workgroups (64-query blocks up to 64 rows) and
-    the merge lists more than 256 slices per query. The thresholds start
-    from the 32,768-row sample bound; the merged prefilter lists equal those
-    of a 9,472-row batch (7 slices, no bound)."""
+    the merge lists more than 256 slices per query; 384 and 700 rows take
+    170 and 85 slices (3 and 6 query blocks: 510 cells, the slice count
+    rounded down so that every cell is resident at once). The thresholds
+    start from the 32,768-row sample bound; the merged prefilter lists equal
+    those of a 9,472-row batch (6 slices, no bound)."""
     from cuscan_amd.knn import KnnIndex
 
     ref = image_like(1_048_600, 20, 10, 21)
