@@ -1632,8 +1632,9 @@ __global__ void pack_rows_kernel(const float *pos, const int32_t *csr_off, int64
 
 // Exclusive prefix sum of the row degrees in projection order:
 // pref[r] = sum over ranks below r of degree(order[r']), pref[n] = E.
-// Pass 1 scans blocks of kPrefBlock ranks, pass 2 scans the block totals in
-// one workgroup, pass 3 adds them.
+// Pass 1 scans blocks of kPrefBlock ranks and writes each block's total;
+// pass 2 adds to each rank the totals of the blocks before its own (every
+// workgroup sums them itself: at most n / kPrefBlock values, in L2).
 constexpr int kPrefThreads = 256, kPrefPer = 4, kPrefBlock = kPrefThreads * kPrefPer;
 
 __device__ inline int32_t block_exclusive_scan(int32_t x, int32_t *s_warp, int32_t &total) {
@@ -1676,31 +1677,26 @@ __global__ __launch_bounds__(kPrefThreads) void rank_prefix_kernel(const int32_t
     if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
 }
 
-__global__ __launch_bounds__(kPrefThreads) void prefix_blocks_kernel(int32_t *block_sums,
-                                                                     int64_t nb) {
-    __shared__ int32_t s_warp[kPrefThreads / 64];
-    int32_t carry = 0;
-    for (int64_t b0 = 0; b0 < nb; b0 += kPrefThreads) {
-        const int64_t b = b0 + threadIdx.x;
-        const int32_t x = b < nb ? block_sums[b] : 0;
-        int32_t total;
-        const int32_t ex = block_exclusive_scan(x, s_warp, total);
-        if (b < nb) block_sums[b] = carry + ex;
-        carry += total;
-    }
-}
-
 // Adds the block offsets and scatters each rank into the units whose first
-// or last slot it holds (rank r covers slots [pref[r], pref[r] + deg)).
+// or last slot it holds (rank r covers slots [pref[r], pref[r] + deg)). A
+// workgroup's ranks lie in one block (kPrefBlock is a multiple of
+// kPrefThreads); it sums the totals of the blocks before it.
 __global__ __launch_bounds__(kPrefThreads) void prefix_units_kernel(int32_t *pref, int64_t n,
                                                                     const int32_t *block_sums,
                                                                     const int32_t *rank_deg,
                                                                     int32_t total_edges,
                                                                     int32_t *unit_k) {
+    static_assert(kPrefBlock % kPrefThreads == 0, "a workgroup's ranks lie in one block");
+    __shared__ int32_t s_warp[kPrefThreads / 64];
     const int64_t r = static_cast<int64_t>(blockIdx.x) * kPrefThreads + threadIdx.x;
+    const int64_t blk = static_cast<int64_t>(blockIdx.x) * kPrefThreads / kPrefBlock;
+    int32_t part = 0;
+    for (int64_t i = threadIdx.x; i < blk; i += kPrefThreads) part += block_sums[i];
+    int32_t before;
+    (void)block_exclusive_scan(part, s_warp, before);   // before = the workgroup's total
     if (r == 0) pref[n] = total_edges;
     if (r >= n) return;
-    const int64_t b = pref[r] + block_sums[r / kPrefBlock];
+    const int64_t b = pref[r] + before;
     pref[r] = static_cast<int32_t>(b);
     const int64_t en = b + rank_deg[r];
     if (en == b) return;
@@ -2598,7 +2594,6 @@ void launch_rank_prefix(const int32_t *rank_deg, int64_t n, int32_t total_edges,
     const int64_t nb = rank_prefix_blocks(n);
     hipLaunchKernelGGL(rank_prefix_kernel, dim3(static_cast<unsigned>(nb)), dim3(kPrefThreads), 0,
                        s, rank_deg, n, pref, block_sums);
-    hipLaunchKernelGGL(prefix_blocks_kernel, dim3(1), dim3(kPrefThreads), 0, s, block_sums, nb);
     hipLaunchKernelGGL(prefix_units_kernel, dim3(static_cast<unsigned>((n + kPrefThreads - 1) /
                                                                        kPrefThreads)),
                        dim3(kPrefThreads), 0, s, pref, n, block_sums, rank_deg, total_edges,
