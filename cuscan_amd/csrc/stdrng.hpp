@@ -94,6 +94,18 @@ public:
     // matters).
     void shuffle(int32_t *a, size_t n, bool permute = true) {
         if (n <= 1) return;
+        if (!permute) {
+            // The words drawn depend only on the chunk boundaries: element 0
+            // draws nothing, and a chunk starting at element i (span i + 1)
+            // draws below(product) once and covers `factors` elements.
+            for (size_t i = 1; i < n;) {
+                uint32_t product, factors;
+                chunk_bound(static_cast<uint32_t>(i + 1), product, factors);
+                (void)below(product);
+                i += factors;
+            }
+            return;
+        }
         uint32_t seen = 0, packed = 0, left = 1;
         for (size_t i = 0; i < n; ++i) {
             const uint32_t span = seen + 1;
@@ -133,34 +145,43 @@ private:
         factors = static_cast<uint32_t>(next - m);
     }
 
-    static inline uint32_t rotl(uint32_t v, int c) { return (v << c) | (v >> (32 - c)); }
+    // The four blocks of a refill are computed side by side: word w of the
+    // four blocks is one 4-lane vector (GCC/Clang vector extension; SSE2 on
+    // x86-64), so every step of a quarter round is one vector operation. The
+    // words are those of four separate block computations.
+    typedef uint32_t v4u __attribute__((vector_size(16)));
 
-    static inline void quarter(uint32_t *x, int a, int b, int c, int d) {
-        x[a] += x[b]; x[d] = rotl(x[d] ^ x[a], 16);
-        x[c] += x[d]; x[b] = rotl(x[b] ^ x[c], 12);
-        x[a] += x[b]; x[d] = rotl(x[d] ^ x[a], 8);
-        x[c] += x[d]; x[b] = rotl(x[b] ^ x[c], 7);
+    static inline v4u rotv(v4u v, int c) { return (v << c) | (v >> (32 - c)); }
+
+    static inline void quarter4(v4u *x, int a, int b, int c, int d) {
+        x[a] += x[b]; x[d] = rotv(x[d] ^ x[a], 16);
+        x[c] += x[d]; x[b] = rotv(x[b] ^ x[c], 12);
+        x[a] += x[b]; x[d] = rotv(x[d] ^ x[a], 8);
+        x[c] += x[d]; x[b] = rotv(x[b] ^ x[c], 7);
     }
 
     void refill() {
-        for (unsigned blk = 0; blk < 4; ++blk) {
-            uint32_t in[16];
-            in[0] = 0x61707865u; in[1] = 0x3320646eu; in[2] = 0x79622d32u; in[3] = 0x6b206574u;
-            std::memcpy(in + 4, key_, sizeof(key_));
-            const uint64_t c = counter_ + blk;
-            in[12] = static_cast<uint32_t>(c);
-            in[13] = static_cast<uint32_t>(c >> 32);
-            in[14] = 0;
-            in[15] = 0;
-            uint32_t x[16];
-            std::memcpy(x, in, sizeof(in));
-            for (int round = 0; round < 6; ++round) {  // 12 rounds
-                quarter(x, 0, 4, 8, 12); quarter(x, 1, 5, 9, 13);
-                quarter(x, 2, 6, 10, 14); quarter(x, 3, 7, 11, 15);
-                quarter(x, 0, 5, 10, 15); quarter(x, 1, 6, 11, 12);
-                quarter(x, 2, 7, 8, 13); quarter(x, 3, 4, 9, 14);
-            }
-            for (int w = 0; w < 16; ++w) words_[blk * 16 + w] = x[w] + in[w];
+        v4u in[16], x[16];
+        const uint32_t sigma[4] = {0x61707865u, 0x3320646eu, 0x79622d32u, 0x6b206574u};
+        for (int w = 0; w < 4; ++w) in[w] = v4u{sigma[w], sigma[w], sigma[w], sigma[w]};
+        for (int w = 0; w < 8; ++w) in[4 + w] = v4u{key_[w], key_[w], key_[w], key_[w]};
+        for (int l = 0; l < 4; ++l) {
+            const uint64_t c = counter_ + static_cast<uint64_t>(l);
+            in[12][l] = static_cast<uint32_t>(c);
+            in[13][l] = static_cast<uint32_t>(c >> 32);
+        }
+        in[14] = v4u{0u, 0u, 0u, 0u};
+        in[15] = v4u{0u, 0u, 0u, 0u};
+        for (int w = 0; w < 16; ++w) x[w] = in[w];
+        for (int round = 0; round < 6; ++round) {  // 12 rounds
+            quarter4(x, 0, 4, 8, 12); quarter4(x, 1, 5, 9, 13);
+            quarter4(x, 2, 6, 10, 14); quarter4(x, 3, 7, 11, 15);
+            quarter4(x, 0, 5, 10, 15); quarter4(x, 1, 6, 11, 12);
+            quarter4(x, 2, 7, 8, 13); quarter4(x, 3, 4, 9, 14);
+        }
+        for (int w = 0; w < 16; ++w) {
+            const v4u o = x[w] + in[w];
+            for (int l = 0; l < 4; ++l) words_[l * 16 + w] = o[l];
         }
         counter_ += 4;
         pos_ = 0;
