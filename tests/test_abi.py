@@ -59,6 +59,7 @@ def _oracle_orders(h, n, seed, momentum, shared, hard, epoch):
     (70, 500, False, False, 200_000),     # transform, global
     (2000, 2000, True, True, 500),        # fit momentum, projection (retired)
     (1500, 1500, True, False, 200_000),   # 1500 < Q
+    (70_000, 70_000, True, False, 200_000),  # spans above 65,535: one draw per element
 ])
 def test_orders_match_oracle_rng(h, n, shared, momentum, q):
     n_eff = h if shared else n
