@@ -880,10 +880,12 @@ __global__ __launch_bounds__(THREADS) void transform_layout_kernel(const Transfo
 // id-order sweep reads it to load only the records of firing edges; the
 // rank-ordered sweep reads every record of its rows (a row's records are
 // contiguous) and tests next_positive <= t itself, which saves the
-// dependent shadow load.
+// dependent shadow load. With a clock schedule (unit_schedule_kernel, the
+// default) both orders read one term-count byte and the CSR column per edge
+// instead, and neither the records nor the shadow.
 // Per-term cost is not what bounds these sweeps: each unit waits on a chain
-// of dependent loads (rows, records, gathers), so occupancy (registers and
-// LDS) and the number of round trips set the rate.
+// of dependent loads (rows, edge words, gathers), so occupancy (registers
+// and LDS) and the number of round trips set the rate.
 constexpr int kUnitEPL = 4;
 constexpr int kUnitEdges = 64 * kUnitEPL;
 constexpr int kUnitTermCap = kUnitEdges * 4;
