@@ -173,7 +173,16 @@ struct umaphip_batch {
         HIP_CHECK(hipStreamSynchronize(stream));
         sched_span = span;
         sched_stride = span ? unit_schedule_stride(total_edges) : 0;
-        sched.alloc(static_cast<size_t>(sched_stride * span));
+        try {
+            sched.alloc(static_cast<size_t>(sched_stride * span));
+        } catch (const Fail &f) {
+            // an optimisation: without the memory the sweeps advance the clocks
+            sched_span = 0;
+            sched_stride = 0;
+            if (f.status != UMAPHIP_MEMORY) throw;
+            (void)hipGetLastError();
+            set_error("%s", "");
+        }
         if (exec) {
             HIP_CHECK(hipGraphExecDestroy(exec));
             exec = nullptr;

@@ -313,7 +313,16 @@ struct umaphip_plan {
         int64_t span = std::max<int64_t>(std::min<int64_t>(p.epochs, kScheduleBytes / sched_stride), 1);
         if (cap > 0) span = std::min<int64_t>(span, cap);
         sched_span = static_cast<int32_t>(span);
-        sched.alloc(static_cast<size_t>(sched_stride * span));
+        try {
+            sched.alloc(static_cast<size_t>(sched_stride * span));
+        } catch (const Fail &f) {
+            // the automatic schedule is an optimisation: without the memory
+            // the sweep advances the clocks itself
+            sched_span = 0;
+            if (cap > 0 || f.status != UMAPHIP_MEMORY) throw;
+            (void)hipGetLastError();
+            set_error("%s", "");
+        }
     }
 
     void mark(std::vector<hipEvent_t> &pool, int &used) {
