@@ -257,6 +257,7 @@ umaphip_status umaphip_plan_set_record_format(umaphip_plan *plan, int32_t compac
  * edge record. span < 0 picks the span automatically (all epochs in one
  * pass while the bytes fit 4 GiB; the default), 0 turns the schedule off,
  * span > 0 caps the epochs per pass. Results are identical in every form.
+ * The automatic schedule stays off when its memory cannot be allocated.
  * Needs a plan on the edge-balanced sweep (Adam, d = 2, more than 1024
  * rows) with negative_sample_rate <= 64 (any span > 0 on another plan is an
  * error; span < 0 leaves it off). */
@@ -670,8 +671,8 @@ umaphip_status umaphip_batch_set_persistent(umaphip_batch *batch, int32_t persis
  * and stores its term count per epoch (one byte per edge and epoch), which
  * the tile sweeps read instead of the four f64 clock words; 0 advances the
  * clocks inside the sweeps. Both give the same bits. A batch allows it when
- * those bytes fit 8 GiB and every count provably fits a byte; otherwise it
- * stays off. umaphip_batch_clock_schedule returns the epochs the schedule
+ * those bytes fit 8 GiB, every count provably fits a byte and the memory can
+ * be allocated; otherwise it stays off. umaphip_batch_clock_schedule returns the epochs the schedule
  * covers (0: off). */
 umaphip_status umaphip_batch_set_clock_schedule(umaphip_batch *batch, int32_t enabled);
 int32_t umaphip_batch_clock_schedule(const umaphip_batch *batch);
