@@ -900,7 +900,10 @@ __global__ __launch_bounds__(64 * WAVES) void knn_sample_select_kernel(const flo
         key[j] = i < m ? float_key(row[i]) : 0u;
     }
     uint32_t prefix = 0u, mask = 0u, want = kTop - 1;   // 0-based rank of the kTop-th smallest
-    for (int shift = 24; shift >= 0; shift -= 8) {
+    // Three byte passes fix the top 24 bits of the kTop-th smallest key; the
+    // bound is the largest value with those bits (at most 2^-15 relative
+    // above it), still an upper bound of every query's final kTop-th best.
+    for (int shift = 24; shift >= 8; shift -= 8) {
         for (int e = tid; e < WAVES * 256; e += T) (&s_hist[0][0])[e] = 0u;
         __syncthreads();
         if (shift == 24) {
@@ -948,7 +951,7 @@ __global__ __launch_bounds__(64 * WAVES) void knn_sample_select_kernel(const flo
         __syncthreads();
     }
     if (tid == 0) {
-        const float kth = key_float(prefix);
+        const float kth = key_float(prefix | 0xFFu);
         // sample values equal the scan's; the widening only guards the bound
         const float bound = m >= kTop && kth < INFINITY ? kth + fabsf(kth) * 0x1.0p-20f : INFINITY;
         tau_key[q] = float_key(bound);
